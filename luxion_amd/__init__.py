@@ -1,0 +1,257 @@
+"""luxion_amd -- MI355X-native wavefront path tracer (Python host binding).
+
+The renderer is the native library libluxion_mi355x.so (gfx950 HIP kernels
+behind the C ABI in include/luxion_mi355x.h); this module is a thin ctypes
+binding.  It mirrors the reference renderer's host interface
+(seabiscuit-iv/luxion):
+
+  reference                                  here
+  Scene(file, envmap)  (src/scene/scene.cpp)  Scene.load_glb(ctx, file, envmap) / Scene(ctx, SceneData)
+  pathtraceInit(Scene*)                       Render(scene, dmis=..., emis=...)
+  pathtrace(pbo, frame, iter)                 Render.pathtrace(iter)
+  copyImageToHost()                           Render.copy_image_to_host()  (per-pixel radiance sum)
+  -dmis / -emis / -e                          dmis= / emis= / envmap=
+
+There is no CPU fallback: every call goes to the GPU library and raises
+LuxionError if it is missing or fails.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _ffi as F
+from ._ffi import LuxionError, check, load
+from . import scenes
+
+__all__ = ["Context", "Scene", "Render", "LuxionError", "scenes", "load", "trace_closest", "trace_occluded"]
+
+
+class Context:
+    """One HIP device + stream with an HBM budget (0 = all free memory)."""
+
+    def __init__(self, device=0, hbm_budget_bytes=0):
+        self.lib = load()
+        self.h = C.c_void_p()
+        check(self.lib.lx_context_create(device, hbm_budget_bytes, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            self.lib.lx_context_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def bytes_in_use(self):
+        return self.lib.lx_context_bytes_in_use(self.h)
+
+    def tuning(self):
+        t = F.lx_tuning()
+        check(self.lib.lx_tuning_get(self.h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in F.lx_tuning._fields_}
+
+    def set_tuning(self, **kw):
+        t = F.lx_tuning()
+        check(self.lib.lx_tuning_get(self.h, C.byref(t)))
+        for k, v in kw.items():
+            setattr(t, k, v)
+        check(self.lib.lx_tuning_set(self.h, C.byref(t)))
+
+
+def build_opts(bvh_width=4, leaf_size=4, builder=1, sah_bins=16, cdf_precision=F.LX_CDF_FP64):
+    o = F.lx_build_opts()
+    o.bvh_width, o.leaf_size, o.builder, o.sah_bins, o.cdf_precision = (bvh_width, leaf_size, builder, sah_bins,
+                                                                        cdf_precision)
+    return o
+
+
+class Scene:
+    """A device-resident scene (meshes, instances, materials, textures, HDRI,
+    emitter / HDRI CDFs, two-level BVH)."""
+
+    def __init__(self, ctx, data=None, build=True, **build_kw):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.h = C.c_void_p()
+        self.data = data
+        if data is not None:
+            desc, keep = data.to_desc()
+            check(self.lib.lx_scene_create(ctx.h, C.byref(desc), C.byref(self.h)))
+            del keep
+            if build:
+                self.build(**build_kw)
+
+    @classmethod
+    def load_glb(cls, ctx, path, envmap=None, build=True, **build_kw):
+        s = cls(ctx)
+        check(s.lib.lx_scene_load_glb(ctx.h, path.encode(), envmap.encode() if envmap else None, C.byref(s.h)))
+        if build:
+            s.build(**build_kw)
+        return s
+
+    def build(self, **kw):
+        o = build_opts(**kw)
+        check(self.lib.lx_scene_build_accel(self.h, C.byref(o)))
+
+    def info(self):
+        i = F.lx_scene_info()
+        check(self.lib.lx_scene_get_info(self.h, C.byref(i)))
+        return {k: getattr(i, k) for k, _ in F.lx_scene_info._fields_}
+
+    def camera(self):
+        c = F.lx_camera_desc()
+        d = C.c_int32()
+        check(self.lib.lx_scene_get_camera(self.h, C.byref(c), C.byref(d)))
+        return dict(width=c.width, height=c.height, position=np.array(c.position[:], np.float32),
+                    view=np.array(c.view[:], np.float32), up=np.array(c.up[:], np.float32),
+                    right=np.array(c.right[:], np.float32),
+                    pixel_length=np.array(c.pixel_length[:], np.float32)), d.value
+
+    def env_cdfs(self):
+        i = self.info()
+        m = np.empty(i["env_height"], np.float32)
+        c = np.empty(i["env_height"] * i["env_width"], np.float32)
+        check(self.lib.lx_scene_read_env_cdfs(self.h, m.ctypes.data, c.ctypes.data))
+        return m, c
+
+    def emitter_cdfs(self):
+        n = C.c_uint32()
+        check(self.lib.lx_scene_read_emitter_cdfs(self.h, None, None, C.byref(n)))
+        inst = np.empty(n.value, np.uint32)
+        pre = np.empty(n.value, np.float32)
+        check(self.lib.lx_scene_read_emitter_cdfs(self.h, inst.ctypes.data, pre.ctypes.data, C.byref(n)))
+        tris = []
+        for k in range(n.value):
+            m = C.c_uint32()
+            check(self.lib.lx_scene_read_instance_tri_cdf(self.h, k, None, C.byref(m)))
+            t = np.empty(m.value, np.float32)
+            check(self.lib.lx_scene_read_instance_tri_cdf(self.h, k, t.ctypes.data, C.byref(m)))
+            tris.append(t)
+        return inst, pre, tris
+
+    def close(self):
+        if self.h:
+            self.lib.lx_scene_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def rays_array(origins, dirs, tmax=1e16):
+    n = len(origins)
+    r = np.zeros((n, 8), np.float32)
+    r[:, 0:3] = origins
+    r[:, 4:7] = dirs
+    r[:, 7] = tmax
+    return np.ascontiguousarray(r)
+
+
+def trace_closest(scene, rays, alpha=True, iteration=1):
+    """rays: (n, 8) float32 [ox oy oz tmin dx dy dz tmax]; returns a
+    structured array (t, instance, prim, u, v)."""
+    rays = np.ascontiguousarray(rays, np.float32)
+    out = np.zeros(len(rays), dtype=[("t", "<f4"), ("instance", "<u4"), ("prim", "<u4"), ("u", "<f4"),
+                                     ("v", "<f4")])
+    check(scene.lib.lx_trace_closest(scene.h, rays.ctypes.data, len(rays), int(alpha), iteration, out.ctypes.data))
+    return out
+
+
+def trace_occluded(scene, rays, alpha=True, iteration=1):
+    rays = np.ascontiguousarray(rays, np.float32)
+    out = np.zeros(len(rays), np.uint8)
+    check(scene.lib.lx_trace_occluded(scene.h, rays.ctypes.data, len(rays), int(alpha), iteration, out.ctypes.data))
+    return out
+
+
+class Render:
+    """pathtraceInit equivalent: per-render device buffers (path state, hit
+    records, accumulation)."""
+
+    def __init__(self, scene, max_depth=0, dmis=False, emis=False, envmap_intensity=1.0, seed=0, color_mode=2,
+                 camera=None):
+        self.scene = scene
+        self.lib = scene.lib
+        self.h = C.c_void_p()
+        o = F.lx_render_opts()
+        o.max_depth, o.dmis, o.emis = max_depth, int(dmis), int(emis)
+        o.envmap_intensity, o.seed, o.color_mode = envmap_intensity, seed, color_mode
+        self._cam = scenes.camera_desc(camera) if camera is not None else None
+        o.camera = C.pointer(self._cam) if self._cam is not None else None
+        check(self.lib.lx_render_create(scene.h, C.byref(o), C.byref(self.h)))
+        cam, depth = scene.camera()
+        self.width = camera["width"] if camera else cam["width"]
+        self.height = camera["height"] if camera else cam["height"]
+        self.depth = max_depth if max_depth > 0 else depth
+        self.next_iter = 1
+
+    def iterate(self, first_iter, n):
+        check(self.lib.lx_render_iterate(self.h, first_iter, n))
+        self.next_iter = first_iter + n
+
+    def pathtrace(self, iteration=None):
+        """One sample per pixel for the given 1-based iteration (reference
+        pathtrace(pbo, frame, iter))."""
+        it = self.next_iter if iteration is None else iteration
+        self.iterate(it, 1)
+
+    def sync(self):
+        check(self.lib.lx_render_sync(self.h))
+
+    def accum(self):
+        out = np.empty((self.height, self.width, 3), np.float32)
+        spp = C.c_uint32()
+        check(self.lib.lx_render_read_accum(self.h, out.ctypes.data, C.byref(spp)))
+        return out, spp.value
+
+    def copy_image_to_host(self):
+        return self.accum()[0]
+
+    def ldr(self, tonemap=F.LX_TONEMAP_ACES):
+        out = np.empty((self.height, self.width, 4), np.uint8)
+        check(self.lib.lx_render_read_ldr(self.h, out.ctypes.data, tonemap))
+        return out
+
+    def reset(self):
+        check(self.lib.lx_render_reset(self.h))
+        self.next_iter = 1
+
+    def set_timing(self, events=True, count_work=False):
+        flags = (F.LX_TIMING_EVENTS if events else 0) | (F.LX_TIMING_COUNT_WORK if count_work else 0)
+        check(self.lib.lx_render_set_timing(self.h, flags))
+
+    def stats(self):
+        s = F.lx_render_stats()
+        check(self.lib.lx_render_stats_get(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in F.lx_render_stats._fields_}
+
+    def debug_bounce(self, iteration):
+        N = self.width * self.height
+        D = self.depth
+        surv = np.zeros(D * N, np.uint32)
+        sc = np.zeros(D, np.uint32)
+        rec = np.zeros(D * N * 8, np.float32)
+        rc = np.zeros(D, np.uint32)
+        check(self.lib.lx_debug_bounce(self.h, iteration, surv.ctypes.data, sc.ctypes.data, rec.ctypes.data,
+                                       rc.ctypes.data))
+        survivors = [surv[d * N:d * N + sc[d]].copy() for d in range(D)]
+        records = [rec[d * N * 8:(d * N + rc[d]) * 8].reshape(-1, 8).copy() for d in range(D)]
+        return survivors, records
+
+    def close(self):
+        if self.h:
+            self.lib.lx_render_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

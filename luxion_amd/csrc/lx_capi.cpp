@@ -1,0 +1,578 @@
+/*
+ * lx_capi.cpp -- extern "C" entry points declared in include/luxion_mi355x.h.
+ * Every call converts exceptions into an lx_status and a per-thread message;
+ * nothing terminates the process (the reference exits on every error,
+ * include/common.h:16-36).
+ */
+#include "lx_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <numeric>
+
+static thread_local std::string g_err;
+
+void lx_set_error(const std::string& msg) { g_err = msg; }
+
+#define LX_TRY try {
+#define LX_CATCH                                                                                           \
+    }                                                                                                      \
+    catch (const LxError& e) {                                                                             \
+        lx_set_error(e.msg);                                                                               \
+        return e.status;                                                                                   \
+    }                                                                                                      \
+    catch (const std::bad_alloc&) {                                                                        \
+        lx_set_error("host allocation failed");                                                            \
+        return LX_ERR_OOM;                                                                                 \
+    }                                                                                                      \
+    catch (const std::exception& e) {                                                                      \
+        lx_set_error(e.what());                                                                            \
+        return LX_ERR_INVALID;                                                                             \
+    }                                                                                                      \
+    lx_set_error("");                                                                                      \
+    return LX_OK;
+
+/* ------------------------------------------------------------- context */
+void lx_context::set_device() const { LX_HIP(hipSetDevice(device)); }
+
+void* lx_context::dalloc(size_t bytes) {
+    uint64_t cur = in_use.load();
+    if (budget && cur + bytes > budget)
+        throw LxError{LX_ERR_BUDGET, "HBM budget exceeded: " + std::to_string(cur + bytes) + " > " +
+                                         std::to_string(budget) + " bytes"};
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) throw LxError{LX_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e)};
+    in_use += bytes;
+    return p;
+}
+
+void lx_context::dfree(void* p, size_t bytes) {
+    if (!p) return;
+    (void)hipFree(p);
+    in_use -= bytes;
+}
+
+extern "C" {
+
+const char* lx_last_error(void) { return g_err.c_str(); }
+int lx_abi_version(void) { return LX_ABI_VERSION; }
+
+double lx_now_ms(void) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+lx_status lx_context_create(int device, uint64_t budget, lx_context** out) {
+    LX_TRY
+    LX_REQUIRE(out, LX_ERR_INVALID, "null output pointer");
+    *out = nullptr;
+    int n = 0;
+    LX_HIP(hipGetDeviceCount(&n));
+    LX_REQUIRE(device >= 0 && device < n, LX_ERR_INVALID, "device ordinal out of range");
+    lx_context* c = new lx_context();
+    c->device = device;
+    try {
+        c->set_device();
+        LX_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        hipDeviceProp_t prop;
+        LX_HIP(hipGetDeviceProperties(&prop, device));
+        c->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        size_t fr = 0, tot = 0;
+        LX_HIP(hipMemGetInfo(&fr, &tot));
+        c->budget = budget ? budget : (uint64_t)fr;
+        c->tuning.block_size = LX_TRACE_BLOCK;
+        c->tuning.bvh_width = 4;
+        c->tuning.queue_chunk = 64;
+        c->tuning.waves_per_cu = 16;
+        c->tuning.use_graph = 1;
+    } catch (...) {
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+        delete c;
+        throw;
+    }
+    *out = c;
+    LX_CATCH
+}
+
+void lx_context_destroy(lx_context* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipStreamDestroy(c->stream);
+    }
+    delete c;
+}
+
+uint64_t lx_context_bytes_in_use(const lx_context* c) { return c ? c->in_use.load() : 0; }
+
+lx_status lx_tuning_get(const lx_context* c, lx_tuning* out) {
+    LX_TRY
+    LX_REQUIRE(c && out, LX_ERR_INVALID, "null argument");
+    *out = c->tuning;
+    LX_CATCH
+}
+
+lx_status lx_tuning_set(lx_context* c, const lx_tuning* t) {
+    LX_TRY
+    LX_REQUIRE(c && t, LX_ERR_INVALID, "null argument");
+    LX_REQUIRE(t->waves_per_cu >= 1 && t->waves_per_cu <= 32, LX_ERR_INVALID, "waves_per_cu out of range");
+    LX_REQUIRE(t->bvh_width == 4 || t->bvh_width == 8, LX_ERR_INVALID, "bvh_width must be 4 or 8");
+    c->tuning = *t;
+    LX_CATCH
+}
+
+/* --------------------------------------------------------------- scene */
+lx_status lx_scene_create(lx_context* ctx, const lx_scene_desc* desc, lx_scene** out) {
+    LX_TRY
+    LX_REQUIRE(ctx && desc && out, LX_ERR_INVALID, "null argument");
+    *out = nullptr;
+    lx_scene* s = new lx_scene();
+    s->ctx = ctx;
+    try {
+        lx_scene_init(s, desc);
+    } catch (...) {
+        delete s;
+        throw;
+    }
+    *out = s;
+    LX_CATCH
+}
+
+lx_status lx_scene_build_accel(lx_scene* s, const lx_build_opts* o) {
+    LX_TRY
+    LX_REQUIRE(s, LX_ERR_INVALID, "null scene");
+    std::lock_guard<std::mutex> lk(s->ctx->mu);
+    lx_scene_accel(s, o);
+    LX_CATCH
+}
+
+lx_status lx_scene_get_camera(const lx_scene* s, lx_camera_desc* out, int32_t* depth) {
+    LX_TRY
+    LX_REQUIRE(s, LX_ERR_INVALID, "null scene");
+    if (out) *out = s->cam;
+    if (depth) *depth = s->trace_depth;
+    LX_CATCH
+}
+
+lx_status lx_scene_get_info(const lx_scene* s, lx_scene_info* out) {
+    LX_TRY
+    LX_REQUIRE(s && out, LX_ERR_INVALID, "null argument");
+    *out = s->info;
+    LX_CATCH
+}
+
+lx_status lx_scene_read_env_cdfs(const lx_scene* s, float* marginal, float* conditional) {
+    LX_TRY
+    LX_REQUIRE(s && s->has_env, LX_ERR_STATE, "scene has no environment map");
+    LX_REQUIRE(s->accel_built, LX_ERR_STATE, "call lx_scene_build_accel first");
+    if (marginal) std::memcpy(marginal, s->marginal.data(), s->marginal.size() * 4);
+    if (conditional) std::memcpy(conditional, s->conditional.data(), s->conditional.size() * 4);
+    LX_CATCH
+}
+
+lx_status lx_scene_read_emitter_cdfs(const lx_scene* s, uint32_t* inst, float* prefix, uint32_t* num) {
+    LX_TRY
+    LX_REQUIRE(s && s->accel_built, LX_ERR_STATE, "call lx_scene_build_accel first");
+    if (num) *num = (uint32_t)s->emissive.size();
+    if (inst) std::memcpy(inst, s->emissive.data(), s->emissive.size() * 4);
+    if (prefix) std::memcpy(prefix, s->emissive_prefix.data(), s->emissive_prefix.size() * 4);
+    LX_CATCH
+}
+
+lx_status lx_scene_read_instance_tri_cdf(const lx_scene* s, uint32_t k, float* tri_prefix, uint32_t* n) {
+    LX_TRY
+    LX_REQUIRE(s && s->accel_built, LX_ERR_STATE, "call lx_scene_build_accel first");
+    LX_REQUIRE(k < s->emissive.size(), LX_ERR_NOT_FOUND, "emissive index out of range");
+    uint32_t a = s->tri_cdf_off[k], b = s->tri_cdf_off[k + 1];
+    if (n) *n = b - a;
+    if (tri_prefix) std::memcpy(tri_prefix, s->tri_cdf.data() + a, (size_t)(b - a) * 4);
+    LX_CATCH
+}
+
+void lx_scene_destroy(lx_scene* s) {
+    if (!s) return;
+    if (s->ctx) (void)hipStreamSynchronize(s->ctx->stream);
+    delete s;
+}
+
+/* ------------------------------------------------------------ test hooks */
+static lx_status trace_hook(lx_scene* s, const lx_ray* rays, uint32_t n, int any, int alpha, uint32_t iteration,
+                            lx_hit* hits, uint8_t* occ) {
+    LX_TRY
+    LX_REQUIRE(s && s->accel_built, LX_ERR_STATE, "call lx_scene_build_accel first");
+    LX_REQUIRE(rays && (any ? occ != nullptr : hits != nullptr), LX_ERR_INVALID, "null argument");
+    if (n == 0) return LX_OK;
+    lx_context* ctx = s->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->set_device();
+    uint32_t grid = std::max(1u, std::min((n + LX_TRACE_BLOCK - 1) / LX_TRACE_BLOCK, (uint32_t)ctx->num_cus * 8u));
+    DevBuf dr, dh, dsp, dov;
+    dr.upload(ctx, rays, (size_t)n * sizeof(lx_ray));
+    dh.alloc(ctx, any ? (size_t)n : (size_t)n * sizeof(lx_hit));
+    dsp.alloc(ctx, (size_t)lx_spill_words(grid) * 4);
+    dov.alloc(ctx, 256);
+    LX_HIP(lx_launch_fill_u32(dov.as<uint32_t>(), 64, 0u, ctx->stream));
+    LX_HIP(lx_launch_trace_rays(s->d_scene.as<LxDevScene>(), dr.as<lx_ray>(), n, any, alpha, iteration,
+                                any ? nullptr : dh.as<lx_hit>(), any ? dh.as<uint8_t>() : nullptr,
+                                dsp.as<uint32_t>(), dov.as<uint32_t>(), grid, ctx->stream));
+    uint32_t ov = 0;
+    if (any) LX_HIP(hipMemcpyAsync(occ, dh.ptr, n, hipMemcpyDeviceToHost, ctx->stream));
+    else LX_HIP(hipMemcpyAsync(hits, dh.ptr, (size_t)n * sizeof(lx_hit), hipMemcpyDeviceToHost, ctx->stream));
+    LX_HIP(hipMemcpyAsync(&ov, dov.ptr, 4, hipMemcpyDeviceToHost, ctx->stream));
+    LX_HIP(hipStreamSynchronize(ctx->stream));
+    LX_REQUIRE(ov == 0, LX_ERR_STATE, "traversal stack overflow (BVH deeper than the supported stack)");
+    LX_CATCH
+}
+
+lx_status lx_trace_closest(lx_scene* s, const lx_ray* rays, uint32_t n, int32_t alpha, uint32_t iteration,
+                           lx_hit* hits) {
+    return trace_hook(s, rays, n, 0, alpha, iteration, hits, nullptr);
+}
+
+lx_status lx_trace_occluded(lx_scene* s, const lx_ray* rays, uint32_t n, int32_t alpha, uint32_t iteration,
+                            uint8_t* occ) {
+    return trace_hook(s, rays, n, 1, alpha, iteration, nullptr, occ);
+}
+
+lx_status lx_debug_math_eval(lx_context* ctx, int32_t fn, const float* x, const float* y, float* out, uint32_t n) {
+    LX_TRY
+    LX_REQUIRE(ctx && x && out, LX_ERR_INVALID, "null argument");
+    if (n == 0) return LX_OK;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->set_device();
+    DevBuf dx, dy, dout;
+    dx.upload(ctx, x, (size_t)n * 4);
+    if (y) dy.upload(ctx, y, (size_t)n * 4);
+    dout.alloc(ctx, (size_t)n * 4);
+    LX_HIP(lx_launch_math_eval(fn, dx.as<float>(), y ? dy.as<float>() : nullptr, dout.as<float>(), n, ctx->stream));
+    LX_HIP(hipMemcpyAsync(out, dout.ptr, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    LX_HIP(hipStreamSynchronize(ctx->stream));
+    LX_CATCH
+}
+
+/* -------------------------------------------------------------- render */
+static void render_grids(lx_context* ctx, uint32_t cap, uint32_t& tg, uint32_t& sg, uint32_t& cg) {
+    uint32_t waves = (uint32_t)std::max(1, ctx->tuning.waves_per_cu);
+    tg = (uint32_t)ctx->num_cus * std::max(1u, waves * 64u / LX_TRACE_BLOCK);
+    tg = std::min(tg, std::max(1u, (cap + LX_TRACE_BLOCK - 1) / LX_TRACE_BLOCK));
+    sg = std::min((uint32_t)ctx->num_cus * 8u, std::max(1u, (cap + LX_SHADE_BLOCK - 1) / LX_SHADE_BLOCK));
+    cg = std::min((uint32_t)ctx->num_cus * 8u, std::max(1u, (cap + 255u) / 256u));
+}
+
+static void validate_flags(const lx_scene* s, int dmis, int emis) {
+    if (dmis)
+        LX_REQUIRE(!s->emissive.empty() && !(s->total_area < 1e-5f), LX_ERR_INVALID,
+                   "-dmis requires the scene to contain emissive geometry");
+    if (emis) LX_REQUIRE(s->has_env, LX_ERR_INVALID, "-emis requires an environment map, pass one with -e");
+}
+
+static bool scene_has_normal_maps(const lx_scene* s) {
+    for (auto& m : s->mats)
+        if (m.normal_tex >= 0) return true;
+    return false;
+}
+
+lx_status lx_render_create(lx_scene* s, const lx_render_opts* o, lx_render** out) {
+    LX_TRY
+    LX_REQUIRE(s && o && out, LX_ERR_INVALID, "null argument");
+    LX_REQUIRE(s->accel_built, LX_ERR_STATE, "call lx_scene_build_accel first");
+    *out = nullptr;
+    validate_flags(s, o->dmis, o->emis);
+    lx_context* ctx = s->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->set_device();
+    lx_render* r = new lx_render();
+    try {
+        r->scene = s;
+        r->ctx = ctx;
+        r->opts = *o;
+        r->cam = o->camera ? *o->camera : s->cam;
+        LX_REQUIRE(r->cam.width > 0 && r->cam.height > 0, LX_ERR_INVALID, "camera resolution must be positive");
+        r->depth = o->max_depth > 0 ? o->max_depth : s->trace_depth;
+        LX_REQUIRE(r->depth <= LX_MAX_DEPTH, LX_ERR_INVALID, "max_depth above the supported maximum");
+        r->npix = (uint32_t)r->cam.width * (uint32_t)r->cam.height;
+        r->wb.alloc(ctx, r->npix, o->dmis != 0, o->emis != 0, scene_has_normal_maps(s));
+        r->accum.alloc(ctx, (size_t)r->npix * 12);
+        LX_HIP(lx_launch_fill_u32(r->accum.as<uint32_t>(), (size_t)r->npix * 3, 0u, ctx->stream));
+        r->jobs.alloc(ctx, sizeof(LxJob));
+        r->job_spp.alloc(ctx, 16);
+        r->counters.alloc(ctx, sizeof(LxCounters));
+        LX_HIP(lx_launch_fill_u32(r->counters.as<uint32_t>(), sizeof(LxCounters) / 4, 0u, ctx->stream));
+        r->sched.alloc(ctx, sizeof(LxSched));
+        render_grids(ctx, r->npix, r->trace_grid, r->shade_grid, r->cam_grid);
+        r->spill.alloc(ctx, (size_t)lx_spill_words(r->trace_grid) * 4);
+        LxJob& J = r->job;
+        std::memset(&J, 0, sizeof(J));
+        J.scene = s->d_scene.as<LxDevScene>();
+        J.cam = r->cam;
+        J.num_pix = r->npix;
+        J.accum_off = 0;
+        J.depth = r->depth;
+        J.dmis = o->dmis;
+        J.emis = o->emis;
+        J.intensity = o->envmap_intensity;
+        J.seed = o->seed;
+        LX_HIP(hipStreamSynchronize(ctx->stream));
+    } catch (...) {
+        delete r;
+        throw;
+    }
+    *out = r;
+    LX_CATCH
+}
+
+static void upload_job(lx_render* r, uint32_t first_iter, uint32_t n) {
+    r->job.iter = (int32_t)first_iter - 1;
+    r->job.spp_done = 0;
+    r->job.active = 0;
+    int32_t spp = (int32_t)n;
+    LX_HIP(hipMemcpyAsync(r->jobs.ptr, &r->job, sizeof(LxJob), hipMemcpyHostToDevice, r->ctx->stream));
+    LX_HIP(hipMemcpyAsync(r->job_spp.ptr, &spp, 4, hipMemcpyHostToDevice, r->ctx->stream));
+}
+
+static void enqueue(lx_render* r, hipEvent_t* ev) {
+    lx_enqueue_round(r->ctx, r->jobs.as<LxJob>(), r->job_spp.as<int32_t>(), 1, r->counters.as<LxCounters>(),
+                     r->sched.as<LxSched>(), r->npix, r->wb, r->accum.as<float>(), r->spill.as<uint32_t>(), r->depth,
+                     r->trace_grid, r->shade_grid, r->cam_grid, ev, r->counting);
+}
+
+lx_status lx_render_iterate(lx_render* r, uint32_t first_iter, uint32_t n) {
+    LX_TRY
+    LX_REQUIRE(r, LX_ERR_INVALID, "null render");
+    LX_REQUIRE(first_iter >= 1, LX_ERR_INVALID, "iterations are 1-based");
+    if (n == 0) return LX_OK;
+    lx_context* ctx = r->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->set_device();
+    upload_job(r, first_iter, n);
+    if (r->timing || r->counting) {
+        /* eager launches; with timing, one event between consecutive kernels
+         * of every round, read back after a single synchronisation */
+        int per = 2 * r->depth + 1;
+        std::vector<hipEvent_t> ev;
+        if (r->timing) {
+            ev.resize((size_t)per * n + 2 * n);
+            for (auto& e : ev) LX_HIP(hipEventCreate(&e));
+        }
+        for (uint32_t k = 0; k < n; k++) {
+            hipEvent_t* e = r->timing ? &ev[(size_t)k * (per + 2)] : nullptr;
+            if (e) LX_HIP(hipEventRecord(e[0], ctx->stream));
+            enqueue(r, e ? e + 1 : nullptr);
+            if (e) LX_HIP(hipEventRecord(e[per + 1], ctx->stream));
+        }
+        LX_HIP(hipStreamSynchronize(ctx->stream));
+        if (r->timing) {
+            for (uint32_t k = 0; k < n; k++) {
+                hipEvent_t* e = &ev[(size_t)k * (per + 2)];
+                float ms;
+                for (int b = 0; b < r->depth; b++) {
+                    LX_HIP(hipEventElapsedTime(&ms, e[1 + 2 * b], e[2 + 2 * b]));
+                    r->trace_ms += ms;
+                    LX_HIP(hipEventElapsedTime(&ms, e[2 + 2 * b], e[3 + 2 * b]));
+                    r->shade_ms += ms;
+                }
+                LX_HIP(hipEventElapsedTime(&ms, e[0], e[1]));
+                r->other_ms += ms;
+                LX_HIP(hipEventElapsedTime(&ms, e[per], e[per + 1]));
+                r->other_ms += ms;
+            }
+            r->trace_launches += r->depth * n;
+            r->shade_launches += r->depth * n;
+            for (auto& e : ev) (void)hipEventDestroy(e);
+        }
+    } else if (ctx->tuning.use_graph) {
+        if (!r->graph) {
+            LX_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            try {
+                enqueue(r, nullptr);
+            } catch (...) {
+                hipGraph_t g;
+                (void)hipStreamEndCapture(ctx->stream, &g);
+                throw;
+            }
+            LX_HIP(hipStreamEndCapture(ctx->stream, &r->graph_src));
+            LX_HIP(hipGraphInstantiate(&r->graph, r->graph_src, nullptr, nullptr, 0));
+        }
+        for (uint32_t k = 0; k < n; k++) LX_HIP(hipGraphLaunch(r->graph, ctx->stream));
+    } else {
+        for (uint32_t k = 0; k < n; k++) enqueue(r, nullptr);
+    }
+    r->spp += n;
+    LX_CATCH
+}
+
+lx_status lx_render_sync(lx_render* r) {
+    LX_TRY
+    LX_REQUIRE(r, LX_ERR_INVALID, "null render");
+    r->ctx->set_device();
+    LX_HIP(hipStreamSynchronize(r->ctx->stream));
+    uint32_t ov = 0;
+    LX_HIP(hipMemcpy(&ov, (uint8_t*)r->counters.ptr + offsetof(LxCounters, stack_overflow), 4, hipMemcpyDeviceToHost));
+    LX_REQUIRE(ov == 0, LX_ERR_STATE, "traversal stack overflow (BVH deeper than the supported stack)");
+    LX_CATCH
+}
+
+lx_status lx_render_read_accum(lx_render* r, float* rgb, uint32_t* spp) {
+    LX_TRY
+    LX_REQUIRE(r, LX_ERR_INVALID, "null render");
+    lx_status st = lx_render_sync(r);
+    if (st != LX_OK) return st;
+    if (rgb)
+        LX_HIP(hipMemcpy(rgb, r->accum.ptr, (size_t)r->npix * 12, hipMemcpyDeviceToHost));
+    if (spp) *spp = r->spp;
+    LX_CATCH
+}
+
+lx_status lx_render_read_ldr(lx_render* r, uint8_t* rgba, int32_t tonemap) {
+    LX_TRY
+    LX_REQUIRE(r && rgba, LX_ERR_INVALID, "null argument");
+    LX_REQUIRE(r->spp > 0, LX_ERR_STATE, "no samples accumulated");
+    std::lock_guard<std::mutex> lk(r->ctx->mu);
+    r->ctx->set_device();
+    if (!r->ldr.ptr) r->ldr.alloc(r->ctx, (size_t)r->npix * 4);
+    LX_HIP(lx_launch_tonemap(r->accum.as<float>(), r->npix, (float)r->spp, tonemap, r->ldr.as<uint8_t>(),
+                             r->ctx->stream));
+    LX_HIP(hipMemcpyAsync(rgba, r->ldr.ptr, (size_t)r->npix * 4, hipMemcpyDeviceToHost, r->ctx->stream));
+    LX_HIP(hipStreamSynchronize(r->ctx->stream));
+    LX_CATCH
+}
+
+lx_status lx_render_reset(lx_render* r) {
+    LX_TRY
+    LX_REQUIRE(r, LX_ERR_INVALID, "null render");
+    std::lock_guard<std::mutex> lk(r->ctx->mu);
+    r->ctx->set_device();
+    LX_HIP(lx_launch_fill_u32(r->accum.as<uint32_t>(), (size_t)r->npix * 3, 0u, r->ctx->stream));
+    LX_HIP(lx_launch_fill_u32(r->counters.as<uint32_t>(), sizeof(LxCounters) / 4, 0u, r->ctx->stream));
+    LX_HIP(hipStreamSynchronize(r->ctx->stream));
+    r->spp = 0;
+    r->trace_ms = r->shade_ms = r->other_ms = 0;
+    r->trace_launches = r->shade_launches = 0;
+    LX_CATCH
+}
+
+lx_status lx_render_set_timing(lx_render* r, int32_t flags) {
+    LX_TRY
+    LX_REQUIRE(r, LX_ERR_INVALID, "null render");
+    r->timing = (flags & LX_TIMING_EVENTS) != 0;
+    r->counting = (flags & LX_TIMING_COUNT_WORK) != 0;
+    LX_CATCH
+}
+
+lx_status lx_render_stats_get(lx_render* r, lx_render_stats* out) {
+    LX_TRY
+    LX_REQUIRE(r && out, LX_ERR_INVALID, "null argument");
+    lx_status st = lx_render_sync(r);
+    if (st != LX_OK) return st;
+    LxCounters c;
+    LX_HIP(hipMemcpy(&c, r->counters.ptr, sizeof(c), hipMemcpyDeviceToHost));
+    std::memset(out, 0, sizeof(*out));
+    out->radiance_rays = c.rad_rays;
+    out->dl_shadow_rays = c.dl_rays;
+    out->env_shadow_rays = c.env_rays;
+    out->path_bounces = c.bounces;
+    out->iterations = r->spp;
+    out->trace_ms = r->trace_ms;
+    out->shade_ms = r->shade_ms;
+    out->other_ms = r->other_ms;
+    out->trace_launches = r->trace_launches;
+    out->shade_launches = r->shade_launches;
+    out->closest_nodes = c.closest_nodes;
+    out->closest_tris = c.closest_tris;
+    out->closest_insts = c.closest_insts;
+    out->any_nodes = c.any_nodes;
+    out->any_tris = c.any_tris;
+    out->any_insts = c.any_insts;
+    LX_CATCH
+}
+
+void lx_render_destroy(lx_render* r) {
+    if (!r) return;
+    (void)hipSetDevice(r->ctx->device);
+    (void)hipStreamSynchronize(r->ctx->stream);
+    if (r->graph) (void)hipGraphExecDestroy(r->graph);
+    if (r->graph_src) (void)hipGraphDestroy(r->graph_src);
+    delete r;
+}
+
+lx_status lx_debug_bounce(lx_render* r, uint32_t iter, uint32_t* survivors, uint32_t* survivor_counts,
+                          float* records, uint32_t* record_counts) {
+    LX_TRY
+    LX_REQUIRE(r && survivor_counts && record_counts, LX_ERR_INVALID, "null argument");
+    LX_REQUIRE(iter >= 1, LX_ERR_INVALID, "iterations are 1-based");
+    lx_context* ctx = r->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->set_device();
+    hipStream_t s = ctx->stream;
+    upload_job(r, iter, 1);
+    LxJob* dj = r->jobs.as<LxJob>();
+    LxCounters* dc = r->counters.as<LxCounters>();
+    LX_HIP(lx_launch_begin_round(dj, r->job_spp.as<int32_t>(), 1, dc, r->sched.as<LxSched>(), r->npix, s));
+    LX_HIP(lx_launch_camera(dj, r->sched.as<LxSched>(), dc, r->wb.A, r->cam_grid, s));
+    LxPathBuf* cur = &r->wb.A;
+    LxPathBuf* nxt = &r->wb.B;
+    size_t N = r->npix;
+    std::vector<float> t(N), nx(N), ny(N), nz(N), u(N), v(N);
+    std::vector<int32_t> mat(N);
+    std::vector<uint32_t> pix(N);
+    for (int b = 0; b < r->depth; b++) {
+        LX_HIP(lx_launch_trace(dj, *cur, r->wb.R, dc, b, r->spill.as<uint32_t>(), r->trace_grid, s));
+        LX_HIP(lx_launch_shade(dj, *cur, *nxt, r->wb.R, dc, b, r->accum.as<float>(), r->shade_grid, s));
+        LX_HIP(hipStreamSynchronize(s));
+        uint32_t cnt[2];
+        LX_HIP(hipMemcpy(cnt, (uint8_t*)r->counters.ptr + 4 * b, 8, hipMemcpyDeviceToHost));
+        uint32_t nb = cnt[0];
+        record_counts[b] = nb;
+        if (nb) {
+            LX_HIP(hipMemcpy(t.data(), r->wb.R.t, nb * 4, hipMemcpyDeviceToHost));
+            LX_HIP(hipMemcpy(nx.data(), r->wb.R.nx, nb * 4, hipMemcpyDeviceToHost));
+            LX_HIP(hipMemcpy(ny.data(), r->wb.R.ny, nb * 4, hipMemcpyDeviceToHost));
+            LX_HIP(hipMemcpy(nz.data(), r->wb.R.nz, nb * 4, hipMemcpyDeviceToHost));
+            LX_HIP(hipMemcpy(u.data(), r->wb.R.u, nb * 4, hipMemcpyDeviceToHost));
+            LX_HIP(hipMemcpy(v.data(), r->wb.R.v, nb * 4, hipMemcpyDeviceToHost));
+            LX_HIP(hipMemcpy(mat.data(), r->wb.R.mat, nb * 4, hipMemcpyDeviceToHost));
+            LX_HIP(hipMemcpy(pix.data(), cur->pix, nb * 4, hipMemcpyDeviceToHost));
+            std::vector<uint32_t> order(nb);
+            std::iota(order.begin(), order.end(), 0u);
+            std::sort(order.begin(), order.end(),
+                      [&](uint32_t a, uint32_t c) { return (pix[a] & 0x7fffffffu) < (pix[c] & 0x7fffffffu); });
+            if (records)
+                for (uint32_t k = 0; k < nb; k++) {
+                    uint32_t q = order[k];
+                    bool hit = t[q] > 0.0f;
+                    float* o = records + ((size_t)b * N + k) * 8;
+                    o[0] = hit ? t[q] : -1.0f;
+                    o[1] = hit ? nx[q] : 0.0f;
+                    o[2] = hit ? ny[q] : 0.0f;
+                    o[3] = hit ? nz[q] : 0.0f;
+                    o[4] = hit ? u[q] : 0.0f;
+                    o[5] = hit ? v[q] : 0.0f;
+                    o[6] = hit ? (float)mat[q] : -1.0f;
+                    o[7] = (float)(pix[q] & 0x7fffffffu);
+                }
+        }
+        bool compact = b + 1 < r->depth;
+        uint32_t ns = compact ? cnt[1] : 0u;
+        survivor_counts[b] = ns;
+        if (ns) {
+            std::vector<uint32_t> sp(ns);
+            LX_HIP(hipMemcpy(sp.data(), nxt->pix, ns * 4, hipMemcpyDeviceToHost));
+            for (auto& x : sp) x &= 0x7fffffffu;
+            std::sort(sp.begin(), sp.end());
+            if (survivors) std::memcpy(survivors + (size_t)b * N, sp.data(), ns * 4);
+        }
+        std::swap(cur, nxt);
+        if (nb == 0) {
+            for (int b2 = b + 1; b2 < r->depth; b2++) survivor_counts[b2] = record_counts[b2] = 0;
+            break;
+        }
+    }
+    LX_HIP(lx_launch_end_round(dj, 1, s));
+    LX_HIP(hipStreamSynchronize(s));
+    r->spp += 1;
+    LX_CATCH
+}
+
+} /* extern "C" */

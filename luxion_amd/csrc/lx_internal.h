@@ -1,0 +1,197 @@
+/*
+ * lx_internal.h -- host-side objects behind the C ABI.
+ */
+#ifndef LX_INTERNAL_H
+#define LX_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "lx_kernels.h"
+#include "lx_types.h"
+
+/* ------------------------------------------------------------- errors */
+void lx_set_error(const std::string& msg);
+
+struct LxError {
+    lx_status status;
+    std::string msg;
+};
+
+#define LX_HIP(call)                                                                                       \
+    do {                                                                                                   \
+        hipError_t e_ = (call);                                                                            \
+        if (e_ != hipSuccess)                                                                              \
+            throw LxError{LX_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)};                 \
+    } while (0)
+
+#define LX_REQUIRE(cond, st, msg)                                                                          \
+    do {                                                                                                   \
+        if (!(cond)) throw LxError{(st), (msg)};                                                           \
+    } while (0)
+
+/* ------------------------------------------------------------ context */
+struct lx_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0;
+    std::atomic<uint64_t> in_use{0};
+    int num_cus = 256;
+    lx_tuning tuning{};
+    std::mutex mu; /* serialises device work from the served form's thread */
+
+    void* dalloc(size_t bytes);           /* charged against the budget */
+    void dfree(void* p, size_t bytes);
+    void set_device() const;
+};
+
+/* RAII device buffer charged to a context. */
+struct DevBuf {
+    lx_context* ctx = nullptr;
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept { *this = std::move(o); }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        release();
+        ctx = o.ctx;
+        ptr = o.ptr;
+        bytes = o.bytes;
+        o.ptr = nullptr;
+        o.bytes = 0;
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(lx_context* c, size_t n) {
+        release();
+        ctx = c;
+        bytes = n;
+        ptr = n ? c->dalloc(n) : nullptr;
+    }
+    void upload(lx_context* c, const void* src, size_t n) {
+        alloc(c, n);
+        if (n) LX_HIP(hipMemcpyAsync(ptr, src, n, hipMemcpyHostToDevice, c->stream));
+    }
+    void release() {
+        if (ptr && ctx) ctx->dfree(ptr, bytes);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    template <class T>
+    T* as() const {
+        return (T*)ptr;
+    }
+};
+
+/* ---------------------------------------------------------- BVH build */
+struct BvhBuildResult {
+    std::vector<LxNode4> nodes;   /* local indices */
+    std::vector<float4> tris;     /* leaf-ordered (v0|prim, e1, e2) */
+    uint32_t root = 0;            /* child ref (node index or leaf ref), local */
+    int depth = 0;
+    double sah = 0.0;
+};
+
+/* Host binned-SAH build + collapse to BVH4 (builder 1). */
+void lx_build_blas_host(const float* pos, const uint32_t* idx, uint32_t ntri, int leaf_size, int bins,
+                        BvhBuildResult& out);
+/* TLAS over instance world boxes (one instance per leaf). */
+void lx_build_tlas_host(const std::vector<float>& boxes /* 6 per instance */, BvhBuildResult& out);
+/* GPU LBVH + binned-SAH refinement (builder 0); returns false if the GPU
+ * path is unavailable for this input. */
+bool lx_build_blas_gpu(lx_context* ctx, const float* pos, const uint32_t* idx, uint32_t ntri, int leaf_size, int bins,
+                       BvhBuildResult& out);
+void lx_pad_box(float lo[3], float hi[3]);
+
+/* -------------------------------------------------------------- scene */
+struct HostMesh {
+    std::vector<float> pos, nrm, uv;
+    std::vector<uint32_t> idx;
+    bool has_n = false, has_uv = false;
+};
+
+struct HostTex {
+    uint32_t w = 0, h = 0;
+    int32_t fmt = 0;
+    std::vector<uint8_t> data;
+};
+
+struct lx_scene {
+    lx_context* ctx = nullptr;
+    std::vector<HostMesh> meshes;
+    std::vector<lx_instance_desc> insts;
+    std::vector<lx_material_desc> mats;
+    std::vector<HostTex> texs;
+    bool has_env = false;
+    HostTex env;
+    lx_camera_desc cam{};
+    int trace_depth = 8;
+    int cdf_mode = LX_CDF_FP64;
+
+    /* host precompute */
+    std::vector<float> inv; /* 12 per instance */
+    std::vector<float> marginal, conditional;
+    std::vector<uint32_t> emissive;
+    std::vector<float> emissive_prefix;
+    std::vector<float> tri_cdf;
+    std::vector<uint32_t> tri_cdf_off;
+    float total_area = 0.0f;
+
+    /* device */
+    DevBuf d_nodes, d_tris, d_inst, d_idx, d_pos, d_nrm, d_uv, d_mats, d_texdesc, d_texels, d_env, d_marg, d_cond,
+        d_emis, d_emis_prefix, d_tri_cdf, d_tri_cdf_off, d_scene;
+    std::vector<DevBuf> d_tex;
+    LxDevScene dev{};
+    bool accel_built = false;
+    lx_scene_info info{};
+};
+
+void lx_scene_init(lx_scene* s, const lx_scene_desc* d); /* copies + precompute + uploads */
+void lx_scene_accel(lx_scene* s, const lx_build_opts* o);
+void lx_affine_inverse(const float* m, float* out);
+
+/* ------------------------------------------------------------- render */
+struct WaveBufs {
+    DevBuf pathA, pathB, rec;
+    LxPathBuf A{}, B{};
+    LxRecBuf R{};
+    uint32_t cap = 0;
+    void alloc(lx_context* ctx, uint32_t cap, bool dmis, bool emis, bool tangents);
+};
+
+struct lx_render {
+    lx_scene* scene = nullptr;
+    lx_context* ctx = nullptr;
+    lx_render_opts opts{};
+    lx_camera_desc cam{};
+    int depth = 8;
+    uint32_t npix = 0;
+    uint32_t spp = 0;
+    WaveBufs wb;
+    DevBuf accum, jobs, job_spp, counters, sched, spill, ldr;
+    LxJob job{};
+    uint32_t trace_grid = 0, shade_grid = 0, cam_grid = 0;
+    bool timing = false;
+    bool counting = false;
+    double trace_ms = 0, shade_ms = 0, other_ms = 0;
+    uint32_t trace_launches = 0, shade_launches = 0;
+    hipGraphExec_t graph = nullptr;
+    hipGraph_t graph_src = nullptr;
+};
+
+/* Records one round (begin, camera, bounces, end) on the stream. */
+void lx_enqueue_round(lx_context* ctx, LxJob* d_jobs, const int32_t* d_spp, uint32_t njobs, LxCounters* d_cnt,
+                      LxSched* d_sched, uint32_t path_cap, WaveBufs& wb, float* d_accum, uint32_t* d_spill,
+                      int max_depth, uint32_t trace_grid, uint32_t shade_grid, uint32_t cam_grid,
+                      hipEvent_t* ev /* 2*max_depth+1 events or NULL */, bool count = false);
+uint32_t lx_spill_words(uint32_t trace_grid);
+
+#endif

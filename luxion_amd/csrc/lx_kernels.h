@@ -1,0 +1,45 @@
+/*
+ * lx_kernels.h -- host-side launch entry points of the gfx950 kernels.
+ */
+#ifndef LX_KERNELS_H
+#define LX_KERNELS_H
+
+#include <hip/hip_runtime.h>
+#include "lx_types.h"
+
+#ifndef LX_TRACE_BLOCK
+#define LX_TRACE_BLOCK 128
+#endif
+#ifndef LX_SHADE_BLOCK
+#define LX_SHADE_BLOCK 256
+#endif
+#define LX_SCHED_MAX 2048
+#ifndef LX_STACK
+#define LX_STACK 24 /* per-lane LDS stack entries */
+#endif
+#define LX_SPILL 64 /* per-lane global spill entries beyond LX_STACK */
+
+struct LxSched {
+    uint32_t n_active;
+    uint32_t pad[3];
+    uint32_t idx[LX_SCHED_MAX];
+    uint32_t off[LX_SCHED_MAX];
+};
+
+hipError_t lx_launch_begin_round(LxJob* jobs, const int32_t* job_spp, uint32_t njobs, LxCounters* C, LxSched* sched,
+                                 uint32_t cap, hipStream_t s);
+hipError_t lx_launch_end_round(LxJob* jobs, uint32_t njobs, hipStream_t s);
+hipError_t lx_launch_camera(const LxJob* jobs, const LxSched* sched, const LxCounters* C, const LxPathBuf& P,
+                            uint32_t grid, hipStream_t s);
+hipError_t lx_launch_trace(const LxJob* jobs, const LxPathBuf& P, const LxRecBuf& R, LxCounters* C, int bounce,
+                           uint32_t* spill, uint32_t grid, hipStream_t s, bool count = false);
+hipError_t lx_launch_shade(const LxJob* jobs, const LxPathBuf& P, const LxPathBuf& Q, const LxRecBuf& R, LxCounters* C,
+                           int bounce, float* accum, uint32_t grid, hipStream_t s);
+hipError_t lx_launch_trace_rays(const LxDevScene* S, const lx_ray* rays, uint32_t n, int any, int alpha,
+                                uint32_t iteration, lx_hit* hits, uint8_t* occ, uint32_t* spill, uint32_t* overflow,
+                                uint32_t grid, hipStream_t s);
+hipError_t lx_launch_tonemap(const float* accum, uint32_t npix, float spp, int mode, uint8_t* out, hipStream_t s);
+hipError_t lx_launch_math_eval(int fn, const float* x, const float* y, float* out, uint32_t n, hipStream_t s);
+hipError_t lx_launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
+
+#endif

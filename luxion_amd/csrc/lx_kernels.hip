@@ -1,0 +1,586 @@
+/*
+ * lx_kernels.hip -- the wavefront bounce loop on gfx950.
+ *
+ * One round = one sample for every active job:
+ *   k_begin_round   device-side job scheduling: active jobs, path offsets,
+ *                   counters (no host round trip, so the round can be a
+ *                   captured hipGraph)
+ *   k_camera        generateRayFromCamera (src/kernels/generate_ray_from_camera.cu:8-39)
+ *   per bounce b = 0 .. maxdepth-1:
+ *     k_trace       sampleDirectLight (src/kernels/sample_direct_light.cu:9-101)
+ *                   + the raygen's radiance / direct-light / environment rays
+ *                   (src/optixshaders/optix_triangle.cu:52-143) + hit attributes
+ *     k_shade       shadePath + update_throughput_materials + compaction
+ *                   (src/kernels/shade_path.cu:94-364) fused with finalGather
+ *                   (src/kernels/image_kernels.cu:49-60): a path adds its colour
+ *                   to the accumulation buffer when it stops
+ *   k_end_round     spp bookkeeping
+ * Every kernel reads its path count from device memory and grid-strides over
+ * fixed launch shapes.  Paths are compacted by copying their state to the next
+ * buffer at a slot from one atomic per wave64 (ballot + mbcnt).
+ */
+#include "lx_trace_dev.h"
+#include "lx_kernels.h"
+
+#define LX_WAVE 64
+
+LXD uint32_t lx_lane() { return __lane_id(); }
+
+/* ------------------------------------------------------------ scheduling */
+__global__ void k_begin_round(LxJob* __restrict__ jobs, const int32_t* __restrict__ job_spp, uint32_t njobs,
+                              LxCounters* __restrict__ C, LxSched* __restrict__ sched, uint32_t path_cap) {
+    /* single thread: jobs that still need samples take consecutive path
+     * ranges until path_cap paths (or LX_SCHED_MAX jobs) are scheduled; the
+     * rest wait for a later round */
+    __shared__ uint32_t s_total;
+    if (threadIdx.x == 0) {
+        uint32_t off = 0, na = 0;
+        for (uint32_t j = 0; j < njobs; j++) {
+            LxJob& J = jobs[j];
+            bool want = J.spp_done < job_spp[j];
+            if (want && na < LX_SCHED_MAX && off + J.num_pix <= path_cap) {
+                J.active = 1;
+                J.path_off = off;
+                J.iter = J.iter + 1;
+                sched->idx[na] = j;
+                sched->off[na] = off;
+                na++;
+                off += J.num_pix;
+            } else {
+                J.active = 0;
+            }
+        }
+        sched->n_active = na;
+        s_total = off;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i <= LX_MAX_DEPTH; i += blockDim.x) C->count[i] = (i == 0) ? s_total : 0u;
+}
+
+__global__ void k_end_round(LxJob* __restrict__ jobs, uint32_t njobs) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < njobs && jobs[j].active) jobs[j].spp_done += 1;
+}
+
+/* generateRayFromCamera, src/kernels/generate_ray_from_camera.cu:8-39 */
+__global__ __launch_bounds__(256) void k_camera(const LxJob* __restrict__ jobs, const LxSched* __restrict__ sched,
+                                                const LxCounters* __restrict__ C, LxPathBuf P) {
+    __shared__ uint32_t s_off[LX_SCHED_MAX];
+    __shared__ uint32_t s_idx[LX_SCHED_MAX];
+    uint32_t nact = sched->n_active;
+    for (uint32_t k = threadIdx.x; k < nact; k += blockDim.x) {
+        s_off[k] = sched->off[k];
+        s_idx[k] = sched->idx[k];
+    }
+    __syncthreads();
+    uint32_t total = C->count[0];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        /* binary search over the active jobs' offsets */
+        uint32_t lo = 0, hi = nact - 1;
+        while (lo < hi) {
+            uint32_t mid = (lo + hi + 1) >> 1;
+            if (s_off[mid] <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        uint32_t j = s_idx[lo];
+        const LxJob& J = jobs[j];
+        uint32_t index = i - J.path_off;
+        int W = J.cam.width;
+        int x = (int)(index % (uint32_t)W), y = (int)(index / (uint32_t)W);
+        LxRng rng = lx_make_rng(J.iter, (int)index, J.depth, J.seed);
+        float x1 = rng.u01() - 0.5f;
+        float x2 = rng.u01() - 0.5f;
+        float pX = (float(x) + x1 + 0.5f) - (float)J.cam.width * 0.5f;
+        float pY = (float(y) + x2 + 0.5f) - (float)J.cam.height * 0.5f;
+        f3 view = mk3(J.cam.view[0], J.cam.view[1], J.cam.view[2]);
+        f3 right = mk3(J.cam.right[0], J.cam.right[1], J.cam.right[2]);
+        f3 up = mk3(J.cam.up[0], J.cam.up[1], J.cam.up[2]);
+        f3 dir = normalize((view - ((right * J.cam.pixel_length[0]) * pX)) - ((up * J.cam.pixel_length[1]) * pY));
+        P.ox[i] = J.cam.position[0];
+        P.oy[i] = J.cam.position[1];
+        P.oz[i] = J.cam.position[2];
+        P.dx[i] = dir.x;
+        P.dy[i] = dir.y;
+        P.dz[i] = dir.z;
+        P.tx[i] = 1.0f;
+        P.ty[i] = 1.0f;
+        P.tz[i] = 1.0f;
+        P.cx[i] = 0.0f;
+        P.cy[i] = 0.0f;
+        P.cz[i] = 0.0f;
+        P.pdf[i] = -1.0f;
+        P.pix[i] = index;
+        P.job[i] = j;
+    }
+}
+
+/* ------------------------------------------------------------ trace */
+LXD void lx_wave_add(unsigned long long* dst, uint32_t v) {
+    /* wave64 reduction, one atomic per wave */
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, LX_WAVE);
+    if (lx_lane() == 0 && v) atomicAdd(dst, (unsigned long long)v);
+}
+
+/* CNT = true is the counting replay: identical work plus per-ray node,
+ * triangle and instance visit counts for the algorithmic-bytes model. */
+template <bool CNT>
+__global__ __launch_bounds__(LX_TRACE_BLOCK) void k_trace(const LxJob* __restrict__ jobs, LxPathBuf P, LxRecBuf R,
+                                                           LxCounters* __restrict__ C, int bounce,
+                                                           uint32_t* __restrict__ spill) {
+    __shared__ uint32_t s_stack[LX_STACK * LX_TRACE_BLOCK];
+    uint32_t n = C->count[bounce];
+    uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t gsz = gridDim.x * blockDim.x;
+    LxStack st;
+    st.lds = s_stack + threadIdx.x;
+    st.stride = LX_TRACE_BLOCK;
+    st.spill = spill + gtid;
+    st.sstride = gsz;
+    uint32_t nrad = 0, ndl = 0, nenv = 0;
+    LxWork wc = {0, 0, 0}, wa = {0, 0, 0};
+    for (uint32_t i = gtid; i < n; i += gsz) {
+        const LxJob& J = jobs[P.job[i]];
+        const LxDevScene& S = *J.scene;
+        uint32_t pix = P.pix[i] & 0x7fffffffu;
+        f3 o = mk3(P.ox[i], P.oy[i], P.oz[i]);
+        f3 d = mk3(P.dx[i], P.dy[i], P.dz[i]);
+        f3 lp = mk3(0, 0, 0), ed = mk3(0, 0, 0);
+        if (J.dmis || J.emis) {
+            /* sampleDirectLight: RNG keyed on the pre-increment depth */
+            LxRng rng = lx_make_rng(J.iter, (int)pix, bounce, J.seed);
+            if (J.dmis) lp = lx_sample_light_point(S, rng);
+            if (J.emis) ed = lx_sample_env_dir(S, rng);
+        }
+        LxHitD h = lx_trace<false, CNT>(S, o, d, 1e16f, pix, (uint32_t)J.iter, LX_INST_ALPHA, st, &C->stack_overflow,
+                                        &wc);
+        nrad++;
+        if (h.inst != 0xffffffffu) {
+            f3 N, Tg;
+            f2 uv;
+            int mat;
+            bool want_t = S.mats[S.inst[h.inst].material].normal_tex >= 0;
+            lx_hit_attrs(S, h, want_t, N, Tg, uv, mat);
+            R.t[i] = h.t;
+            R.nx[i] = N.x;
+            R.ny[i] = N.y;
+            R.nz[i] = N.z;
+            R.u[i] = uv.x;
+            R.v[i] = uv.y;
+            R.mat[i] = mat;
+            if (want_t) {
+                R.tgx[i] = Tg.x;
+                R.tgy[i] = Tg.y;
+                R.tgz[i] = Tg.z;
+            }
+            f3 hp = o + (h.t * d);
+            f3 so = hp + (N * 1e-4f);
+            if (J.dmis) {
+                f3 ld = lp - so;
+                float dist = length(ld);
+                ld = ld * (1.0f / dist);
+                LxHitD hd = lx_trace<false, CNT>(S, so, ld, dist * 1.0001f + 1e-4f, pix, (uint32_t)J.iter,
+                                                 LX_INST_ALPHA, st, &C->stack_overflow, &wc);
+                ndl++;
+                if (hd.inst != 0xffffffffu) {
+                    f3 dn;
+                    f2 duv;
+                    int dm;
+                    lx_dl_attrs(S, hd, dn, duv, dm);
+                    R.dl_t[i] = hd.t;
+                    R.dl_nx[i] = dn.x;
+                    R.dl_ny[i] = dn.y;
+                    R.dl_nz[i] = dn.z;
+                    R.dl_u[i] = duv.x;
+                    R.dl_v[i] = duv.y;
+                    R.dl_mat[i] = dm;
+                } else {
+                    R.dl_t[i] = -1.0f;
+                }
+                R.lpx[i] = lp.x;
+                R.lpy[i] = lp.y;
+                R.lpz[i] = lp.z;
+            }
+            if (J.emis) {
+                LxHitD he = lx_trace<true, CNT>(S, so, ed, 1e16f, pix, (uint32_t)J.iter, LX_INST_ALPHA, st,
+                                                &C->stack_overflow, &wa);
+                nenv++;
+                R.env_t[i] = he.inst != 0xffffffffu ? 1.0f : -1.0f;
+                R.edx[i] = ed.x;
+                R.edy[i] = ed.y;
+                R.edz[i] = ed.z;
+            } else {
+                R.env_t[i] = 1.0f;
+            }
+        } else {
+            R.t[i] = -1.0f;
+            if (J.dmis) R.dl_t[i] = -1.0f;
+            R.env_t[i] = -0.5f;
+        }
+    }
+    lx_wave_add(&C->rad_rays, nrad);
+    lx_wave_add(&C->dl_rays, ndl);
+    lx_wave_add(&C->env_rays, nenv);
+    if (CNT) {
+        lx_wave_add(&C->closest_nodes, wc.nodes);
+        lx_wave_add(&C->closest_tris, wc.tris);
+        lx_wave_add(&C->closest_insts, wc.insts);
+        lx_wave_add(&C->any_nodes, wa.nodes);
+        lx_wave_add(&C->any_tris, wa.tris);
+        lx_wave_add(&C->any_insts, wa.insts);
+    }
+}
+
+/* ------------------------------------------------------------ shade */
+/* shadePath (src/kernels/shade_path.cu:94-364) for one path.  Updates the
+ * path registers in place; returns true if the path is pushed to the next
+ * queue. */
+LXD bool lx_shade_path(const LxJob& J, const LxDevScene& S, const LxRecBuf& R, uint32_t i, int depth, bool compact,
+                       f3& o, f3& d, f3& thr, f3& col, float& last_pdf, bool& spec, uint32_t pix) {
+    LxRng rng = lx_make_rng(J.iter, (int)pix, depth, J.seed);
+    bool kill = false;
+    float t = R.t[i];
+    f3 N = mk3(0, 0, 0);
+    f3 sample_dir = mk3(0, 0, 0);
+    bool has_exr = S.has_env != 0;
+    if (t > 0.0f) {
+        N = mk3(R.nx[i], R.ny[i], R.nz[i]);
+        f2 uv = mk2(R.u[i], R.v[i]);
+        const lx_material_desc& mat = S.mats[R.mat[i]];
+        f3 color = lx_get_albedo(S, mat, uv);
+        f3 normal = N;
+        if (mat.normal_tex >= 0) normal = lx_get_normal(S, mat, N, mk3(R.tgx[i], R.tgy[i], R.tgz[i]), uv);
+        float rough, metal;
+        lx_get_mr(S, mat, uv, rough, metal);
+        f3 emission = lx_get_emission(S, mat, uv);
+        bool is_specular = rough < 0.05f;
+        f3 nf = dot(-d, normal) < 0.0f ? -normal : normal;
+        sample_dir = lx_sample_ct(-d, normal, rough, metal, rng, color, mat.ior, mat.transmission);
+        bool emissive = length(emission) > 0.00001f;
+        if (J.emis) {
+            if (emissive) {
+                float w = 1.0f;
+                if (depth > 1 && !spec) {
+                    float pb = last_pdf;
+                    float pe = lx_envmap_pdf(S, d);
+                    w = (pb * pb) / (pb * pb + pe * pe);
+                }
+                col = col + thr * (w * emission);
+                kill = true;
+            } else if (R.env_t[i] < 0.0f && !is_specular && has_exr) {
+                f3 ed = mk3(R.edx[i], R.edy[i], R.edz[i]);
+                float pe = lx_envmap_pdf(S, ed);
+                float pb = lx_pdf(-d, ed, normal, rough, metal, color, mat.ior, mat.transmission);
+                float w = (pe * pe) / (pe * pe + pb * pb);
+                f3 brdf = lx_brdf(-d, normal, ed, color, rough, metal, mat.ior, mat.transmission);
+                f3 L = lx_env_lookup(S, normalize(ed), J.intensity);
+                float cs = dot(nf, ed);
+                if (cs > 0.0f && pe > 1e-8f) {
+                    f3 contrib = ((L * brdf) * cs) / pe;
+                    col = col + (thr * contrib) * w;
+                }
+            }
+        } else if (J.dmis) {
+            if (emissive) {
+                float w = 1.0f;
+                if (depth > 1 && !spec) {
+                    float pb = last_pdf;
+                    float dist_sq = t * t;
+                    float cl = dot(N, -d);
+                    if (cl > 0.0001f) {
+                        float pa = 1.0f / S.total_area;
+                        float psa = pa * (dist_sq / cl);
+                        w = (pb * pb) / (pb * pb + psa * psa);
+                    } else {
+                        w = 0.0f;
+                    }
+                }
+                col = col + thr * (w * emission);
+                kill = true;
+            } else if (R.dl_t[i] > 0.0f && !is_specular) {
+                f3 sp = (t * d) + o;
+                f3 lv = mk3(R.lpx[i], R.lpy[i], R.lpz[i]) - sp;
+                float dsq = dot(lv, lv);
+                float dist = lxm_sqrt(dsq);
+                f3 ld = lv / dist;
+                float cs = dot(nf, ld);
+                float cl = dot(mk3(R.dl_nx[i], R.dl_ny[i], R.dl_nz[i]), -ld);
+                if (cs > 0.0f && cl > 0.0f) {
+                    float pa = 1.0f / S.total_area;
+                    float psa = pa * (dsq / cl);
+                    float pb = lx_pdf(-d, ld, normal, rough, metal, color, mat.ior, mat.transmission);
+                    float w = (psa * psa) / (psa * psa + pb * pb);
+                    f3 brdf = lx_brdf(-d, normal, ld, color, rough, metal, mat.ior, mat.transmission);
+                    f3 Le = lx_get_emission(S, S.mats[R.dl_mat[i]], mk2(R.dl_u[i], R.dl_v[i]));
+                    f3 contrib = ((Le * brdf) * cs) / psa;
+                    col = col + (thr * contrib) * w;
+                }
+            }
+        } else if (emissive) {
+            col = col + thr * emission;
+            kill = true;
+        }
+        if (!kill) {
+            /* update_throughput_materials, src/update_throughput_materials.cu:8-67 */
+            bool transmitted = dot(-d, normal) * dot(sample_dir, normal) < 0.0f;
+            bool nspec = is_specular || transmitted;
+            f3 ct = lx_ct_eval_cos(-d, normal, sample_dir, color, rough, metal, mat.ior, mat.transmission);
+            float pdf = lx_pdf(-d, sample_dir, normal, rough, metal, color, mat.ior, mat.transmission);
+            thr = thr * (ct / pdf);
+            last_pdf = pdf;
+            spec = nspec;
+            if (depth >= 3) {
+                float mt = lxm_max(thr.x, lxm_max(thr.y, thr.z));
+                float sv = lxm_clamp(mt, 0.0f, 1.0f);
+                if (rng.u01() >= sv) kill = true;
+                else thr = thr / sv;
+            }
+        }
+    } else if (has_exr) {
+        f3 dn = normalize(d);
+        f3 L = lx_env_lookup(S, dn, J.intensity);
+        float w = 1.0f;
+        if (J.emis && depth > 1 && !spec) {
+            float pb = last_pdf;
+            float pe = lx_envmap_pdf(S, dn);
+            w = (pb * pb) / (pb * pb + pe * pe);
+        }
+        col = col + (thr * L) * w;
+    }
+    if (t == -1.0f) {
+        kill = true;
+    } else {
+        /* getPointOnRay (include/intersections.h:29-32) and the spawn offset */
+        f3 hp = o + ((t - .0001f) * normalize(d));
+        float l2 = dot(sample_dir, sample_dir);
+        if (!(l2 > 1e-12f) || !lxm_isfinite(l2)) return false;
+        d = sample_dir;
+        if (dot(d, N) > 0.0f) o = hp + (N * 0.01f);
+        else o = hp - (N * 0.01f);
+    }
+    return compact && !kill;
+}
+
+__global__ __launch_bounds__(LX_SHADE_BLOCK) void k_shade(const LxJob* __restrict__ jobs, LxPathBuf P, LxPathBuf Q,
+                                                           LxRecBuf R, LxCounters* __restrict__ C, int bounce,
+                                                           float* __restrict__ accum) {
+    uint32_t n = C->count[bounce];
+    uint32_t wave_base = blockIdx.x * blockDim.x + (threadIdx.x & ~(LX_WAVE - 1));
+    uint32_t gsz = gridDim.x * blockDim.x;
+    uint32_t lane = lx_lane();
+    int depth = bounce + 1;
+    uint32_t nsh = 0;
+    for (uint32_t base = wave_base; base < n; base += gsz) {
+        uint32_t i = base + lane;
+        bool active = i < n;
+        bool survive = false;
+        f3 o, d, thr, col;
+        float pdf = 0.0f;
+        uint32_t pixw = 0, job = 0;
+        bool spec = false;
+        if (active) {
+            job = P.job[i];
+            const LxJob& J = jobs[job];
+            pixw = P.pix[i];
+            spec = (pixw >> 31) != 0u;
+            uint32_t pix = pixw & 0x7fffffffu;
+            o = mk3(P.ox[i], P.oy[i], P.oz[i]);
+            d = mk3(P.dx[i], P.dy[i], P.dz[i]);
+            thr = mk3(P.tx[i], P.ty[i], P.tz[i]);
+            col = mk3(P.cx[i], P.cy[i], P.cz[i]);
+            pdf = P.pdf[i];
+            bool compact = depth < J.depth;
+            survive = lx_shade_path(J, *J.scene, R, i, depth, compact, o, d, thr, col, pdf, spec, pix);
+            if (!survive) {
+                /* finalGather: the path's colour is final */
+                float* a = accum + 3u * (size_t)(J.accum_off + pix);
+                a[0] += col.x;
+                a[1] += col.y;
+                a[2] += col.z;
+            }
+            pixw = pix | (spec ? 0x80000000u : 0u);
+            nsh++;
+        }
+        uint64_t mask = __ballot(survive);
+        uint32_t cnt = (uint32_t)__popcll(mask);
+        uint32_t obase = 0;
+        if (cnt) {
+            if (lane == 0) obase = atomicAdd(&C->count[bounce + 1], cnt);
+            obase = __shfl(obase, 0, LX_WAVE);
+        }
+        if (survive) {
+            uint32_t slot = obase + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+            Q.ox[slot] = o.x;
+            Q.oy[slot] = o.y;
+            Q.oz[slot] = o.z;
+            Q.dx[slot] = d.x;
+            Q.dy[slot] = d.y;
+            Q.dz[slot] = d.z;
+            Q.tx[slot] = thr.x;
+            Q.ty[slot] = thr.y;
+            Q.tz[slot] = thr.z;
+            Q.cx[slot] = col.x;
+            Q.cy[slot] = col.y;
+            Q.cz[slot] = col.z;
+            Q.pdf[slot] = pdf;
+            Q.pix[slot] = pixw;
+            Q.job[slot] = job;
+        }
+    }
+    lx_wave_add(&C->bounces, nsh);
+}
+
+/* ------------------------------------------------------------ test hooks */
+__global__ __launch_bounds__(LX_TRACE_BLOCK) void k_trace_rays(const LxDevScene* __restrict__ Sp,
+                                                                const lx_ray* __restrict__ rays, uint32_t n,
+                                                                int any, int alpha, uint32_t iteration,
+                                                                lx_hit* __restrict__ hits, uint8_t* __restrict__ occ,
+                                                                uint32_t* __restrict__ spill,
+                                                                uint32_t* __restrict__ overflow) {
+    __shared__ uint32_t s_stack[LX_STACK * LX_TRACE_BLOCK];
+    uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t gsz = gridDim.x * blockDim.x;
+    LxStack st;
+    st.lds = s_stack + threadIdx.x;
+    st.stride = LX_TRACE_BLOCK;
+    st.spill = spill + gtid;
+    st.sstride = gsz;
+    LxDevScene S = *Sp;
+    uint32_t amask = alpha ? LX_INST_ALPHA : 0u;
+    for (uint32_t i = gtid; i < n; i += gsz) {
+        lx_ray r = rays[i];
+        f3 o = mk3(r.origin[0], r.origin[1], r.origin[2]);
+        f3 d = mk3(r.dir[0], r.dir[1], r.dir[2]);
+        if (any) {
+            LxHitD h = lx_trace<true>(S, o, d, r.tmax, i, iteration, amask, st, overflow);
+            occ[i] = h.inst != 0xffffffffu ? 1 : 0;
+        } else {
+            LxHitD h = lx_trace<false>(S, o, d, r.tmax, i, iteration, amask, st, overflow);
+            lx_hit o2;
+            o2.t = h.inst == 0xffffffffu ? -1.0f : h.t;
+            o2.instance = h.inst;
+            o2.prim = h.prim;
+            o2.u = h.u;
+            o2.v = h.v;
+            hits[i] = o2;
+        }
+    }
+}
+
+/* sendImageToPBO, src/kernels/image_kernels.cu:6-46: average, tonemap,
+ * gamma 0.45, RGBA8.  Tonemaps from src/tonemapping.cu:3-30. */
+LXD f3 lx_aces(f3 x) {
+    const float a = 2.51f, b = 0.03f, c = 2.43f, dd = 0.59f, e = 0.14f;
+    f3 num = x * ((a * x) + mk3(b, b, b));
+    f3 den = (x * ((c * x) + mk3(dd, dd, dd))) + mk3(e, e, e);
+    f3 r = mk3(num.x / den.x, num.y / den.y, num.z / den.z);
+    return mk3(lxm_clamp(r.x, 0.0f, 1.0f), lxm_clamp(r.y, 0.0f, 1.0f), lxm_clamp(r.z, 0.0f, 1.0f));
+}
+LXD f3 lx_agx(f3 v) {
+    v = mk3(lxm_max(v.x, 1e-6f), lxm_max(v.y, 1e-6f), lxm_max(v.z, 1e-6f));
+    v.x = (lxm_log2(v.x) + 10.0f) / 16.0f;
+    v.y = (lxm_log2(v.y) + 10.0f) / 16.0f;
+    v.z = (lxm_log2(v.z) + 10.0f) / 16.0f;
+    v = mk3(lxm_clamp(v.x, 0.0f, 1.0f), lxm_clamp(v.y, 0.0f, 1.0f), lxm_clamp(v.z, 0.0f, 1.0f));
+    f3 x2 = v * v;
+    f3 x4 = x2 * x2;
+    v = (((((15.5f * x4) * v - 40.14f * x4) + (31.96f * x2) * v) - 6.86f * x2) + 0.429f * v) + mk3(0.0322f, 0.0322f, 0.0322f);
+    f3 r;
+    r.x = v.x * 1.1968790f - v.y * 0.0980208f - v.z * 0.0990297f;
+    r.y = -v.x * 0.0528968f + v.y * 1.1519031f - v.z * 0.0989611f;
+    r.z = -v.x * 0.0529716f - v.y * 0.0980434f + v.z * 1.1510736f;
+    return mk3(lxm_clamp(r.x, 0.0f, 1.0f), lxm_clamp(r.y, 0.0f, 1.0f), lxm_clamp(r.z, 0.0f, 1.0f));
+}
+
+__global__ void k_tonemap(const float* __restrict__ accum, uint32_t npix, float spp, int mode, uint8_t* __restrict__ out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npix) return;
+    f3 p = mk3(accum[3 * i], accum[3 * i + 1], accum[3 * i + 2]);
+    float inv = 1.0f / spp;
+    p = p * inv;
+    if (mode == LX_TONEMAP_REINHARD) p = mk3(p.x / (p.x + 1.0f), p.y / (p.y + 1.0f), p.z / (p.z + 1.0f));
+    else if (mode == LX_TONEMAP_AGX) p = lx_agx(p);
+    else p = lx_aces(p);
+    p = mk3(lxm_pow(p.x, 0.45f), lxm_pow(p.y, 0.45f), lxm_pow(p.z, 0.45f));
+    int r = (int)((double)p.x * 255.0), g = (int)((double)p.y * 255.0), b = (int)((double)p.z * 255.0);
+    r = r < 0 ? 0 : (r > 255 ? 255 : r);
+    g = g < 0 ? 0 : (g > 255 ? 255 : g);
+    b = b < 0 ? 0 : (b > 255 ? 255 : b);
+    out[4 * i + 0] = (uint8_t)r;
+    out[4 * i + 1] = (uint8_t)g;
+    out[4 * i + 2] = (uint8_t)b;
+    out[4 * i + 3] = 0;
+}
+
+__global__ void k_math_eval(int fn, const float* __restrict__ x, const float* __restrict__ y, float* __restrict__ out,
+                            uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float a = x[i], b = y ? y[i] : 0.0f, r = 0.0f;
+    switch (fn) {
+    case 0: r = lxm_sin(a); break;
+    case 1: r = lxm_cos(a); break;
+    case 2: r = lxm_atan(a); break;
+    case 3: r = lxm_atan2(a, b); break;
+    case 4: r = lxm_acos(a); break;
+    case 5: r = lxm_log2(a); break;
+    case 6: r = lxm_exp2(a); break;
+    case 7: r = lxm_pow(a, b); break;
+    case 8: r = lxm_sqrt(a); break;
+    case 9: r = a / b; break;
+    default: r = 0.0f;
+    }
+    out[i] = r;
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+/* ------------------------------------------------------------ launchers */
+hipError_t lx_launch_begin_round(LxJob* jobs, const int32_t* job_spp, uint32_t njobs, LxCounters* C, LxSched* sched,
+                                 uint32_t cap, hipStream_t s) {
+    hipLaunchKernelGGL(k_begin_round, dim3(1), dim3(64), 0, s, jobs, job_spp, njobs, C, sched, cap);
+    return hipGetLastError();
+}
+hipError_t lx_launch_end_round(LxJob* jobs, uint32_t njobs, hipStream_t s) {
+    hipLaunchKernelGGL(k_end_round, dim3((njobs + 255) / 256), dim3(256), 0, s, jobs, njobs);
+    return hipGetLastError();
+}
+hipError_t lx_launch_camera(const LxJob* jobs, const LxSched* sched, const LxCounters* C, const LxPathBuf& P,
+                            uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_camera, dim3(grid), dim3(256), 0, s, jobs, sched, C, P);
+    return hipGetLastError();
+}
+hipError_t lx_launch_trace(const LxJob* jobs, const LxPathBuf& P, const LxRecBuf& R, LxCounters* C, int bounce,
+                           uint32_t* spill, uint32_t grid, hipStream_t s, bool count) {
+    if (count) hipLaunchKernelGGL(k_trace<true>, dim3(grid), dim3(LX_TRACE_BLOCK), 0, s, jobs, P, R, C, bounce, spill);
+    else hipLaunchKernelGGL(k_trace<false>, dim3(grid), dim3(LX_TRACE_BLOCK), 0, s, jobs, P, R, C, bounce, spill);
+    return hipGetLastError();
+}
+hipError_t lx_launch_shade(const LxJob* jobs, const LxPathBuf& P, const LxPathBuf& Q, const LxRecBuf& R, LxCounters* C,
+                           int bounce, float* accum, uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(LX_SHADE_BLOCK), 0, s, jobs, P, Q, R, C, bounce, accum);
+    return hipGetLastError();
+}
+hipError_t lx_launch_trace_rays(const LxDevScene* S, const lx_ray* rays, uint32_t n, int any, int alpha,
+                                uint32_t iteration, lx_hit* hits, uint8_t* occ, uint32_t* spill, uint32_t* overflow,
+                                uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_trace_rays, dim3(grid), dim3(LX_TRACE_BLOCK), 0, s, S, rays, n, any, alpha, iteration, hits,
+                       occ, spill, overflow);
+    return hipGetLastError();
+}
+hipError_t lx_launch_tonemap(const float* accum, uint32_t npix, float spp, int mode, uint8_t* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_tonemap, dim3((npix + 255) / 256), dim3(256), 0, s, accum, npix, spp, mode, out);
+    return hipGetLastError();
+}
+hipError_t lx_launch_math_eval(int fn, const float* x, const float* y, float* out, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_math_eval, dim3((n + 255) / 256), dim3(256), 0, s, fn, x, y, out, n);
+    return hipGetLastError();
+}
+hipError_t lx_launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 65535) blocks = 65535;
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)blocks), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}

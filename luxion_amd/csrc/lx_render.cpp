@@ -1,0 +1,106 @@
+/*
+ * lx_render.cpp -- bounce scheduler for one render job.
+ *
+ * Replaces pathtraceInit / pathtrace / pathtraceFree / copyImageToHost
+ * (src/pathtrace/pathtrace_buffers.cu:21-190, src/pathtrace.cu:97-367).
+ * The reference synchronises with the host every bounce (a blocking read of
+ * the live-path count, pathtrace.cu:317, plus a device-wide sync in
+ * checkCUDAError); here each round is a fixed sequence of launches whose
+ * sizes come from device counters, so a round is captured once into a
+ * hipGraph and replayed per sample with no host round trip.
+ */
+#include "lx_internal.h"
+
+#include <algorithm>
+#include <chrono>
+
+uint32_t lx_spill_words(uint32_t trace_grid) { return (uint32_t)LX_SPILL * trace_grid * LX_TRACE_BLOCK; }
+
+static void carve(uint8_t*& p, float** f, size_t n) {
+    *f = (float*)p;
+    p += ((n * 4 + 255) / 256) * 256;
+}
+
+void WaveBufs::alloc(lx_context* ctx, uint32_t c, bool dmis, bool emis, bool tangents) {
+    (void)emis;
+    cap = c;
+    size_t arr = ((size_t)c * 4 + 255) / 256 * 256;
+    pathA.alloc(ctx, 15 * arr);
+    pathB.alloc(ctx, 15 * arr);
+    uint8_t* pa = (uint8_t*)pathA.ptr;
+    uint8_t* pb = (uint8_t*)pathB.ptr;
+    LxPathBuf* bufs[2] = {&A, &B};
+    uint8_t* bases[2] = {pa, pb};
+    for (int k = 0; k < 2; k++) {
+        uint8_t* p = bases[k];
+        LxPathBuf& P = *bufs[k];
+        carve(p, &P.ox, c);
+        carve(p, &P.oy, c);
+        carve(p, &P.oz, c);
+        carve(p, &P.dx, c);
+        carve(p, &P.dy, c);
+        carve(p, &P.dz, c);
+        carve(p, &P.tx, c);
+        carve(p, &P.ty, c);
+        carve(p, &P.tz, c);
+        carve(p, &P.cx, c);
+        carve(p, &P.cy, c);
+        carve(p, &P.cz, c);
+        carve(p, &P.pdf, c);
+        carve(p, (float**)&P.pix, c);
+        carve(p, (float**)&P.job, c);
+    }
+    /* records: 7 always, 3 tangent, 4 environment, 10 direct light */
+    size_t nrec = 7 + 4 + (tangents ? 3 : 0) + (dmis ? 10 : 0);
+    rec.alloc(ctx, nrec * arr);
+    uint8_t* p = (uint8_t*)rec.ptr;
+    std::memset(&R, 0, sizeof(R));
+    carve(p, &R.t, c);
+    carve(p, &R.nx, c);
+    carve(p, &R.ny, c);
+    carve(p, &R.nz, c);
+    carve(p, &R.u, c);
+    carve(p, &R.v, c);
+    carve(p, (float**)&R.mat, c);
+    carve(p, &R.env_t, c);
+    carve(p, &R.edx, c);
+    carve(p, &R.edy, c);
+    carve(p, &R.edz, c);
+    if (tangents) {
+        carve(p, &R.tgx, c);
+        carve(p, &R.tgy, c);
+        carve(p, &R.tgz, c);
+    }
+    if (dmis) {
+        carve(p, &R.dl_t, c);
+        carve(p, &R.dl_nx, c);
+        carve(p, &R.dl_ny, c);
+        carve(p, &R.dl_nz, c);
+        carve(p, &R.dl_u, c);
+        carve(p, &R.dl_v, c);
+        carve(p, (float**)&R.dl_mat, c);
+        carve(p, &R.lpx, c);
+        carve(p, &R.lpy, c);
+        carve(p, &R.lpz, c);
+    }
+}
+
+void lx_enqueue_round(lx_context* ctx, LxJob* d_jobs, const int32_t* d_spp, uint32_t njobs, LxCounters* d_cnt,
+                      LxSched* d_sched, uint32_t path_cap, WaveBufs& wb, float* d_accum, uint32_t* d_spill,
+                      int max_depth, uint32_t trace_grid, uint32_t shade_grid, uint32_t cam_grid, hipEvent_t* ev,
+                      bool count) {
+    hipStream_t s = ctx->stream;
+    LX_HIP(lx_launch_begin_round(d_jobs, d_spp, njobs, d_cnt, d_sched, path_cap, s));
+    LX_HIP(lx_launch_camera(d_jobs, d_sched, d_cnt, wb.A, cam_grid, s));
+    LxPathBuf* cur = &wb.A;
+    LxPathBuf* nxt = &wb.B;
+    for (int b = 0; b < max_depth; b++) {
+        if (ev) LX_HIP(hipEventRecord(ev[2 * b], s));
+        LX_HIP(lx_launch_trace(d_jobs, *cur, wb.R, d_cnt, b, d_spill, trace_grid, s, count));
+        if (ev) LX_HIP(hipEventRecord(ev[2 * b + 1], s));
+        LX_HIP(lx_launch_shade(d_jobs, *cur, *nxt, wb.R, d_cnt, b, d_accum, shade_grid, s));
+        std::swap(cur, nxt);
+    }
+    if (ev) LX_HIP(hipEventRecord(ev[2 * max_depth], s));
+    LX_HIP(lx_launch_end_round(d_jobs, njobs, s));
+}

@@ -1,0 +1,470 @@
+/*
+ * lx_scene.cpp -- scene ingest, emitter / HDRI CDF precompute, device upload
+ * and acceleration-structure assembly.
+ *
+ * Replaces Scene::Scene + Scene::precompute_emissive_mesh_area +
+ * Scene::precompute_hdri_emission (src/scene/scene.cpp:22-49,
+ * src/scene/scene_precompute.cpp:71-193), Mesh::make_mesh_device
+ * (src/mesh/mesh_device.cu:11-45) and the GAS / IAS builds
+ * (src/myoptix/optix_accel.cpp:9-110, src/myoptix/optix_ias.cpp:9-63).
+ *
+ * The CDF prefix sums run in FP64 by default (normalised values rounded to
+ * fp32 once); LX_CDF_FP32_REFERENCE reproduces the reference's fp32 sums.
+ */
+#include "lx_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cfloat>
+
+#include "../../include/luxion_detmath.h"
+
+namespace {
+
+struct H3 {
+    float x, y, z;
+};
+static inline H3 h3(float x, float y, float z) { return H3{x, y, z}; }
+static inline H3 hsub(H3 a, H3 b) { return h3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline float hdot(H3 a, H3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline H3 hcross(H3 x, H3 y) {
+    return h3(x.y * y.z - y.y * x.z, x.z * y.x - y.z * x.x, x.x * y.y - y.x * x.y);
+}
+static inline H3 hxf(const float* m, H3 p) {
+    return h3((m[0] * p.x + m[1] * p.y) + (m[2] * p.z + m[3]), (m[4] * p.x + m[5] * p.y) + (m[6] * p.z + m[7]),
+              (m[8] * p.x + m[9] * p.y) + (m[10] * p.z + m[11]));
+}
+
+static double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+} // namespace
+
+/* Affine inverse in double, rounded once to float (the reference inverts in
+ * fp32 with glm::inverse, src/scene/scene_load_gltf.cpp:436). */
+void lx_affine_inverse(const float* m, float* out) {
+    double a = m[0], b = m[1], c = m[2], d = m[4], e = m[5], f = m[6], g = m[8], h = m[9], i = m[10];
+    double A = e * i - f * h, B = -(d * i - f * g), C = d * h - e * g;
+    double det = a * A + b * B + c * C;
+    LX_REQUIRE(det != 0.0 && std::isfinite(det), LX_ERR_INVALID, "instance transform is singular");
+    double inv_det = 1.0 / det;
+    double r[9] = {A * inv_det,
+                   -(b * i - c * h) * inv_det,
+                   (b * f - c * e) * inv_det,
+                   B * inv_det,
+                   (a * i - c * g) * inv_det,
+                   -(a * f - c * d) * inv_det,
+                   C * inv_det,
+                   -(a * h - b * g) * inv_det,
+                   (a * e - b * d) * inv_det};
+    double tx = m[3], ty = m[7], tz = m[11];
+    for (int row = 0; row < 3; row++) {
+        out[row * 4 + 0] = (float)r[row * 3 + 0];
+        out[row * 4 + 1] = (float)r[row * 3 + 1];
+        out[row * 4 + 2] = (float)r[row * 3 + 2];
+        out[row * 4 + 3] = (float)(-(r[row * 3 + 0] * tx + r[row * 3 + 1] * ty + r[row * 3 + 2] * tz));
+    }
+}
+
+static size_t texel_bytes(int fmt) { return fmt == LX_TEX_RGBA8 ? 4 : (fmt == LX_TEX_RGBA16 ? 8 : 16); }
+
+static void copy_tex(const lx_texture_desc& d, HostTex& t) {
+    LX_REQUIRE(d.width > 0 && d.height > 0 && d.data, LX_ERR_INVALID, "texture without data");
+    LX_REQUIRE(d.format >= LX_TEX_RGBA8 && d.format <= LX_TEX_RGBA32F, LX_ERR_INVALID, "unknown texture format");
+    t.w = d.width;
+    t.h = d.height;
+    t.fmt = d.format;
+    size_t n = (size_t)d.width * d.height * texel_bytes(d.format);
+    t.data.assign((const uint8_t*)d.data, (const uint8_t*)d.data + n);
+}
+
+static float texel_f(const HostTex& t, size_t k, int c) {
+    if (t.fmt == LX_TEX_RGBA8) return ((float)t.data[4 * k + c]) / 255.0f;
+    if (t.fmt == LX_TEX_RGBA16) {
+        uint16_t v;
+        std::memcpy(&v, &t.data[8 * k + 2 * c], 2);
+        return ((float)v) / 65535.0f;
+    }
+    float v;
+    std::memcpy(&v, &t.data[16 * k + 4 * c], 4);
+    return v;
+}
+
+/* precompute_emissive_mesh_area, scene_precompute.cpp:71-122 */
+static void precompute_emitters(lx_scene* s) {
+    s->emissive.clear();
+    s->emissive_prefix.clear();
+    s->tri_cdf.clear();
+    s->tri_cdf_off.assign(1, 0u);
+    float area_f = 0.0f;
+    double area_d = 0.0;
+    std::vector<double> pref_d;
+    for (uint32_t gi = 0; gi < s->insts.size(); gi++) {
+        const lx_instance_desc& g = s->insts[gi];
+        const lx_material_desc& mat = s->mats[g.material];
+        H3 ec = h3(mat.emission_color[0], mat.emission_color[1], mat.emission_color[2]);
+        if (!(lxm_sqrt(hdot(ec, ec)) * mat.emission_strength > 1e-5f)) continue;
+        const HostMesh& m = s->meshes[g.mesh];
+        size_t nt = m.idx.size() / 3;
+        size_t base = s->tri_cdf.size();
+        s->tri_cdf.resize(base + nt);
+        float cum_f = 0.0f;
+        double cum_d = 0.0;
+        std::vector<double> cd(nt);
+        for (size_t t = 0; t < nt; t++) {
+            const float* P = m.pos.data();
+            uint32_t i0 = m.idx[3 * t], i1 = m.idx[3 * t + 1], i2 = m.idx[3 * t + 2];
+            H3 A = hxf(g.transform, h3(P[3 * i0], P[3 * i0 + 1], P[3 * i0 + 2]));
+            H3 B = hxf(g.transform, h3(P[3 * i1], P[3 * i1 + 1], P[3 * i1 + 2]));
+            H3 C = hxf(g.transform, h3(P[3 * i2], P[3 * i2 + 1], P[3 * i2 + 2]));
+            H3 cr = hcross(hsub(B, A), hsub(C, A));
+            float a = 0.5f * lxm_sqrt(hdot(cr, cr));
+            cum_f += a;
+            cum_d += (double)a;
+            s->tri_cdf[base + t] = cum_f;
+            cd[t] = cum_d;
+        }
+        if (s->cdf_mode == LX_CDF_FP32_REFERENCE) {
+            for (size_t t = 0; t < nt; t++) s->tri_cdf[base + t] /= cum_f;
+            area_f += cum_f;
+            s->emissive_prefix.push_back(area_f);
+        } else {
+            for (size_t t = 0; t < nt; t++) s->tri_cdf[base + t] = (float)(cd[t] / cum_d);
+            area_d += cum_d;
+            pref_d.push_back(area_d);
+        }
+        s->emissive.push_back(gi);
+        s->tri_cdf_off.push_back((uint32_t)s->tri_cdf.size());
+    }
+    if (s->cdf_mode == LX_CDF_FP32_REFERENCE) {
+        for (auto& p : s->emissive_prefix) p /= area_f;
+        s->total_area = area_f;
+    } else {
+        for (double p : pref_d) s->emissive_prefix.push_back((float)(p / area_d));
+        s->total_area = (float)area_d;
+    }
+}
+
+/* precompute_hdri_emission, scene_precompute.cpp:125-193 */
+static void precompute_env(lx_scene* s) {
+    int W = (int)s->env.w, H = (int)s->env.h;
+    const float PIF = 3.1415926535897932384626422832795028841971f;
+    s->conditional.assign((size_t)W * H, 0.0f);
+    s->marginal.assign(H, 0.0f);
+    float total_f = 0.0f;
+    double total_d = 0.0;
+    std::vector<double> marg_d(H), row_d(W);
+    for (int y = 0; y < H; y++) {
+        float row_f = 0.0f;
+        double rs = 0.0;
+        float theta = PIF * (y + 0.5f) / H;
+        float st = lxm_sin(theta);
+        for (int x = 0; x < W; x++) {
+            size_t k = (size_t)y * W + x;
+            float r = texel_f(s->env, k, 0), g = texel_f(s->env, k, 1), b = texel_f(s->env, k, 2),
+                  a = texel_f(s->env, k, 3);
+            float e = lxm_sqrt((r * r + g * g) + (b * b + a * a));
+            float w = e * st;
+            row_f += w;
+            rs += (double)w;
+            s->conditional[k] = row_f;
+            row_d[x] = rs;
+        }
+        float* row = &s->conditional[(size_t)y * W];
+        if (s->cdf_mode == LX_CDF_FP32_REFERENCE) {
+            if (row_f > 0.0f)
+                for (int x = 0; x < W; x++) row[x] /= row_f;
+            else
+                for (int x = 0; x < W; x++) row[x] = (float)(x + 1) / (float)W;
+            row[W - 1] = 1.0f;
+            total_f += row_f;
+            s->marginal[y] = total_f;
+        } else {
+            if (rs > 0.0)
+                for (int x = 0; x < W; x++) row[x] = (float)(row_d[x] / rs);
+            else
+                for (int x = 0; x < W; x++) row[x] = (float)(x + 1) / (float)W;
+            row[W - 1] = 1.0f;
+            total_d += rs;
+            marg_d[y] = total_d;
+        }
+    }
+    if (s->cdf_mode == LX_CDF_FP32_REFERENCE) {
+        LX_REQUIRE(total_f != 0.0f, LX_ERR_INVALID, "environment map has zero emission");
+        for (int y = 0; y < H; y++) s->marginal[y] /= total_f;
+    } else {
+        LX_REQUIRE(total_d != 0.0, LX_ERR_INVALID, "environment map has zero emission");
+        for (int y = 0; y < H; y++) s->marginal[y] = (float)(marg_d[y] / total_d);
+    }
+    s->marginal[H - 1] = 1.0f;
+}
+
+static void upload_textures(lx_scene* s) {
+    lx_context* ctx = s->ctx;
+    std::vector<LxTexDesc> td(s->texs.size());
+    s->d_tex.resize(s->texs.size());
+    for (size_t i = 0; i < s->texs.size(); i++) {
+        s->d_tex[i].upload(ctx, s->texs[i].data.data(), s->texs[i].data.size());
+        td[i].data = s->d_tex[i].ptr;
+        td[i].w = (int)s->texs[i].w;
+        td[i].h = (int)s->texs[i].h;
+        td[i].fmt = s->texs[i].fmt;
+    }
+    if (!td.empty()) s->d_texdesc.upload(ctx, td.data(), td.size() * sizeof(LxTexDesc));
+    if (s->has_env) {
+        s->d_env.upload(ctx, s->env.data.data(), s->env.data.size());
+        s->dev.env.data = s->d_env.ptr;
+        s->dev.env.w = (int)s->env.w;
+        s->dev.env.h = (int)s->env.h;
+        s->dev.env.fmt = s->env.fmt;
+        s->d_marg.upload(ctx, s->marginal.data(), s->marginal.size() * 4);
+        s->d_cond.upload(ctx, s->conditional.data(), s->conditional.size() * 4);
+    }
+}
+
+void lx_scene_init(lx_scene* s, const lx_scene_desc* d) {
+    LX_REQUIRE(d != nullptr, LX_ERR_INVALID, "null scene descriptor");
+    LX_REQUIRE(d->num_meshes > 0 && d->meshes, LX_ERR_INVALID, "scene needs at least one mesh");
+    LX_REQUIRE(d->num_instances > 0 && d->instances, LX_ERR_INVALID, "scene needs at least one instance");
+    LX_REQUIRE(d->num_materials > 0 && d->materials, LX_ERR_INVALID, "scene needs at least one material");
+    s->meshes.resize(d->num_meshes);
+    for (uint32_t i = 0; i < d->num_meshes; i++) {
+        const lx_mesh_desc& md = d->meshes[i];
+        LX_REQUIRE(md.positions && md.indices && md.num_tris > 0 && md.num_verts > 0, LX_ERR_INVALID,
+                   "mesh " + std::to_string(i) + " has no geometry");
+        HostMesh& m = s->meshes[i];
+        m.pos.assign(md.positions, md.positions + 3 * (size_t)md.num_verts);
+        m.idx.assign(md.indices, md.indices + 3 * (size_t)md.num_tris);
+        for (uint32_t v : m.idx) LX_REQUIRE(v < md.num_verts, LX_ERR_INVALID, "mesh index out of range");
+        m.has_n = md.normals != nullptr;
+        if (m.has_n) m.nrm.assign(md.normals, md.normals + 3 * (size_t)md.num_verts);
+        m.has_uv = md.uvs != nullptr;
+        if (m.has_uv) m.uv.assign(md.uvs, md.uvs + 2 * (size_t)md.num_verts);
+    }
+    s->mats.assign(d->materials, d->materials + d->num_materials);
+    s->texs.resize(d->num_textures);
+    for (uint32_t i = 0; i < d->num_textures; i++) copy_tex(d->textures[i], s->texs[i]);
+    for (const auto& m : s->mats) {
+        int ts[4] = {m.albedo_tex, m.metallic_roughness_tex, m.normal_tex, m.emissive_tex};
+        for (int t : ts) LX_REQUIRE(t < (int)d->num_textures, LX_ERR_INVALID, "material texture index out of range");
+    }
+    s->insts.assign(d->instances, d->instances + d->num_instances);
+    s->inv.resize(12 * s->insts.size());
+    for (size_t i = 0; i < s->insts.size(); i++) {
+        LX_REQUIRE(s->insts[i].mesh < d->num_meshes, LX_ERR_INVALID, "instance mesh index out of range");
+        LX_REQUIRE(s->insts[i].material < d->num_materials, LX_ERR_INVALID, "instance material out of range");
+        lx_affine_inverse(s->insts[i].transform, &s->inv[12 * i]);
+    }
+    s->cam = d->camera;
+    s->trace_depth = d->trace_depth > 0 ? d->trace_depth : 8;
+    LX_REQUIRE(s->trace_depth <= LX_MAX_DEPTH, LX_ERR_INVALID, "trace depth above the supported maximum");
+    s->has_env = d->envmap != nullptr;
+    if (s->has_env) {
+        LX_REQUIRE(d->envmap->format == LX_TEX_RGBA32F, LX_ERR_INVALID, "environment map must be RGBA32F");
+        copy_tex(*d->envmap, s->env);
+    }
+    s->info.num_meshes = d->num_meshes;
+    s->info.num_instances = d->num_instances;
+    s->info.num_materials = d->num_materials;
+    s->info.num_textures = d->num_textures;
+    for (auto& m : s->meshes) s->info.num_unique_tris += m.idx.size() / 3;
+    for (auto& g : s->insts) s->info.num_instanced_tris += s->meshes[g.mesh].idx.size() / 3;
+}
+
+/* Precompute and upload everything; builds the BVHs. */
+void lx_scene_accel(lx_scene* s, const lx_build_opts* o) {
+    lx_build_opts opt{4, 4, 1, 16, LX_CDF_FP64};
+    if (o) opt = *o;
+    if (opt.bvh_width != 4 && opt.bvh_width != 8) opt.bvh_width = 4;
+    if (opt.leaf_size <= 0) opt.leaf_size = 4;
+    if (opt.sah_bins <= 0) opt.sah_bins = 16;
+    LX_REQUIRE(opt.bvh_width == 4, LX_ERR_UNSUPPORTED, "this build traverses BVH4 nodes only");
+    lx_context* ctx = s->ctx;
+    ctx->set_device();
+    double t0 = now_ms();
+    s->cdf_mode = opt.cdf_precision;
+    if (s->has_env) precompute_env(s);
+    precompute_emitters(s);
+
+    /* attribute pools */
+    std::vector<uint32_t> idx_off(s->meshes.size()), vtx_off(s->meshes.size());
+    std::vector<uint32_t> all_idx;
+    std::vector<float> all_pos, all_nrm, all_uv;
+    bool any_n = false, any_uv = false;
+    for (auto& m : s->meshes) {
+        any_n |= m.has_n;
+        any_uv |= m.has_uv;
+    }
+    for (size_t i = 0; i < s->meshes.size(); i++) {
+        const HostMesh& m = s->meshes[i];
+        idx_off[i] = (uint32_t)all_idx.size();
+        vtx_off[i] = (uint32_t)(all_pos.size() / 3);
+        all_idx.insert(all_idx.end(), m.idx.begin(), m.idx.end());
+        all_pos.insert(all_pos.end(), m.pos.begin(), m.pos.end());
+        size_t nv = m.pos.size() / 3;
+        if (any_n) {
+            if (m.has_n) all_nrm.insert(all_nrm.end(), m.nrm.begin(), m.nrm.end());
+            else all_nrm.resize(all_nrm.size() + 3 * nv, 0.0f);
+        }
+        if (any_uv) {
+            if (m.has_uv) all_uv.insert(all_uv.end(), m.uv.begin(), m.uv.end());
+            else all_uv.resize(all_uv.size() + 2 * nv, 0.0f);
+        }
+    }
+
+    /* BLAS per mesh */
+    std::vector<LxNode4> nodes;
+    std::vector<float4> tris;
+    std::vector<uint32_t> blas_root(s->meshes.size());
+    std::vector<std::array<float, 6>> mesh_box(s->meshes.size());
+    int max_depth = 0;
+    double sah = 0.0;
+    for (size_t i = 0; i < s->meshes.size(); i++) {
+        const HostMesh& m = s->meshes[i];
+        BvhBuildResult r;
+        uint32_t nt = (uint32_t)(m.idx.size() / 3);
+        bool ok = false;
+        if (opt.builder == 0) ok = lx_build_blas_gpu(ctx, m.pos.data(), m.idx.data(), nt, opt.leaf_size, opt.sah_bins, r);
+        if (!ok) lx_build_blas_host(m.pos.data(), m.idx.data(), nt, opt.leaf_size, opt.sah_bins, r);
+        uint32_t noff = (uint32_t)nodes.size(), toff = (uint32_t)(tris.size() / 3);
+        LX_REQUIRE((size_t)toff + nt < (1u << 28) - 1, LX_ERR_INVALID, "scene too large for the leaf encoding");
+        for (LxNode4 n : r.nodes) {
+            int* c = &n.child.x;
+            for (int k = 0; k < 4; k++) {
+                uint32_t v = (uint32_t)c[k];
+                if (v == LX_CHILD_EMPTY) continue;
+                if (v & LX_LEAF_BIT) v = (v & ~LX_LEAF_FIRST_MASK) | ((v & LX_LEAF_FIRST_MASK) + toff);
+                else v += noff;
+                c[k] = (int)v;
+            }
+            nodes.push_back(n);
+        }
+        uint32_t rr = r.root;
+        if (rr & LX_LEAF_BIT) rr = (rr & ~LX_LEAF_FIRST_MASK) | ((rr & LX_LEAF_FIRST_MASK) + toff);
+        else rr += noff;
+        blas_root[i] = rr;
+        tris.insert(tris.end(), r.tris.begin(), r.tris.end());
+        max_depth = std::max(max_depth, r.depth);
+        sah += r.sah;
+        /* object-space box of the mesh */
+        std::array<float, 6> b = {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX};
+        for (size_t v = 0; v < m.pos.size() / 3; v++)
+            for (int a = 0; a < 3; a++) {
+                b[a] = std::min(b[a], m.pos[3 * v + a]);
+                b[3 + a] = std::max(b[3 + a], m.pos[3 * v + a]);
+            }
+        mesh_box[i] = b;
+    }
+
+    /* TLAS over conservative world boxes of the instances */
+    std::vector<float> wboxes(6 * s->insts.size());
+    std::vector<LxInstance> dinst(s->insts.size());
+    for (size_t i = 0; i < s->insts.size(); i++) {
+        const lx_instance_desc& g = s->insts[i];
+        const std::array<float, 6>& b = mesh_box[g.mesh];
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int c = 0; c < 8; c++) {
+            double p[3] = {(c & 1) ? b[3] : b[0], (c & 2) ? b[4] : b[1], (c & 4) ? b[5] : b[2]};
+            for (int r = 0; r < 3; r++) {
+                double w = (double)g.transform[4 * r] * p[0] + (double)g.transform[4 * r + 1] * p[1] +
+                           (double)g.transform[4 * r + 2] * p[2] + (double)g.transform[4 * r + 3];
+                lo[r] = std::min(lo[r], w);
+                hi[r] = std::max(hi[r], w);
+            }
+        }
+        float flo[3], fhi[3];
+        for (int r = 0; r < 3; r++) {
+            flo[r] = std::nextafter((float)lo[r], -FLT_MAX);
+            fhi[r] = std::nextafter((float)hi[r], FLT_MAX);
+        }
+        lx_pad_box(flo, fhi);
+        for (int r = 0; r < 3; r++) {
+            wboxes[6 * i + r] = flo[r];
+            wboxes[6 * i + 3 + r] = fhi[r];
+        }
+        LxInstance& I = dinst[i];
+        std::memcpy(I.inv, &s->inv[12 * i], 48);
+        std::memcpy(I.m, g.transform, 48);
+        I.mesh = g.mesh;
+        I.material = g.material;
+        I.blas_root = blas_root[g.mesh];
+        I.flags = s->mats[g.material].alpha_mode != LX_ALPHA_OPAQUE ? LX_INST_ALPHA : 0u;
+        I.idx_off = idx_off[g.mesh];
+        I.vtx_off = vtx_off[g.mesh];
+        I.num_tris = (uint32_t)(s->meshes[g.mesh].idx.size() / 3);
+        I.has_n_uv = (s->meshes[g.mesh].has_n ? 1u : 0u) | (s->meshes[g.mesh].has_uv ? 2u : 0u);
+    }
+    BvhBuildResult tl;
+    lx_build_tlas_host(wboxes, tl);
+    uint32_t noff = (uint32_t)nodes.size();
+    for (LxNode4 n : tl.nodes) {
+        int* c = &n.child.x;
+        for (int k = 0; k < 4; k++) {
+            uint32_t v = (uint32_t)c[k];
+            if (v == LX_CHILD_EMPTY || (v & LX_LEAF_BIT)) continue;
+            c[k] = (int)(v + noff);
+        }
+        n.meta.x = 1;
+        nodes.push_back(n);
+    }
+    uint32_t tlas_root = (tl.root & LX_LEAF_BIT) ? tl.root : tl.root + noff;
+
+    /* uploads */
+    s->d_nodes.upload(ctx, nodes.data(), nodes.size() * sizeof(LxNode4));
+    s->d_tris.upload(ctx, tris.data(), tris.size() * sizeof(float4));
+    s->d_inst.upload(ctx, dinst.data(), dinst.size() * sizeof(LxInstance));
+    s->d_idx.upload(ctx, all_idx.data(), all_idx.size() * 4);
+    s->d_pos.upload(ctx, all_pos.data(), all_pos.size() * 4);
+    if (any_n) s->d_nrm.upload(ctx, all_nrm.data(), all_nrm.size() * 4);
+    if (any_uv) s->d_uv.upload(ctx, all_uv.data(), all_uv.size() * 4);
+    s->d_mats.upload(ctx, s->mats.data(), s->mats.size() * sizeof(lx_material_desc));
+    upload_textures(s);
+    if (!s->emissive.empty()) {
+        s->d_emis.upload(ctx, s->emissive.data(), s->emissive.size() * 4);
+        s->d_emis_prefix.upload(ctx, s->emissive_prefix.data(), s->emissive_prefix.size() * 4);
+        s->d_tri_cdf.upload(ctx, s->tri_cdf.data(), s->tri_cdf.size() * 4);
+        s->d_tri_cdf_off.upload(ctx, s->tri_cdf_off.data(), s->tri_cdf_off.size() * 4);
+    }
+    LxDevScene& D = s->dev;
+    D.nodes = s->d_nodes.ptr;
+    D.tris = s->d_tris.as<float4>();
+    D.inst = s->d_inst.as<LxInstance>();
+    D.idx = s->d_idx.as<uint32_t>();
+    D.pos = s->d_pos.as<float>();
+    D.nrm = s->d_nrm.as<float>();
+    D.uv = s->d_uv.as<float>();
+    D.mats = s->d_mats.as<lx_material_desc>();
+    D.texs = s->d_texdesc.as<LxTexDesc>();
+    D.marginal = s->d_marg.as<float>();
+    D.conditional = s->d_cond.as<float>();
+    D.emissive_inst = s->d_emis.as<uint32_t>();
+    D.emissive_prefix = s->d_emis_prefix.as<float>();
+    D.tri_cdf = s->d_tri_cdf.as<float>();
+    D.tri_cdf_off = s->d_tri_cdf_off.as<uint32_t>();
+    D.tlas_root = tlas_root;
+    D.num_inst = (uint32_t)s->insts.size();
+    D.num_emissive = (uint32_t)s->emissive.size();
+    D.total_area = s->total_area;
+    D.has_env = s->has_env ? 1 : 0;
+    D.bvh_width = 4;
+    s->d_scene.upload(ctx, &D, sizeof(LxDevScene));
+    LX_HIP(hipStreamSynchronize(ctx->stream));
+    s->accel_built = true;
+    s->info.num_emissive_instances = (uint32_t)s->emissive.size();
+    s->info.total_emissive_area = s->total_area;
+    s->info.env_width = s->has_env ? (int)s->env.w : 0;
+    s->info.env_height = s->has_env ? (int)s->env.h : 0;
+    s->info.bvh_nodes = (uint32_t)nodes.size();
+    s->info.bvh_depth = max_depth + tl.depth + 1;
+    s->info.build_ms = now_ms() - t0;
+    s->info.sah_cost = sah;
+    uint64_t bytes = 0;
+    DevBuf* bufs[] = {&s->d_nodes, &s->d_tris, &s->d_inst,  &s->d_idx,  &s->d_pos,         &s->d_nrm,
+                      &s->d_uv,    &s->d_mats, &s->d_texdesc, &s->d_env, &s->d_marg, &s->d_cond,
+                      &s->d_emis,  &s->d_emis_prefix, &s->d_tri_cdf, &s->d_tri_cdf_off, &s->d_scene};
+    for (DevBuf* b : bufs) bytes += b->bytes;
+    for (auto& t : s->d_tex) bytes += t.bytes;
+    s->info.device_bytes = bytes;
+}

@@ -1,0 +1,365 @@
+/*
+ * lx_trace_dev.h -- software two-level BVH traversal for gfx950.
+ *
+ * Replaces OptiX's hardware traversal (optixTrace in
+ * src/optixshaders/optix_triangle.cu:68-135 of the reference) with:
+ *  - a TLAS of BVH4/BVH8 nodes over instance world boxes, one instance per
+ *    leaf; entering a leaf maps the ray to object space once and pushes a
+ *    sentinel that restores the world ray on the way back;
+ *  - per-mesh BLAS trees with up to 8 triangles per leaf, triangles stored in
+ *    leaf order as (v0 | prim, e1, e2) so one leaf is a run of coalescable
+ *    16-byte loads;
+ *  - a per-lane short stack in LDS (column per lane: conflict-free for a
+ *    wave64), spilling to a per-lane global region past its depth;
+ *  - robust slab tests: t = (lo - o) * (1/d), far distance inflated by
+ *    2^-21 relative, and culling against the best hit only beyond a 2^-16
+ *    relative margin, so culling never changes the result of the triangle
+ *    test; the closest hit is the lexicographic minimum of
+ *    (t, instance, primitive) over all accepted candidates and does not
+ *    depend on traversal order or tree shape.
+ * The triangle test is the reference's software Moller-Trumbore
+ * (src/intersections/mesh_isect.cu:77-137) computed in object space.
+ */
+#ifndef LX_TRACE_DEV_H
+#define LX_TRACE_DEV_H
+
+#include "lx_shade_dev.h"
+
+#include "lx_kernels.h"
+
+struct LxHitD {
+    float t;
+    uint32_t inst, prim;
+    float u, v;
+};
+
+/* hash_u32, optix_triangle.cu:41-49 */
+LXD uint32_t lx_hash_u32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+/* __anyhit__ms_all, optix_triangle.cu:317-366.  Keyed on the pixel index
+ * (deterministic) instead of the compacted launch index. */
+LXD bool lx_any_hit_accept(const LxDevScene& S, const LxInstance& I, uint32_t inst, uint32_t prim, float bu,
+                           float bv, f3 wdir, uint32_t key, uint32_t iteration) {
+    const lx_material_desc& mat = S.mats[I.material];
+    if (mat.alpha_mode == LX_ALPHA_OPAQUE) return true;
+    float alpha = mat.alpha;
+    if (mat.albedo_tex >= 0) {
+        f2 uv = mk2(0.0f, 0.0f);
+        if (I.has_n_uv & 2u) {
+            const uint32_t* ix = S.idx + I.idx_off + 3u * prim;
+            const float* UV = S.uv + 2u * (size_t)I.vtx_off;
+            f2 A = mk2(UV[2 * ix[0]], UV[2 * ix[0] + 1]);
+            f2 B = mk2(UV[2 * ix[1]], UV[2 * ix[1] + 1]);
+            f2 C = mk2(UV[2 * ix[2]], UV[2 * ix[2] + 1]);
+            float bA = 1.0f - bu - bv;
+            uv = mk2((bA * A.x + bu * B.x) + bv * C.x, (bA * A.y + bu * B.y) + bv * C.y);
+        }
+        uv = mk2(uv.x * mat.albedo_xf.scale[0], uv.y * mat.albedo_xf.scale[1]);
+        if (mat.albedo_xf.rotation != 0.0f) {
+            float c = lxm_cos(mat.albedo_xf.rotation), s = lxm_sin(mat.albedo_xf.rotation);
+            uv = mk2(c * uv.x - s * uv.y, s * uv.x + c * uv.y);
+        }
+        uv = mk2(uv.x + mat.albedo_xf.offset[0], uv.y + mat.albedo_xf.offset[1]);
+        float t[4];
+        lx_tex_sample(S.texs[mat.albedo_tex], uv.x, uv.y, t);
+        alpha *= t[3];
+    }
+    if (mat.alpha_mode == LX_ALPHA_MASK) return !(alpha < mat.alpha_cutoff);
+    uint32_t h = lx_hash_u32(key ^ lx_hash_u32(iteration));
+    h = lx_hash_u32(h ^ lxm_f2u(wdir.x));
+    h = lx_hash_u32(h ^ lxm_f2u(wdir.y));
+    h = lx_hash_u32(h ^ lxm_f2u(wdir.z));
+    h = lx_hash_u32(h ^ inst);
+    h = lx_hash_u32(h ^ prim);
+    float rnd = (h >> 8) * (1.0f / 16777216.0f);
+    return !(rnd >= alpha);
+}
+
+/* Moller-Trumbore with the reference's epsilon bounds (mesh_isect.cu:77-137). */
+LXD bool lx_mt(f3 o, f3 d, const float4& A, const float4& E1, const float4& E2, float tmax, float& t, float& u,
+               float& v) {
+    const float eps = (float)(1.1920929E-7F);
+    f3 e1 = mk3(E1.x, E1.y, E1.z), e2 = mk3(E2.x, E2.y, E2.z);
+    f3 c = cross(d, e2);
+    float det = dot(e1, c);
+    if (det > -eps && det < eps) return false;
+    float inv_det = 1.0f / det;
+    f3 a = o - mk3(A.x, A.y, A.z);
+    u = inv_det * dot(a, c);
+    if (u < -eps || u > 1.0f + eps) return false;
+    c = cross(a, e1);
+    v = inv_det * dot(d, c);
+    if ((v < 0 && lxm_abs(v) > eps) || (u + v > 1 && lxm_abs(u + v - 1) > eps)) return false;
+    t = inv_det * dot(e2, c);
+    if (!(t > eps)) return false;
+    if (!(t < tmax)) return false;
+    return true;
+}
+
+LXD bool lx_better(float t, uint32_t inst, uint32_t prim, const LxHitD& h) {
+    if (t < h.t) return true;
+    if (t == h.t) {
+        if (inst < h.inst) return true;
+        if (inst == h.inst && prim < h.prim) return true;
+    }
+    return false;
+}
+
+#define LX_FAR_INFLATE 1.00000048f  /* 1 + 2^-21 */
+#define LX_CULL_MARGIN 1.00001526f  /* 1 + 2^-16 */
+
+struct LxStack {
+    uint32_t* lds;     /* this lane's column: entry k at lds[k * stride] */
+    uint32_t stride;
+    uint32_t* spill;   /* this lane's spill column: entry k at spill[k * sstride] */
+    uint32_t sstride;
+    int sp;
+    LXD void push(uint32_t v) {
+        if (sp < LX_STACK) lds[sp * stride] = v;
+        else spill[(sp - LX_STACK) * sstride] = v;
+        sp++;
+    }
+    LXD uint32_t pop() {
+        sp--;
+        return sp < LX_STACK ? lds[sp * stride] : spill[(sp - LX_STACK) * sstride];
+    }
+};
+
+/* Slab test of four children of a BVH4 node; writes entry distances (or a
+ * negative "miss" flag as +inf) and returns a 4-bit hit mask. */
+LXD uint32_t lx_box4(const LxNode4* __restrict__ nd, f3 o, f3 ix, float tcut, float* tn) {
+    float4 lx = nd->lox, hx = nd->hix, ly = nd->loy, hy = nd->hiy, lz = nd->loz, hz = nd->hiz;
+    int4 ch = nd->child;
+    const float lxs[4] = {lx.x, lx.y, lx.z, lx.w}, hxs[4] = {hx.x, hx.y, hx.z, hx.w};
+    const float lys[4] = {ly.x, ly.y, ly.z, ly.w}, hys[4] = {hy.x, hy.y, hy.z, hy.w};
+    const float lzs[4] = {lz.x, lz.y, lz.z, lz.w}, hzs[4] = {hz.x, hz.y, hz.z, hz.w};
+    const uint32_t cs[4] = {(uint32_t)ch.x, (uint32_t)ch.y, (uint32_t)ch.z, (uint32_t)ch.w};
+    uint32_t mask = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        float ax = (lxs[k] - o.x) * ix.x, bx = (hxs[k] - o.x) * ix.x;
+        float ay = (lys[k] - o.y) * ix.y, by = (hys[k] - o.y) * ix.y;
+        float az = (lzs[k] - o.z) * ix.z, bz = (hzs[k] - o.z) * ix.z;
+        float n0 = __builtin_fminf(ax, bx), f0 = __builtin_fmaxf(ax, bx);
+        float n1 = __builtin_fminf(ay, by), f1 = __builtin_fmaxf(ay, by);
+        float n2 = __builtin_fminf(az, bz), f2v = __builtin_fmaxf(az, bz);
+        float tnear = __builtin_fmaxf(__builtin_fmaxf(n0, n1), __builtin_fmaxf(n2, 0.0f));
+        float tfar = __builtin_fminf(__builtin_fminf(f0, f1), __builtin_fminf(f2v, tcut));
+        bool h = (tnear <= tfar * LX_FAR_INFLATE) && cs[k] != LX_CHILD_EMPTY;
+        tn[k] = tnear;
+        mask |= h ? (1u << k) : 0u;
+    }
+    return mask;
+}
+
+/* Closest (ANY = false) or any (ANY = true) accepted hit along the ray. */
+/* Per-ray work counters of the counting replay (bytes model). */
+struct LxWork {
+    uint32_t nodes, tris, insts;
+};
+
+template <bool ANY, bool CNT = false>
+LXD LxHitD lx_trace(const LxDevScene& S, f3 wo, f3 wd, float tmax, uint32_t key, uint32_t iteration,
+                    uint32_t alpha_mask, LxStack& st, uint32_t* overflow, LxWork* work = nullptr) {
+    LxHitD best;
+    best.t = __builtin_inff();
+    best.inst = 0xffffffffu;
+    best.prim = 0xffffffffu;
+    best.u = best.v = 0.0f;
+    const LxNode4* __restrict__ nodes = (const LxNode4*)S.nodes;
+    f3 iwd = mk3(1.0f / wd.x, 1.0f / wd.y, 1.0f / wd.z);
+    f3 o = wo, d = wd, ix = iwd;
+    bool in_blas = false;
+    uint32_t cur_inst = 0;
+    uint32_t inst_flags = 0;
+    st.sp = 0;
+    uint32_t node = S.tlas_root;
+    while (true) {
+        bool do_pop = false;
+        if (node == LX_SENTINEL) {
+            o = wo;
+            d = wd;
+            ix = iwd;
+            in_blas = false;
+            do_pop = true;
+        } else if (!(node & LX_LEAF_BIT)) {
+            float tcut = (best.inst == 0xffffffffu ? tmax : best.t) * LX_CULL_MARGIN;
+            float tn[4];
+            uint32_t m = lx_box4(nodes + node, o, ix, tcut, tn);
+            if (CNT) work->nodes++;
+            if (m == 0u) {
+                do_pop = true;
+            } else {
+                int4 ch = nodes[node].child;
+                uint32_t cs[4] = {(uint32_t)ch.x, (uint32_t)ch.y, (uint32_t)ch.z, (uint32_t)ch.w};
+                /* nearest child next; the rest pushed far to near */
+                float bt = __builtin_inff();
+                int bk = -1;
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if ((m >> k) & 1u)
+                        if (tn[k] < bt) {
+                            bt = tn[k];
+                            bk = k;
+                        }
+                m &= ~(1u << bk);
+                while (m) {
+                    float ft = -1.0f;
+                    int fk = -1;
+#pragma unroll
+                    for (int k = 0; k < 4; k++)
+                        if ((m >> k) & 1u)
+                            if (tn[k] > ft) {
+                                ft = tn[k];
+                                fk = k;
+                            }
+                    m &= ~(1u << fk);
+                    if (st.sp >= LX_STACK + LX_SPILL) {
+                        atomicOr(overflow, 1u);
+                        continue;
+                    }
+                    st.push(cs[fk]);
+                }
+                node = cs[bk];
+                continue;
+            }
+        } else if (!in_blas) {
+            cur_inst = node & LX_LEAF_FIRST_MASK;
+            const LxInstance& I = S.inst[cur_inst];
+            o = xf_point(I.inv, wo);
+            d = xf_vector(I.inv, wd);
+            ix = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+            inst_flags = I.flags & alpha_mask;
+            in_blas = true;
+            if (st.sp >= LX_STACK + LX_SPILL) {
+                atomicOr(overflow, 1u);
+                do_pop = true;
+            } else {
+                st.push(LX_SENTINEL);
+                if (CNT) work->insts++;
+                node = I.blas_root;
+                continue;
+            }
+        } else {
+            uint32_t first = node & LX_LEAF_FIRST_MASK;
+            uint32_t cnt = ((node >> 28) & 7u) + 1u;
+            for (uint32_t k = 0; k < cnt; k++) {
+                const float4* tp = S.tris + 3u * (size_t)(first + k);
+                float4 A = tp[0], E1 = tp[1], E2 = tp[2];
+                float t, u, v;
+                if (CNT) work->tris++;
+                if (!lx_mt(o, d, A, E1, E2, tmax, t, u, v)) continue;
+                uint32_t prim = lxm_f2u(A.w);
+                if (!ANY && !lx_better(t, cur_inst, prim, best)) continue;
+                if ((inst_flags & LX_INST_ALPHA) &&
+                    !lx_any_hit_accept(S, S.inst[cur_inst], cur_inst, prim, u, v, wd, key, iteration))
+                    continue;
+                best.t = t;
+                best.inst = cur_inst;
+                best.prim = prim;
+                best.u = u;
+                best.v = v;
+                if (ANY) return best;
+            }
+            do_pop = true;
+        }
+        if (do_pop) {
+            if (st.sp == 0) break;
+            node = st.pop();
+        }
+    }
+    return best;
+}
+
+/* __closesthit__ch, optix_triangle.cu:156-243: interpolated normal (not
+ * flipped toward the ray), uv, uv-derivative tangent, object -> world. */
+LXD void lx_hit_attrs(const LxDevScene& S, const LxHitD& h, bool want_tangent, f3& N, f3& Tg, f2& uv, int& mat) {
+    const LxInstance& I = S.inst[h.inst];
+    const uint32_t* ix = S.idx + I.idx_off + 3u * h.prim;
+    uint32_t i0 = ix[0], i1 = ix[1], i2 = ix[2];
+    const float* P = S.pos + 3u * (size_t)I.vtx_off;
+    f3 A = mk3(P[3 * i0], P[3 * i0 + 1], P[3 * i0 + 2]);
+    f3 B = mk3(P[3 * i1], P[3 * i1 + 1], P[3 * i1 + 2]);
+    f3 C = mk3(P[3 * i2], P[3 * i2 + 1], P[3 * i2 + 2]);
+    f3 e1 = B - A, e2 = C - A;
+    f3 normal;
+    float bA = 1.0f - h.u - h.v;
+    if (I.has_n_uv & 1u) {
+        const float* Nn = S.nrm + 3u * (size_t)I.vtx_off;
+        f3 nA = mk3(Nn[3 * i0], Nn[3 * i0 + 1], Nn[3 * i0 + 2]);
+        f3 nB = mk3(Nn[3 * i1], Nn[3 * i1 + 1], Nn[3 * i1 + 2]);
+        f3 nC = mk3(Nn[3 * i2], Nn[3 * i2 + 1], Nn[3 * i2 + 2]);
+        normal = normalize(((bA * nA) + (h.u * nB)) + (h.v * nC));
+    } else {
+        normal = normalize(cross(e1, e2));
+    }
+    f3 tangent = mk3(1.0f, 0.0f, 0.0f);
+    uv = mk2(0.0f, 0.0f);
+    if (I.has_n_uv & 2u) {
+        const float* U = S.uv + 2u * (size_t)I.vtx_off;
+        f2 a = mk2(U[2 * i0], U[2 * i0 + 1]), b = mk2(U[2 * i1], U[2 * i1 + 1]), c = mk2(U[2 * i2], U[2 * i2 + 1]);
+        uv = mk2((bA * a.x + h.u * b.x) + h.v * c.x, (bA * a.y + h.u * b.y) + h.v * c.y);
+        if (want_tangent) {
+            f2 d1 = mk2(b.x - a.x, b.y - a.y), d2 = mk2(c.x - a.x, c.y - a.y);
+            float det = (d1.x * d2.y - d2.x * d1.y);
+            float f = (lxm_abs(det) < 1e-10f) ? 0.0f : 1.0f / det;
+            if (f != 0.0f)
+                tangent = mk3(f * (d2.y * e1.x - d1.y * e2.x), f * (d2.y * e1.y - d1.y * e2.y),
+                              f * (d2.y * e1.z - d1.y * e2.z));
+        }
+    }
+    normal = normalize(xf_normal(I.inv, normal));
+    if (want_tangent) {
+        tangent = normalize(xf_vector(I.m, tangent));
+        f3 T = tangent - dot(tangent, normal) * normal;
+        float len2 = dot(T, T);
+        if (len2 > 1e-10f) {
+            tangent = T * (1.0f / lxm_sqrt(len2));
+        } else {
+            tangent = (lxm_abs(normal.x) > 0.9f) ? mk3(0, 1, 0) : mk3(1, 0, 0);
+            tangent = normalize(cross(tangent, normal));
+        }
+    }
+    N = normal;
+    Tg = tangent;
+    mat = (int)I.material;
+}
+
+/* __closesthit__ch_direct_light, optix_triangle.cu:259-305 */
+LXD void lx_dl_attrs(const LxDevScene& S, const LxHitD& h, f3& N, f2& uv, int& mat) {
+    const LxInstance& I = S.inst[h.inst];
+    const uint32_t* ix = S.idx + I.idx_off + 3u * h.prim;
+    uint32_t i0 = ix[0], i1 = ix[1], i2 = ix[2];
+    float bA = 1.0f - h.u - h.v;
+    f3 normal;
+    if (I.has_n_uv & 1u) {
+        const float* Nn = S.nrm + 3u * (size_t)I.vtx_off;
+        f3 nA = mk3(Nn[3 * i0], Nn[3 * i0 + 1], Nn[3 * i0 + 2]);
+        f3 nB = mk3(Nn[3 * i1], Nn[3 * i1 + 1], Nn[3 * i1 + 2]);
+        f3 nC = mk3(Nn[3 * i2], Nn[3 * i2 + 1], Nn[3 * i2 + 2]);
+        normal = normalize(((bA * nA) + (h.u * nB)) + (h.v * nC));
+    } else {
+        const float* P = S.pos + 3u * (size_t)I.vtx_off;
+        f3 A = mk3(P[3 * i0], P[3 * i0 + 1], P[3 * i0 + 2]);
+        f3 B = mk3(P[3 * i1], P[3 * i1 + 1], P[3 * i1 + 2]);
+        f3 C = mk3(P[3 * i2], P[3 * i2 + 1], P[3 * i2 + 2]);
+        normal = normalize(cross(B - A, C - A));
+    }
+    uv = mk2(0.0f, 0.0f);
+    if (I.has_n_uv & 2u) {
+        const float* U = S.uv + 2u * (size_t)I.vtx_off;
+        f2 a = mk2(U[2 * i0], U[2 * i0 + 1]), b = mk2(U[2 * i1], U[2 * i1 + 1]), c = mk2(U[2 * i2], U[2 * i2 + 1]);
+        uv = mk2((bA * a.x + h.u * b.x) + h.v * c.x, (bA * a.y + h.u * b.y) + h.v * c.y);
+    }
+    N = normalize(xf_normal(I.inv, normal));
+    mat = (int)I.material;
+}
+
+#endif

@@ -1,0 +1,27 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) and the built libluxion_mi355x.so")
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    from oracle import lxo
+    lxo.lib()
+    return lxo
+
+
+@pytest.fixture(scope="session")
+def ctx():
+    import luxion_amd as lx
+    c = lx.Context(0)
+    yield c
+    c.close()
