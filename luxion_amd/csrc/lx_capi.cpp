@@ -95,8 +95,10 @@ lx_status lx_context_create(int device, uint64_t budget, lx_context** out) {
     LX_CATCH
 }
 
-void lx_context_destroy(lx_context* c) {
-    if (!c) return;
+/* Objects are reference counted (render -> scene -> context) so handles can
+ * be destroyed in any order; the last release frees. */
+void lx_context_release(lx_context* c) {
+    if (!c || --c->refs > 0) return;
     (void)hipSetDevice(c->device);
     if (c->stream) {
         (void)hipStreamSynchronize(c->stream);
@@ -104,6 +106,19 @@ void lx_context_destroy(lx_context* c) {
     }
     delete c;
 }
+
+void lx_scene_release(lx_scene* s) {
+    if (!s || --s->refs > 0) return;
+    lx_context* c = s->ctx;
+    if (c) {
+        (void)hipSetDevice(c->device);
+        (void)hipStreamSynchronize(c->stream);
+    }
+    delete s; /* device buffers return to the context here */
+    lx_context_release(c);
+}
+
+void lx_context_destroy(lx_context* c) { lx_context_release(c); }
 
 uint64_t lx_context_bytes_in_use(const lx_context* c) { return c ? c->in_use.load() : 0; }
 
@@ -136,6 +151,7 @@ lx_status lx_scene_create(lx_context* ctx, const lx_scene_desc* desc, lx_scene**
         delete s;
         throw;
     }
+    ctx->refs++;
     *out = s;
     LX_CATCH
 }
@@ -191,11 +207,7 @@ lx_status lx_scene_read_instance_tri_cdf(const lx_scene* s, uint32_t k, float* t
     LX_CATCH
 }
 
-void lx_scene_destroy(lx_scene* s) {
-    if (!s) return;
-    if (s->ctx) (void)hipStreamSynchronize(s->ctx->stream);
-    delete s;
-}
+void lx_scene_destroy(lx_scene* s) { lx_scene_release(s); }
 
 /* ------------------------------------------------------------ test hooks */
 static lx_status trace_hook(lx_scene* s, const lx_ray* rays, uint32_t n, int any, int alpha, uint32_t iteration,
@@ -319,6 +331,7 @@ lx_status lx_render_create(lx_scene* s, const lx_render_opts* o, lx_render** out
         delete r;
         throw;
     }
+    s->refs++;
     *out = r;
     LX_CATCH
 }
@@ -327,9 +340,9 @@ static void upload_job(lx_render* r, uint32_t first_iter, uint32_t n) {
     r->job.iter = (int32_t)first_iter - 1;
     r->job.spp_done = 0;
     r->job.active = 0;
-    int32_t spp = (int32_t)n;
+    r->host_spp = (int32_t)n; /* source of an async copy: must outlive the call */
     LX_HIP(hipMemcpyAsync(r->jobs.ptr, &r->job, sizeof(LxJob), hipMemcpyHostToDevice, r->ctx->stream));
-    LX_HIP(hipMemcpyAsync(r->job_spp.ptr, &spp, 4, hipMemcpyHostToDevice, r->ctx->stream));
+    LX_HIP(hipMemcpyAsync(r->job_spp.ptr, &r->host_spp, 4, hipMemcpyHostToDevice, r->ctx->stream));
 }
 
 static void enqueue(lx_render* r, hipEvent_t* ev) {
@@ -494,7 +507,9 @@ void lx_render_destroy(lx_render* r) {
     (void)hipStreamSynchronize(r->ctx->stream);
     if (r->graph) (void)hipGraphExecDestroy(r->graph);
     if (r->graph_src) (void)hipGraphDestroy(r->graph_src);
+    lx_scene* s = r->scene;
     delete r;
+    lx_scene_release(s);
 }
 
 lx_status lx_debug_bounce(lx_render* r, uint32_t iter, uint32_t* survivors, uint32_t* survivor_counts,

@@ -45,6 +45,7 @@ struct lx_context {
     int num_cus = 256;
     lx_tuning tuning{};
     std::mutex mu; /* serialises device work from the served form's thread */
+    std::atomic<int> refs{1}; /* user handle + live scenes */
 
     void* dalloc(size_t bytes);           /* charged against the budget */
     void dfree(void* p, size_t bytes);
@@ -126,6 +127,7 @@ struct HostTex {
 
 struct lx_scene {
     lx_context* ctx = nullptr;
+    std::atomic<int> refs{1}; /* user handle + live renders / jobs */
     std::vector<HostMesh> meshes;
     std::vector<lx_instance_desc> insts;
     std::vector<lx_material_desc> mats;
@@ -155,6 +157,8 @@ struct lx_scene {
 };
 
 void lx_scene_init(lx_scene* s, const lx_scene_desc* d); /* copies + precompute + uploads */
+extern "C" void lx_scene_release(lx_scene* s);
+extern "C" void lx_context_release(lx_context* c);
 void lx_scene_accel(lx_scene* s, const lx_build_opts* o);
 void lx_affine_inverse(const float* m, float* out);
 
@@ -178,6 +182,7 @@ struct lx_render {
     WaveBufs wb;
     DevBuf accum, jobs, job_spp, counters, sched, spill, ldr;
     LxJob job{};
+    int32_t host_spp = 0;
     uint32_t trace_grid = 0, shade_grid = 0, cam_grid = 0;
     bool timing = false;
     bool counting = false;
