@@ -34,7 +34,17 @@ def main():
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--level", type=int, default=8)
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="only run the plain 16-byte copy kernel over this many MiB (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import ctypes
+        import luxion_amd as lx
+        ctx = lx.Context(0)
+        g = ctypes.c_double()
+        lx.check(ctx.lib.lx_debug_copy_bandwidth(ctx.h, args.copy_calibration << 20, 4, ctypes.byref(g)))
+        print(json.dumps({"copy_mib": args.copy_calibration, "dispatches": 5, "gbps": round(g.value, 1)}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus above 1 is not supported: each GPU "
                                    "would render its own share (spp or jobs) and reduce the accumulation"}))

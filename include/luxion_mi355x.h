@@ -299,6 +299,12 @@ lx_status lx_debug_bounce(lx_render* r, uint32_t iter, uint32_t* survivors_out, 
  * GPU and host sequences agree bit for bit. */
 lx_status lx_debug_math_eval(lx_context* ctx, int32_t fn, const float* x, const float* y, float* out, uint32_t n);
 
+/* Plain device-to-device copy kernel of `bytes` bytes run `iters` times on
+ * the context stream (16-byte loads / stores); returns the event-timed copy
+ * bandwidth (read + write bytes per second / 1e9).  Used to calibrate HBM
+ * traffic counters and as the achievable-bandwidth reference. */
+lx_status lx_debug_copy_bandwidth(lx_context* ctx, uint64_t bytes, uint32_t iters, double* gbps);
+
 /* ----------------------------------------------------------- batched form */
 /* One render job of the batched / served forms.  All jobs of a batch share
  * one persistent launch per bounce. */

@@ -162,6 +162,7 @@ SIGNATURES = {
     "lx_trace_occluded": (C.c_int, [_P, _P, C.c_uint32, C.c_int32, C.c_uint32, _P]),
     "lx_debug_bounce": (C.c_int, [_P, C.c_uint32, _P, _P, _P, _P]),
     "lx_debug_math_eval": (C.c_int, [_P, C.c_int32, _P, _P, _P, C.c_uint32]),
+    "lx_debug_copy_bandwidth": (C.c_int, [_P, C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]),
     "lx_tuning_get": (C.c_int, [_P, C.POINTER(lx_tuning)]),
     "lx_tuning_set": (C.c_int, [_P, C.POINTER(lx_tuning)]),
     "lx_autotune": (C.c_int, [_P, _P, C.c_uint32, C.c_char_p, C.POINTER(lx_tuning)]),

@@ -264,6 +264,33 @@ lx_status lx_debug_math_eval(lx_context* ctx, int32_t fn, const float* x, const 
     LX_CATCH
 }
 
+lx_status lx_debug_copy_bandwidth(lx_context* ctx, uint64_t bytes, uint32_t iters, double* gbps) {
+    LX_TRY
+    LX_REQUIRE(ctx && gbps && bytes >= 16 && iters > 0, LX_ERR_INVALID, "bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->set_device();
+    size_t n16 = (size_t)(bytes / 16);
+    DevBuf a, b;
+    a.alloc(ctx, n16 * 16);
+    b.alloc(ctx, n16 * 16);
+    LX_HIP(lx_launch_fill_u32(a.as<uint32_t>(), n16 * 4, 0x3f800000u, ctx->stream));
+    uint32_t grid = (uint32_t)ctx->num_cus * 16u;
+    LX_HIP(lx_launch_copy16(a.ptr, b.ptr, n16, grid, ctx->stream)); /* warm */
+    hipEvent_t e0, e1;
+    LX_HIP(hipEventCreate(&e0));
+    LX_HIP(hipEventCreate(&e1));
+    LX_HIP(hipEventRecord(e0, ctx->stream));
+    for (uint32_t k = 0; k < iters; k++) LX_HIP(lx_launch_copy16(a.ptr, b.ptr, n16, grid, ctx->stream));
+    LX_HIP(hipEventRecord(e1, ctx->stream));
+    LX_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    LX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *gbps = 2.0 * (double)n16 * 16.0 * iters / (ms * 1e-3) / 1e9;
+    LX_CATCH
+}
+
 /* -------------------------------------------------------------- render */
 static void render_grids(lx_context* ctx, uint32_t cap, uint32_t& tg, uint32_t& sg, uint32_t& cg) {
     uint32_t waves = (uint32_t)std::max(1, ctx->tuning.waves_per_cu);

@@ -532,6 +532,16 @@ __global__ void k_math_eval(int fn, const float* __restrict__ x, const float* __
     out[i] = r;
 }
 
+__global__ void k_copy16(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+hipError_t lx_launch_copy16(const void* src, void* dst, size_t n16, uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, s, (const float4*)src, (float4*)dst, n16);
+    return hipGetLastError();
+}
+
 __global__ void k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
