@@ -334,6 +334,7 @@ typedef struct lx_tuning {
     int32_t queue_chunk;   /* paths claimed per wave from the work queue */
     int32_t waves_per_cu;  /* persistent grid size */
     int32_t use_graph;     /* capture the bounce loop in a hipGraph */
+    uint32_t max_paths_per_round; /* batched form path capacity, 0 = from the budget */
 } lx_tuning;
 
 lx_status lx_tuning_get(const lx_context* ctx, lx_tuning* out);
@@ -352,8 +353,9 @@ lx_status lx_batch_run(lx_context* ctx, const lx_job* jobs, uint32_t n, lx_job_r
 typedef struct lx_server_opts {
     uint64_t hbm_budget_bytes; /* admission budget for per-job buffers, 0 = context budget */
     int32_t max_jobs_in_flight;
-    int32_t spp_per_round;     /* samples each admitted job advances per round */
-    uint32_t max_paths_per_round; /* 0 = no cap */
+    int32_t spp_per_round;     /* samples each admitted job advances per round (1) */
+    uint32_t max_paths_per_round; /* 0 = sized from the budget */
+    int32_t max_depth;         /* deepest job accepted; bounces per round (0 = 16) */
 } lx_server_opts;
 
 lx_status lx_server_start(lx_context* ctx, const lx_server_opts* opts, lx_server** out);

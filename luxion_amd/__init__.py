@@ -255,3 +255,88 @@ class Render:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------------------ batched form
+def make_job(scene, camera, spp, max_depth=0, dmis=False, emis=False, envmap_intensity=1.0, seed=0, first_iter=1,
+             deadline_ms=0.0):
+    j = F.lx_job()
+    j.scene = scene.h.value
+    j.camera = scenes.camera_desc(camera)
+    j.spp, j.max_depth, j.dmis, j.emis = spp, max_depth, int(dmis), int(emis)
+    j.envmap_intensity, j.seed, j.first_iter, j.deadline_ms = envmap_intensity, seed, first_iter, deadline_ms
+    return j
+
+
+def batch_run(ctx, jobs):
+    """Render every job to completion, all jobs sharing each bounce launch.
+    Returns one dict per job: image (H, W, 3 radiance sums), spp, status,
+    timestamps (context clock, ms)."""
+    n = len(jobs)
+    arr = (F.lx_job * n)(*jobs)
+    res = (F.lx_job_result * n)()
+    imgs = []
+    for k, j in enumerate(jobs):
+        img = np.zeros((j.camera.height, j.camera.width, 3), np.float32)
+        imgs.append(img)
+        res[k].rgb_sum = img.ctypes.data
+    check(ctx.lib.lx_batch_run(ctx.h, arr, n, res))
+    return [dict(image=imgs[k], spp=res[k].spp_done, status=res[k].status, submit_ms=res[k].submit_ms,
+                 start_ms=res[k].start_ms, finish_ms=res[k].finish_ms, rays=res[k].rays) for k in range(n)]
+
+
+# ------------------------------------------------------------- served form
+class Server:
+    """Continuous-batching render server (one request thread in the
+    library).  Results are copied into per-job numpy images owned here."""
+
+    def __init__(self, ctx, hbm_budget_bytes=0, max_jobs=64, max_paths=0, max_depth=16):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.h = C.c_void_p()
+        o = F.lx_server_opts()
+        o.hbm_budget_bytes, o.max_jobs_in_flight, o.spp_per_round = hbm_budget_bytes, max_jobs, 1
+        o.max_paths_per_round, o.max_depth = max_paths, max_depth
+        check(self.lib.lx_server_start(ctx.h, C.byref(o), C.byref(self.h)))
+        self._jobs = {}
+
+    def submit(self, job):
+        img = np.zeros((job.camera.height, job.camera.width, 3), np.float32)
+        res = F.lx_job_result()
+        res.rgb_sum = img.ctypes.data
+        jid = C.c_uint64()
+        check(self.lib.lx_server_submit(self.h, C.byref(job), C.byref(res), C.byref(jid)))
+        self._jobs[jid.value] = (job, res, img)
+        return jid.value
+
+    def poll(self, jid, timeout_ms=-1.0):
+        d = C.c_int32()
+        check(self.lib.lx_server_poll(self.h, jid, timeout_ms, C.byref(d)))
+        return bool(d.value)
+
+    def result(self, jid):
+        job, res, img = self._jobs[jid]
+        return dict(image=img, spp=res.spp_done, status=res.status, submit_ms=res.submit_ms, start_ms=res.start_ms,
+                    finish_ms=res.finish_ms)
+
+    def cancel(self, jid):
+        check(self.lib.lx_server_cancel(self.h, jid))
+
+    def inject_budget_failure(self, n=1):
+        check(self.lib.lx_server_inject_budget_failure(self.h, n))
+
+    def stats(self):
+        s = F.lx_server_stats()
+        check(self.lib.lx_server_stats_get(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in F.lx_server_stats._fields_}
+
+    def stop(self):
+        if self.h:
+            check(self.lib.lx_server_stop(self.h))
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.stop()
+        except Exception:
+            pass

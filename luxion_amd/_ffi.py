@@ -117,12 +117,12 @@ class lx_job_result(C.Structure):
 
 class lx_tuning(C.Structure):
     _fields_ = [("block_size", C.c_int32), ("bvh_width", C.c_int32), ("queue_chunk", C.c_int32),
-                ("waves_per_cu", C.c_int32), ("use_graph", C.c_int32)]
+                ("waves_per_cu", C.c_int32), ("use_graph", C.c_int32), ("max_paths_per_round", C.c_uint32)]
 
 
 class lx_server_opts(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_jobs_in_flight", C.c_int32),
-                ("spp_per_round", C.c_int32), ("max_paths_per_round", C.c_uint32)]
+                ("spp_per_round", C.c_int32), ("max_paths_per_round", C.c_uint32), ("max_depth", C.c_int32)]
 
 
 class lx_server_stats(C.Structure):

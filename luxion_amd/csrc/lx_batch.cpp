@@ -1,0 +1,209 @@
+/*
+ * lx_batch.cpp -- batched form: many render jobs (any scenes, resolutions,
+ * spp, depths and flags) share one launch per bounce.
+ *
+ * The reference renders one job per process, serially (run_best_scenes.ps1:22-50).
+ * Here every round takes one sample of every job that still needs samples
+ * (up to a path capacity); k_begin_round does the scheduling on the device,
+ * so the round is captured once into a hipGraph and replayed.  Each job's
+ * pixels keep their own RNG streams (iteration, pixel, depth, job seed), so a
+ * job's image is bit-identical whether it runs alone or in a batch.
+ */
+#include "lx_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+static double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+void lx_validate_job(const lx_job& j) {
+    LX_REQUIRE(j.scene != nullptr, LX_ERR_INVALID, "job without scene");
+    lx_scene* s = j.scene;
+    LX_REQUIRE(s->accel_built, LX_ERR_STATE, "job scene: call lx_scene_build_accel first");
+    LX_REQUIRE(j.camera.width > 0 && j.camera.height > 0, LX_ERR_INVALID, "job camera resolution must be positive");
+    LX_REQUIRE(j.spp >= 1, LX_ERR_INVALID, "job spp must be >= 1");
+    LX_REQUIRE(j.first_iter >= 1, LX_ERR_INVALID, "iterations are 1-based");
+    int depth = j.max_depth > 0 ? j.max_depth : s->trace_depth;
+    LX_REQUIRE(depth <= LX_MAX_DEPTH, LX_ERR_INVALID, "max_depth above the supported maximum");
+    if (j.dmis)
+        LX_REQUIRE(!s->emissive.empty() && !(s->total_area < 1e-5f), LX_ERR_INVALID,
+                   "-dmis requires the scene to contain emissive geometry");
+    if (j.emis) LX_REQUIRE(s->has_env, LX_ERR_INVALID, "-emis requires an environment map, pass one with -e");
+}
+
+LxJob lx_make_dev_job(const lx_job& j, uint32_t accum_off) {
+    LxJob J;
+    std::memset(&J, 0, sizeof(J));
+    J.scene = j.scene->d_scene.as<LxDevScene>();
+    J.cam = j.camera;
+    J.num_pix = (uint32_t)j.camera.width * (uint32_t)j.camera.height;
+    J.accum_off = accum_off;
+    J.depth = j.max_depth > 0 ? j.max_depth : j.scene->trace_depth;
+    J.dmis = j.dmis;
+    J.emis = j.emis;
+    J.intensity = j.envmap_intensity;
+    J.seed = j.seed;
+    J.iter = (int32_t)j.first_iter - 1;
+    J.spp_done = 0;
+    return J;
+}
+
+/* Host replica of k_begin_round's greedy schedule: which round finishes each
+ * job, and how many rounds the batch needs. */
+int lx_simulate_rounds(const std::vector<uint32_t>& npix, const std::vector<int32_t>& spp, uint32_t cap,
+                       std::vector<int>* finish_round) {
+    size_t n = npix.size();
+    std::vector<int32_t> done(n, 0);
+    if (finish_round) finish_round->assign(n, -1);
+    int rounds = 0;
+    while (true) {
+        uint32_t off = 0, na = 0;
+        bool any = false;
+        for (size_t j = 0; j < n; j++) {
+            if (done[j] < spp[j] && na < LX_SCHED_MAX && off + npix[j] <= cap) {
+                off += npix[j];
+                na++;
+                done[j]++;
+                any = true;
+                if (done[j] == spp[j] && finish_round) (*finish_round)[j] = rounds;
+            }
+        }
+        if (!any) break;
+        rounds++;
+    }
+    return rounds;
+}
+
+uint32_t lx_batch_path_cap(lx_context* ctx, const std::vector<uint32_t>& npix, bool dmis, uint64_t extra_bytes) {
+    uint64_t total = 0, mx = 0;
+    for (uint32_t p : npix) {
+        total += p;
+        mx = std::max<uint64_t>(mx, p);
+    }
+    /* bytes per path slot: two path buffers + records (+ emitter records) + spill share */
+    uint64_t per = 2 * 15 * 4 + (11 + 3 + (dmis ? 10 : 0)) * 4;
+    uint64_t free_b = ctx->budget > ctx->in_use.load() + extra_bytes ? ctx->budget - ctx->in_use.load() - extra_bytes : 0;
+    uint64_t cap = std::min<uint64_t>(total, free_b / per / 2);
+    if (ctx->tuning.max_paths_per_round) cap = std::min<uint64_t>(cap, ctx->tuning.max_paths_per_round);
+    cap = std::min<uint64_t>(cap, (1ull << 31) - 1);
+    LX_REQUIRE(cap >= mx, LX_ERR_BUDGET, "HBM budget too small for the largest job's paths");
+    return (uint32_t)cap;
+}
+
+extern "C" lx_status lx_batch_run(lx_context* ctx, const lx_job* jobs, uint32_t n, lx_job_result* results) {
+    try {
+        LX_REQUIRE(ctx && jobs && results && n > 0, LX_ERR_INVALID, "null argument");
+        double t_submit = now_ms();
+        for (uint32_t i = 0; i < n; i++) lx_validate_job(jobs[i]);
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->set_device();
+        std::vector<LxJob> hj(n);
+        std::vector<int32_t> hspp(n);
+        std::vector<uint32_t> npix(n);
+        uint64_t accum_pix = 0;
+        bool any_dmis = false, any_tan = false;
+        int max_depth = 1;
+        for (uint32_t i = 0; i < n; i++) {
+            hj[i] = lx_make_dev_job(jobs[i], (uint32_t)accum_pix);
+            npix[i] = hj[i].num_pix;
+            accum_pix += npix[i];
+            hspp[i] = jobs[i].spp;
+            any_dmis |= jobs[i].dmis != 0;
+            for (auto& m : jobs[i].scene->mats) any_tan |= m.normal_tex >= 0;
+            max_depth = std::max(max_depth, hj[i].depth);
+        }
+        LX_REQUIRE(accum_pix < (1ull << 32), LX_ERR_INVALID, "batch too large");
+        DevBuf d_jobs, d_spp, d_cnt, d_sched, d_spill, d_accum;
+        d_accum.alloc(ctx, accum_pix * 12);
+        uint32_t cap = lx_batch_path_cap(ctx, npix, any_dmis, (uint64_t)lx_spill_words(ctx->num_cus * 16u) * 4 + (1u << 20));
+        WaveBufs wb;
+        wb.alloc(ctx, cap, any_dmis, true, any_tan);
+        uint32_t waves = (uint32_t)std::max(1, ctx->tuning.waves_per_cu);
+        uint32_t tg = std::min((uint32_t)ctx->num_cus * std::max(1u, waves * 64u / LX_TRACE_BLOCK),
+                               std::max(1u, (cap + LX_TRACE_BLOCK - 1) / LX_TRACE_BLOCK));
+        uint32_t sg = std::min((uint32_t)ctx->num_cus * 8u, std::max(1u, (cap + LX_SHADE_BLOCK - 1) / LX_SHADE_BLOCK));
+        uint32_t cg = std::min((uint32_t)ctx->num_cus * 8u, std::max(1u, (cap + 255u) / 256u));
+        d_spill.alloc(ctx, (size_t)lx_spill_words(tg) * 4);
+        d_jobs.upload(ctx, hj.data(), n * sizeof(LxJob));
+        d_spp.upload(ctx, hspp.data(), n * 4);
+        d_cnt.alloc(ctx, sizeof(LxCounters));
+        d_sched.alloc(ctx, sizeof(LxSched));
+        hipStream_t s = ctx->stream;
+        LX_HIP(lx_launch_fill_u32(d_accum.as<uint32_t>(), accum_pix * 3, 0u, s));
+        LX_HIP(lx_launch_fill_u32(d_cnt.as<uint32_t>(), sizeof(LxCounters) / 4, 0u, s));
+        std::vector<int> fin;
+        int rounds = lx_simulate_rounds(npix, hspp, cap, &fin);
+        /* one captured round, replayed; an event after every round dates job completion */
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        bool use_graph = ctx->tuning.use_graph != 0;
+        if (use_graph) {
+            LX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            try {
+                lx_enqueue_round(ctx, d_jobs.as<LxJob>(), d_spp.as<int32_t>(), n, d_cnt.as<LxCounters>(),
+                                 d_sched.as<LxSched>(), cap, wb, d_accum.as<float>(), d_spill.as<uint32_t>(), max_depth,
+                                 tg, sg, cg, nullptr);
+            } catch (...) {
+                hipGraph_t tmp;
+                (void)hipStreamEndCapture(s, &tmp);
+                throw;
+            }
+            LX_HIP(hipStreamEndCapture(s, &g));
+            LX_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+        }
+        std::vector<hipEvent_t> ev(rounds + 1);
+        for (auto& e : ev) LX_HIP(hipEventCreate(&e));
+        LX_HIP(hipEventRecord(ev[0], s));
+        double t_start = now_ms();
+        for (int k = 0; k < rounds; k++) {
+            if (use_graph) LX_HIP(hipGraphLaunch(ge, s));
+            else
+                lx_enqueue_round(ctx, d_jobs.as<LxJob>(), d_spp.as<int32_t>(), n, d_cnt.as<LxCounters>(),
+                                 d_sched.as<LxSched>(), cap, wb, d_accum.as<float>(), d_spill.as<uint32_t>(), max_depth,
+                                 tg, sg, cg, nullptr);
+            LX_HIP(hipEventRecord(ev[k + 1], s));
+        }
+        LX_HIP(hipStreamSynchronize(s));
+        double t_end = now_ms();
+        LxCounters c;
+        LX_HIP(hipMemcpy(&c, d_cnt.ptr, sizeof(c), hipMemcpyDeviceToHost));
+        std::vector<float> round_ms(rounds + 1, 0.0f);
+        for (int k = 1; k <= rounds; k++) LX_HIP(hipEventElapsedTime(&round_ms[k], ev[0], ev[k]));
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        if (ge) (void)hipGraphExecDestroy(ge);
+        if (g) (void)hipGraphDestroy(g);
+        LX_REQUIRE(c.stack_overflow == 0, LX_ERR_STATE, "traversal stack overflow");
+        std::vector<LxJob> done(n);
+        LX_HIP(hipMemcpy(done.data(), d_jobs.ptr, n * sizeof(LxJob), hipMemcpyDeviceToHost));
+        uint64_t tot_rays = c.rad_rays + c.dl_rays + c.env_rays;
+        for (uint32_t i = 0; i < n; i++) {
+            lx_job_result& r = results[i];
+            if (r.rgb_sum)
+                LX_HIP(hipMemcpy(r.rgb_sum, (uint8_t*)d_accum.ptr + (size_t)hj[i].accum_off * 12, (size_t)npix[i] * 12,
+                                 hipMemcpyDeviceToHost));
+            r.status = done[i].spp_done == hspp[i] ? LX_OK : LX_ERR_STATE;
+            r.spp_done = (uint32_t)done[i].spp_done;
+            /* rays are counted for the batch; attributed to jobs by path-samples */
+            r.rays = (uint64_t)((double)tot_rays * ((double)npix[i] * hspp[i]) /
+                                std::max(1.0, [&]() {
+                                    double t = 0;
+                                    for (uint32_t k = 0; k < n; k++) t += (double)npix[k] * hspp[k];
+                                    return t;
+                                }()));
+            r.submit_ms = t_submit;
+            r.start_ms = t_start;
+            r.finish_ms = fin[i] >= 0 ? t_start + round_ms[fin[i] + 1] : t_end;
+        }
+        lx_set_error("");
+        return LX_OK;
+    } catch (const LxError& e) {
+        lx_set_error(e.msg);
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        lx_set_error("host allocation failed");
+        return LX_ERR_OOM;
+    }
+}

@@ -199,4 +199,12 @@ void lx_enqueue_round(lx_context* ctx, LxJob* d_jobs, const int32_t* d_spp, uint
                       hipEvent_t* ev /* 2*max_depth+1 events or NULL */, bool count = false);
 uint32_t lx_spill_words(uint32_t trace_grid);
 
+
+
+/* ------------------------------------------------------ batched / served */
+void lx_validate_job(const lx_job& j);
+LxJob lx_make_dev_job(const lx_job& j, uint32_t accum_off);
+int lx_simulate_rounds(const std::vector<uint32_t>& npix, const std::vector<int32_t>& spp, uint32_t cap,
+                       std::vector<int>* finish_round);
+uint32_t lx_batch_path_cap(lx_context* ctx, const std::vector<uint32_t>& npix, bool dmis, uint64_t extra_bytes);
 #endif

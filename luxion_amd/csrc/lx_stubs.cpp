@@ -12,12 +12,4 @@ lx_status lx_scene_load_glb(lx_context*, const char*, const char*, lx_scene**) {
 }
 lx_status lx_autotune(lx_context*, const lx_job*, uint32_t, const char*, lx_tuning*) { return LX_ERR_UNSUPPORTED; }
 lx_status lx_tuning_load(lx_context*, const char*) { return LX_ERR_UNSUPPORTED; }
-lx_status lx_batch_run(lx_context*, const lx_job*, uint32_t, lx_job_result*) { return LX_ERR_UNSUPPORTED; }
-lx_status lx_server_start(lx_context*, const lx_server_opts*, lx_server**) { return LX_ERR_UNSUPPORTED; }
-lx_status lx_server_submit(lx_server*, const lx_job*, lx_job_result*, uint64_t*) { return LX_ERR_UNSUPPORTED; }
-lx_status lx_server_poll(lx_server*, uint64_t, double, int32_t*) { return LX_ERR_UNSUPPORTED; }
-lx_status lx_server_cancel(lx_server*, uint64_t) { return LX_ERR_UNSUPPORTED; }
-lx_status lx_server_stats_get(lx_server*, lx_server_stats*) { return LX_ERR_UNSUPPORTED; }
-lx_status lx_server_inject_budget_failure(lx_server*, uint32_t) { return LX_ERR_UNSUPPORTED; }
-lx_status lx_server_stop(lx_server*) { return LX_ERR_UNSUPPORTED; }
 }
