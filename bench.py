@@ -20,6 +20,25 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBPS = 8000.0
 NODE_BYTES, TRI_BYTES, INST_BYTES = 128, 48, 64
 RAY_BYTES, HIT_BYTES, OCC_BYTES = 32, 16, 4
+# per path-bounce shading: path state read + write, three hit records, three
+# index triples, normals and UVs, the material, the next ray and two shadow
+# rays, one queue word (texels not counted)
+SHADE_BYTES = 2 * 48 + 3 * 16 + 12 + 36 + 24 + 64 + 32 + 2 * 32 + 4
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (a line 'traffic_bytes_per_launch:
+    N' in profiles/<kernel>_cfg1_1080p_emis.md), or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_cfg1_1080p_emis.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.strip().startswith("traffic_bytes_per_launch:"):
+                    return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
 
 
 def log(msg):
@@ -34,6 +53,8 @@ def main():
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--level", type=int, default=8)
+    ap.add_argument("--wavefront-bounces", type=int, default=None,
+                    help="bounces launched one kernel each before the tail kernel (default: the context's)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run the plain 16-byte copy kernel over this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -66,6 +87,9 @@ def main():
         f"build {info['build_ms']:.0f} ms")
     depth = data.trace_depth
 
+    if args.wavefront_bounces is not None:
+        ctx.set_tuning(wavefront_bounces=args.wavefront_bounces)
+    tuning = ctx.tuning()
     r = lx.Render(sc, emis=True)
     r.iterate(1, args.warmup)
     r.sync()
@@ -98,20 +122,37 @@ def main():
     rc.set_timing(events=False, count_work=True)
     rc.iterate(first, args.steps)
     sc_ = rc.stats()
+    assert sc_["radiance_rays"] == st["radiance_rays"], "counting replay traced a different ray batch"
+    rc.close()
+
+    def trace_bytes(n_closest, n_any, c):
+        return (n_closest * (RAY_BYTES + HIT_BYTES) + n_any * (RAY_BYTES + OCC_BYTES) +
+                INST_BYTES * (c["closest_insts"] + c["any_insts"]) +
+                NODE_BYTES * (c["closest_nodes"] + c["any_nodes"]) + TRI_BYTES * (c["closest_tris"] + c["any_tris"]))
+
     n_closest = sc_["radiance_rays"] + sc_["dl_shadow_rays"]
     n_any = sc_["env_shadow_rays"]
-    alg_bytes = (n_closest * (RAY_BYTES + HIT_BYTES) + n_any * (RAY_BYTES + OCC_BYTES) +
-                 INST_BYTES * (sc_["closest_insts"] + sc_["any_insts"]) +
-                 NODE_BYTES * (sc_["closest_nodes"] + sc_["any_nodes"]) +
-                 TRI_BYTES * (sc_["closest_tris"] + sc_["any_tris"]))
-    assert sc_["radiance_rays"] == st["radiance_rays"], "counting replay traced a different ray batch"
-    trace_launch_ms = st["trace_ms"] / max(1, st["trace_launches"])
-    bytes_per_launch = alg_bytes / max(1, st["trace_launches"])
-    achieved = bytes_per_launch / (trace_launch_ms * 1e-3) / 1e9
-    rays_total = max(1, rays)
+    tail = {k[5:]: v for k, v in sc_.items() if k.startswith("tail_") and k[5:] in
+            ("closest_nodes", "closest_tris", "closest_insts", "any_nodes", "any_tris", "any_insts")}
+    wave = {k: sc_[k] - tail[k] for k in tail}
+    tail_bytes = trace_bytes(sc_["tail_closest_rays"], sc_["tail_any_rays"], tail)
+    wave_bytes = trace_bytes(n_closest - sc_["tail_closest_rays"], n_any - sc_["tail_any_rays"], wave)
     per_ray = {"nodes_closest": sc_["closest_nodes"] / max(1, n_closest),
                "tris_closest": sc_["closest_tris"] / max(1, n_closest),
                "nodes_any": sc_["any_nodes"] / max(1, n_any), "tris_any": sc_["any_tris"] / max(1, n_any)}
+    # the dominant kernel: the tail kernel (trace + shade of every bounce it
+    # runs) or, when every bounce is its own launch, the trace kernel
+    if st["tail_ms"] >= st["trace_ms"]:
+        kname, kms, klaunch = "k_finish", st["tail_ms"], st["tail_launches"]
+        shaded = sc_["tail_bounces"]
+        kbytes = tail_bytes + SHADE_BYTES * shaded
+    else:
+        kname, kms, klaunch = "k_trace", st["trace_ms"], st["trace_launches"]
+        kbytes = wave_bytes
+    launch_ms = kms / max(1, klaunch)
+    bytes_per_launch = kbytes / max(1, klaunch)
+    achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
+    traffic = measured_traffic(kname)
 
     value = rays / t_timed / 1e6
     value_graph = (st_graph["radiance_rays"] + st_graph["dl_shadow_rays"] + st_graph["env_shadow_rays"]) \
@@ -164,11 +205,13 @@ def main():
         "graph_replay": {"value": round(value_graph, 2), "ms_per_step": round(t_graph * 1e3 / args.steps, 3)},
         "kernel_ms_per_step": {"trace": round(st["trace_ms"] / args.steps, 3),
                                "shade": round(st["shade_ms"] / args.steps, 3),
+                               "tail": round(st["tail_ms"] / args.steps, 3),
                                "other": round(st["other_ms"] / args.steps, 3)},
-        "roofline": {"bound": "hbm", "kernel": "k_trace", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
-                     "bytes_per_launch": int(bytes_per_launch), "avg_launch_ms": round(trace_launch_ms, 4),
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
+                     "bytes_per_launch": int(bytes_per_launch), "avg_launch_ms": round(launch_ms, 4),
                      "per_ray": {k: round(v, 2) for k, v in per_ray.items()}},
+        "tuning": tuning,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -247,6 +247,14 @@ typedef struct lx_render_stats {
      * shadow) and any-hit (environment shadow) rays */
     uint64_t closest_nodes, closest_tris, closest_insts;
     uint64_t any_nodes, any_tris, any_insts;
+    /* the tail kernel's share of the above (bounces at and after
+     * lx_tuning.wavefront_bounces), its rays, and its event-timed device time */
+    uint64_t tail_closest_rays, tail_any_rays;
+    uint64_t tail_closest_nodes, tail_closest_tris, tail_closest_insts;
+    uint64_t tail_any_nodes, tail_any_tris, tail_any_insts;
+    double tail_ms;
+    uint32_t tail_launches;
+    uint64_t tail_bounces; /* path-bounces shaded by the tail kernel */
 } lx_render_stats;
 /* Counters since the last reset. */
 lx_status lx_render_stats_get(lx_render* r, lx_render_stats* out);
@@ -329,12 +337,17 @@ typedef struct lx_job_result {
 } lx_job_result;
 
 typedef struct lx_tuning {
-    int32_t block_size;    /* threads per workgroup of the bounce kernels */
+    int32_t block_size;    /* threads per workgroup of the tail kernel: 64, 128 or 256 */
     int32_t bvh_width;     /* 4 or 8 */
     int32_t queue_chunk;   /* paths claimed per wave from the work queue */
     int32_t waves_per_cu;  /* persistent grid size */
     int32_t use_graph;     /* capture the bounce loop in a hipGraph */
     uint32_t max_paths_per_round; /* batched form path capacity, 0 = from the budget */
+    int32_t wavefront_bounces; /* bounces run as one launch each; the rest run in the
+                                  tail kernel, one lane per path to its end (same bits);
+                                  negative = every bounce its own launch */
+    int32_t tail_min_waves;    /* 0, or 4: the tail kernel is compiled for at least
+                                  4 waves per SIMD (fewer registers, some spills) */
 } lx_tuning;
 
 lx_status lx_tuning_get(const lx_context* ctx, lx_tuning* out);

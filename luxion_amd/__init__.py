@@ -54,6 +54,11 @@ class Context:
         check(self.lib.lx_tuning_get(self.h, C.byref(t)))
         return {k: getattr(t, k) for k, _ in F.lx_tuning._fields_}
 
+    def tuning(self):
+        t = F.lx_tuning()
+        check(self.lib.lx_tuning_get(self.h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in F.lx_tuning._fields_}
+
     def set_tuning(self, **kw):
         t = F.lx_tuning()
         check(self.lib.lx_tuning_get(self.h, C.byref(t)))

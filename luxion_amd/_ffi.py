@@ -92,7 +92,11 @@ class lx_render_stats(C.Structure):
                 ("shade_ms", C.c_double), ("other_ms", C.c_double), ("trace_launches", C.c_uint32),
                 ("shade_launches", C.c_uint32), ("closest_nodes", C.c_uint64), ("closest_tris", C.c_uint64),
                 ("closest_insts", C.c_uint64), ("any_nodes", C.c_uint64), ("any_tris", C.c_uint64),
-                ("any_insts", C.c_uint64)]
+                ("any_insts", C.c_uint64), ("tail_closest_rays", C.c_uint64), ("tail_any_rays", C.c_uint64),
+                ("tail_closest_nodes", C.c_uint64), ("tail_closest_tris", C.c_uint64),
+                ("tail_closest_insts", C.c_uint64), ("tail_any_nodes", C.c_uint64), ("tail_any_tris", C.c_uint64),
+                ("tail_any_insts", C.c_uint64), ("tail_ms", C.c_double), ("tail_launches", C.c_uint32),
+                ("tail_bounces", C.c_uint64)]
 
 
 class lx_ray(C.Structure):
@@ -117,7 +121,8 @@ class lx_job_result(C.Structure):
 
 class lx_tuning(C.Structure):
     _fields_ = [("block_size", C.c_int32), ("bvh_width", C.c_int32), ("queue_chunk", C.c_int32),
-                ("waves_per_cu", C.c_int32), ("use_graph", C.c_int32), ("max_paths_per_round", C.c_uint32)]
+                ("waves_per_cu", C.c_int32), ("use_graph", C.c_int32), ("max_paths_per_round", C.c_uint32),
+                ("wavefront_bounces", C.c_int32), ("tail_min_waves", C.c_int32)]
 
 
 class lx_server_opts(C.Structure):

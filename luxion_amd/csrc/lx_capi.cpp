@@ -86,6 +86,8 @@ lx_status lx_context_create(int device, uint64_t budget, lx_context** out) {
         c->tuning.queue_chunk = 64;
         c->tuning.waves_per_cu = 16;
         c->tuning.use_graph = 1;
+        c->tuning.wavefront_bounces = LX_DEFAULT_WAVEFRONT_BOUNCES;
+        c->tuning.tail_min_waves = 4;
     } catch (...) {
         if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
@@ -134,6 +136,9 @@ lx_status lx_tuning_set(lx_context* c, const lx_tuning* t) {
     LX_REQUIRE(c && t, LX_ERR_INVALID, "null argument");
     LX_REQUIRE(t->waves_per_cu >= 1 && t->waves_per_cu <= 32, LX_ERR_INVALID, "waves_per_cu out of range");
     LX_REQUIRE(t->bvh_width == 4 || t->bvh_width == 8, LX_ERR_INVALID, "bvh_width must be 4 or 8");
+    LX_REQUIRE(t->block_size == 64 || t->block_size == 128 || t->block_size == 256, LX_ERR_INVALID,
+               "block_size must be 64, 128 or 256");
+    LX_REQUIRE(t->tail_min_waves == 0 || t->tail_min_waves == 4, LX_ERR_INVALID, "tail_min_waves must be 0 or 4");
     c->tuning = *t;
     LX_CATCH
 }
@@ -404,22 +409,28 @@ lx_status lx_render_iterate(lx_render* r, uint32_t first_iter, uint32_t n) {
         }
         LX_HIP(hipStreamSynchronize(ctx->stream));
         if (r->timing) {
+            int wf = lx_wavefront_bounces(ctx, r->depth);
             for (uint32_t k = 0; k < n; k++) {
                 hipEvent_t* e = &ev[(size_t)k * (per + 2)];
                 float ms;
-                for (int b = 0; b < r->depth; b++) {
+                for (int b = 0; b < wf; b++) {
                     LX_HIP(hipEventElapsedTime(&ms, e[1 + 2 * b], e[2 + 2 * b]));
                     r->trace_ms += ms;
                     LX_HIP(hipEventElapsedTime(&ms, e[2 + 2 * b], e[3 + 2 * b]));
                     r->shade_ms += ms;
+                }
+                if (wf < r->depth) {
+                    LX_HIP(hipEventElapsedTime(&ms, e[1 + 2 * wf], e[2 + 2 * wf]));
+                    r->tail_ms += ms;
                 }
                 LX_HIP(hipEventElapsedTime(&ms, e[0], e[1]));
                 r->other_ms += ms;
                 LX_HIP(hipEventElapsedTime(&ms, e[per], e[per + 1]));
                 r->other_ms += ms;
             }
-            r->trace_launches += r->depth * n;
-            r->shade_launches += r->depth * n;
+            r->trace_launches += wf * n;
+            r->shade_launches += wf * n;
+            if (wf < r->depth) r->tail_launches += n;
             for (auto& e : ev) (void)hipEventDestroy(e);
         }
     } else if (ctx->tuning.use_graph) {
@@ -488,8 +499,8 @@ lx_status lx_render_reset(lx_render* r) {
     LX_HIP(lx_launch_fill_u32(r->counters.as<uint32_t>(), sizeof(LxCounters) / 4, 0u, r->ctx->stream));
     LX_HIP(hipStreamSynchronize(r->ctx->stream));
     r->spp = 0;
-    r->trace_ms = r->shade_ms = r->other_ms = 0;
-    r->trace_launches = r->shade_launches = 0;
+    r->trace_ms = r->shade_ms = r->other_ms = r->tail_ms = 0;
+    r->trace_launches = r->shade_launches = r->tail_launches = 0;
     LX_CATCH
 }
 
@@ -519,12 +530,24 @@ lx_status lx_render_stats_get(lx_render* r, lx_render_stats* out) {
     out->other_ms = r->other_ms;
     out->trace_launches = r->trace_launches;
     out->shade_launches = r->shade_launches;
-    out->closest_nodes = c.closest_nodes;
-    out->closest_tris = c.closest_tris;
-    out->closest_insts = c.closest_insts;
-    out->any_nodes = c.any_nodes;
-    out->any_tris = c.any_tris;
-    out->any_insts = c.any_insts;
+    const LxWorkCnt &w0 = c.work[0], &w1 = c.work[1];
+    out->closest_nodes = w0.closest_nodes + w1.closest_nodes;
+    out->closest_tris = w0.closest_tris + w1.closest_tris;
+    out->closest_insts = w0.closest_insts + w1.closest_insts;
+    out->any_nodes = w0.any_nodes + w1.any_nodes;
+    out->any_tris = w0.any_tris + w1.any_tris;
+    out->any_insts = w0.any_insts + w1.any_insts;
+    out->tail_closest_rays = w1.closest_rays;
+    out->tail_any_rays = w1.any_rays;
+    out->tail_closest_nodes = w1.closest_nodes;
+    out->tail_closest_tris = w1.closest_tris;
+    out->tail_closest_insts = w1.closest_insts;
+    out->tail_any_nodes = w1.any_nodes;
+    out->tail_any_tris = w1.any_tris;
+    out->tail_any_insts = w1.any_insts;
+    out->tail_ms = r->tail_ms;
+    out->tail_launches = r->tail_launches;
+    out->tail_bounces = c.tail_bounces;
     LX_CATCH
 }
 

@@ -16,6 +16,12 @@
 #include "lx_kernels.h"
 #include "lx_types.h"
 
+/* bounces before the tail kernel takes over in a fresh context; negative
+ * runs every bounce as its own launch */
+#ifndef LX_DEFAULT_WAVEFRONT_BOUNCES
+#define LX_DEFAULT_WAVEFRONT_BOUNCES 0
+#endif
+
 /* ------------------------------------------------------------- errors */
 void lx_set_error(const std::string& msg);
 
@@ -186,8 +192,8 @@ struct lx_render {
     uint32_t trace_grid = 0, shade_grid = 0, cam_grid = 0;
     bool timing = false;
     bool counting = false;
-    double trace_ms = 0, shade_ms = 0, other_ms = 0;
-    uint32_t trace_launches = 0, shade_launches = 0;
+    double trace_ms = 0, shade_ms = 0, other_ms = 0, tail_ms = 0;
+    uint32_t trace_launches = 0, shade_launches = 0, tail_launches = 0;
     hipGraphExec_t graph = nullptr;
     hipGraph_t graph_src = nullptr;
 };
@@ -198,6 +204,10 @@ void lx_enqueue_round(lx_context* ctx, LxJob* d_jobs, const int32_t* d_spp, uint
                       int max_depth, uint32_t trace_grid, uint32_t shade_grid, uint32_t cam_grid,
                       hipEvent_t* ev /* 2*max_depth+1 events or NULL */, bool count = false);
 uint32_t lx_spill_words(uint32_t trace_grid);
+/* bounces launched one kernel each before the tail kernel takes over
+ * (tuning.wavefront_bounces; negative or above max_depth = all of them,
+ * 0 = the tail kernel runs every bounce) */
+int lx_wavefront_bounces(const lx_context* ctx, int max_depth);
 
 
 

@@ -6,25 +6,36 @@
  *                   counters (no host round trip, so the round can be a
  *                   captured hipGraph)
  *   k_camera        generateRayFromCamera (src/kernels/generate_ray_from_camera.cu:8-39)
- *   per bounce b = 0 .. maxdepth-1:
- *     k_trace       sampleDirectLight (src/kernels/sample_direct_light.cu:9-101)
- *                   + the raygen's radiance / direct-light / environment rays
- *                   (src/optixshaders/optix_triangle.cu:52-143) + hit attributes
- *     k_shade       shadePath + update_throughput_materials + compaction
- *                   (src/kernels/shade_path.cu:94-364) fused with finalGather
- *                   (src/kernels/image_kernels.cu:49-60): a path adds its colour
- *                   to the accumulation buffer when it stops
+ *   bounces b < B (wavefront):
+ *     k_trace       sampleDirectLight + raygen rays + hit attributes
+ *                   (src/kernels/sample_direct_light.cu, src/optixshaders/optix_triangle.cu)
+ *     k_shade       shadePath + RR + compaction (src/kernels/shade_path.cu:94-364) fused
+ *                   with finalGather (src/kernels/image_kernels.cu:49-60): a path adds its
+ *                   colour to the accumulation buffer when it stops
+ *   bounces b >= B (tail):
+ *     k_finish      each lane takes a live path and carries it through all its
+ *                   remaining bounces (trace + shade in registers), refilling from
+ *                   the queue when its path stops; the per-bounce launch barrier,
+ *                   which makes every late bounce as long as its slowest path, is gone
  *   k_end_round     spp bookkeeping
  * Every kernel reads its path count from device memory and grid-strides over
  * fixed launch shapes.  Paths are compacted by copying their state to the next
- * buffer at a slot from one atomic per wave64 (ballot + mbcnt).
+ * buffer at a slot from one atomic per wave64 (ballot + mbcnt).  A path's
+ * arithmetic does not depend on the bounce at which the tail kernel takes
+ * over, so every choice of B gives the same bits.
  */
-#include "lx_trace_dev.h"
+#include "lx_path_dev.h"
 #include "lx_kernels.h"
 
 #define LX_WAVE 64
 
 LXD uint32_t lx_lane() { return __lane_id(); }
+
+LXD void lx_wave_add(unsigned long long* dst, uint32_t v) {
+    /* wave64 reduction, one atomic per wave */
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, LX_WAVE);
+    if (lx_lane() == 0 && v) atomicAdd(dst, (unsigned long long)v);
+}
 
 /* ------------------------------------------------------------ scheduling */
 __global__ void k_begin_round(LxJob* __restrict__ jobs, const int32_t* __restrict__ job_spp, uint32_t njobs,
@@ -52,6 +63,7 @@ __global__ void k_begin_round(LxJob* __restrict__ jobs, const int32_t* __restric
         }
         sched->n_active = na;
         s_total = off;
+        C->tail_next = 0;
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i <= LX_MAX_DEPTH; i += blockDim.x) C->count[i] = (i == 0) ? s_total : 0u;
@@ -114,12 +126,68 @@ __global__ __launch_bounds__(256) void k_camera(const LxJob* __restrict__ jobs, 
     }
 }
 
-/* ------------------------------------------------------------ trace */
-LXD void lx_wave_add(unsigned long long* dst, uint32_t v) {
-    /* wave64 reduction, one atomic per wave */
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, LX_WAVE);
-    if (lx_lane() == 0 && v) atomicAdd(dst, (unsigned long long)v);
+/* ------------------------------------------------------------ counters */
+template <bool CNT>
+LXD void lx_flush_counts(LxCounters* C, LxWorkCnt* W, const LxRayCount& rc, uint32_t nsh, const LxWork& wc,
+                         const LxWork& wa) {
+    lx_wave_add(&C->rad_rays, rc.rad);
+    lx_wave_add(&C->dl_rays, rc.dl);
+    lx_wave_add(&C->env_rays, rc.env);
+    lx_wave_add(&C->bounces, nsh);
+    if (CNT) {
+        lx_wave_add(&W->closest_rays, rc.rad + rc.dl);
+        lx_wave_add(&W->any_rays, rc.env);
+        lx_wave_add(&W->closest_nodes, wc.nodes);
+        lx_wave_add(&W->closest_tris, wc.tris);
+        lx_wave_add(&W->closest_insts, wc.insts);
+        lx_wave_add(&W->any_nodes, wa.nodes);
+        lx_wave_add(&W->any_tris, wa.tris);
+        lx_wave_add(&W->any_insts, wa.insts);
+    }
 }
+
+/* ------------------------------------------------------------ trace */
+/* record sink writing straight to the SoA record buffers */
+struct LxRecStore {
+    const LxRecBuf& R;
+    uint32_t i;
+    LXD void hit(float t, f3 N, f2 uv, int mat, bool tan, f3 Tg) {
+        R.t[i] = t;
+        R.nx[i] = N.x;
+        R.ny[i] = N.y;
+        R.nz[i] = N.z;
+        R.u[i] = uv.x;
+        R.v[i] = uv.y;
+        R.mat[i] = mat;
+        if (tan) {
+            R.tgx[i] = Tg.x;
+            R.tgy[i] = Tg.y;
+            R.tgz[i] = Tg.z;
+        }
+    }
+    LXD void miss() { R.t[i] = -1.0f; }
+    LXD void dl(float t, f3 N, f2 uv, int mat) {
+        R.dl_t[i] = t;
+        R.dl_nx[i] = N.x;
+        R.dl_ny[i] = N.y;
+        R.dl_nz[i] = N.z;
+        R.dl_u[i] = uv.x;
+        R.dl_v[i] = uv.y;
+        R.dl_mat[i] = mat;
+    }
+    LXD void dl_miss() { R.dl_t[i] = -1.0f; }
+    LXD void light(f3 lp) {
+        R.lpx[i] = lp.x;
+        R.lpy[i] = lp.y;
+        R.lpz[i] = lp.z;
+    }
+    LXD void env_dir(f3 ed) {
+        R.edx[i] = ed.x;
+        R.edy[i] = ed.y;
+        R.edz[i] = ed.z;
+    }
+    LXD void env(float env_t) { R.env_t[i] = env_t; }
+};
 
 /* CNT = true is the counting replay: identical work plus per-ray node,
  * triangle and instance visit counts for the algorithmic-bytes model. */
@@ -136,228 +204,47 @@ __global__ __launch_bounds__(LX_TRACE_BLOCK) void k_trace(const LxJob* __restric
     st.stride = LX_TRACE_BLOCK;
     st.spill = spill + gtid;
     st.sstride = gsz;
-    uint32_t nrad = 0, ndl = 0, nenv = 0;
+    LxRayCount rc = {0, 0, 0};
     LxWork wc = {0, 0, 0}, wa = {0, 0, 0};
     for (uint32_t i = gtid; i < n; i += gsz) {
         const LxJob& J = jobs[P.job[i]];
         const LxDevScene& S = *J.scene;
         uint32_t pix = P.pix[i] & 0x7fffffffu;
-        f3 o = mk3(P.ox[i], P.oy[i], P.oz[i]);
-        f3 d = mk3(P.dx[i], P.dy[i], P.dz[i]);
-        f3 lp = mk3(0, 0, 0), ed = mk3(0, 0, 0);
-        if (J.dmis || J.emis) {
-            /* sampleDirectLight: RNG keyed on the pre-increment depth */
-            LxRng rng = lx_make_rng(J.iter, (int)pix, bounce, J.seed);
-            if (J.dmis) lp = lx_sample_light_point(S, rng);
-            if (J.emis) ed = lx_sample_env_dir(S, rng);
-        }
-        LxHitD h = lx_trace<false, CNT>(S, o, d, 1e16f, pix, (uint32_t)J.iter, LX_INST_ALPHA, st, &C->stack_overflow,
-                                        &wc);
-        nrad++;
-        if (h.inst != 0xffffffffu) {
-            f3 N, Tg;
-            f2 uv;
-            int mat;
-            bool want_t = S.mats[S.inst[h.inst].material].normal_tex >= 0;
-            lx_hit_attrs(S, h, want_t, N, Tg, uv, mat);
-            R.t[i] = h.t;
-            R.nx[i] = N.x;
-            R.ny[i] = N.y;
-            R.nz[i] = N.z;
-            R.u[i] = uv.x;
-            R.v[i] = uv.y;
-            R.mat[i] = mat;
-            if (want_t) {
-                R.tgx[i] = Tg.x;
-                R.tgy[i] = Tg.y;
-                R.tgz[i] = Tg.z;
-            }
-            f3 hp = o + (h.t * d);
-            f3 so = hp + (N * 1e-4f);
-            if (J.dmis) {
-                f3 ld = lp - so;
-                float dist = length(ld);
-                ld = ld * (1.0f / dist);
-                LxHitD hd = lx_trace<false, CNT>(S, so, ld, dist * 1.0001f + 1e-4f, pix, (uint32_t)J.iter,
-                                                 LX_INST_ALPHA, st, &C->stack_overflow, &wc);
-                ndl++;
-                if (hd.inst != 0xffffffffu) {
-                    f3 dn;
-                    f2 duv;
-                    int dm;
-                    lx_dl_attrs(S, hd, dn, duv, dm);
-                    R.dl_t[i] = hd.t;
-                    R.dl_nx[i] = dn.x;
-                    R.dl_ny[i] = dn.y;
-                    R.dl_nz[i] = dn.z;
-                    R.dl_u[i] = duv.x;
-                    R.dl_v[i] = duv.y;
-                    R.dl_mat[i] = dm;
-                } else {
-                    R.dl_t[i] = -1.0f;
-                }
-                R.lpx[i] = lp.x;
-                R.lpy[i] = lp.y;
-                R.lpz[i] = lp.z;
-            }
-            if (J.emis) {
-                LxHitD he = lx_trace<true, CNT>(S, so, ed, 1e16f, pix, (uint32_t)J.iter, LX_INST_ALPHA, st,
-                                                &C->stack_overflow, &wa);
-                nenv++;
-                R.env_t[i] = he.inst != 0xffffffffu ? 1.0f : -1.0f;
-                R.edx[i] = ed.x;
-                R.edy[i] = ed.y;
-                R.edz[i] = ed.z;
-            } else {
-                R.env_t[i] = 1.0f;
-            }
-        } else {
-            R.t[i] = -1.0f;
-            if (J.dmis) R.dl_t[i] = -1.0f;
-            R.env_t[i] = -0.5f;
-        }
+        LxRecStore out{R, i};
+        lx_trace_path<CNT>(J, S, mk3(P.ox[i], P.oy[i], P.oz[i]), mk3(P.dx[i], P.dy[i], P.dz[i]), pix, bounce, st,
+                           &C->stack_overflow, out, rc, &wc, &wa);
     }
-    lx_wave_add(&C->rad_rays, nrad);
-    lx_wave_add(&C->dl_rays, ndl);
-    lx_wave_add(&C->env_rays, nenv);
-    if (CNT) {
-        lx_wave_add(&C->closest_nodes, wc.nodes);
-        lx_wave_add(&C->closest_tris, wc.tris);
-        lx_wave_add(&C->closest_insts, wc.insts);
-        lx_wave_add(&C->any_nodes, wa.nodes);
-        lx_wave_add(&C->any_tris, wa.tris);
-        lx_wave_add(&C->any_insts, wa.insts);
-    }
+    lx_flush_counts<CNT>(C, &C->work[0], rc, 0, wc, wa);
 }
 
 /* ------------------------------------------------------------ shade */
-/* shadePath (src/kernels/shade_path.cu:94-364) for one path.  Updates the
- * path registers in place; returns true if the path is pushed to the next
- * queue. */
-LXD bool lx_shade_path(const LxJob& J, const LxDevScene& S, const LxRecBuf& R, uint32_t i, int depth, bool compact,
-                       f3& o, f3& d, f3& thr, f3& col, float& last_pdf, bool& spec, uint32_t pix) {
-    LxRng rng = lx_make_rng(J.iter, (int)pix, depth, J.seed);
-    bool kill = false;
-    float t = R.t[i];
-    f3 N = mk3(0, 0, 0);
-    f3 sample_dir = mk3(0, 0, 0);
-    bool has_exr = S.has_env != 0;
-    if (t > 0.0f) {
-        N = mk3(R.nx[i], R.ny[i], R.nz[i]);
-        f2 uv = mk2(R.u[i], R.v[i]);
-        const lx_material_desc& mat = S.mats[R.mat[i]];
-        f3 color = lx_get_albedo(S, mat, uv);
-        f3 normal = N;
-        if (mat.normal_tex >= 0) normal = lx_get_normal(S, mat, N, mk3(R.tgx[i], R.tgy[i], R.tgz[i]), uv);
-        float rough, metal;
-        lx_get_mr(S, mat, uv, rough, metal);
-        f3 emission = lx_get_emission(S, mat, uv);
-        bool is_specular = rough < 0.05f;
-        f3 nf = dot(-d, normal) < 0.0f ? -normal : normal;
-        sample_dir = lx_sample_ct(-d, normal, rough, metal, rng, color, mat.ior, mat.transmission);
-        bool emissive = length(emission) > 0.00001f;
-        if (J.emis) {
-            if (emissive) {
-                float w = 1.0f;
-                if (depth > 1 && !spec) {
-                    float pb = last_pdf;
-                    float pe = lx_envmap_pdf(S, d);
-                    w = (pb * pb) / (pb * pb + pe * pe);
-                }
-                col = col + thr * (w * emission);
-                kill = true;
-            } else if (R.env_t[i] < 0.0f && !is_specular && has_exr) {
-                f3 ed = mk3(R.edx[i], R.edy[i], R.edz[i]);
-                float pe = lx_envmap_pdf(S, ed);
-                float pb = lx_pdf(-d, ed, normal, rough, metal, color, mat.ior, mat.transmission);
-                float w = (pe * pe) / (pe * pe + pb * pb);
-                f3 brdf = lx_brdf(-d, normal, ed, color, rough, metal, mat.ior, mat.transmission);
-                f3 L = lx_env_lookup(S, normalize(ed), J.intensity);
-                float cs = dot(nf, ed);
-                if (cs > 0.0f && pe > 1e-8f) {
-                    f3 contrib = ((L * brdf) * cs) / pe;
-                    col = col + (thr * contrib) * w;
-                }
-            }
-        } else if (J.dmis) {
-            if (emissive) {
-                float w = 1.0f;
-                if (depth > 1 && !spec) {
-                    float pb = last_pdf;
-                    float dist_sq = t * t;
-                    float cl = dot(N, -d);
-                    if (cl > 0.0001f) {
-                        float pa = 1.0f / S.total_area;
-                        float psa = pa * (dist_sq / cl);
-                        w = (pb * pb) / (pb * pb + psa * psa);
-                    } else {
-                        w = 0.0f;
-                    }
-                }
-                col = col + thr * (w * emission);
-                kill = true;
-            } else if (R.dl_t[i] > 0.0f && !is_specular) {
-                f3 sp = (t * d) + o;
-                f3 lv = mk3(R.lpx[i], R.lpy[i], R.lpz[i]) - sp;
-                float dsq = dot(lv, lv);
-                float dist = lxm_sqrt(dsq);
-                f3 ld = lv / dist;
-                float cs = dot(nf, ld);
-                float cl = dot(mk3(R.dl_nx[i], R.dl_ny[i], R.dl_nz[i]), -ld);
-                if (cs > 0.0f && cl > 0.0f) {
-                    float pa = 1.0f / S.total_area;
-                    float psa = pa * (dsq / cl);
-                    float pb = lx_pdf(-d, ld, normal, rough, metal, color, mat.ior, mat.transmission);
-                    float w = (psa * psa) / (psa * psa + pb * pb);
-                    f3 brdf = lx_brdf(-d, normal, ld, color, rough, metal, mat.ior, mat.transmission);
-                    f3 Le = lx_get_emission(S, S.mats[R.dl_mat[i]], mk2(R.dl_u[i], R.dl_v[i]));
-                    f3 contrib = ((Le * brdf) * cs) / psa;
-                    col = col + (thr * contrib) * w;
-                }
-            }
-        } else if (emissive) {
-            col = col + thr * emission;
-            kill = true;
-        }
-        if (!kill) {
-            /* update_throughput_materials, src/update_throughput_materials.cu:8-67 */
-            bool transmitted = dot(-d, normal) * dot(sample_dir, normal) < 0.0f;
-            bool nspec = is_specular || transmitted;
-            f3 ct = lx_ct_eval_cos(-d, normal, sample_dir, color, rough, metal, mat.ior, mat.transmission);
-            float pdf = lx_pdf(-d, sample_dir, normal, rough, metal, color, mat.ior, mat.transmission);
-            thr = thr * (ct / pdf);
-            last_pdf = pdf;
-            spec = nspec;
-            if (depth >= 3) {
-                float mt = lxm_max(thr.x, lxm_max(thr.y, thr.z));
-                float sv = lxm_clamp(mt, 0.0f, 1.0f);
-                if (rng.u01() >= sv) kill = true;
-                else thr = thr / sv;
-            }
-        }
-    } else if (has_exr) {
-        f3 dn = normalize(d);
-        f3 L = lx_env_lookup(S, dn, J.intensity);
-        float w = 1.0f;
-        if (J.emis && depth > 1 && !spec) {
-            float pb = last_pdf;
-            float pe = lx_envmap_pdf(S, dn);
-            w = (pb * pb) / (pb * pb + pe * pe);
-        }
-        col = col + (thr * L) * w;
+LXD void lx_load_rec(const LxJob& J, const LxDevScene& S, const LxRecBuf& R, uint32_t i, LxRec& r) {
+    r.t = R.t[i];
+    r.env_t = R.env_t[i];
+    if (r.t > 0.0f) {
+        r.N = mk3(R.nx[i], R.ny[i], R.nz[i]);
+        r.uv = mk2(R.u[i], R.v[i]);
+        r.mat = R.mat[i];
+        if (S.mats[r.mat].normal_tex >= 0) r.Tg = mk3(R.tgx[i], R.tgy[i], R.tgz[i]);
+        if (J.emis) r.ed = mk3(R.edx[i], R.edy[i], R.edz[i]);
     }
-    if (t == -1.0f) {
-        kill = true;
-    } else {
-        /* getPointOnRay (include/intersections.h:29-32) and the spawn offset */
-        f3 hp = o + ((t - .0001f) * normalize(d));
-        float l2 = dot(sample_dir, sample_dir);
-        if (!(l2 > 1e-12f) || !lxm_isfinite(l2)) return false;
-        d = sample_dir;
-        if (dot(d, N) > 0.0f) o = hp + (N * 0.01f);
-        else o = hp - (N * 0.01f);
+    if (J.dmis) {
+        r.dl_t = R.dl_t[i];
+        if (r.dl_t > 0.0f) {
+            r.dlN = mk3(R.dl_nx[i], R.dl_ny[i], R.dl_nz[i]);
+            r.dluv = mk2(R.dl_u[i], R.dl_v[i]);
+            r.dlmat = R.dl_mat[i];
+        }
+        r.lp = mk3(R.lpx[i], R.lpy[i], R.lpz[i]);
     }
-    return compact && !kill;
+}
+
+LXD void lx_accumulate(float* __restrict__ accum, const LxJob& J, uint32_t pix, f3 col) {
+    /* finalGather: the path's colour is final */
+    float* a = accum + 3u * (size_t)(J.accum_off + pix);
+    a[0] += col.x;
+    a[1] += col.y;
+    a[2] += col.z;
 }
 
 __global__ __launch_bounds__(LX_SHADE_BLOCK) void k_shade(const LxJob* __restrict__ jobs, LxPathBuf P, LxPathBuf Q,
@@ -380,6 +267,7 @@ __global__ __launch_bounds__(LX_SHADE_BLOCK) void k_shade(const LxJob* __restric
         if (active) {
             job = P.job[i];
             const LxJob& J = jobs[job];
+            const LxDevScene& S = *J.scene;
             pixw = P.pix[i];
             spec = (pixw >> 31) != 0u;
             uint32_t pix = pixw & 0x7fffffffu;
@@ -388,15 +276,11 @@ __global__ __launch_bounds__(LX_SHADE_BLOCK) void k_shade(const LxJob* __restric
             thr = mk3(P.tx[i], P.ty[i], P.tz[i]);
             col = mk3(P.cx[i], P.cy[i], P.cz[i]);
             pdf = P.pdf[i];
+            LxRec r;
+            lx_load_rec(J, S, R, i, r);
             bool compact = depth < J.depth;
-            survive = lx_shade_path(J, *J.scene, R, i, depth, compact, o, d, thr, col, pdf, spec, pix);
-            if (!survive) {
-                /* finalGather: the path's colour is final */
-                float* a = accum + 3u * (size_t)(J.accum_off + pix);
-                a[0] += col.x;
-                a[1] += col.y;
-                a[2] += col.z;
-            }
+            survive = lx_shade_path(J, S, r, depth, compact, o, d, thr, col, pdf, spec, pix);
+            if (!survive) lx_accumulate(accum, J, pix, col);
             pixw = pix | (spec ? 0x80000000u : 0u);
             nsh++;
         }
@@ -426,7 +310,82 @@ __global__ __launch_bounds__(LX_SHADE_BLOCK) void k_shade(const LxJob* __restric
             Q.job[slot] = job;
         }
     }
-    lx_wave_add(&C->bounces, nsh);
+    LxRayCount rc = {0, 0, 0};
+    LxWork z = {0, 0, 0};
+    lx_flush_counts<false>(C, nullptr, rc, nsh, z, z);
+}
+
+/* ------------------------------------------------------------ tail */
+/* BLOCK threads per workgroup; OCC > 0 asks the register allocator for at
+ * least OCC waves per SIMD (spilling to scratch if it must). */
+template <bool CNT, int BLOCK, int OCC>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1)))
+void k_finish(const LxJob* __restrict__ jobs, LxPathBuf P, LxCounters* __restrict__ C, int bounce0,
+              uint32_t* __restrict__ spill, float* __restrict__ accum) {
+    __shared__ uint32_t s_stack[LX_STACK * BLOCK];
+    uint32_t n = C->count[bounce0];
+    uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t gsz = gridDim.x * blockDim.x;
+    uint32_t lane = lx_lane();
+    LxStack st;
+    st.lds = s_stack + threadIdx.x;
+    st.stride = BLOCK;
+    st.spill = spill + gtid;
+    st.sstride = gsz;
+    LxRayCount rc = {0, 0, 0};
+    LxWork wc = {0, 0, 0}, wa = {0, 0, 0};
+    uint32_t nsh = 0;
+    bool have = false, exhausted = false;
+    uint32_t job = 0, pix = 0;
+    int b = bounce0;
+    f3 o = mk3(0, 0, 0), d = mk3(0, 0, 0), thr = mk3(0, 0, 0), col = mk3(0, 0, 0);
+    float pdf = 0.0f;
+    bool spec = false;
+    while (true) {
+        /* refill idle lanes from the live-path queue: one atomic per wave */
+        uint64_t idle = __ballot(!have);
+        if (idle && !exhausted) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(&C->tail_next, (uint32_t)__popcll(idle));
+            base = __shfl(base, 0, LX_WAVE);
+            if (base >= n) exhausted = true;
+            if (!have) {
+                uint32_t my = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                if (my < n) {
+                    job = P.job[my];
+                    uint32_t pw = P.pix[my];
+                    spec = (pw >> 31) != 0u;
+                    pix = pw & 0x7fffffffu;
+                    o = mk3(P.ox[my], P.oy[my], P.oz[my]);
+                    d = mk3(P.dx[my], P.dy[my], P.dz[my]);
+                    thr = mk3(P.tx[my], P.ty[my], P.tz[my]);
+                    col = mk3(P.cx[my], P.cy[my], P.cz[my]);
+                    pdf = P.pdf[my];
+                    b = bounce0;
+                    have = true;
+                }
+            }
+        }
+        if (__ballot(have) == 0ull) break;
+        if (have) {
+            const LxJob& J = jobs[job];
+            const LxDevScene& S = *J.scene;
+            LxRec r;
+            LxRecSink sink{r};
+            lx_trace_path<CNT>(J, S, o, d, pix, b, st, &C->stack_overflow, sink, rc, &wc, &wa);
+            int depth = b + 1;
+            bool cont = lx_shade_path(J, S, r, depth, depth < J.depth, o, d, thr, col, pdf, spec, pix);
+            nsh++;
+            if (cont) {
+                b++;
+            } else {
+                lx_accumulate(accum, J, pix, col);
+                have = false;
+            }
+        }
+    }
+    lx_flush_counts<CNT>(C, &C->work[1], rc, nsh, wc, wa);
+    lx_wave_add(&C->tail_bounces, nsh);
 }
 
 /* ------------------------------------------------------------ test hooks */
@@ -537,11 +496,6 @@ __global__ void k_copy16(const float4* __restrict__ src, float4* __restrict__ ds
         dst[i] = src[i];
 }
 
-hipError_t lx_launch_copy16(const void* src, void* dst, size_t n16, uint32_t grid, hipStream_t s) {
-    hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, s, (const float4*)src, (float4*)dst, n16);
-    return hipGetLastError();
-}
-
 __global__ void k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
@@ -572,6 +526,31 @@ hipError_t lx_launch_shade(const LxJob* jobs, const LxPathBuf& P, const LxPathBu
     hipLaunchKernelGGL(k_shade, dim3(grid), dim3(LX_SHADE_BLOCK), 0, s, jobs, P, Q, R, C, bounce, accum);
     return hipGetLastError();
 }
+template <int BLOCK, int OCC>
+static void lx_finish_at(const LxJob* jobs, const LxPathBuf& P, LxCounters* C, int bounce0, uint32_t* spill,
+                         float* accum, uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL((k_finish<false, BLOCK, OCC>), dim3(grid), dim3(BLOCK), 0, s, jobs, P, C, bounce0, spill, accum);
+}
+hipError_t lx_launch_finish(const LxJob* jobs, const LxPathBuf& P, LxCounters* C, int bounce0, uint32_t* spill,
+                            float* accum, uint32_t threads, int block, int occ, hipStream_t s, bool count) {
+    /* threads = grid * block lanes, the same for every shape, so one spill
+     * allocation serves them all */
+    if (threads < (uint32_t)block) block = 128;
+    if (count) {
+        hipLaunchKernelGGL((k_finish<true, 128, 0>), dim3(threads / 128), dim3(128), 0, s, jobs, P, C, bounce0, spill,
+                           accum);
+    } else if (block == 64) {
+        if (occ >= 4) lx_finish_at<64, 4>(jobs, P, C, bounce0, spill, accum, threads / 64, s);
+        else lx_finish_at<64, 0>(jobs, P, C, bounce0, spill, accum, threads / 64, s);
+    } else if (block == 256) {
+        if (occ >= 4) lx_finish_at<256, 4>(jobs, P, C, bounce0, spill, accum, threads / 256, s);
+        else lx_finish_at<256, 0>(jobs, P, C, bounce0, spill, accum, threads / 256, s);
+    } else {
+        if (occ >= 4) lx_finish_at<128, 4>(jobs, P, C, bounce0, spill, accum, threads / 128, s);
+        else lx_finish_at<128, 0>(jobs, P, C, bounce0, spill, accum, threads / 128, s);
+    }
+    return hipGetLastError();
+}
 hipError_t lx_launch_trace_rays(const LxDevScene* S, const lx_ray* rays, uint32_t n, int any, int alpha,
                                 uint32_t iteration, lx_hit* hits, uint8_t* occ, uint32_t* spill, uint32_t* overflow,
                                 uint32_t grid, hipStream_t s) {
@@ -585,6 +564,10 @@ hipError_t lx_launch_tonemap(const float* accum, uint32_t npix, float spp, int m
 }
 hipError_t lx_launch_math_eval(int fn, const float* x, const float* y, float* out, uint32_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_math_eval, dim3((n + 255) / 256), dim3(256), 0, s, fn, x, y, out, n);
+    return hipGetLastError();
+}
+hipError_t lx_launch_copy16(const void* src, void* dst, size_t n16, uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, s, (const float4*)src, (float4*)dst, n16);
     return hipGetLastError();
 }
 hipError_t lx_launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {

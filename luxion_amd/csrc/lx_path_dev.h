@@ -1,0 +1,243 @@
+/*
+ * lx_path_dev.h -- one bounce of one path, in registers.
+ *
+ * lx_trace_path: sampleDirectLight (src/kernels/sample_direct_light.cu:9-101)
+ *   + the raygen's radiance / direct-light / environment rays and the
+ *   closest-hit attribute programs (src/optixshaders/optix_triangle.cu:52-305).
+ * lx_shade_path: shadePath (src/kernels/shade_path.cu:94-364) with
+ *   update_throughput_materials (src/update_throughput_materials.cu:8-67).
+ * The wavefront kernels store the record between the two in SoA buffers; the
+ * tail kernel keeps it in registers.  Both give the same bits.
+ */
+#ifndef LX_PATH_DEV_H
+#define LX_PATH_DEV_H
+
+#include "lx_trace_dev.h"
+
+struct LxRec {
+    float t;
+    f3 N;
+    f2 uv;
+    int mat;
+    f3 Tg;
+    float env_t;
+    f3 ed;
+    float dl_t;
+    f3 dlN;
+    f2 dluv;
+    int dlmat;
+    f3 lp;
+};
+
+struct LxRayCount {
+    uint32_t rad, dl, env;
+};
+
+/* Sink receives the record piece by piece (hit, emitter shadow hit, light
+ * point, environment occlusion) so a caller that stores to memory can free
+ * the registers early; LxRecSink keeps them in an LxRec. */
+struct LxRecSink {
+    LxRec& r;
+    LXD void hit(float t, f3 N, f2 uv, int mat, bool tan, f3 Tg) {
+        r.t = t;
+        r.N = N;
+        r.uv = uv;
+        r.mat = mat;
+        if (tan) r.Tg = Tg;
+    }
+    LXD void miss() { r.t = -1.0f; }
+    LXD void dl(float t, f3 N, f2 uv, int mat) {
+        r.dl_t = t;
+        r.dlN = N;
+        r.dluv = uv;
+        r.dlmat = mat;
+    }
+    LXD void dl_miss() { r.dl_t = -1.0f; }
+    LXD void light(f3 lp) { r.lp = lp; }
+    LXD void env_dir(f3 ed) { r.ed = ed; }
+    LXD void env(float env_t) { r.env_t = env_t; }
+};
+
+template <bool CNT, class Sink>
+LXD void lx_trace_path(const LxJob& J, const LxDevScene& S, f3 o, f3 d, uint32_t pix, int bounce, LxStack& st,
+                       uint32_t* overflow, Sink& out, LxRayCount& rc, LxWork* wc, LxWork* wa) {
+    f3 lp = mk3(0, 0, 0), ed = mk3(0, 0, 0);
+    if (J.dmis || J.emis) {
+        /* sampleDirectLight: RNG keyed on the pre-increment depth */
+        LxRng rng = lx_make_rng(J.iter, (int)pix, bounce, J.seed);
+        if (J.dmis) lp = lx_sample_light_point(S, rng);
+        if (J.emis) ed = lx_sample_env_dir(S, rng);
+    }
+    LxHitD h = lx_trace<false, CNT>(S, o, d, 1e16f, pix, (uint32_t)J.iter, LX_INST_ALPHA, st, overflow, wc);
+    rc.rad++;
+    if (J.dmis) out.light(lp);
+    if (J.emis) out.env_dir(ed);
+    if (h.inst != 0xffffffffu) {
+        bool want_t = S.mats[S.inst[h.inst].material].normal_tex >= 0;
+        f3 N, Tg;
+        f2 uv;
+        int mat;
+        lx_hit_attrs(S, h, want_t, N, Tg, uv, mat);
+        out.hit(h.t, N, uv, mat, want_t, Tg);
+        f3 hp = o + (h.t * d);
+        f3 so = hp + (N * 1e-4f);
+        if (J.dmis) {
+            f3 ld = lp - so;
+            float dist = length(ld);
+            ld = ld * (1.0f / dist);
+            LxHitD hd = lx_trace<false, CNT>(S, so, ld, dist * 1.0001f + 1e-4f, pix, (uint32_t)J.iter, LX_INST_ALPHA,
+                                             st, overflow, wc);
+            rc.dl++;
+            if (hd.inst != 0xffffffffu) {
+                f3 dN;
+                f2 duv;
+                int dmat;
+                lx_dl_attrs(S, hd, dN, duv, dmat);
+                out.dl(hd.t, dN, duv, dmat);
+            } else {
+                out.dl_miss();
+            }
+        }
+        if (J.emis) {
+            LxHitD he = lx_trace<true, CNT>(S, so, ed, 1e16f, pix, (uint32_t)J.iter, LX_INST_ALPHA, st, overflow, wa);
+            rc.env++;
+            out.env(he.inst != 0xffffffffu ? 1.0f : -1.0f);
+        } else {
+            out.env(1.0f);
+        }
+    } else {
+        out.miss();
+        if (J.dmis) out.dl_miss();
+        out.env(-0.5f);
+    }
+}
+
+/* Updates the path registers; returns true if the path continues to the
+ * next bounce (compaction push), false when it stops. */
+LXD bool lx_shade_path(const LxJob& J, const LxDevScene& S, const LxRec& R, int depth, bool compact, f3& o, f3& d,
+                       f3& thr, f3& col, float& last_pdf, bool& spec, uint32_t pix) {
+    LxRng rng = lx_make_rng(J.iter, (int)pix, depth, J.seed);
+    bool kill = false;
+    float t = R.t;
+    f3 N = mk3(0, 0, 0);
+    f3 sample_dir = mk3(0, 0, 0);
+    bool has_exr = S.has_env != 0;
+    if (t > 0.0f) {
+        N = R.N;
+        f2 uv = R.uv;
+        const lx_material_desc& mat = S.mats[R.mat];
+        f3 color = lx_get_albedo(S, mat, uv);
+        f3 normal = N;
+        if (mat.normal_tex >= 0) normal = lx_get_normal(S, mat, N, R.Tg, uv);
+        float rough, metal;
+        lx_get_mr(S, mat, uv, rough, metal);
+        f3 emission = lx_get_emission(S, mat, uv);
+        bool is_specular = rough < 0.05f;
+        f3 nf = dot(-d, normal) < 0.0f ? -normal : normal;
+        sample_dir = lx_sample_ct(-d, normal, rough, metal, rng, color, mat.ior, mat.transmission);
+        bool emissive = length(emission) > 0.00001f;
+        if (J.emis) {
+            if (emissive) {
+                float w = 1.0f;
+                if (depth > 1 && !spec) {
+                    float pb = last_pdf;
+                    float pe = lx_envmap_pdf(S, d);
+                    w = (pb * pb) / (pb * pb + pe * pe);
+                }
+                col = col + thr * (w * emission);
+                kill = true;
+            } else if (R.env_t < 0.0f && !is_specular && has_exr) {
+                f3 ed = R.ed;
+                float pe = lx_envmap_pdf(S, ed);
+                float pb = lx_pdf(-d, ed, normal, rough, metal, color, mat.ior, mat.transmission);
+                float w = (pe * pe) / (pe * pe + pb * pb);
+                f3 brdf = lx_brdf(-d, normal, ed, color, rough, metal, mat.ior, mat.transmission);
+                f3 L = lx_env_lookup(S, normalize(ed), J.intensity);
+                float cs = dot(nf, ed);
+                if (cs > 0.0f && pe > 1e-8f) {
+                    f3 contrib = ((L * brdf) * cs) / pe;
+                    col = col + (thr * contrib) * w;
+                }
+            }
+        } else if (J.dmis) {
+            if (emissive) {
+                float w = 1.0f;
+                if (depth > 1 && !spec) {
+                    float pb = last_pdf;
+                    float dist_sq = t * t;
+                    float cl = dot(N, -d);
+                    if (cl > 0.0001f) {
+                        float pa = 1.0f / S.total_area;
+                        float psa = pa * (dist_sq / cl);
+                        w = (pb * pb) / (pb * pb + psa * psa);
+                    } else {
+                        w = 0.0f;
+                    }
+                }
+                col = col + thr * (w * emission);
+                kill = true;
+            } else if (R.dl_t > 0.0f && !is_specular) {
+                f3 sp = (t * d) + o;
+                f3 lv = R.lp - sp;
+                float dsq = dot(lv, lv);
+                float dist = lxm_sqrt(dsq);
+                f3 ld = lv / dist;
+                float cs = dot(nf, ld);
+                float cl = dot(R.dlN, -ld);
+                if (cs > 0.0f && cl > 0.0f) {
+                    float pa = 1.0f / S.total_area;
+                    float psa = pa * (dsq / cl);
+                    float pb = lx_pdf(-d, ld, normal, rough, metal, color, mat.ior, mat.transmission);
+                    float w = (psa * psa) / (psa * psa + pb * pb);
+                    f3 brdf = lx_brdf(-d, normal, ld, color, rough, metal, mat.ior, mat.transmission);
+                    f3 Le = lx_get_emission(S, S.mats[R.dlmat], R.dluv);
+                    f3 contrib = ((Le * brdf) * cs) / psa;
+                    col = col + (thr * contrib) * w;
+                }
+            }
+        } else if (emissive) {
+            col = col + thr * emission;
+            kill = true;
+        }
+        if (!kill) {
+            /* update_throughput_materials, src/update_throughput_materials.cu:8-67 */
+            bool transmitted = dot(-d, normal) * dot(sample_dir, normal) < 0.0f;
+            bool nspec = is_specular || transmitted;
+            f3 ct = lx_ct_eval_cos(-d, normal, sample_dir, color, rough, metal, mat.ior, mat.transmission);
+            float pdf = lx_pdf(-d, sample_dir, normal, rough, metal, color, mat.ior, mat.transmission);
+            thr = thr * (ct / pdf);
+            last_pdf = pdf;
+            spec = nspec;
+            if (depth >= 3) {
+                float mt = lxm_max(thr.x, lxm_max(thr.y, thr.z));
+                float sv = lxm_clamp(mt, 0.0f, 1.0f);
+                if (rng.u01() >= sv) kill = true;
+                else thr = thr / sv;
+            }
+        }
+    } else if (has_exr) {
+        f3 dn = normalize(d);
+        f3 L = lx_env_lookup(S, dn, J.intensity);
+        float w = 1.0f;
+        if (J.emis && depth > 1 && !spec) {
+            float pb = last_pdf;
+            float pe = lx_envmap_pdf(S, dn);
+            w = (pb * pb) / (pb * pb + pe * pe);
+        }
+        col = col + (thr * L) * w;
+    }
+    if (t == -1.0f) {
+        kill = true;
+    } else {
+        /* getPointOnRay (include/intersections.h:29-32) and the spawn offset */
+        f3 hp = o + ((t - .0001f) * normalize(d));
+        float l2 = dot(sample_dir, sample_dir);
+        if (!(l2 > 1e-12f) || !lxm_isfinite(l2)) return false;
+        d = sample_dir;
+        if (dot(d, N) > 0.0f) o = hp + (N * 0.01f);
+        else o = hp - (N * 0.01f);
+    }
+    return compact && !kill;
+}
+
+#endif

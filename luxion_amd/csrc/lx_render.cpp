@@ -90,17 +90,33 @@ void lx_enqueue_round(lx_context* ctx, LxJob* d_jobs, const int32_t* d_spp, uint
                       int max_depth, uint32_t trace_grid, uint32_t shade_grid, uint32_t cam_grid, hipEvent_t* ev,
                       bool count) {
     hipStream_t s = ctx->stream;
+    int wf = lx_wavefront_bounces(ctx, max_depth);
     LX_HIP(lx_launch_begin_round(d_jobs, d_spp, njobs, d_cnt, d_sched, path_cap, s));
     LX_HIP(lx_launch_camera(d_jobs, d_sched, d_cnt, wb.A, cam_grid, s));
     LxPathBuf* cur = &wb.A;
     LxPathBuf* nxt = &wb.B;
-    for (int b = 0; b < max_depth; b++) {
+    for (int b = 0; b < wf; b++) {
         if (ev) LX_HIP(hipEventRecord(ev[2 * b], s));
         LX_HIP(lx_launch_trace(d_jobs, *cur, wb.R, d_cnt, b, d_spill, trace_grid, s, count));
         if (ev) LX_HIP(hipEventRecord(ev[2 * b + 1], s));
         LX_HIP(lx_launch_shade(d_jobs, *cur, *nxt, wb.R, d_cnt, b, d_accum, shade_grid, s));
         std::swap(cur, nxt);
     }
+    if (wf < max_depth) {
+        /* ev[2wf] -> ev[2wf+1] brackets the tail kernel; the remaining events
+         * mark empty intervals so the layout matches the all-wavefront round */
+        if (ev) LX_HIP(hipEventRecord(ev[2 * wf], s));
+        LX_HIP(lx_launch_finish(d_jobs, *cur, d_cnt, wf, d_spill, d_accum, trace_grid * LX_TRACE_BLOCK,
+                                ctx->tuning.block_size, ctx->tuning.tail_min_waves, s, count));
+        if (ev)
+            for (int k = 2 * wf + 1; k < 2 * max_depth; k++) LX_HIP(hipEventRecord(ev[k], s));
+    }
     if (ev) LX_HIP(hipEventRecord(ev[2 * max_depth], s));
     LX_HIP(lx_launch_end_round(d_jobs, njobs, s));
+}
+
+int lx_wavefront_bounces(const lx_context* ctx, int max_depth) {
+    int wf = ctx->tuning.wavefront_bounces;
+    if (wf < 0 || wf > max_depth) wf = max_depth;
+    return wf;
 }

@@ -125,12 +125,18 @@ struct LxRecBuf {
 
 /* Per-round counters in device memory. */
 #define LX_MAX_DEPTH 64
+/* counting replay: rays and node / triangle / instance visits of closest-hit
+ * (radiance + direct-light) and any-hit (environment) rays of one kernel */
+struct LxWorkCnt {
+    unsigned long long closest_rays, any_rays;
+    unsigned long long closest_nodes, closest_tris, closest_insts, any_nodes, any_tris, any_insts;
+};
 struct LxCounters {
     uint32_t count[LX_MAX_DEPTH + 1]; /* live paths entering bounce d */
+    uint32_t tail_next;               /* tail kernel's queue head */
     unsigned long long rad_rays, dl_rays, env_rays, bounces;
-    /* counting replay: node / triangle / instance visits of closest-hit
-     * (radiance + direct-light) and any-hit (environment) rays */
-    unsigned long long closest_nodes, closest_tris, closest_insts, any_nodes, any_tris, any_insts;
+    unsigned long long tail_bounces; /* path-bounces shaded by the tail kernel */
+    LxWorkCnt work[2]; /* [0] wavefront trace kernel, [1] tail kernel */
     uint32_t stack_overflow;
     uint32_t pad[3];
 };
