@@ -53,6 +53,12 @@ def main():
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--level", type=int, default=8)
+    ap.add_argument("--batched", type=int, default=256,
+                    help="jobs in the batched-form mix on the same scene (0 = skip)")
+    ap.add_argument("--served", choices=["off", "light", "full"], default="light",
+                    help="served-form Poisson sweep: light = the HDRI scene + Cornell box resident; full = the "
+                         "HDRI scene + instanced interior + 30M-triangle grid resident")
+    ap.add_argument("--served-jobs", type=int, default=48, help="jobs per arrival rate")
     ap.add_argument("--wavefront-bounces", type=int, default=None,
                     help="bounces launched one kernel each before the tail kernel (default: the context's)")
     ap.add_argument("--copy-calibration", type=int, default=0,
@@ -179,6 +185,43 @@ def main():
                          f"{first}..{first + iters - 1}), oracle/lxo_oracle.cpp restatement with its "
                          f"reference-style binary SAH BVH, {threads} threads; BVH build excluded"}
 
+    batched = served = None
+    if args.batched:
+        from luxion_amd import workloads
+        spec = workloads.mixed_jobs([(sc, dict(emis=True), (0.0, 0.0, 0.0), 3.0)], args.batched, seed=3)
+        log(f"batched: {len(spec)} jobs, {workloads.path_samples(spec) / 1e6:.1f} M path-samples")
+        batched, _ = workloads.run_batched(ctx, spec)
+        log(f"batched: {batched}")
+    if args.served != "off":
+        from luxion_amd import workloads
+        specs = [(sc, dict(emis=True), (0.0, 0.0, 0.0), 3.0)]
+        resident = [data.name]
+        if args.served == "light":
+            cb = lx.Scene(ctx, scenes.cornell(256, 256))
+            specs.append((cb, dict(dmis=True), (0.0, 1.0, 0.0), 3.4))
+            resident.append("cornell")
+        else:
+            t0 = time.time()
+            ii = scenes.instanced_interior()
+            sc2 = lx.Scene(ctx, ii)
+            gr = scenes.displaced_grid()
+            sc3 = lx.Scene(ctx, gr)
+            log(f"served scenes built in {time.time() - t0:.1f} s")
+            specs += [(sc2, dict(dmis=True), (0.0, 3.0, 0.0), 40.0), (sc3, dict(emis=True), (0.0, 0.0, 0.0), 14.0)]
+            resident += [f"{ii.name} ({ii.num_tris} unique tris)", f"{gr.name} ({gr.num_tris} tris)"]
+        srv = lx.Server(ctx, max_jobs=64, max_depth=16)
+        try:
+            sweep = []
+            for k, lam in enumerate((50, 100, 200, 500, 1000, 2000)):
+                spec = workloads.mixed_jobs(specs, args.served_jobs, seed=100 + k)
+                res_ = workloads.run_served(ctx, spec, lam, seed=200 + k, srv=srv)
+                log(f"served lambda={lam}: {res_}")
+                sweep.append(res_)
+        finally:
+            srv.stop()
+        served = {"resident_scenes": resident, "job_mix": "side U{64..512}, spp U{1..64}, seeded orbit cameras",
+                  "sweep": sweep}
+
     out = {
         "metric": "Mrays/s + spp/s at 1920x1080 (1 GPU); p50/p99 job latency under mixed load",
         "value": round(value, 2),
@@ -212,6 +255,8 @@ def main():
                      "bytes_per_launch": int(bytes_per_launch), "avg_launch_ms": round(launch_ms, 4),
                      "per_ray": {k: round(v, 2) for k, v in per_ray.items()}},
         "tuning": tuning,
+        "batched": batched,
+        "served": served,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

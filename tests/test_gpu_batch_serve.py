@@ -118,3 +118,15 @@ def test_server_failure_handling_isolated(ctx, scenes_pair):
         assert st["jobs_rejected"] == 1 and st["jobs_failed"] == 2 and st["jobs_done"] == 1
     finally:
         srv.stop()
+
+
+def test_workload_drivers_small(ctx, scenes_pair):
+    from luxion_amd import workloads
+    corn, ico = scenes_pair
+    specs = [(ico, dict(emis=True), (0.0, 0.0, 0.0), 3.0), (corn, dict(dmis=True), (0.0, 1.0, 0.0), 3.4)]
+    spec = workloads.mixed_jobs(specs, 6, seed=5, side=(16, 48), spp=(1, 4))
+    b, res = workloads.run_batched(ctx, spec)
+    assert b["images_bitexact"] and b["speedup_vs_per_job"] > 0
+    s = workloads.run_served(ctx, workloads.mixed_jobs(specs, 6, seed=6, side=(16, 48), spp=(1, 4)), 500.0,
+                             max_depth=8)
+    assert s["p99_ms"] >= s["p50_ms"] > 0 and s["jobs"] == 6
