@@ -164,7 +164,7 @@ lx_status lx_scene_load_glb(lx_context* ctx, const char* glb_path, const char* e
 
 enum { LX_CDF_FP64 = 0, LX_CDF_FP32_REFERENCE = 1 };
 typedef struct lx_build_opts {
-    int32_t bvh_width;      /* 4 (default) or 8 */
+    int32_t bvh_width;      /* 4 or 8; 0 = the context tuning's bvh_width */
     int32_t leaf_size;      /* max triangles per leaf, 1..8 (default 4) */
     int32_t builder;        /* 0 = GPU LBVH + binned SAH (default), 1 = host binned SAH */
     int32_t sah_bins;       /* binned-SAH bins (default 16) */
@@ -337,8 +337,8 @@ typedef struct lx_job_result {
 } lx_job_result;
 
 typedef struct lx_tuning {
-    int32_t block_size;    /* threads per workgroup of the tail kernel: 64, 128 or 256 */
-    int32_t bvh_width;     /* 4 or 8 */
+    int32_t block_size;    /* threads per workgroup of the tail kernel: 128 or 256 */
+    int32_t bvh_width;     /* 4 or 8: scenes built with bvh_width 0 and servers */
     int32_t queue_chunk;   /* paths claimed per wave from the work queue */
     int32_t waves_per_cu;  /* persistent grid size */
     int32_t use_graph;     /* capture the bounce loop in a hipGraph */

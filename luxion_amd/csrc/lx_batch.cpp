@@ -98,6 +98,10 @@ extern "C" lx_status lx_batch_run(lx_context* ctx, const lx_job* jobs, uint32_t 
         LX_REQUIRE(ctx && jobs && results && n > 0, LX_ERR_INVALID, "null argument");
         double t_submit = now_ms();
         for (uint32_t i = 0; i < n; i++) lx_validate_job(jobs[i]);
+        int width = jobs[0].scene->dev.bvh_width;
+        for (uint32_t i = 0; i < n; i++)
+            LX_REQUIRE(jobs[i].scene->dev.bvh_width == width, LX_ERR_INVALID,
+                       "every job of a batch must reference scenes built at one BVH width");
         std::lock_guard<std::mutex> lk(ctx->mu);
         ctx->set_device();
         std::vector<LxJob> hj(n);
@@ -145,7 +149,7 @@ extern "C" lx_status lx_batch_run(lx_context* ctx, const lx_job* jobs, uint32_t 
             try {
                 lx_enqueue_round(ctx, d_jobs.as<LxJob>(), d_spp.as<int32_t>(), n, d_cnt.as<LxCounters>(),
                                  d_sched.as<LxSched>(), cap, wb, d_accum.as<float>(), d_spill.as<uint32_t>(), max_depth,
-                                 tg, sg, cg, nullptr);
+                                 tg, sg, cg, width, nullptr);
             } catch (...) {
                 hipGraph_t tmp;
                 (void)hipStreamEndCapture(s, &tmp);
@@ -163,7 +167,7 @@ extern "C" lx_status lx_batch_run(lx_context* ctx, const lx_job* jobs, uint32_t 
             else
                 lx_enqueue_round(ctx, d_jobs.as<LxJob>(), d_spp.as<int32_t>(), n, d_cnt.as<LxCounters>(),
                                  d_sched.as<LxSched>(), cap, wb, d_accum.as<float>(), d_spill.as<uint32_t>(), max_depth,
-                                 tg, sg, cg, nullptr);
+                                 tg, sg, cg, width, nullptr);
             LX_HIP(hipEventRecord(ev[k + 1], s));
         }
         LX_HIP(hipStreamSynchronize(s));

@@ -1,6 +1,6 @@
 /*
- * lx_bvh_host.cpp -- host binned-SAH builder and the binary -> 4-wide
- * collapse shared by the host and GPU builders.
+ * lx_bvh_host.cpp -- host binned-SAH builder and the binary -> 4- or
+ * 8-wide collapse shared by the host and GPU builders.
  *
  * Replaces the reference's per-mesh OptiX GAS build (src/myoptix/optix_accel.cpp:9-110)
  * and IAS build (src/myoptix/optix_ias.cpp:9-63).  The reference's own
@@ -200,9 +200,9 @@ void lx_pad_box(float lo[3], float hi[3]) {
     }
 }
 
-/* Collapse the binary tree into 4-wide nodes: each wide node adopts the
- * children of its largest-area internal children until it has four. */
-static uint32_t collapse(const std::vector<BNode>& bn, int32_t bi, std::vector<LxNode4>& out,
+/* Collapse the binary tree into W-wide nodes (W = 4 or 8): each wide node
+ * adopts the children of its largest-area internal children until it has W. */
+static uint32_t collapse(const std::vector<BNode>& bn, int32_t bi, int W, std::vector<LxWideNode>& out,
                          const std::function<uint32_t(uint32_t, uint32_t)>& leaf_ref, int depth, int& max_depth,
                          double& sah, double root_area) {
     if (depth > max_depth) max_depth = depth;
@@ -212,9 +212,9 @@ static uint32_t collapse(const std::vector<BNode>& bn, int32_t bi, std::vector<L
         return leaf_ref(b.first, b.count);
     }
     sah += (double)area(b.lo, b.hi) / root_area;
-    int32_t kids[4] = {b.left, b.right, -1, -1};
+    int32_t kids[8] = {b.left, b.right, -1, -1, -1, -1, -1, -1};
     int nk = 2;
-    while (nk < 4) {
+    while (nk < W) {
         int best = -1;
         float ba = -1.0f;
         for (int k = 0; k < nk; k++) {
@@ -233,34 +233,23 @@ static uint32_t collapse(const std::vector<BNode>& bn, int32_t bi, std::vector<L
     }
     uint32_t me = (uint32_t)out.size();
     out.emplace_back();
-    float lo[4][3], hi[4][3];
-    uint32_t refs[4];
-    for (int k = 0; k < 4; k++) {
+    LxWideNode n;
+    std::memset(&n, 0, sizeof(n));
+    n.n = (uint32_t)nk;
+    for (int k = 0; k < 8; k++) {
         if (k < nk) {
             const BNode& c = bn[kids[k]];
             for (int a = 0; a < 3; a++) {
-                lo[k][a] = c.lo[a];
-                hi[k][a] = c.hi[a];
+                n.lo[k][a] = c.lo[a];
+                n.hi[k][a] = c.hi[a];
             }
-            lx_pad_box(lo[k], hi[k]);
-            refs[k] = collapse(bn, kids[k], out, leaf_ref, depth + 1, max_depth, sah, root_area);
+            lx_pad_box(n.lo[k], n.hi[k]);
+            n.child[k] = collapse(bn, kids[k], W, out, leaf_ref, depth + 1, max_depth, sah, root_area);
         } else {
-            for (int a = 0; a < 3; a++) {
-                lo[k][a] = 0.0f;
-                hi[k][a] = 0.0f;
-            }
-            refs[k] = LX_CHILD_EMPTY;
+            n.child[k] = LX_CHILD_EMPTY;
         }
     }
-    LxNode4& n = out[me];
-    n.lox = float4{lo[0][0], lo[1][0], lo[2][0], lo[3][0]};
-    n.hix = float4{hi[0][0], hi[1][0], hi[2][0], hi[3][0]};
-    n.loy = float4{lo[0][1], lo[1][1], lo[2][1], lo[3][1]};
-    n.hiy = float4{hi[0][1], hi[1][1], hi[2][1], hi[3][1]};
-    n.loz = float4{lo[0][2], lo[1][2], lo[2][2], lo[3][2]};
-    n.hiz = float4{hi[0][2], hi[1][2], hi[2][2], hi[3][2]};
-    n.child = int4{(int)refs[0], (int)refs[1], (int)refs[2], (int)refs[3]};
-    n.meta = int4{0, nk, 0, 0};
+    out[me] = n;
     return me;
 }
 
@@ -268,7 +257,7 @@ static uint32_t make_leaf_ref(uint32_t first, uint32_t count) {
     return LX_LEAF_BIT | ((count - 1u) << 28) | first;
 }
 
-void lx_build_blas_host(const float* pos, const uint32_t* idx, uint32_t ntri, int leaf_size, int bins,
+void lx_build_blas_host(const float* pos, const uint32_t* idx, uint32_t ntri, int leaf_size, int bins, int width,
                         BvhBuildResult& out) {
     LX_REQUIRE(ntri > 0, LX_ERR_INVALID, "mesh has no triangles");
     LX_REQUIRE(ntri < (1u << 28) - 1, LX_ERR_INVALID, "mesh too large for the leaf encoding");
@@ -308,12 +297,12 @@ void lx_build_blas_host(const float* pos, const uint32_t* idx, uint32_t ntri, in
     int md = 0;
     double sah = 0.0;
     double ra = std::max(1e-30, (double)area(B.nodes[root].lo, B.nodes[root].hi));
-    out.root = collapse(B.nodes, root, out.nodes, make_leaf_ref, 0, md, sah, ra);
+    out.root = collapse(B.nodes, root, width == 8 ? 8 : 4, out.nodes, make_leaf_ref, 0, md, sah, ra);
     out.depth = md;
     out.sah = sah;
 }
 
-void lx_build_tlas_host(const std::vector<float>& boxes, BvhBuildResult& out) {
+void lx_build_tlas_host(const std::vector<float>& boxes, int width, BvhBuildResult& out) {
     uint32_t n = (uint32_t)(boxes.size() / 6);
     LX_REQUIRE(n > 0, LX_ERR_INVALID, "scene has no instances");
     std::vector<Prim> prims(n);
@@ -339,7 +328,7 @@ void lx_build_tlas_host(const std::vector<float>& boxes, BvhBuildResult& out) {
         (void)count;
         return make_leaf_ref(prims[first].id, 1);
     };
-    out.root = collapse(B.nodes, root, out.nodes, ref, 0, md, sah, ra);
+    out.root = collapse(B.nodes, root, width == 8 ? 8 : 4, out.nodes, ref, 0, md, sah, ra);
     out.depth = md;
     out.sah = sah;
 }

@@ -136,8 +136,7 @@ lx_status lx_tuning_set(lx_context* c, const lx_tuning* t) {
     LX_REQUIRE(c && t, LX_ERR_INVALID, "null argument");
     LX_REQUIRE(t->waves_per_cu >= 1 && t->waves_per_cu <= 32, LX_ERR_INVALID, "waves_per_cu out of range");
     LX_REQUIRE(t->bvh_width == 4 || t->bvh_width == 8, LX_ERR_INVALID, "bvh_width must be 4 or 8");
-    LX_REQUIRE(t->block_size == 64 || t->block_size == 128 || t->block_size == 256, LX_ERR_INVALID,
-               "block_size must be 64, 128 or 256");
+    LX_REQUIRE(t->block_size == 128 || t->block_size == 256, LX_ERR_INVALID, "block_size must be 128 or 256");
     LX_REQUIRE(t->tail_min_waves == 0 || t->tail_min_waves == 4, LX_ERR_INVALID, "tail_min_waves must be 0 or 4");
     c->tuning = *t;
     LX_CATCH
@@ -233,7 +232,7 @@ static lx_status trace_hook(lx_scene* s, const lx_ray* rays, uint32_t n, int any
     LX_HIP(lx_launch_fill_u32(dov.as<uint32_t>(), 64, 0u, ctx->stream));
     LX_HIP(lx_launch_trace_rays(s->d_scene.as<LxDevScene>(), dr.as<lx_ray>(), n, any, alpha, iteration,
                                 any ? nullptr : dh.as<lx_hit>(), any ? dh.as<uint8_t>() : nullptr,
-                                dsp.as<uint32_t>(), dov.as<uint32_t>(), grid, ctx->stream));
+                                dsp.as<uint32_t>(), dov.as<uint32_t>(), grid, s->dev.bvh_width, ctx->stream));
     uint32_t ov = 0;
     if (any) LX_HIP(hipMemcpyAsync(occ, dh.ptr, n, hipMemcpyDeviceToHost, ctx->stream));
     else LX_HIP(hipMemcpyAsync(hits, dh.ptr, (size_t)n * sizeof(lx_hit), hipMemcpyDeviceToHost, ctx->stream));
@@ -380,7 +379,7 @@ static void upload_job(lx_render* r, uint32_t first_iter, uint32_t n) {
 static void enqueue(lx_render* r, hipEvent_t* ev) {
     lx_enqueue_round(r->ctx, r->jobs.as<LxJob>(), r->job_spp.as<int32_t>(), 1, r->counters.as<LxCounters>(),
                      r->sched.as<LxSched>(), r->npix, r->wb, r->accum.as<float>(), r->spill.as<uint32_t>(), r->depth,
-                     r->trace_grid, r->shade_grid, r->cam_grid, ev, r->counting);
+                     r->trace_grid, r->shade_grid, r->cam_grid, r->scene->dev.bvh_width, ev, r->counting);
 }
 
 lx_status lx_render_iterate(lx_render* r, uint32_t first_iter, uint32_t n) {
@@ -583,7 +582,8 @@ lx_status lx_debug_bounce(lx_render* r, uint32_t iter, uint32_t* survivors, uint
     std::vector<int32_t> mat(N);
     std::vector<uint32_t> pix(N);
     for (int b = 0; b < r->depth; b++) {
-        LX_HIP(lx_launch_trace(dj, *cur, r->wb.R, dc, b, r->spill.as<uint32_t>(), r->trace_grid, s));
+        LX_HIP(lx_launch_trace(dj, *cur, r->wb.R, dc, b, r->spill.as<uint32_t>(), r->trace_grid,
+                               r->scene->dev.bvh_width, s));
         LX_HIP(lx_launch_shade(dj, *cur, *nxt, r->wb.R, dc, b, r->accum.as<float>(), r->shade_grid, s));
         LX_HIP(hipStreamSynchronize(s));
         uint32_t cnt[2];

@@ -99,8 +99,16 @@ struct DevBuf {
 };
 
 /* ---------------------------------------------------------- BVH build */
+/* Builder output node, up to eight children; packed into LxNode4 or LxNode8
+ * by the scene once every tree's indices are final. */
+struct LxWideNode {
+    float lo[8][3], hi[8][3];
+    uint32_t child[8]; /* node index, leaf ref or LX_CHILD_EMPTY */
+    uint32_t n;
+};
+
 struct BvhBuildResult {
-    std::vector<LxNode4> nodes;   /* local indices */
+    std::vector<LxWideNode> nodes; /* local indices */
     std::vector<float4> tris;     /* leaf-ordered (v0|prim, e1, e2) */
     uint32_t root = 0;            /* child ref (node index or leaf ref), local */
     int depth = 0;
@@ -108,14 +116,14 @@ struct BvhBuildResult {
 };
 
 /* Host binned-SAH build + collapse to BVH4 (builder 1). */
-void lx_build_blas_host(const float* pos, const uint32_t* idx, uint32_t ntri, int leaf_size, int bins,
+void lx_build_blas_host(const float* pos, const uint32_t* idx, uint32_t ntri, int leaf_size, int bins, int width,
                         BvhBuildResult& out);
 /* TLAS over instance world boxes (one instance per leaf). */
-void lx_build_tlas_host(const std::vector<float>& boxes /* 6 per instance */, BvhBuildResult& out);
+void lx_build_tlas_host(const std::vector<float>& boxes /* 6 per instance */, int width, BvhBuildResult& out);
 /* GPU LBVH + binned-SAH refinement (builder 0); returns false if the GPU
  * path is unavailable for this input. */
 bool lx_build_blas_gpu(lx_context* ctx, const float* pos, const uint32_t* idx, uint32_t ntri, int leaf_size, int bins,
-                       BvhBuildResult& out);
+                       int width, BvhBuildResult& out);
 void lx_pad_box(float lo[3], float hi[3]);
 
 /* -------------------------------------------------------------- scene */
@@ -202,6 +210,7 @@ struct lx_render {
 void lx_enqueue_round(lx_context* ctx, LxJob* d_jobs, const int32_t* d_spp, uint32_t njobs, LxCounters* d_cnt,
                       LxSched* d_sched, uint32_t path_cap, WaveBufs& wb, float* d_accum, uint32_t* d_spill,
                       int max_depth, uint32_t trace_grid, uint32_t shade_grid, uint32_t cam_grid,
+                      int width /* BVH width of every referenced scene */,
                       hipEvent_t* ev /* 2*max_depth+1 events or NULL */, bool count = false);
 uint32_t lx_spill_words(uint32_t trace_grid);
 /* bounces launched one kernel each before the tail kernel takes over

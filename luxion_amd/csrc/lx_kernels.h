@@ -31,17 +31,19 @@ hipError_t lx_launch_begin_round(LxJob* jobs, const int32_t* job_spp, uint32_t n
 hipError_t lx_launch_end_round(LxJob* jobs, uint32_t njobs, hipStream_t s);
 hipError_t lx_launch_camera(const LxJob* jobs, const LxSched* sched, const LxCounters* C, const LxPathBuf& P,
                             uint32_t grid, hipStream_t s);
+/* width: the BVH width (4 or 8) of every scene the jobs reference */
 hipError_t lx_launch_trace(const LxJob* jobs, const LxPathBuf& P, const LxRecBuf& R, LxCounters* C, int bounce,
-                           uint32_t* spill, uint32_t grid, hipStream_t s, bool count = false);
+                           uint32_t* spill, uint32_t grid, int width, hipStream_t s, bool count = false);
 hipError_t lx_launch_shade(const LxJob* jobs, const LxPathBuf& P, const LxPathBuf& Q, const LxRecBuf& R, LxCounters* C,
                            int bounce, float* accum, uint32_t grid, hipStream_t s);
-/* threads: total lanes (a multiple of 256); block: 64, 128 or 256; occ: 0 or
- * 4 minimum waves per SIMD */
+/* threads: total lanes (a multiple of 128); block: 128 or 256; occ: 0 or 4
+ * minimum waves per SIMD */
 hipError_t lx_launch_finish(const LxJob* jobs, const LxPathBuf& P, LxCounters* C, int bounce0, uint32_t* spill,
-                            float* accum, uint32_t threads, int block, int occ, hipStream_t s, bool count = false);
+                            float* accum, uint32_t threads, int block, int occ, int width, hipStream_t s,
+                            bool count = false);
 hipError_t lx_launch_trace_rays(const LxDevScene* S, const lx_ray* rays, uint32_t n, int any, int alpha,
                                 uint32_t iteration, lx_hit* hits, uint8_t* occ, uint32_t* spill, uint32_t* overflow,
-                                uint32_t grid, hipStream_t s);
+                                uint32_t grid, int width, hipStream_t s);
 hipError_t lx_launch_tonemap(const float* accum, uint32_t npix, float spp, int mode, uint8_t* out, hipStream_t s);
 hipError_t lx_launch_math_eval(int fn, const float* x, const float* y, float* out, uint32_t n, hipStream_t s);
 hipError_t lx_launch_copy16(const void* src, void* dst, size_t n16, uint32_t grid, hipStream_t s);

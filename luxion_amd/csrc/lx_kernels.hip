@@ -191,7 +191,7 @@ struct LxRecStore {
 
 /* CNT = true is the counting replay: identical work plus per-ray node,
  * triangle and instance visit counts for the algorithmic-bytes model. */
-template <bool CNT>
+template <bool CNT, int W>
 __global__ __launch_bounds__(LX_TRACE_BLOCK) void k_trace(const LxJob* __restrict__ jobs, LxPathBuf P, LxRecBuf R,
                                                            LxCounters* __restrict__ C, int bounce,
                                                            uint32_t* __restrict__ spill) {
@@ -211,7 +211,7 @@ __global__ __launch_bounds__(LX_TRACE_BLOCK) void k_trace(const LxJob* __restric
         const LxDevScene& S = *J.scene;
         uint32_t pix = P.pix[i] & 0x7fffffffu;
         LxRecStore out{R, i};
-        lx_trace_path<CNT>(J, S, mk3(P.ox[i], P.oy[i], P.oz[i]), mk3(P.dx[i], P.dy[i], P.dz[i]), pix, bounce, st,
+        lx_trace_path<CNT, W>(J, S, mk3(P.ox[i], P.oy[i], P.oz[i]), mk3(P.dx[i], P.dy[i], P.dz[i]), pix, bounce, st,
                            &C->stack_overflow, out, rc, &wc, &wa);
     }
     lx_flush_counts<CNT>(C, &C->work[0], rc, 0, wc, wa);
@@ -318,7 +318,7 @@ __global__ __launch_bounds__(LX_SHADE_BLOCK) void k_shade(const LxJob* __restric
 /* ------------------------------------------------------------ tail */
 /* BLOCK threads per workgroup; OCC > 0 asks the register allocator for at
  * least OCC waves per SIMD (spilling to scratch if it must). */
-template <bool CNT, int BLOCK, int OCC>
+template <bool CNT, int BLOCK, int OCC, int W>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1)))
 void k_finish(const LxJob* __restrict__ jobs, LxPathBuf P, LxCounters* __restrict__ C, int bounce0,
               uint32_t* __restrict__ spill, float* __restrict__ accum) {
@@ -372,7 +372,7 @@ void k_finish(const LxJob* __restrict__ jobs, LxPathBuf P, LxCounters* __restric
             const LxDevScene& S = *J.scene;
             LxRec r;
             LxRecSink sink{r};
-            lx_trace_path<CNT>(J, S, o, d, pix, b, st, &C->stack_overflow, sink, rc, &wc, &wa);
+            lx_trace_path<CNT, W>(J, S, o, d, pix, b, st, &C->stack_overflow, sink, rc, &wc, &wa);
             int depth = b + 1;
             bool cont = lx_shade_path(J, S, r, depth, depth < J.depth, o, d, thr, col, pdf, spec, pix);
             nsh++;
@@ -389,6 +389,7 @@ void k_finish(const LxJob* __restrict__ jobs, LxPathBuf P, LxCounters* __restric
 }
 
 /* ------------------------------------------------------------ test hooks */
+template <int W>
 __global__ __launch_bounds__(LX_TRACE_BLOCK) void k_trace_rays(const LxDevScene* __restrict__ Sp,
                                                                 const lx_ray* __restrict__ rays, uint32_t n,
                                                                 int any, int alpha, uint32_t iteration,
@@ -410,10 +411,10 @@ __global__ __launch_bounds__(LX_TRACE_BLOCK) void k_trace_rays(const LxDevScene*
         f3 o = mk3(r.origin[0], r.origin[1], r.origin[2]);
         f3 d = mk3(r.dir[0], r.dir[1], r.dir[2]);
         if (any) {
-            LxHitD h = lx_trace<true>(S, o, d, r.tmax, i, iteration, amask, st, overflow);
+            LxHitD h = lx_trace<true, false, W>(S, o, d, r.tmax, i, iteration, amask, st, overflow);
             occ[i] = h.inst != 0xffffffffu ? 1 : 0;
         } else {
-            LxHitD h = lx_trace<false>(S, o, d, r.tmax, i, iteration, amask, st, overflow);
+            LxHitD h = lx_trace<false, false, W>(S, o, d, r.tmax, i, iteration, amask, st, overflow);
             lx_hit o2;
             o2.t = h.inst == 0xffffffffu ? -1.0f : h.t;
             o2.instance = h.inst;
@@ -516,9 +517,15 @@ hipError_t lx_launch_camera(const LxJob* jobs, const LxSched* sched, const LxCou
     return hipGetLastError();
 }
 hipError_t lx_launch_trace(const LxJob* jobs, const LxPathBuf& P, const LxRecBuf& R, LxCounters* C, int bounce,
-                           uint32_t* spill, uint32_t grid, hipStream_t s, bool count) {
-    if (count) hipLaunchKernelGGL(k_trace<true>, dim3(grid), dim3(LX_TRACE_BLOCK), 0, s, jobs, P, R, C, bounce, spill);
-    else hipLaunchKernelGGL(k_trace<false>, dim3(grid), dim3(LX_TRACE_BLOCK), 0, s, jobs, P, R, C, bounce, spill);
+                           uint32_t* spill, uint32_t grid, int width, hipStream_t s, bool count) {
+    dim3 g(grid), b(LX_TRACE_BLOCK);
+    if (width == 8) {
+        if (count) hipLaunchKernelGGL((k_trace<true, 8>), g, b, 0, s, jobs, P, R, C, bounce, spill);
+        else hipLaunchKernelGGL((k_trace<false, 8>), g, b, 0, s, jobs, P, R, C, bounce, spill);
+    } else {
+        if (count) hipLaunchKernelGGL((k_trace<true, 4>), g, b, 0, s, jobs, P, R, C, bounce, spill);
+        else hipLaunchKernelGGL((k_trace<false, 4>), g, b, 0, s, jobs, P, R, C, bounce, spill);
+    }
     return hipGetLastError();
 }
 hipError_t lx_launch_shade(const LxJob* jobs, const LxPathBuf& P, const LxPathBuf& Q, const LxRecBuf& R, LxCounters* C,
@@ -526,36 +533,43 @@ hipError_t lx_launch_shade(const LxJob* jobs, const LxPathBuf& P, const LxPathBu
     hipLaunchKernelGGL(k_shade, dim3(grid), dim3(LX_SHADE_BLOCK), 0, s, jobs, P, Q, R, C, bounce, accum);
     return hipGetLastError();
 }
-template <int BLOCK, int OCC>
+template <int BLOCK, int OCC, int W>
 static void lx_finish_at(const LxJob* jobs, const LxPathBuf& P, LxCounters* C, int bounce0, uint32_t* spill,
                          float* accum, uint32_t grid, hipStream_t s) {
-    hipLaunchKernelGGL((k_finish<false, BLOCK, OCC>), dim3(grid), dim3(BLOCK), 0, s, jobs, P, C, bounce0, spill, accum);
+    hipLaunchKernelGGL((k_finish<false, BLOCK, OCC, W>), dim3(grid), dim3(BLOCK), 0, s, jobs, P, C, bounce0, spill,
+                       accum);
+}
+template <int W>
+static void lx_finish_w(const LxJob* jobs, const LxPathBuf& P, LxCounters* C, int bounce0, uint32_t* spill,
+                        float* accum, uint32_t threads, int block, int occ, hipStream_t s, bool count) {
+    if (count) {
+        hipLaunchKernelGGL((k_finish<true, 128, 0, W>), dim3(threads / 128), dim3(128), 0, s, jobs, P, C, bounce0,
+                           spill, accum);
+    } else if (block == 256 && threads >= 256) {
+        if (occ >= 4) lx_finish_at<256, 4, W>(jobs, P, C, bounce0, spill, accum, threads / 256, s);
+        else lx_finish_at<256, 0, W>(jobs, P, C, bounce0, spill, accum, threads / 256, s);
+    } else {
+        if (occ >= 4) lx_finish_at<128, 4, W>(jobs, P, C, bounce0, spill, accum, threads / 128, s);
+        else lx_finish_at<128, 0, W>(jobs, P, C, bounce0, spill, accum, threads / 128, s);
+    }
 }
 hipError_t lx_launch_finish(const LxJob* jobs, const LxPathBuf& P, LxCounters* C, int bounce0, uint32_t* spill,
-                            float* accum, uint32_t threads, int block, int occ, hipStream_t s, bool count) {
+                            float* accum, uint32_t threads, int block, int occ, int width, hipStream_t s, bool count) {
     /* threads = grid * block lanes, the same for every shape, so one spill
      * allocation serves them all */
-    if (threads < (uint32_t)block) block = 128;
-    if (count) {
-        hipLaunchKernelGGL((k_finish<true, 128, 0>), dim3(threads / 128), dim3(128), 0, s, jobs, P, C, bounce0, spill,
-                           accum);
-    } else if (block == 64) {
-        if (occ >= 4) lx_finish_at<64, 4>(jobs, P, C, bounce0, spill, accum, threads / 64, s);
-        else lx_finish_at<64, 0>(jobs, P, C, bounce0, spill, accum, threads / 64, s);
-    } else if (block == 256) {
-        if (occ >= 4) lx_finish_at<256, 4>(jobs, P, C, bounce0, spill, accum, threads / 256, s);
-        else lx_finish_at<256, 0>(jobs, P, C, bounce0, spill, accum, threads / 256, s);
-    } else {
-        if (occ >= 4) lx_finish_at<128, 4>(jobs, P, C, bounce0, spill, accum, threads / 128, s);
-        else lx_finish_at<128, 0>(jobs, P, C, bounce0, spill, accum, threads / 128, s);
-    }
+    if (width == 8) lx_finish_w<8>(jobs, P, C, bounce0, spill, accum, threads, block, occ, s, count);
+    else lx_finish_w<4>(jobs, P, C, bounce0, spill, accum, threads, block, occ, s, count);
     return hipGetLastError();
 }
 hipError_t lx_launch_trace_rays(const LxDevScene* S, const lx_ray* rays, uint32_t n, int any, int alpha,
                                 uint32_t iteration, lx_hit* hits, uint8_t* occ, uint32_t* spill, uint32_t* overflow,
-                                uint32_t grid, hipStream_t s) {
-    hipLaunchKernelGGL(k_trace_rays, dim3(grid), dim3(LX_TRACE_BLOCK), 0, s, S, rays, n, any, alpha, iteration, hits,
-                       occ, spill, overflow);
+                                uint32_t grid, int width, hipStream_t s) {
+    if (width == 8)
+        hipLaunchKernelGGL(k_trace_rays<8>, dim3(grid), dim3(LX_TRACE_BLOCK), 0, s, S, rays, n, any, alpha, iteration,
+                           hits, occ, spill, overflow);
+    else
+        hipLaunchKernelGGL(k_trace_rays<4>, dim3(grid), dim3(LX_TRACE_BLOCK), 0, s, S, rays, n, any, alpha, iteration,
+                           hits, occ, spill, overflow);
     return hipGetLastError();
 }
 hipError_t lx_launch_tonemap(const float* accum, uint32_t npix, float spp, int mode, uint8_t* out, hipStream_t s) {

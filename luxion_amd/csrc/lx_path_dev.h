@@ -58,7 +58,7 @@ struct LxRecSink {
     LXD void env(float env_t) { r.env_t = env_t; }
 };
 
-template <bool CNT, class Sink>
+template <bool CNT, int W, class Sink>
 LXD void lx_trace_path(const LxJob& J, const LxDevScene& S, f3 o, f3 d, uint32_t pix, int bounce, LxStack& st,
                        uint32_t* overflow, Sink& out, LxRayCount& rc, LxWork* wc, LxWork* wa) {
     f3 lp = mk3(0, 0, 0), ed = mk3(0, 0, 0);
@@ -68,7 +68,7 @@ LXD void lx_trace_path(const LxJob& J, const LxDevScene& S, f3 o, f3 d, uint32_t
         if (J.dmis) lp = lx_sample_light_point(S, rng);
         if (J.emis) ed = lx_sample_env_dir(S, rng);
     }
-    LxHitD h = lx_trace<false, CNT>(S, o, d, 1e16f, pix, (uint32_t)J.iter, LX_INST_ALPHA, st, overflow, wc);
+    LxHitD h = lx_trace<false, CNT, W>(S, o, d, 1e16f, pix, (uint32_t)J.iter, LX_INST_ALPHA, st, overflow, wc);
     rc.rad++;
     if (J.dmis) out.light(lp);
     if (J.emis) out.env_dir(ed);
@@ -85,7 +85,7 @@ LXD void lx_trace_path(const LxJob& J, const LxDevScene& S, f3 o, f3 d, uint32_t
             f3 ld = lp - so;
             float dist = length(ld);
             ld = ld * (1.0f / dist);
-            LxHitD hd = lx_trace<false, CNT>(S, so, ld, dist * 1.0001f + 1e-4f, pix, (uint32_t)J.iter, LX_INST_ALPHA,
+            LxHitD hd = lx_trace<false, CNT, W>(S, so, ld, dist * 1.0001f + 1e-4f, pix, (uint32_t)J.iter, LX_INST_ALPHA,
                                              st, overflow, wc);
             rc.dl++;
             if (hd.inst != 0xffffffffu) {
@@ -99,7 +99,7 @@ LXD void lx_trace_path(const LxJob& J, const LxDevScene& S, f3 o, f3 d, uint32_t
             }
         }
         if (J.emis) {
-            LxHitD he = lx_trace<true, CNT>(S, so, ed, 1e16f, pix, (uint32_t)J.iter, LX_INST_ALPHA, st, overflow, wa);
+            LxHitD he = lx_trace<true, CNT, W>(S, so, ed, 1e16f, pix, (uint32_t)J.iter, LX_INST_ALPHA, st, overflow, wa);
             rc.env++;
             out.env(he.inst != 0xffffffffu ? 1.0f : -1.0f);
         } else {

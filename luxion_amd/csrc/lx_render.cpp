@@ -87,8 +87,8 @@ void WaveBufs::alloc(lx_context* ctx, uint32_t c, bool dmis, bool emis, bool tan
 
 void lx_enqueue_round(lx_context* ctx, LxJob* d_jobs, const int32_t* d_spp, uint32_t njobs, LxCounters* d_cnt,
                       LxSched* d_sched, uint32_t path_cap, WaveBufs& wb, float* d_accum, uint32_t* d_spill,
-                      int max_depth, uint32_t trace_grid, uint32_t shade_grid, uint32_t cam_grid, hipEvent_t* ev,
-                      bool count) {
+                      int max_depth, uint32_t trace_grid, uint32_t shade_grid, uint32_t cam_grid, int width,
+                      hipEvent_t* ev, bool count) {
     hipStream_t s = ctx->stream;
     int wf = lx_wavefront_bounces(ctx, max_depth);
     LX_HIP(lx_launch_begin_round(d_jobs, d_spp, njobs, d_cnt, d_sched, path_cap, s));
@@ -97,7 +97,7 @@ void lx_enqueue_round(lx_context* ctx, LxJob* d_jobs, const int32_t* d_spp, uint
     LxPathBuf* nxt = &wb.B;
     for (int b = 0; b < wf; b++) {
         if (ev) LX_HIP(hipEventRecord(ev[2 * b], s));
-        LX_HIP(lx_launch_trace(d_jobs, *cur, wb.R, d_cnt, b, d_spill, trace_grid, s, count));
+        LX_HIP(lx_launch_trace(d_jobs, *cur, wb.R, d_cnt, b, d_spill, trace_grid, width, s, count));
         if (ev) LX_HIP(hipEventRecord(ev[2 * b + 1], s));
         LX_HIP(lx_launch_shade(d_jobs, *cur, *nxt, wb.R, d_cnt, b, d_accum, shade_grid, s));
         std::swap(cur, nxt);
@@ -107,7 +107,7 @@ void lx_enqueue_round(lx_context* ctx, LxJob* d_jobs, const int32_t* d_spp, uint
          * mark empty intervals so the layout matches the all-wavefront round */
         if (ev) LX_HIP(hipEventRecord(ev[2 * wf], s));
         LX_HIP(lx_launch_finish(d_jobs, *cur, d_cnt, wf, d_spill, d_accum, trace_grid * LX_TRACE_BLOCK,
-                                ctx->tuning.block_size, ctx->tuning.tail_min_waves, s, count));
+                                ctx->tuning.block_size, ctx->tuning.tail_min_waves, width, s, count));
         if (ev)
             for (int k = 2 * wf + 1; k < 2 * max_depth; k++) LX_HIP(hipEventRecord(ev[k], s));
     }

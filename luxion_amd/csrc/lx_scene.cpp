@@ -273,14 +273,48 @@ void lx_scene_init(lx_scene* s, const lx_scene_desc* d) {
     for (auto& g : s->insts) s->info.num_instanced_tris += s->meshes[g.mesh].idx.size() / 3;
 }
 
+/* Node k's child boxes in per-axis SoA float4 groups (four children per
+ * group), children refs, and meta.x = 1 for TLAS nodes (index >= tlas_first).
+ * Empty child slots get zero boxes and LX_CHILD_EMPTY. */
+template <class N, int W>
+static std::vector<N> pack_nodes(const std::vector<LxWideNode>& in, uint32_t tlas_first) {
+    std::vector<N> out(in.size());
+    for (size_t i = 0; i < in.size(); i++) {
+        const LxWideNode& w = in[i];
+        N& n = out[i];
+        std::memset(&n, 0, sizeof(N));
+        float* lox = (float*)&n.lox;
+        float* hix = (float*)&n.hix;
+        float* loy = (float*)&n.loy;
+        float* hiy = (float*)&n.hiy;
+        float* loz = (float*)&n.loz;
+        float* hiz = (float*)&n.hiz;
+        uint32_t* ch = (uint32_t*)&n.child;
+        for (int k = 0; k < W; k++) {
+            bool used = k < 8 && w.child[k] != LX_CHILD_EMPTY;
+            lox[k] = used ? w.lo[k][0] : 0.0f;
+            loy[k] = used ? w.lo[k][1] : 0.0f;
+            loz[k] = used ? w.lo[k][2] : 0.0f;
+            hix[k] = used ? w.hi[k][0] : 0.0f;
+            hiy[k] = used ? w.hi[k][1] : 0.0f;
+            hiz[k] = used ? w.hi[k][2] : 0.0f;
+            ch[k] = used ? w.child[k] : LX_CHILD_EMPTY;
+        }
+        int* meta = (int*)&n.meta;
+        meta[0] = i >= tlas_first ? 1 : 0;
+        meta[1] = (int)w.n;
+    }
+    return out;
+}
+
 /* Precompute and upload everything; builds the BVHs. */
 void lx_scene_accel(lx_scene* s, const lx_build_opts* o) {
     lx_build_opts opt{4, 4, 1, 16, LX_CDF_FP64};
     if (o) opt = *o;
-    if (opt.bvh_width != 4 && opt.bvh_width != 8) opt.bvh_width = 4;
+    if (opt.bvh_width == 0) opt.bvh_width = s->ctx->tuning.bvh_width;
+    LX_REQUIRE(opt.bvh_width == 4 || opt.bvh_width == 8, LX_ERR_INVALID, "bvh_width must be 4 or 8 (0: the tuning's)");
     if (opt.leaf_size <= 0) opt.leaf_size = 4;
     if (opt.sah_bins <= 0) opt.sah_bins = 16;
-    LX_REQUIRE(opt.bvh_width == 4, LX_ERR_UNSUPPORTED, "this build traverses BVH4 nodes only");
     lx_context* ctx = s->ctx;
     ctx->set_device();
     double t0 = now_ms();
@@ -315,7 +349,7 @@ void lx_scene_accel(lx_scene* s, const lx_build_opts* o) {
     }
 
     /* BLAS per mesh */
-    std::vector<LxNode4> nodes;
+    std::vector<LxWideNode> nodes;
     std::vector<float4> tris;
     std::vector<uint32_t> blas_root(s->meshes.size());
     std::vector<std::array<float, 6>> mesh_box(s->meshes.size());
@@ -326,18 +360,18 @@ void lx_scene_accel(lx_scene* s, const lx_build_opts* o) {
         BvhBuildResult r;
         uint32_t nt = (uint32_t)(m.idx.size() / 3);
         bool ok = false;
-        if (opt.builder == 0) ok = lx_build_blas_gpu(ctx, m.pos.data(), m.idx.data(), nt, opt.leaf_size, opt.sah_bins, r);
-        if (!ok) lx_build_blas_host(m.pos.data(), m.idx.data(), nt, opt.leaf_size, opt.sah_bins, r);
+        if (opt.builder == 0)
+            ok = lx_build_blas_gpu(ctx, m.pos.data(), m.idx.data(), nt, opt.leaf_size, opt.sah_bins, opt.bvh_width, r);
+        if (!ok) lx_build_blas_host(m.pos.data(), m.idx.data(), nt, opt.leaf_size, opt.sah_bins, opt.bvh_width, r);
         uint32_t noff = (uint32_t)nodes.size(), toff = (uint32_t)(tris.size() / 3);
         LX_REQUIRE((size_t)toff + nt < (1u << 28) - 1, LX_ERR_INVALID, "scene too large for the leaf encoding");
-        for (LxNode4 n : r.nodes) {
-            int* c = &n.child.x;
-            for (int k = 0; k < 4; k++) {
-                uint32_t v = (uint32_t)c[k];
+        for (LxWideNode n : r.nodes) {
+            for (int k = 0; k < 8; k++) {
+                uint32_t v = n.child[k];
                 if (v == LX_CHILD_EMPTY) continue;
                 if (v & LX_LEAF_BIT) v = (v & ~LX_LEAF_FIRST_MASK) | ((v & LX_LEAF_FIRST_MASK) + toff);
                 else v += noff;
-                c[k] = (int)v;
+                n.child[k] = v;
             }
             nodes.push_back(n);
         }
@@ -397,22 +431,26 @@ void lx_scene_accel(lx_scene* s, const lx_build_opts* o) {
         I.has_n_uv = (s->meshes[g.mesh].has_n ? 1u : 0u) | (s->meshes[g.mesh].has_uv ? 2u : 0u);
     }
     BvhBuildResult tl;
-    lx_build_tlas_host(wboxes, tl);
+    lx_build_tlas_host(wboxes, opt.bvh_width, tl);
     uint32_t noff = (uint32_t)nodes.size();
-    for (LxNode4 n : tl.nodes) {
-        int* c = &n.child.x;
-        for (int k = 0; k < 4; k++) {
-            uint32_t v = (uint32_t)c[k];
+    for (LxWideNode n : tl.nodes) {
+        for (int k = 0; k < 8; k++) {
+            uint32_t v = n.child[k];
             if (v == LX_CHILD_EMPTY || (v & LX_LEAF_BIT)) continue;
-            c[k] = (int)(v + noff);
+            n.child[k] = v + noff;
         }
-        n.meta.x = 1;
         nodes.push_back(n);
     }
     uint32_t tlas_root = (tl.root & LX_LEAF_BIT) ? tl.root : tl.root + noff;
 
-    /* uploads */
-    s->d_nodes.upload(ctx, nodes.data(), nodes.size() * sizeof(LxNode4));
+    /* uploads: nodes packed at the traversal's width */
+    if (opt.bvh_width == 8) {
+        std::vector<LxNode8> packed = pack_nodes<LxNode8, 8>(nodes, noff);
+        s->d_nodes.upload(ctx, packed.data(), packed.size() * sizeof(LxNode8));
+    } else {
+        std::vector<LxNode4> packed = pack_nodes<LxNode4, 4>(nodes, noff);
+        s->d_nodes.upload(ctx, packed.data(), packed.size() * sizeof(LxNode4));
+    }
     s->d_tris.upload(ctx, tris.data(), tris.size() * sizeof(float4));
     s->d_inst.upload(ctx, dinst.data(), dinst.size() * sizeof(LxInstance));
     s->d_idx.upload(ctx, all_idx.data(), all_idx.size() * 4);
@@ -448,7 +486,7 @@ void lx_scene_accel(lx_scene* s, const lx_build_opts* o) {
     D.num_emissive = (uint32_t)s->emissive.size();
     D.total_area = s->total_area;
     D.has_env = s->has_env ? 1 : 0;
-    D.bvh_width = 4;
+    D.bvh_width = opt.bvh_width;
     s->d_scene.upload(ctx, &D, sizeof(LxDevScene));
     LX_HIP(hipStreamSynchronize(ctx->stream));
     s->accel_built = true;

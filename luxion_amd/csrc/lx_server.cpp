@@ -84,6 +84,7 @@ struct lx_server {
     lx_server_opts opts{};
     uint32_t slots = 0, cap = 0, pool_pix = 0;
     int max_depth = LX_MAX_DEPTH;
+    int width = 4; /* BVH width every job's scene must have */
     std::thread th;
     std::mutex m;
     std::condition_variable cv, done_cv;
@@ -276,6 +277,7 @@ lx_status lx_server_start(lx_context* ctx, const lx_server_opts* o, lx_server** 
         S->opts = opts;
         S->slots = (uint32_t)opts.max_jobs_in_flight;
         S->max_depth = opts.max_depth;
+        S->width = ctx->tuning.bvh_width == 8 ? 8 : 4;
         std::lock_guard<std::mutex> dl(ctx->mu);
         ctx->set_device();
         uint64_t avail = ctx->budget > ctx->in_use.load() ? ctx->budget - ctx->in_use.load() : 0;
@@ -312,7 +314,7 @@ lx_status lx_server_start(lx_context* ctx, const lx_server_opts* o, lx_server** 
         try {
             lx_enqueue_round(ctx, S->d_jobs.as<LxJob>(), S->d_spp.as<int32_t>(), S->slots, S->d_cnt.as<LxCounters>(),
                              S->d_sched.as<LxSched>(), S->cap, S->wb, S->d_accum.as<float>(),
-                             S->d_spill.as<uint32_t>(), S->max_depth, S->tg, S->sg, S->cg, nullptr);
+                             S->d_spill.as<uint32_t>(), S->max_depth, S->tg, S->sg, S->cg, S->width, nullptr);
         } catch (...) {
             hipGraph_t tmp;
             (void)hipStreamEndCapture(s, &tmp);
@@ -339,6 +341,8 @@ lx_status lx_server_submit(lx_server* S, const lx_job* job, lx_job_result* res, 
         lx_validate_job(*job);
         int depth = job->max_depth > 0 ? job->max_depth : job->scene->trace_depth;
         LX_REQUIRE(depth <= S->max_depth, LX_ERR_INVALID, "job depth above the server's maximum");
+        LX_REQUIRE(job->scene->dev.bvh_width == S->width, LX_ERR_INVALID,
+                   "job scene BVH width differs from the server's (the context tuning's bvh_width at start)");
         SrvJob* j = new SrvJob();
         j->job = *job;
         j->res = res;

@@ -3,7 +3,9 @@
  */
 #include "lx_internal.h"
 
-bool lx_build_blas_gpu(lx_context*, const float*, const uint32_t*, uint32_t, int, int, BvhBuildResult&) { return false; }
+bool lx_build_blas_gpu(lx_context*, const float*, const uint32_t*, uint32_t, int, int, int, BvhBuildResult&) {
+    return false;
+}
 
 extern "C" {
 lx_status lx_scene_load_glb(lx_context*, const char*, const char*, lx_scene**) {

@@ -132,18 +132,41 @@ struct LxStack {
     }
 };
 
-/* Slab test of four children of a BVH4 node; writes entry distances (or a
- * negative "miss" flag as +inf) and returns a 4-bit hit mask. */
-LXD uint32_t lx_box4(const LxNode4* __restrict__ nd, f3 o, f3 ix, float tcut, float* tn) {
-    float4 lx = nd->lox, hx = nd->hix, ly = nd->loy, hy = nd->hiy, lz = nd->loz, hz = nd->hiz;
-    int4 ch = nd->child;
-    const float lxs[4] = {lx.x, lx.y, lx.z, lx.w}, hxs[4] = {hx.x, hx.y, hx.z, hx.w};
-    const float lys[4] = {ly.x, ly.y, ly.z, ly.w}, hys[4] = {hy.x, hy.y, hy.z, hy.w};
-    const float lzs[4] = {lz.x, lz.y, lz.z, lz.w}, hzs[4] = {hz.x, hz.y, hz.z, hz.w};
-    const uint32_t cs[4] = {(uint32_t)ch.x, (uint32_t)ch.y, (uint32_t)ch.z, (uint32_t)ch.w};
+template <int W>
+struct LxNodeT;
+template <>
+struct LxNodeT<4> {
+    typedef LxNode4 type;
+};
+template <>
+struct LxNodeT<8> {
+    typedef LxNode8 type;
+};
+
+/* Slab test of the W children of a BVH node; writes entry distances and
+ * returns a W-bit hit mask. */
+template <int W>
+LXD uint32_t lx_boxw(const typename LxNodeT<W>::type* __restrict__ nd, f3 o, f3 ix, float tcut, float* tn,
+                     uint32_t* cs) {
+    float lxs[W], hxs[W], lys[W], hys[W], lzs[W], hzs[W];
+#pragma unroll
+    for (int g = 0; g < W / 4; g++) {
+        const float4* L = (const float4*)&nd->lox;
+        float4 a = L[0 * (W / 4) + g], b = L[1 * (W / 4) + g], c = L[2 * (W / 4) + g];
+        float4 d = L[3 * (W / 4) + g], e = L[4 * (W / 4) + g], f = L[5 * (W / 4) + g];
+        int4 ch = ((const int4*)&nd->child)[g];
+        lxs[4 * g + 0] = a.x, lxs[4 * g + 1] = a.y, lxs[4 * g + 2] = a.z, lxs[4 * g + 3] = a.w;
+        hxs[4 * g + 0] = b.x, hxs[4 * g + 1] = b.y, hxs[4 * g + 2] = b.z, hxs[4 * g + 3] = b.w;
+        lys[4 * g + 0] = c.x, lys[4 * g + 1] = c.y, lys[4 * g + 2] = c.z, lys[4 * g + 3] = c.w;
+        hys[4 * g + 0] = d.x, hys[4 * g + 1] = d.y, hys[4 * g + 2] = d.z, hys[4 * g + 3] = d.w;
+        lzs[4 * g + 0] = e.x, lzs[4 * g + 1] = e.y, lzs[4 * g + 2] = e.z, lzs[4 * g + 3] = e.w;
+        hzs[4 * g + 0] = f.x, hzs[4 * g + 1] = f.y, hzs[4 * g + 2] = f.z, hzs[4 * g + 3] = f.w;
+        cs[4 * g + 0] = (uint32_t)ch.x, cs[4 * g + 1] = (uint32_t)ch.y;
+        cs[4 * g + 2] = (uint32_t)ch.z, cs[4 * g + 3] = (uint32_t)ch.w;
+    }
     uint32_t mask = 0;
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < W; k++) {
         float ax = (lxs[k] - o.x) * ix.x, bx = (hxs[k] - o.x) * ix.x;
         float ay = (lys[k] - o.y) * ix.y, by = (hys[k] - o.y) * ix.y;
         float az = (lzs[k] - o.z) * ix.z, bz = (hzs[k] - o.z) * ix.z;
@@ -165,7 +188,7 @@ struct LxWork {
     uint32_t nodes, tris, insts;
 };
 
-template <bool ANY, bool CNT = false>
+template <bool ANY, bool CNT = false, int W = 4>
 LXD LxHitD lx_trace(const LxDevScene& S, f3 wo, f3 wd, float tmax, uint32_t key, uint32_t iteration,
                     uint32_t alpha_mask, LxStack& st, uint32_t* overflow, LxWork* work = nullptr) {
     LxHitD best;
@@ -173,7 +196,8 @@ LXD LxHitD lx_trace(const LxDevScene& S, f3 wo, f3 wd, float tmax, uint32_t key,
     best.inst = 0xffffffffu;
     best.prim = 0xffffffffu;
     best.u = best.v = 0.0f;
-    const LxNode4* __restrict__ nodes = (const LxNode4*)S.nodes;
+    typedef typename LxNodeT<W>::type Node;
+    const Node* __restrict__ nodes = (const Node*)S.nodes;
     f3 iwd = mk3(1.0f / wd.x, 1.0f / wd.y, 1.0f / wd.z);
     f3 o = wo, d = wd, ix = iwd;
     bool in_blas = false;
@@ -191,19 +215,18 @@ LXD LxHitD lx_trace(const LxDevScene& S, f3 wo, f3 wd, float tmax, uint32_t key,
             do_pop = true;
         } else if (!(node & LX_LEAF_BIT)) {
             float tcut = (best.inst == 0xffffffffu ? tmax : best.t) * LX_CULL_MARGIN;
-            float tn[4];
-            uint32_t m = lx_box4(nodes + node, o, ix, tcut, tn);
+            float tn[W];
+            uint32_t cs[W];
+            uint32_t m = lx_boxw<W>(nodes + node, o, ix, tcut, tn, cs);
             if (CNT) work->nodes++;
             if (m == 0u) {
                 do_pop = true;
             } else {
-                int4 ch = nodes[node].child;
-                uint32_t cs[4] = {(uint32_t)ch.x, (uint32_t)ch.y, (uint32_t)ch.z, (uint32_t)ch.w};
                 /* nearest child next; the rest pushed far to near */
                 float bt = __builtin_inff();
                 int bk = -1;
 #pragma unroll
-                for (int k = 0; k < 4; k++)
+                for (int k = 0; k < W; k++)
                     if ((m >> k) & 1u)
                         if (tn[k] < bt) {
                             bt = tn[k];
@@ -214,7 +237,7 @@ LXD LxHitD lx_trace(const LxDevScene& S, f3 wo, f3 wd, float tmax, uint32_t key,
                     float ft = -1.0f;
                     int fk = -1;
 #pragma unroll
-                    for (int k = 0; k < 4; k++)
+                    for (int k = 0; k < W; k++)
                         if ((m >> k) & 1u)
                             if (tn[k] > ft) {
                                 ft = tn[k];

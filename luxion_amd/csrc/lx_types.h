@@ -35,12 +35,14 @@ struct alignas(128) LxNode4 {
     int4 meta; /* meta.x: 1 for TLAS nodes */
 };
 
-/* BVH8 node (256 B): same scheme with eight children. */
+/* BVH8 node (256 B): the same scheme with eight children; each bound is
+ * eight consecutive floats (children 0-3, then 4-7). */
 struct alignas(256) LxNode8 {
     float4 lox[2], hix[2], loy[2], hiy[2], loz[2], hiz[2];
     int4 child[2];
     int4 meta[2];
 };
+static_assert(sizeof(LxNode4) == 128 && sizeof(LxNode8) == 256, "node layout");
 
 struct alignas(16) LxInstance {
     float inv[12]; /* world -> object, row-major 3x4 */
