@@ -339,7 +339,8 @@ typedef struct lx_job_result {
 typedef struct lx_tuning {
     int32_t block_size;    /* threads per workgroup of the tail kernel: 128 or 256 */
     int32_t bvh_width;     /* 4 or 8: scenes built with bvh_width 0 and servers */
-    int32_t queue_chunk;   /* paths claimed per wave from the work queue */
+    int32_t queue_chunk;   /* tail kernel: a wave claims paths from the queue once this
+                              many of its lanes (1..64) are idle */
     int32_t waves_per_cu;  /* persistent grid size */
     int32_t use_graph;     /* capture the bounce loop in a hipGraph */
     uint32_t max_paths_per_round; /* batched form path capacity, 0 = from the budget */

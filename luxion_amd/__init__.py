@@ -59,6 +59,18 @@ class Context:
         check(self.lib.lx_tuning_get(self.h, C.byref(t)))
         return {k: getattr(t, k) for k, _ in F.lx_tuning._fields_}
 
+    def autotune(self, jobs, json_path=None):
+        """Time the batched form over `jobs` for each launch shape (and BVH
+        width, when the jobs' scenes can be rebuilt); keep the fastest.
+        Returns the chosen tuning."""
+        arr = (F.lx_job * len(jobs))(*jobs)
+        t = F.lx_tuning()
+        check(self.lib.lx_autotune(self.h, arr, len(jobs), json_path.encode() if json_path else None, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in F.lx_tuning._fields_}
+
+    def load_tuning(self, json_path):
+        check(self.lib.lx_tuning_load(self.h, json_path.encode()))
+
     def set_tuning(self, **kw):
         t = F.lx_tuning()
         check(self.lib.lx_tuning_get(self.h, C.byref(t)))

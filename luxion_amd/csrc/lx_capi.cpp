@@ -15,24 +15,6 @@ static thread_local std::string g_err;
 
 void lx_set_error(const std::string& msg) { g_err = msg; }
 
-#define LX_TRY try {
-#define LX_CATCH                                                                                           \
-    }                                                                                                      \
-    catch (const LxError& e) {                                                                             \
-        lx_set_error(e.msg);                                                                               \
-        return e.status;                                                                                   \
-    }                                                                                                      \
-    catch (const std::bad_alloc&) {                                                                        \
-        lx_set_error("host allocation failed");                                                            \
-        return LX_ERR_OOM;                                                                                 \
-    }                                                                                                      \
-    catch (const std::exception& e) {                                                                      \
-        lx_set_error(e.what());                                                                            \
-        return LX_ERR_INVALID;                                                                             \
-    }                                                                                                      \
-    lx_set_error("");                                                                                      \
-    return LX_OK;
-
 /* ------------------------------------------------------------- context */
 void lx_context::set_device() const { LX_HIP(hipSetDevice(device)); }
 
@@ -83,7 +65,7 @@ lx_status lx_context_create(int device, uint64_t budget, lx_context** out) {
         c->budget = budget ? budget : (uint64_t)fr;
         c->tuning.block_size = LX_TRACE_BLOCK;
         c->tuning.bvh_width = 4;
-        c->tuning.queue_chunk = 64;
+        c->tuning.queue_chunk = 32;
         c->tuning.waves_per_cu = 16;
         c->tuning.use_graph = 1;
         c->tuning.wavefront_bounces = LX_DEFAULT_WAVEFRONT_BOUNCES;
@@ -134,10 +116,7 @@ lx_status lx_tuning_get(const lx_context* c, lx_tuning* out) {
 lx_status lx_tuning_set(lx_context* c, const lx_tuning* t) {
     LX_TRY
     LX_REQUIRE(c && t, LX_ERR_INVALID, "null argument");
-    LX_REQUIRE(t->waves_per_cu >= 1 && t->waves_per_cu <= 32, LX_ERR_INVALID, "waves_per_cu out of range");
-    LX_REQUIRE(t->bvh_width == 4 || t->bvh_width == 8, LX_ERR_INVALID, "bvh_width must be 4 or 8");
-    LX_REQUIRE(t->block_size == 128 || t->block_size == 256, LX_ERR_INVALID, "block_size must be 128 or 256");
-    LX_REQUIRE(t->tail_min_waves == 0 || t->tail_min_waves == 4, LX_ERR_INVALID, "tail_min_waves must be 0 or 4");
+    lx_tuning_check(*t);
     c->tuning = *t;
     LX_CATCH
 }
@@ -163,6 +142,9 @@ lx_status lx_scene_create(lx_context* ctx, const lx_scene_desc* desc, lx_scene**
 lx_status lx_scene_build_accel(lx_scene* s, const lx_build_opts* o) {
     LX_TRY
     LX_REQUIRE(s, LX_ERR_INVALID, "null scene");
+    /* a rebuild replaces the device buffers that renders and jobs point at */
+    LX_REQUIRE(!s->accel_built || s->refs.load() == 1, LX_ERR_STATE,
+               "scene is in use by renders or jobs: destroy them before rebuilding");
     std::lock_guard<std::mutex> lk(s->ctx->mu);
     lx_scene_accel(s, o);
     LX_CATCH

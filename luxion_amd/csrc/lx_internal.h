@@ -25,6 +25,25 @@
 /* ------------------------------------------------------------- errors */
 void lx_set_error(const std::string& msg);
 
+/* C-ABI bodies: errors become a status and a message */
+#define LX_TRY try {
+#define LX_CATCH                                                                                           \
+    }                                                                                                      \
+    catch (const LxError& e) {                                                                             \
+        lx_set_error(e.msg);                                                                               \
+        return e.status;                                                                                   \
+    }                                                                                                      \
+    catch (const std::bad_alloc&) {                                                                        \
+        lx_set_error("host allocation failed");                                                            \
+        return LX_ERR_OOM;                                                                                 \
+    }                                                                                                      \
+    catch (const std::exception& e) {                                                                      \
+        lx_set_error(e.what());                                                                            \
+        return LX_ERR_INVALID;                                                                             \
+    }                                                                                                      \
+    lx_set_error("");                                                                                      \
+    return LX_OK;
+
 struct LxError {
     lx_status status;
     std::string msg;
@@ -167,6 +186,7 @@ struct lx_scene {
     std::vector<DevBuf> d_tex;
     LxDevScene dev{};
     bool accel_built = false;
+    lx_build_opts build_opts{}; /* of the last build (the autotuner rebuilds at other widths) */
     lx_scene_info info{};
 };
 
@@ -221,6 +241,7 @@ int lx_wavefront_bounces(const lx_context* ctx, int max_depth);
 
 
 /* ------------------------------------------------------ batched / served */
+void lx_tuning_check(const lx_tuning& t); /* throws LX_ERR_INVALID */
 void lx_validate_job(const lx_job& j);
 LxJob lx_make_dev_job(const lx_job& j, uint32_t accum_off);
 int lx_simulate_rounds(const std::vector<uint32_t>& npix, const std::vector<int32_t>& spp, uint32_t cap,

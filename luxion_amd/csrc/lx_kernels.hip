@@ -321,7 +321,7 @@ __global__ __launch_bounds__(LX_SHADE_BLOCK) void k_shade(const LxJob* __restric
 template <bool CNT, int BLOCK, int OCC, int W>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1)))
 void k_finish(const LxJob* __restrict__ jobs, LxPathBuf P, LxCounters* __restrict__ C, int bounce0,
-              uint32_t* __restrict__ spill, float* __restrict__ accum) {
+              uint32_t* __restrict__ spill, float* __restrict__ accum, uint32_t chunk) {
     __shared__ uint32_t s_stack[LX_STACK * BLOCK];
     uint32_t n = C->count[bounce0];
     uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -342,9 +342,10 @@ void k_finish(const LxJob* __restrict__ jobs, LxPathBuf P, LxCounters* __restric
     float pdf = 0.0f;
     bool spec = false;
     while (true) {
-        /* refill idle lanes from the live-path queue: one atomic per wave */
+        /* refill idle lanes from the live-path queue, one atomic per wave,
+         * once at least `chunk` lanes are idle (or all of them are) */
         uint64_t idle = __ballot(!have);
-        if (idle && !exhausted) {
+        if (!exhausted && idle && ((uint32_t)__popcll(idle) >= chunk || idle == ~0ull)) {
             uint32_t base = 0;
             if (lane == 0) base = atomicAdd(&C->tail_next, (uint32_t)__popcll(idle));
             base = __shfl(base, 0, LX_WAVE);
@@ -535,30 +536,33 @@ hipError_t lx_launch_shade(const LxJob* jobs, const LxPathBuf& P, const LxPathBu
 }
 template <int BLOCK, int OCC, int W>
 static void lx_finish_at(const LxJob* jobs, const LxPathBuf& P, LxCounters* C, int bounce0, uint32_t* spill,
-                         float* accum, uint32_t grid, hipStream_t s) {
+                         float* accum, uint32_t grid, uint32_t chunk, hipStream_t s) {
     hipLaunchKernelGGL((k_finish<false, BLOCK, OCC, W>), dim3(grid), dim3(BLOCK), 0, s, jobs, P, C, bounce0, spill,
-                       accum);
+                       accum, chunk);
 }
 template <int W>
 static void lx_finish_w(const LxJob* jobs, const LxPathBuf& P, LxCounters* C, int bounce0, uint32_t* spill,
-                        float* accum, uint32_t threads, int block, int occ, hipStream_t s, bool count) {
+                        float* accum, uint32_t threads, int block, int occ, uint32_t chunk, hipStream_t s,
+                        bool count) {
     if (count) {
         hipLaunchKernelGGL((k_finish<true, 128, 0, W>), dim3(threads / 128), dim3(128), 0, s, jobs, P, C, bounce0,
-                           spill, accum);
+                           spill, accum, chunk);
     } else if (block == 256 && threads >= 256) {
-        if (occ >= 4) lx_finish_at<256, 4, W>(jobs, P, C, bounce0, spill, accum, threads / 256, s);
-        else lx_finish_at<256, 0, W>(jobs, P, C, bounce0, spill, accum, threads / 256, s);
+        if (occ >= 4) lx_finish_at<256, 4, W>(jobs, P, C, bounce0, spill, accum, threads / 256, chunk, s);
+        else lx_finish_at<256, 0, W>(jobs, P, C, bounce0, spill, accum, threads / 256, chunk, s);
     } else {
-        if (occ >= 4) lx_finish_at<128, 4, W>(jobs, P, C, bounce0, spill, accum, threads / 128, s);
-        else lx_finish_at<128, 0, W>(jobs, P, C, bounce0, spill, accum, threads / 128, s);
+        if (occ >= 4) lx_finish_at<128, 4, W>(jobs, P, C, bounce0, spill, accum, threads / 128, chunk, s);
+        else lx_finish_at<128, 0, W>(jobs, P, C, bounce0, spill, accum, threads / 128, chunk, s);
     }
 }
 hipError_t lx_launch_finish(const LxJob* jobs, const LxPathBuf& P, LxCounters* C, int bounce0, uint32_t* spill,
-                            float* accum, uint32_t threads, int block, int occ, int width, hipStream_t s, bool count) {
+                            float* accum, uint32_t threads, int block, int occ, int width, int chunk, hipStream_t s,
+                            bool count) {
     /* threads = grid * block lanes, the same for every shape, so one spill
      * allocation serves them all */
-    if (width == 8) lx_finish_w<8>(jobs, P, C, bounce0, spill, accum, threads, block, occ, s, count);
-    else lx_finish_w<4>(jobs, P, C, bounce0, spill, accum, threads, block, occ, s, count);
+    uint32_t c = chunk < 1 ? 1u : (chunk > 64 ? 64u : (uint32_t)chunk);
+    if (width == 8) lx_finish_w<8>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, s, count);
+    else lx_finish_w<4>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, s, count);
     return hipGetLastError();
 }
 hipError_t lx_launch_trace_rays(const LxDevScene* S, const lx_ray* rays, uint32_t n, int any, int alpha,

@@ -107,7 +107,8 @@ void lx_enqueue_round(lx_context* ctx, LxJob* d_jobs, const int32_t* d_spp, uint
          * mark empty intervals so the layout matches the all-wavefront round */
         if (ev) LX_HIP(hipEventRecord(ev[2 * wf], s));
         LX_HIP(lx_launch_finish(d_jobs, *cur, d_cnt, wf, d_spill, d_accum, trace_grid * LX_TRACE_BLOCK,
-                                ctx->tuning.block_size, ctx->tuning.tail_min_waves, width, s, count));
+                                ctx->tuning.block_size, ctx->tuning.tail_min_waves, width, ctx->tuning.queue_chunk,
+                                s, count));
         if (ev)
             for (int k = 2 * wf + 1; k < 2 * max_depth; k++) LX_HIP(hipEventRecord(ev[k], s));
     }

@@ -318,6 +318,7 @@ void lx_scene_accel(lx_scene* s, const lx_build_opts* o) {
     lx_context* ctx = s->ctx;
     ctx->set_device();
     double t0 = now_ms();
+    s->build_opts = opt;
     s->cdf_mode = opt.cdf_precision;
     if (s->has_env) precompute_env(s);
     precompute_emitters(s);

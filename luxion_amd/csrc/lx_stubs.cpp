@@ -12,6 +12,4 @@ lx_status lx_scene_load_glb(lx_context*, const char*, const char*, lx_scene**) {
     lx_set_error("glTF loading is not available in this build");
     return LX_ERR_UNSUPPORTED;
 }
-lx_status lx_autotune(lx_context*, const lx_job*, uint32_t, const char*, lx_tuning*) { return LX_ERR_UNSUPPORTED; }
-lx_status lx_tuning_load(lx_context*, const char*) { return LX_ERR_UNSUPPORTED; }
 }

@@ -57,3 +57,12 @@ def test_null_arguments_are_rejected_without_gpu():
     assert lib.lx_scene_create(None, None, None) == 1
     assert b"null" in lib.lx_last_error()
     assert lib.lx_render_iterate(None, 1, 1) == 1
+
+
+def test_tuning_json_errors_without_gpu(tmp_path):
+    """lx_tuning_load validates before touching a context: a null context is
+    rejected, as is a missing file (no device needed for either)."""
+    from luxion_amd import _ffi as F
+    lib = F.load()
+    assert lib.lx_tuning_load(None, str(tmp_path / "missing.json").encode()) == 1  # LX_ERR_INVALID
+    assert b"null" in lib.lx_last_error()

@@ -93,3 +93,21 @@ def test_tail_in_batch_equals_alone(tuned):
     for r, w in zip(res, want):
         assert r["status"] == 0
         assert np.array_equal(r["image"], w)
+
+
+def test_autotune_picks_valid_tuning_same_image(tuned, tmp_path):
+    data = scenes.icosphere_hdri(64, 48, level=4, pieces=8, env_size=(128, 64))
+    sc = lx.Scene(tuned, data)
+    cam = data.camera
+    jobs = [lx.make_job(sc, cam, spp=2, emis=True), lx.make_job(sc, cam, spp=1, emis=True, seed=9)]
+    before = lx.batch_run(tuned, jobs)
+    path = str(tmp_path / "tuning.json")
+    best = tuned.autotune(jobs, path)
+    assert best["bvh_width"] in (4, 8) and best["block_size"] in (128, 256)
+    assert tuned.tuning() == best
+    after = lx.batch_run(tuned, jobs)
+    for a, b in zip(before, after):
+        assert np.array_equal(a["image"], b["image"])
+    tuned.set_tuning(wavefront_bounces=-1, queue_chunk=32)
+    tuned.load_tuning(path)
+    assert tuned.tuning() == best
