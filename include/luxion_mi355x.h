@@ -158,9 +158,16 @@ typedef struct lx_scene_desc {
  * (src/scene/scene_precompute.cpp:71-193). */
 lx_status lx_scene_create(lx_context* ctx, const lx_scene_desc* desc, lx_scene** out);
 
-/* Replaces Scene::Scene for .glb input (src/scene/scene_load_gltf.cpp:61-529):
- * glTF binary + optional EXR environment map. */
+/* Replaces Scene::Scene for .glb input (src/scene/scene.cpp:22-49 ->
+ * src/scene/scene_load_gltf.cpp:61-529): glTF binary + optional EXR
+ * environment map (NULL or "" for none).  Textures: PNG and baseline JPEG;
+ * EXR: scanline NONE / RLE / ZIPS / ZIP / PIZ. */
 lx_status lx_scene_load_glb(lx_context* ctx, const char* glb_path, const char* exr_path, lx_scene** out);
+/* Test hook, no device needed: the descriptor lx_scene_load_glb would pass to
+ * lx_scene_create.  Its arrays stay valid until the next call on the same
+ * thread.  env_rgba_head receives the first head_n floats of the EXR. */
+lx_status lx_debug_parse_glb(const char* glb_path, const char* exr_path, lx_scene_desc* out,
+                             float* env_rgba_head, uint32_t head_n);
 
 enum { LX_CDF_FP64 = 0, LX_CDF_FP32_REFERENCE = 1 };
 typedef struct lx_build_opts {
@@ -188,6 +195,7 @@ typedef struct lx_scene_info {
     int32_t bvh_depth;
     double build_ms;
     double sah_cost;
+    char name[128]; /* glTF scene name ("GLTF_DEFAULT_NAME" when unnamed), or "" */
 } lx_scene_info;
 lx_status lx_scene_get_info(const lx_scene* scene, lx_scene_info* out);
 
@@ -264,6 +272,13 @@ lx_status lx_render_stats_get(lx_render* r, lx_render_stats* out);
  * 0 restores graph replay. */
 enum { LX_TIMING_EVENTS = 1, LX_TIMING_COUNT_WORK = 2 };
 lx_status lx_render_set_timing(lx_render* r, int32_t flags);
+/* saveImage (src/app/save_image.cpp:43-84, src/image.cpp:23-43): average the
+ * accumulation, host ACES, gamma 0.45, clamp * 255 truncated, mirrored
+ * horizontally, written as an 8-bit RGB PNG at `path` (extension included). */
+lx_status lx_render_save_png(lx_render* r, const char* path);
+/* Test hook, no device: the same output path over a host accumulation
+ * buffer (w * h * 3 radiance sums, index x + y * w). */
+lx_status lx_debug_write_png(const char* path, const float* accum, uint32_t w, uint32_t h, uint32_t spp);
 void lx_render_destroy(lx_render* r);
 
 /* -------------------------------------------------------------- test hooks */

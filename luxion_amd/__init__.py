@@ -240,6 +240,11 @@ class Render:
         check(self.lib.lx_render_reset(self.h))
         self.next_iter = 1
 
+    def save_png(self, path):
+        """Averaged, ACES-tonemapped, gamma 0.45, horizontally mirrored 8-bit
+        PNG, as the reference's saveImage writes it."""
+        check(self.lib.lx_render_save_png(self.h, path.encode()))
+
     def set_timing(self, events=True, count_work=False):
         flags = (F.LX_TIMING_EVENTS if events else 0) | (F.LX_TIMING_COUNT_WORK if count_work else 0)
         check(self.lib.lx_render_set_timing(self.h, flags))
@@ -272,6 +277,58 @@ class Render:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------------------- glTF input
+def parse_glb(path, envmap=None, env_head=0):
+    """What lx_scene_load_glb builds from a .glb (+ EXR), read back on the
+    host without a device: meshes, instances, materials, textures, camera."""
+    lib = load()
+    d = F.lx_scene_desc()
+    head = np.zeros(max(1, env_head), np.float32)
+    check(lib.lx_debug_parse_glb(path.encode(), envmap.encode() if envmap else None, C.byref(d),
+                                 head.ctypes.data, env_head))
+
+    def arr(ptr, n, dt):
+        if not ptr or n == 0:
+            return None
+        return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(dt)), (n,)).copy()
+
+    meshes = []
+    for k in range(d.num_meshes):
+        m = d.meshes[k]
+        meshes.append((arr(m.positions, 3 * m.num_verts, C.c_float).reshape(-1, 3),
+                       None if not m.normals else arr(m.normals, 3 * m.num_verts, C.c_float).reshape(-1, 3),
+                       None if not m.uvs else arr(m.uvs, 2 * m.num_verts, C.c_float).reshape(-1, 2),
+                       arr(m.indices, 3 * m.num_tris, C.c_uint32)))
+    insts = [(d.instances[k].mesh, d.instances[k].material, np.array(d.instances[k].transform, np.float32).reshape(3, 4))
+             for k in range(d.num_instances)]
+    mats = []
+    for k in range(d.num_materials):
+        m = d.materials[k]
+        e = {f: getattr(m, f) for f, _ in F.lx_material_desc._fields_}
+        for f in ("base_color", "emission_color"):
+            e[f] = tuple(e[f])
+        for f in ("albedo_xf", "normal_xf", "emissive_xf", "metallic_roughness_xf"):
+            x = e[f]
+            e[f] = dict(offset=tuple(x.offset), scale=tuple(x.scale), rotation=x.rotation)
+        mats.append(e)
+    texs = []
+    for k in range(d.num_textures):
+        t = d.textures[k]
+        if t.format == F.LX_TEX_RGBA16:
+            a = arr(t.data, 4 * t.width * t.height, C.c_uint16)
+        else:
+            a = arr(t.data, 4 * t.width * t.height, C.c_uint8)
+        texs.append((a.reshape(t.height, t.width, 4), 16 if t.format == F.LX_TEX_RGBA16 else 8))
+    c = d.camera
+    cam = dict(width=c.width, height=c.height, position=np.array(c.position), view=np.array(c.view),
+               up=np.array(c.up), right=np.array(c.right), pixel_length=tuple(c.pixel_length))
+    env = None
+    if d.envmap:
+        env = (d.envmap[0].width, d.envmap[0].height, head[:env_head].copy())
+    return dict(meshes=meshes, instances=insts, materials=mats, textures=texs, camera=cam,
+                trace_depth=d.trace_depth, env=env)
 
 
 # ------------------------------------------------------------ batched form

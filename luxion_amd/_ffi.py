@@ -77,7 +77,7 @@ class lx_scene_info(C.Structure):
                 ("num_emissive_instances", C.c_uint32), ("total_emissive_area", C.c_float),
                 ("env_width", C.c_int32), ("env_height", C.c_int32), ("device_bytes", C.c_uint64),
                 ("bvh_nodes", C.c_uint32), ("bvh_depth", C.c_int32), ("build_ms", C.c_double),
-                ("sah_cost", C.c_double)]
+                ("sah_cost", C.c_double), ("name", C.c_char * 128)]
 
 
 class lx_render_opts(C.Structure):
@@ -147,6 +147,9 @@ SIGNATURES = {
     "lx_context_bytes_in_use": (C.c_uint64, [_P]),
     "lx_scene_create": (C.c_int, [_P, C.POINTER(lx_scene_desc), _PP]),
     "lx_scene_load_glb": (C.c_int, [_P, C.c_char_p, C.c_char_p, _PP]),
+    "lx_render_save_png": (C.c_int, [_P, C.c_char_p]),
+    "lx_debug_write_png": (C.c_int, [C.c_char_p, _P, C.c_uint32, C.c_uint32, C.c_uint32]),
+    "lx_debug_parse_glb": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(lx_scene_desc), _P, C.c_uint32]),
     "lx_scene_build_accel": (C.c_int, [_P, C.POINTER(lx_build_opts)]),
     "lx_scene_get_camera": (C.c_int, [_P, C.POINTER(lx_camera_desc), C.POINTER(C.c_int32)]),
     "lx_scene_get_info": (C.c_int, [_P, C.POINTER(lx_scene_info)]),

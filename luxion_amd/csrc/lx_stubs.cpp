@@ -7,9 +7,3 @@ bool lx_build_blas_gpu(lx_context*, const float*, const uint32_t*, uint32_t, int
     return false;
 }
 
-extern "C" {
-lx_status lx_scene_load_glb(lx_context*, const char*, const char*, lx_scene**) {
-    lx_set_error("glTF loading is not available in this build");
-    return LX_ERR_UNSUPPORTED;
-}
-}

@@ -184,6 +184,17 @@ def _xf(scale, rotation, offset):
 
 
 def material_desc(m):
+    if "_desc" in m:  # a loader material: every field and per-slot transforms given
+        r = m["_desc"]
+        d = F.lx_material_desc()
+        for k, _ in F.lx_material_desc._fields_:
+            v = r[k]
+            if k.endswith("_xf"):
+                v = _xf(v["scale"], v["rotation"], v["offset"])
+            elif isinstance(v, tuple):
+                v = F.F3(*v)
+            setattr(d, k, v)
+        return d
     d = F.lx_material_desc()
     d.base_color = F.F3(*m["color"])
     xf = _xf(m["uv_scale"], m["uv_rotation"], m["uv_offset"])
@@ -206,6 +217,35 @@ def texture_desc(t):
                 np.dtype(np.float32): F.LX_TEX_RGBA32F}[t.dtype]
     d.data = t.ctypes.data
     return d
+
+
+def from_glb(path, envmap=None):
+    """A SceneData holding exactly what lx_scene_load_glb builds from `path`
+    (+ EXR), read back through the host-only parse hook; lets the oracle and
+    lx_scene_create see the loader's scene."""
+    from . import parse_glb
+    g = parse_glb(path, envmap)
+    env_len = 0 if g["env"] is None else g["env"][0] * g["env"][1] * 4
+    if env_len:
+        g = parse_glb(path, envmap, env_head=env_len)
+    s = SceneData()
+    s.name = path.rsplit("/", 1)[-1]
+    for pos, nrm, uv, idx in g["meshes"]:
+        s.add_mesh(pos, idx, nrm, uv)
+    for m in g["materials"]:
+        s.add_material({"_desc": m})
+    for mesh, mat, xf in g["instances"]:
+        s.add_instance(mesh, mat, np.asarray(xf, np.float32).reshape(12))
+    s.textures = [t for t, _ in g["textures"]]
+    if env_len:
+        w, h, rgba = g["env"]
+        s.env = np.ascontiguousarray(rgba.reshape(h, w, 4), np.float32)
+    c = g["camera"]
+    s.camera = dict(width=c["width"], height=c["height"], position=np.float32(c["position"]),
+                    view=np.float32(c["view"]), up=np.float32(c["up"]), right=np.float32(c["right"]),
+                    pixel_length=np.float32(c["pixel_length"]))
+    s.trace_depth = g["trace_depth"]
+    return s
 
 
 # ----------------------------------------------------------------- cfg0

@@ -1,0 +1,229 @@
+"""lx_scene_load_glb on the host (no device): the loader's scene description
+against the Python restatement in oracle/gltf_ref.py, on the reference's own
+.glb scenes (a few committed under tests/golden/scenes; all of them when the
+reference checkout is present) and on synthetic files that reach every image
+path (PNG colour types / bit depths / interlacing, baseline JPEG sampling
+factors, EXR NONE / RLE / ZIPS / ZIP).
+
+Bars: indices, material and texture indices, PNG texels, EXR texels from
+NONE / RLE / ZIP files: exact.  Positions, uvs: exact (copied floats).
+Normals (normalised per vertex): 1 ulp-level.  Transforms: 1e-6 relative
+(float32 vs float64 composition).  Camera: 4e-7 absolute (the oracle restates
+the float32 arithmetic).  JPEG texels: different decoders (PIL's libjpeg vs
+this build's; the reference uses stb_image, absent here) round differently:
+mean |diff| <= 1 level, max <= 24 (parity unpinned beyond that).  PIZ EXR:
+no independent decoder in this image; checked by invariants (parity
+unpinned).
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import luxion_amd as lx
+from oracle import gltf_ref
+from tests import gltf_util
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIXTURES = sorted(glob.glob(os.path.join(HERE, "golden", "scenes", "*.glb")))
+REF_SCENES = "/root/reference/scenes"
+REF_GLBS = sorted(glob.glob(os.path.join(REF_SCENES, "*.glb"))) if os.path.isdir(REF_SCENES) else []
+
+
+def _compare(got, want, jpeg_tol=False):
+    assert len(got["meshes"]) == len(want["meshes"])
+    for (p, n, t, i), (P, N, T, I) in zip(got["meshes"], want["meshes"]):
+        assert np.array_equal(p, P) and np.array_equal(i, I)
+        assert (n is None) == (N is None) and (t is None) == (T is None)
+        if n is not None:
+            np.testing.assert_allclose(n, N, rtol=0, atol=2e-7)
+        if t is not None:
+            assert np.array_equal(t, T)
+    assert len(got["instances"]) == len(want["instances"])
+    for (m, mat, x), (M, MAT, X) in zip(got["instances"], want["instances"]):
+        assert m == M and mat == MAT
+        np.testing.assert_allclose(x, X, rtol=1e-6, atol=1e-6 * max(1.0, np.abs(X).max()))
+    assert len(got["materials"]) == len(want["materials"])
+    for a, b in zip(got["materials"], want["materials"]):
+        for k, v in b.items():
+            if k.endswith("_xf"):
+                for f in ("offset", "scale"):
+                    assert np.array_equal(np.float32(a[k][f]), np.float32(v[f])), (k, f)
+                assert np.float32(a[k]["rotation"]) == np.float32(v["rotation"])
+            elif isinstance(v, tuple):
+                assert np.array_equal(np.float32(a[k]), np.float32(v)), k
+            else:
+                assert np.float32(a[k]) == np.float32(v), (k, a[k], v)
+    assert len(got["textures"]) == len(want["textures"])
+    for (a, bits), (b, bbits, mime) in zip(got["textures"], want["textures"]):
+        assert bits == bbits and a.shape == b.shape
+        if mime == "image/jpeg":
+            d = np.abs(a.astype(np.int32) - b.astype(np.int32))
+            assert d.mean() <= 1.0 and d.max() <= 24, (d.mean(), d.max())
+        else:
+            assert np.array_equal(a, b)
+    ca, cb = got["camera"], want["camera"]
+    assert (ca["width"], ca["height"]) == (cb["width"], cb["height"])
+    for k in ("position", "view", "up", "right"):
+        np.testing.assert_allclose(np.asarray(ca[k], np.float64), np.asarray(cb[k], np.float64), rtol=0, atol=4e-7)
+    np.testing.assert_allclose(ca["pixel_length"], cb["pixel_length"], rtol=1e-6)
+    assert got["trace_depth"] == 8
+
+
+@pytest.mark.parametrize("path", FIXTURES, ids=os.path.basename)
+def test_committed_scene_matches_oracle(path):
+    _compare(lx.parse_glb(path), gltf_ref.load(path))
+
+
+@pytest.mark.skipif(not REF_GLBS, reason="reference checkout not present")
+@pytest.mark.parametrize("path", REF_GLBS, ids=os.path.basename)
+def test_reference_scene_matches_oracle(path):
+    if os.path.getsize(path) > 16 << 20:
+        pytest.skip("large scene: covered by the smaller ones sharing its image formats")
+    _compare(lx.parse_glb(path), gltf_ref.load(path))
+
+
+def test_cornell_contents_and_defaults():
+    """Closed facts of cornell.glb: seven materials (the default red one
+    first), one camera at 1000x1000, emitter with emissive strength."""
+    s = lx.parse_glb(os.path.join(HERE, "golden", "scenes", "cornell.glb"))
+    assert s["materials"][0]["base_color"] == (1.0, 0.0, 0.0)
+    assert s["materials"][0]["ior"] == 0.0 and s["materials"][0]["alpha"] == 1.0
+    assert (s["camera"]["width"], s["camera"]["height"]) == (1000, 1000)
+    emissive = [m for m in s["materials"][1:] if max(m["emission_color"]) * m["emission_strength"] > 1e-5]
+    assert len(emissive) >= 1
+    assert all(m["ior"] == np.float32(1.55) for m in s["materials"][1:])
+    up = np.asarray(s["camera"]["up"])
+    r = np.asarray(s["camera"]["right"])
+    assert abs(r[1]) < 1e-6  # right is horizontal: re-derived against world +Y
+    assert abs(np.dot(up, r)) < 1e-6
+
+
+@pytest.mark.parametrize("mode,kw", [("RGB", {}), ("RGBA", {}), ("L", {}), ("LA", {}), ("P", {}),
+                                     ("P", {"transparency": 3}), ("1", {}), ("RGB", {"optimize": True}),
+                                     ("RGBA", {"interlace": 1}), ("L", {"bits": 4}), ("L", {"bits": 2}),
+                                     ("I;16", {})])
+def test_png_variants_exact(tmp_path, mode, kw):
+    from PIL import Image
+    rng = np.random.default_rng(hash(mode) % 1000)
+    h, w = 37, 53
+    base = rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
+    if mode == "I;16":
+        im = Image.fromarray(rng.integers(0, 65536, (h, w), dtype=np.uint16))
+    elif mode == "P":
+        im = Image.fromarray(base[..., :3], "RGB").quantize(16)
+    elif mode == "1":
+        im = Image.fromarray(base[..., 0] > 127)
+    else:
+        im = Image.fromarray(base, "RGBA").convert(mode)
+    png = gltf_util.png_bytes(im, **kw)
+    p = str(tmp_path / "t.glb")
+    gltf_util.write_glb(p, [png])
+    got = lx.parse_glb(p)
+    want = gltf_ref.load(p)
+    _compare(got, want)
+
+
+@pytest.mark.parametrize("sub", [0, 1, 2])
+def test_jpeg_baseline_close_to_libjpeg(tmp_path, sub):
+    from PIL import Image
+    y, x = np.mgrid[0:96, 0:120]
+    img = np.stack([128 + 100 * np.sin(x / 9.0), 128 + 90 * np.cos(y / 7.0), (x * y) % 256], -1)
+    im = Image.fromarray(np.clip(img, 0, 255).astype(np.uint8), "RGB")
+    jpg = gltf_util.jpeg_bytes(im, quality=92, subsampling=sub)
+    p = str(tmp_path / "j.glb")
+    gltf_util.write_glb(p, [jpg], mime="image/jpeg")
+    _compare(lx.parse_glb(p), gltf_ref.load(p), jpeg_tol=True)
+    gray = gltf_util.jpeg_bytes(im.convert("L"), quality=85)
+    gltf_util.write_glb(p, [gray], mime="image/jpeg")
+    _compare(lx.parse_glb(p), gltf_ref.load(p), jpeg_tol=True)
+
+
+def test_node_transforms_and_shared_meshes(tmp_path):
+    p = str(tmp_path / "x.glb")
+    gltf_util.write_glb(p, extra_materials=[{"pbrMetallicRoughness": {"baseColorFactor": [0.2, 0.3, 0.4, 0.5]},
+                                             "alphaMode": "BLEND", "extensions": {
+                                                 "KHR_materials_ior": {"ior": 1.3},
+                                                 "KHR_materials_transmission": {"transmissionFactor": 0.7},
+                                                 "KHR_materials_emissive_strength": {"emissiveStrength": 4.0}},
+                                             "emissiveFactor": [1, 0.5, 0.25]}] * 2,
+                        node_xf={"rotation": [0.1826, 0.3651, 0.5477, 0.7303], "scale": [1.0, 2.0, 0.5]},
+                        camera_rot=(-0.3826834, 0.0, 0.0, 0.9238795))
+    got, want = lx.parse_glb(p), gltf_ref.load(p)
+    _compare(got, want)
+    m = got["materials"][1]
+    assert m["alpha_mode"] == 2 and np.float32(m["alpha"]) == np.float32(0.5)
+    assert m["emission_strength"] == 4.0 and np.float32(m["ior"]) == np.float32(1.3)
+
+
+def test_errors_are_statuses(tmp_path):
+    bad = tmp_path / "bad.glb"
+    bad.write_bytes(b"not a glb file at all")
+    with pytest.raises(lx.LuxionError, match="binary glTF"):
+        lx.parse_glb(str(bad))
+    with pytest.raises(lx.LuxionError):
+        lx.parse_glb(str(tmp_path / "missing.glb"))
+    p = str(tmp_path / "ok.glb")
+    gltf_util.write_glb(p)
+    with pytest.raises(lx.LuxionError, match="cannot read"):
+        lx.parse_glb(p, str(tmp_path / "missing.exr"))
+
+
+@pytest.mark.parametrize("comp", [0, 1, 2, 3])
+@pytest.mark.parametrize("chans", ["RGB", "RGBA", "Y"])
+def test_exr_scanline_codecs_exact(tmp_path, comp, chans):
+    rng = np.random.default_rng(comp * 7 + len(chans))
+    H, W = 41, 67
+    planes = {}
+    for c in chans:
+        v = np.exp(rng.normal(0, 2, (H, W))).astype(np.float32)
+        v[5:9, :] = 1.25  # runs for RLE
+        planes[c] = v
+    exr = str(tmp_path / "e.exr")
+    gltf_util.write_exr(exr, planes, comp)
+    p = str(tmp_path / "s.glb")
+    gltf_util.write_glb(p)
+    got = lx.parse_glb(p, exr, env_head=H * W * 4)
+    w, h, rgba = got["env"]
+    assert (w, h) == (W, H)
+    rgba = rgba.reshape(H, W, 4)
+    half = {c: planes[c].astype(np.float16).astype(np.float32) for c in chans}
+    if chans == "Y":
+        for k in range(4):
+            assert np.array_equal(rgba[..., k], half["Y"])
+    else:
+        for k, c in enumerate("RGB"):
+            assert np.array_equal(rgba[..., k], half[c])
+        assert np.array_equal(rgba[..., 3], half["A"] if "A" in half else np.ones((H, W), np.float32))
+    if comp in (0, 2, 3):
+        assert np.array_equal(rgba, gltf_ref.load_exr_zip(exr))
+
+
+def test_white_furnace_exr_matches_oracle():
+    exr = os.path.join(HERE, "golden", "scenes", "white_furnace.exr")
+    glb = os.path.join(HERE, "golden", "scenes", "furnace_test.glb")
+    want = gltf_ref.load_exr_zip(exr)
+    got = lx.parse_glb(glb, exr, env_head=want.size)
+    assert (got["env"][0], got["env"][1]) == (want.shape[1], want.shape[0])
+    assert np.array_equal(got["env"][2].reshape(want.shape), want)
+
+
+@pytest.mark.skipif(not os.path.isdir(os.path.join(REF_SCENES, "exr")), reason="reference checkout not present")
+def test_piz_exr_invariants():
+    """PIZ (4096 x 2048, HALF): finite, non-negative radiance, alpha 1, and no
+    seams at the 32-line block boundaries (a wrong wavelet or Huffman step
+    breaks continuity there)."""
+    exr = os.path.join(REF_SCENES, "exr", "studio_small_09_4k.exr")
+    glb = os.path.join(HERE, "golden", "scenes", "furnace_test.glb")
+    got = lx.parse_glb(glb, exr, env_head=4096 * 2048 * 4)
+    w, h, rgba = got["env"]
+    img = rgba.reshape(h, w, 4)
+    assert (w, h) == (4096, 2048)
+    assert np.isfinite(img).all() and (img[..., :3] >= 0).all() and (img[..., 3] == 1).all()
+    lum = np.log1p(img[..., :3].mean(-1).astype(np.float64))
+    dy = np.abs(np.diff(lum, axis=0)).mean(1)  # mean step between consecutive lines
+    at_seams = dy[31::32]
+    inside = np.delete(dy, np.arange(31, len(dy), 32))
+    assert at_seams.mean() < 3.0 * inside.mean() + 1e-3
+    assert img[..., :3].max() > 1.0  # an HDR capture
