@@ -99,6 +99,9 @@ extern "C" lx_status lx_batch_run(lx_context* ctx, const lx_job* jobs, uint32_t 
         double t_submit = now_ms();
         for (uint32_t i = 0; i < n; i++) lx_validate_job(jobs[i]);
         int width = jobs[0].scene->dev.bvh_width;
+        const LxDevScene* uscene = jobs[0].scene->d_scene.as<LxDevScene>();
+        for (uint32_t i = 1; i < n; i++)
+            if (jobs[i].scene != jobs[0].scene) uscene = nullptr;
         for (uint32_t i = 0; i < n; i++)
             LX_REQUIRE(jobs[i].scene->dev.bvh_width == width, LX_ERR_INVALID,
                        "every job of a batch must reference scenes built at one BVH width");
@@ -149,7 +152,7 @@ extern "C" lx_status lx_batch_run(lx_context* ctx, const lx_job* jobs, uint32_t 
             try {
                 lx_enqueue_round(ctx, d_jobs.as<LxJob>(), d_spp.as<int32_t>(), n, d_cnt.as<LxCounters>(),
                                  d_sched.as<LxSched>(), cap, wb, d_accum.as<float>(), d_spill.as<uint32_t>(), max_depth,
-                                 tg, sg, cg, width, nullptr);
+                                 tg, sg, cg, width, uscene, nullptr);
             } catch (...) {
                 hipGraph_t tmp;
                 (void)hipStreamEndCapture(s, &tmp);
@@ -167,7 +170,7 @@ extern "C" lx_status lx_batch_run(lx_context* ctx, const lx_job* jobs, uint32_t 
             else
                 lx_enqueue_round(ctx, d_jobs.as<LxJob>(), d_spp.as<int32_t>(), n, d_cnt.as<LxCounters>(),
                                  d_sched.as<LxSched>(), cap, wb, d_accum.as<float>(), d_spill.as<uint32_t>(), max_depth,
-                                 tg, sg, cg, width, nullptr);
+                                 tg, sg, cg, width, uscene, nullptr);
             LX_HIP(hipEventRecord(ev[k + 1], s));
         }
         LX_HIP(hipStreamSynchronize(s));

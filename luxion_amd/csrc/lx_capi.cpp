@@ -361,7 +361,8 @@ static void upload_job(lx_render* r, uint32_t first_iter, uint32_t n) {
 static void enqueue(lx_render* r, hipEvent_t* ev) {
     lx_enqueue_round(r->ctx, r->jobs.as<LxJob>(), r->job_spp.as<int32_t>(), 1, r->counters.as<LxCounters>(),
                      r->sched.as<LxSched>(), r->npix, r->wb, r->accum.as<float>(), r->spill.as<uint32_t>(), r->depth,
-                     r->trace_grid, r->shade_grid, r->cam_grid, r->scene->dev.bvh_width, ev, r->counting);
+                     r->trace_grid, r->shade_grid, r->cam_grid, r->scene->dev.bvh_width,
+                     r->scene->d_scene.as<LxDevScene>(), ev, r->counting);
 }
 
 lx_status lx_render_iterate(lx_render* r, uint32_t first_iter, uint32_t n) {

@@ -231,6 +231,7 @@ void lx_enqueue_round(lx_context* ctx, LxJob* d_jobs, const int32_t* d_spp, uint
                       LxSched* d_sched, uint32_t path_cap, WaveBufs& wb, float* d_accum, uint32_t* d_spill,
                       int max_depth, uint32_t trace_grid, uint32_t shade_grid, uint32_t cam_grid,
                       int width /* BVH width of every referenced scene */,
+                      const LxDevScene* uscene /* device scene shared by every job, or NULL */,
                       hipEvent_t* ev /* 2*max_depth+1 events or NULL */, bool count = false);
 uint32_t lx_spill_words(uint32_t trace_grid);
 /* bounces launched one kernel each before the tail kernel takes over

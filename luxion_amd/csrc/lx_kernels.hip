@@ -318,10 +318,13 @@ __global__ __launch_bounds__(LX_SHADE_BLOCK) void k_shade(const LxJob* __restric
 /* ------------------------------------------------------------ tail */
 /* BLOCK threads per workgroup; OCC > 0 asks the register allocator for at
  * least OCC waves per SIMD (spilling to scratch if it must). */
-template <bool CNT, int BLOCK, int OCC, int W>
+/* UNI: every job of the launch renders the scene at `uscene`, which then
+ * lives in scalar registers instead of one copy per lane. */
+template <bool CNT, int BLOCK, int OCC, int W, bool UNI>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1)))
 void k_finish(const LxJob* __restrict__ jobs, LxPathBuf P, LxCounters* __restrict__ C, int bounce0,
-              uint32_t* __restrict__ spill, float* __restrict__ accum, uint32_t chunk) {
+              uint32_t* __restrict__ spill, float* __restrict__ accum, uint32_t chunk,
+              const LxDevScene* __restrict__ uscene) {
     __shared__ uint32_t s_stack[LX_STACK * BLOCK];
     uint32_t n = C->count[bounce0];
     uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -341,16 +344,20 @@ void k_finish(const LxJob* __restrict__ jobs, LxPathBuf P, LxCounters* __restric
     f3 o = mk3(0, 0, 0), d = mk3(0, 0, 0), thr = mk3(0, 0, 0), col = mk3(0, 0, 0);
     float pdf = 0.0f;
     bool spec = false;
+    /* the environment shadow ray of the last shaded bounce, traced together
+     * with the next radiance ray; its colour term waits in pend_col */
+    bool pend = false, pend_add = false, flush = false;
+    f3 pend_o = mk3(0, 0, 0), pend_d = mk3(0, 0, 0), pend_col = mk3(0, 0, 0);
     while (true) {
         /* refill idle lanes from the live-path queue, one atomic per wave,
          * once at least `chunk` lanes are idle (or all of them are) */
-        uint64_t idle = __ballot(!have);
+        uint64_t idle = __ballot(!have && !flush);
         if (!exhausted && idle && ((uint32_t)__popcll(idle) >= chunk || idle == ~0ull)) {
             uint32_t base = 0;
             if (lane == 0) base = atomicAdd(&C->tail_next, (uint32_t)__popcll(idle));
             base = __shfl(base, 0, LX_WAVE);
             if (base >= n) exhausted = true;
-            if (!have) {
+            if (!have && !flush) {
                 uint32_t my = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
                 if (my < n) {
                     job = P.job[my];
@@ -367,21 +374,40 @@ void k_finish(const LxJob* __restrict__ jobs, LxPathBuf P, LxCounters* __restric
                 }
             }
         }
-        if (__ballot(have) == 0ull) break;
-        if (have) {
+        if (__ballot(have || flush) == 0ull) break;
+        if (have || flush) {
             const LxJob& J = jobs[job];
-            const LxDevScene& S = *J.scene;
+            const LxDevScene& S = UNI ? *uscene : *J.scene;
             LxRec r;
             LxRecSink sink{r};
-            lx_trace_path<CNT, W>(J, S, o, d, pix, b, st, &C->stack_overflow, sink, rc, &wc, &wa);
-            int depth = b + 1;
-            bool cont = lx_shade_path(J, S, r, depth, depth < J.depth, o, d, thr, col, pdf, spec, pix);
-            nsh++;
-            if (cont) {
-                b++;
-            } else {
+            bool occluded = false;
+            lx_trace_path_d<CNT, W>(J, S, o, d, pix, b, st, &C->stack_overflow, pend, pend_o, pend_d, have, occluded,
+                                    sink, rc, &wc, &wa);
+            if (pend) {
+                if (!occluded && pend_add) col = col + pend_col;
+                pend = false;
+            }
+            if (flush) {
                 lx_accumulate(accum, J, pix, col);
-                have = false;
+                flush = false;
+            } else {
+                int depth = b + 1;
+                bool cont = lx_shade_path<true>(J, S, r, depth, depth < J.depth, o, d, thr, col, pdf, spec, pix,
+                                                &pend_add, &pend_col);
+                nsh++;
+                if (J.emis && r.t > 0.0f) {
+                    /* this bounce's environment shadow ray, traced next time */
+                    pend = true;
+                    pend_o = r.so;
+                    pend_d = r.ed;
+                }
+                if (cont) {
+                    b++;
+                } else {
+                    have = false;
+                    if (pend) flush = true;
+                    else lx_accumulate(accum, J, pix, col);
+                }
             }
         }
     }
@@ -534,35 +560,40 @@ hipError_t lx_launch_shade(const LxJob* jobs, const LxPathBuf& P, const LxPathBu
     hipLaunchKernelGGL(k_shade, dim3(grid), dim3(LX_SHADE_BLOCK), 0, s, jobs, P, Q, R, C, bounce, accum);
     return hipGetLastError();
 }
-template <int BLOCK, int OCC, int W>
+template <int BLOCK, int OCC, int W, bool UNI>
 static void lx_finish_at(const LxJob* jobs, const LxPathBuf& P, LxCounters* C, int bounce0, uint32_t* spill,
-                         float* accum, uint32_t grid, uint32_t chunk, hipStream_t s) {
-    hipLaunchKernelGGL((k_finish<false, BLOCK, OCC, W>), dim3(grid), dim3(BLOCK), 0, s, jobs, P, C, bounce0, spill,
-                       accum, chunk);
+                         float* accum, uint32_t grid, uint32_t chunk, const LxDevScene* us, hipStream_t s) {
+    hipLaunchKernelGGL((k_finish<false, BLOCK, OCC, W, UNI>), dim3(grid), dim3(BLOCK), 0, s, jobs, P, C, bounce0,
+                       spill, accum, chunk, us);
 }
-template <int W>
+template <int W, bool UNI>
 static void lx_finish_w(const LxJob* jobs, const LxPathBuf& P, LxCounters* C, int bounce0, uint32_t* spill,
-                        float* accum, uint32_t threads, int block, int occ, uint32_t chunk, hipStream_t s,
-                        bool count) {
+                        float* accum, uint32_t threads, int block, int occ, uint32_t chunk, const LxDevScene* us,
+                        hipStream_t s, bool count) {
     if (count) {
-        hipLaunchKernelGGL((k_finish<true, 128, 0, W>), dim3(threads / 128), dim3(128), 0, s, jobs, P, C, bounce0,
-                           spill, accum, chunk);
-    } else if (block == 256 && threads >= 256) {
-        if (occ >= 4) lx_finish_at<256, 4, W>(jobs, P, C, bounce0, spill, accum, threads / 256, chunk, s);
-        else lx_finish_at<256, 0, W>(jobs, P, C, bounce0, spill, accum, threads / 256, chunk, s);
+        hipLaunchKernelGGL((k_finish<true, 128, 0, W, UNI>), dim3(threads / 128), dim3(128), 0, s, jobs, P, C,
+                           bounce0, spill, accum, chunk, us);
+    } else if (occ >= 4) {
+        if (block == 256 && threads >= 256)
+            lx_finish_at<256, 4, W, UNI>(jobs, P, C, bounce0, spill, accum, threads / 256, chunk, us, s);
+        else lx_finish_at<128, 4, W, UNI>(jobs, P, C, bounce0, spill, accum, threads / 128, chunk, us, s);
     } else {
-        if (occ >= 4) lx_finish_at<128, 4, W>(jobs, P, C, bounce0, spill, accum, threads / 128, chunk, s);
-        else lx_finish_at<128, 0, W>(jobs, P, C, bounce0, spill, accum, threads / 128, chunk, s);
+        lx_finish_at<128, 0, W, UNI>(jobs, P, C, bounce0, spill, accum, threads / 128, chunk, us, s);
     }
 }
 hipError_t lx_launch_finish(const LxJob* jobs, const LxPathBuf& P, LxCounters* C, int bounce0, uint32_t* spill,
-                            float* accum, uint32_t threads, int block, int occ, int width, int chunk, hipStream_t s,
-                            bool count) {
+                            float* accum, uint32_t threads, int block, int occ, int width, int chunk,
+                            const LxDevScene* uscene, hipStream_t s, bool count) {
     /* threads = grid * block lanes, the same for every shape, so one spill
      * allocation serves them all */
     uint32_t c = chunk < 1 ? 1u : (chunk > 64 ? 64u : (uint32_t)chunk);
-    if (width == 8) lx_finish_w<8>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, s, count);
-    else lx_finish_w<4>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, s, count);
+    if (uscene) {
+        if (width == 8) lx_finish_w<8, true>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, uscene, s, count);
+        else lx_finish_w<4, true>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, uscene, s, count);
+    } else {
+        if (width == 8) lx_finish_w<8, false>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, nullptr, s, count);
+        else lx_finish_w<4, false>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, nullptr, s, count);
+    }
     return hipGetLastError();
 }
 hipError_t lx_launch_trace_rays(const LxDevScene* S, const lx_ray* rays, uint32_t n, int any, int alpha,

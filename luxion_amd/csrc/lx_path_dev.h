@@ -27,6 +27,7 @@ struct LxRec {
     f2 dluv;
     int dlmat;
     f3 lp;
+    f3 so; /* origin of the environment shadow ray (hit point + 1e-4 N) */
 };
 
 struct LxRayCount {
@@ -56,6 +57,7 @@ struct LxRecSink {
     LXD void light(f3 lp) { r.lp = lp; }
     LXD void env_dir(f3 ed) { r.ed = ed; }
     LXD void env(float env_t) { r.env_t = env_t; }
+    LXD void env_origin(f3 so) { r.so = so; }
 };
 
 template <bool CNT, int W, class Sink>
@@ -112,10 +114,76 @@ LXD void lx_trace_path(const LxJob& J, const LxDevScene& S, f3 o, f3 d, uint32_t
     }
 }
 
+/* The tail kernel's form of lx_trace_path: the environment shadow ray of
+ * the previous bounce (pend_*; its result goes to pend_occluded) is traced in
+ * the same traversal loop as this bounce's radiance ray, and this bounce's
+ * environment shadow ray is not traced here but left in the record (so, ed)
+ * for the next call.  has_rad false traces only the pending ray (a path that
+ * ended with one outstanding).  The direct-light ray follows as before. */
+template <bool CNT, int W>
+LXD void lx_trace_path_d(const LxJob& J, const LxDevScene& S, f3 o, f3 d, uint32_t pix, int bounce, LxStack& st,
+                         uint32_t* overflow, bool pend, f3 pend_o, f3 pend_d, bool has_rad, bool& pend_occluded,
+                         LxRecSink& out, LxRayCount& rc, LxWork* wc, LxWork* wa) {
+    f3 lp = mk3(0, 0, 0), ed = mk3(0, 0, 0);
+    if (has_rad && (J.dmis || J.emis)) {
+        /* sampleDirectLight: RNG keyed on the pre-increment depth */
+        LxRng rng = lx_make_rng(J.iter, (int)pix, bounce, J.seed);
+        if (J.dmis) lp = lx_sample_light_point(S, rng);
+        if (J.emis) ed = lx_sample_env_dir(S, rng);
+    }
+    LxHitD h = lx_trace_pair<CNT, W>(S, pend, pend_o, pend_d, 1e16f, has_rad, o, d, 1e16f, pix, (uint32_t)J.iter,
+                                      LX_INST_ALPHA, st, overflow, pend_occluded, wc, wa);
+    if (pend) rc.env++;
+    if (!has_rad) return;
+    rc.rad++;
+    if (J.dmis) out.light(lp);
+    if (J.emis) out.env_dir(ed);
+    if (h.inst != 0xffffffffu) {
+        bool want_t = S.mats[S.inst[h.inst].material].normal_tex >= 0;
+        f3 N, Tg;
+        f2 uv;
+        int mat;
+        lx_hit_attrs(S, h, want_t, N, Tg, uv, mat);
+        out.hit(h.t, N, uv, mat, want_t, Tg);
+        f3 hp = o + (h.t * d);
+        f3 so = hp + (N * 1e-4f);
+        if (J.dmis) {
+            f3 ld = lp - so;
+            float dist = length(ld);
+            ld = ld * (1.0f / dist);
+            LxHitD hd = lx_trace<false, CNT, W>(S, so, ld, dist * 1.0001f + 1e-4f, pix, (uint32_t)J.iter,
+                                                LX_INST_ALPHA, st, overflow, wc);
+            rc.dl++;
+            if (hd.inst != 0xffffffffu) {
+                f3 dN;
+                f2 duv;
+                int dmat;
+                lx_dl_attrs(S, hd, dN, duv, dmat);
+                out.dl(hd.t, dN, duv, dmat);
+            } else {
+                out.dl_miss();
+            }
+        }
+        out.env_origin(so);
+        out.env(1.0f);
+    } else {
+        out.miss();
+        if (J.dmis) out.dl_miss();
+        out.env(-0.5f);
+    }
+}
+
 /* Updates the path registers; returns true if the path continues to the
- * next bounce (compaction push), false when it stops. */
+ * next bounce (compaction push), false when it stops.  DEFER: the
+ * environment-light term is not added here but returned in pend_add /
+ * pend_col, to be added once the environment shadow ray (traced later) is
+ * known to be unoccluded -- before anything else is added to the colour, so
+ * the sums are the same. */
+template <bool DEFER = false>
 LXD bool lx_shade_path(const LxJob& J, const LxDevScene& S, const LxRec& R, int depth, bool compact, f3& o, f3& d,
-                       f3& thr, f3& col, float& last_pdf, bool& spec, uint32_t pix) {
+                       f3& thr, f3& col, float& last_pdf, bool& spec, uint32_t pix, bool* pend_add = nullptr,
+                       f3* pend_col = nullptr) {
+    if (DEFER) *pend_add = false;
     LxRng rng = lx_make_rng(J.iter, (int)pix, depth, J.seed);
     bool kill = false;
     float t = R.t;
@@ -146,7 +214,7 @@ LXD bool lx_shade_path(const LxJob& J, const LxDevScene& S, const LxRec& R, int 
                 }
                 col = col + thr * (w * emission);
                 kill = true;
-            } else if (R.env_t < 0.0f && !is_specular && has_exr) {
+            } else if ((DEFER || R.env_t < 0.0f) && !is_specular && has_exr) {
                 f3 ed = R.ed;
                 float pe = lx_envmap_pdf(S, ed);
                 float pb = lx_pdf(-d, ed, normal, rough, metal, color, mat.ior, mat.transmission);
@@ -156,7 +224,12 @@ LXD bool lx_shade_path(const LxJob& J, const LxDevScene& S, const LxRec& R, int 
                 float cs = dot(nf, ed);
                 if (cs > 0.0f && pe > 1e-8f) {
                     f3 contrib = ((L * brdf) * cs) / pe;
-                    col = col + (thr * contrib) * w;
+                    if (DEFER) {
+                        *pend_add = true;
+                        *pend_col = (thr * contrib) * w;
+                    } else {
+                        col = col + (thr * contrib) * w;
+                    }
                 }
             }
         } else if (J.dmis) {

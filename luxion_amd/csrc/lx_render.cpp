@@ -88,7 +88,7 @@ void WaveBufs::alloc(lx_context* ctx, uint32_t c, bool dmis, bool emis, bool tan
 void lx_enqueue_round(lx_context* ctx, LxJob* d_jobs, const int32_t* d_spp, uint32_t njobs, LxCounters* d_cnt,
                       LxSched* d_sched, uint32_t path_cap, WaveBufs& wb, float* d_accum, uint32_t* d_spill,
                       int max_depth, uint32_t trace_grid, uint32_t shade_grid, uint32_t cam_grid, int width,
-                      hipEvent_t* ev, bool count) {
+                      const LxDevScene* uscene, hipEvent_t* ev, bool count) {
     hipStream_t s = ctx->stream;
     int wf = lx_wavefront_bounces(ctx, max_depth);
     LX_HIP(lx_launch_begin_round(d_jobs, d_spp, njobs, d_cnt, d_sched, path_cap, s));
@@ -108,7 +108,7 @@ void lx_enqueue_round(lx_context* ctx, LxJob* d_jobs, const int32_t* d_spp, uint
         if (ev) LX_HIP(hipEventRecord(ev[2 * wf], s));
         LX_HIP(lx_launch_finish(d_jobs, *cur, d_cnt, wf, d_spill, d_accum, trace_grid * LX_TRACE_BLOCK,
                                 ctx->tuning.block_size, ctx->tuning.tail_min_waves, width, ctx->tuning.queue_chunk,
-                                s, count));
+                                uscene, s, count));
         if (ev)
             for (int k = 2 * wf + 1; k < 2 * max_depth; k++) LX_HIP(hipEventRecord(ev[k], s));
     }

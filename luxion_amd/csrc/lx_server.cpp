@@ -314,7 +314,8 @@ lx_status lx_server_start(lx_context* ctx, const lx_server_opts* o, lx_server** 
         try {
             lx_enqueue_round(ctx, S->d_jobs.as<LxJob>(), S->d_spp.as<int32_t>(), S->slots, S->d_cnt.as<LxCounters>(),
                              S->d_sched.as<LxSched>(), S->cap, S->wb, S->d_accum.as<float>(),
-                             S->d_spill.as<uint32_t>(), S->max_depth, S->tg, S->sg, S->cg, S->width, nullptr);
+                             S->d_spill.as<uint32_t>(), S->max_depth, S->tg, S->sg, S->cg, S->width, nullptr,
+                             nullptr);
         } catch (...) {
             hipGraph_t tmp;
             (void)hipStreamEndCapture(s, &tmp);

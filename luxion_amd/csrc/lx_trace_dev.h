@@ -301,6 +301,151 @@ LXD LxHitD lx_trace(const LxDevScene& S, f3 wo, f3 wd, float tmax, uint32_t key,
     return best;
 }
 
+/* Two rays of one lane in one traversal loop: first (when has0) an any-hit
+ * ray (o0, d0, tmax0), then (when has1) a closest-hit ray (o1, d1, tmax1);
+ * at least one of them must be present.  A lane that
+ * finishes its first ray starts the second at once instead of idling until
+ * every lane of its wave has finished its first ray.  Each ray's result is
+ * what lx_trace returns for it alone (the candidate set does not depend on
+ * traversal order). */
+template <bool CNT, int W>
+LXD LxHitD lx_trace_pair(const LxDevScene& S, bool has0, f3 o0, f3 d0, float tmax0, bool has1, f3 o1, f3 d1,
+                         float tmax1, uint32_t key, uint32_t iteration, uint32_t alpha_mask, LxStack& st,
+                         uint32_t* overflow, bool& occluded0, LxWork* wclosest, LxWork* wany) {
+    LxHitD best;
+    typedef typename LxNodeT<W>::type Node;
+    const Node* __restrict__ nodes = (const Node*)S.nodes;
+    bool any = has0;
+    f3 wo = any ? o0 : o1, wd = any ? d0 : d1;
+    float tmax = any ? tmax0 : tmax1;
+    f3 iwd, o, d, ix;
+    bool in_blas;
+    uint32_t cur_inst = 0, inst_flags = 0, node;
+    occluded0 = false;
+    /* (re)start the lane on its current ray */
+    auto begin = [&]() {
+        best.t = __builtin_inff();
+        best.inst = 0xffffffffu;
+        best.prim = 0xffffffffu;
+        best.u = best.v = 0.0f;
+        iwd = mk3(1.0f / wd.x, 1.0f / wd.y, 1.0f / wd.z);
+        o = wo;
+        d = wd;
+        ix = iwd;
+        in_blas = false;
+        st.sp = 0;
+        node = S.tlas_root;
+    };
+    begin();
+    while (true) {
+        LxWork* work = any ? wany : wclosest;
+        bool do_pop = false, found_any = false;
+        if (node == LX_SENTINEL) {
+            o = wo;
+            d = wd;
+            ix = iwd;
+            in_blas = false;
+            do_pop = true;
+        } else if (!(node & LX_LEAF_BIT)) {
+            float tcut = (best.inst == 0xffffffffu ? tmax : best.t) * LX_CULL_MARGIN;
+            float tn[W];
+            uint32_t cs[W];
+            uint32_t m = lx_boxw<W>(nodes + node, o, ix, tcut, tn, cs);
+            if (CNT) work->nodes++;
+            if (m == 0u) {
+                do_pop = true;
+            } else {
+                float bt = __builtin_inff();
+                int bk = -1;
+#pragma unroll
+                for (int k = 0; k < W; k++)
+                    if ((m >> k) & 1u)
+                        if (tn[k] < bt) {
+                            bt = tn[k];
+                            bk = k;
+                        }
+                m &= ~(1u << bk);
+                while (m) {
+                    float ft = -1.0f;
+                    int fk = -1;
+#pragma unroll
+                    for (int k = 0; k < W; k++)
+                        if ((m >> k) & 1u)
+                            if (tn[k] > ft) {
+                                ft = tn[k];
+                                fk = k;
+                            }
+                    m &= ~(1u << fk);
+                    if (st.sp >= LX_STACK + LX_SPILL) {
+                        atomicOr(overflow, 1u);
+                        continue;
+                    }
+                    st.push(cs[fk]);
+                }
+                node = cs[bk];
+                continue;
+            }
+        } else if (!in_blas) {
+            cur_inst = node & LX_LEAF_FIRST_MASK;
+            const LxInstance& I = S.inst[cur_inst];
+            o = xf_point(I.inv, wo);
+            d = xf_vector(I.inv, wd);
+            ix = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+            inst_flags = I.flags & alpha_mask;
+            in_blas = true;
+            if (st.sp >= LX_STACK + LX_SPILL) {
+                atomicOr(overflow, 1u);
+                do_pop = true;
+            } else {
+                st.push(LX_SENTINEL);
+                if (CNT) work->insts++;
+                node = I.blas_root;
+                continue;
+            }
+        } else {
+            uint32_t first = node & LX_LEAF_FIRST_MASK;
+            uint32_t cnt = ((node >> 28) & 7u) + 1u;
+            for (uint32_t k = 0; k < cnt; k++) {
+                const float4* tp = S.tris + 3u * (size_t)(first + k);
+                float4 A = tp[0], E1 = tp[1], E2 = tp[2];
+                float t, u, v;
+                if (CNT) work->tris++;
+                if (!lx_mt(o, d, A, E1, E2, tmax, t, u, v)) continue;
+                uint32_t prim = lxm_f2u(A.w);
+                if (!any && !lx_better(t, cur_inst, prim, best)) continue;
+                if ((inst_flags & LX_INST_ALPHA) &&
+                    !lx_any_hit_accept(S, S.inst[cur_inst], cur_inst, prim, u, v, wd, key, iteration))
+                    continue;
+                best.t = t;
+                best.inst = cur_inst;
+                best.prim = prim;
+                best.u = u;
+                best.v = v;
+                if (any) {
+                    found_any = true;
+                    break;
+                }
+            }
+            do_pop = true;
+        }
+        bool ray_done = found_any || (do_pop && st.sp == 0);
+        if (ray_done) {
+            if (!any) break;
+            /* first ray finished: record it and start the second */
+            occluded0 = best.inst != 0xffffffffu;
+            if (!has1) break;
+            any = false;
+            wo = o1;
+            wd = d1;
+            tmax = tmax1;
+            begin();
+            continue;
+        }
+        if (do_pop) node = st.pop();
+    }
+    return best;
+}
+
 /* __closesthit__ch, optix_triangle.cu:156-243: interpolated normal (not
  * flipped toward the ray), uv, uv-derivative tangent, object -> world. */
 LXD void lx_hit_attrs(const LxDevScene& S, const LxHitD& h, bool want_tangent, f3& N, f3& Tg, f2& uv, int& mat) {
