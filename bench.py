@@ -111,7 +111,7 @@ def main():
     st_graph = r.stats()
     log(f"graph replay: {args.steps} steps in {t_graph * 1e3:.1f} ms")
 
-    # timed region: same frames with a HIP event between consecutive kernels
+    # timed region: same frames with HIP events before and after each kernel
     # on the render's stream (per-kernel device time for the roofline)
     r.reset()
     r.set_timing(events=True)

@@ -63,7 +63,7 @@ lx_status lx_context_create(int device, uint64_t budget, lx_context** out) {
         size_t fr = 0, tot = 0;
         LX_HIP(hipMemGetInfo(&fr, &tot));
         c->budget = budget ? budget : (uint64_t)fr;
-        c->tuning.block_size = LX_TRACE_BLOCK;
+        c->tuning.block_size = 256;
         c->tuning.bvh_width = 4;
         c->tuning.queue_chunk = 32;
         c->tuning.waves_per_cu = 16;
@@ -375,8 +375,9 @@ lx_status lx_render_iterate(lx_render* r, uint32_t first_iter, uint32_t n) {
     ctx->set_device();
     upload_job(r, first_iter, n);
     if (r->timing || r->counting) {
-        /* eager launches; with timing, one event between consecutive kernels
-         * of every round, read back after a single synchronisation */
+        /* eager launches; with timing, an event before and after each
+         * kernel of every round (only those the stats read), read back
+         * after a single synchronisation */
         int per = 2 * r->depth + 1;
         std::vector<hipEvent_t> ev;
         if (r->timing) {

@@ -224,6 +224,7 @@ struct lx_render {
     uint32_t trace_launches = 0, shade_launches = 0, tail_launches = 0;
     hipGraphExec_t graph = nullptr;
     hipGraph_t graph_src = nullptr;
+
 };
 
 /* Records one round (begin, camera, bounces, end) on the stream. */
@@ -232,7 +233,10 @@ void lx_enqueue_round(lx_context* ctx, LxJob* d_jobs, const int32_t* d_spp, uint
                       int max_depth, uint32_t trace_grid, uint32_t shade_grid, uint32_t cam_grid,
                       int width /* BVH width of every referenced scene */,
                       const LxDevScene* uscene /* device scene shared by every job, or NULL */,
-                      hipEvent_t* ev /* 2*max_depth+1 events or NULL */, bool count = false);
+                      hipEvent_t* ev /* 2*max_depth+1 slots or NULL: [2b], [2b+1] bracket the trace and
+                                        shade of bounce b < wavefront_bounces, [2w], [2w+1] the tail kernel,
+                                        [2*max_depth] the round's end; other slots are not recorded */,
+                      bool count = false);
 uint32_t lx_spill_words(uint32_t trace_grid);
 /* bounces launched one kernel each before the tail kernel takes over
  * (tuning.wavefront_bounces; negative or above max_depth = all of them,

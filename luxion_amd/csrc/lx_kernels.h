@@ -41,8 +41,8 @@ hipError_t lx_launch_shade(const LxJob* jobs, const LxPathBuf& P, const LxPathBu
  * (1..64) are idle */
 hipError_t lx_launch_finish(const LxJob* jobs, const LxPathBuf& P, LxCounters* C, int bounce0, uint32_t* spill,
                             float* accum, uint32_t threads, int block, int occ, int width, int chunk,
-                            const LxDevScene* uscene /* the scene of every job, or NULL */, hipStream_t s,
-                            bool count = false);
+                            const LxDevScene* uscene /* the scene of every job, or NULL */,
+                            bool single_job /* njobs == 1 */, hipStream_t s, bool count = false);
 hipError_t lx_launch_trace_rays(const LxDevScene* S, const lx_ray* rays, uint32_t n, int any, int alpha,
                                 uint32_t iteration, lx_hit* hits, uint8_t* occ, uint32_t* spill, uint32_t* overflow,
                                 uint32_t grid, int width, hipStream_t s);

@@ -318,9 +318,10 @@ __global__ __launch_bounds__(LX_SHADE_BLOCK) void k_shade(const LxJob* __restric
 /* ------------------------------------------------------------ tail */
 /* BLOCK threads per workgroup; OCC > 0 asks the register allocator for at
  * least OCC waves per SIMD (spilling to scratch if it must). */
-/* UNI: every job of the launch renders the scene at `uscene`, which then
- * lives in scalar registers instead of one copy per lane. */
-template <bool CNT, int BLOCK, int OCC, int W, bool UNI>
+/* UNI 1: every job of the launch renders the scene at `uscene`, which then
+ * lives in scalar registers instead of one copy per lane; UNI 2: in
+ * addition the launch has a single job (jobs[0]), also kept in scalars. */
+template <bool CNT, int BLOCK, int OCC, int W, int UNI>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1)))
 void k_finish(const LxJob* __restrict__ jobs, LxPathBuf P, LxCounters* __restrict__ C, int bounce0,
               uint32_t* __restrict__ spill, float* __restrict__ accum, uint32_t chunk,
@@ -376,7 +377,7 @@ void k_finish(const LxJob* __restrict__ jobs, LxPathBuf P, LxCounters* __restric
         }
         if (__ballot(have || flush) == 0ull) break;
         if (have || flush) {
-            const LxJob& J = jobs[job];
+            const LxJob& J = UNI == 2 ? jobs[0] : jobs[job];
             const LxDevScene& S = UNI ? *uscene : *J.scene;
             LxRec r;
             LxRecSink sink{r};
@@ -560,20 +561,17 @@ hipError_t lx_launch_shade(const LxJob* jobs, const LxPathBuf& P, const LxPathBu
     hipLaunchKernelGGL(k_shade, dim3(grid), dim3(LX_SHADE_BLOCK), 0, s, jobs, P, Q, R, C, bounce, accum);
     return hipGetLastError();
 }
-template <int BLOCK, int OCC, int W, bool UNI>
+template <int BLOCK, int OCC, int W, int UNI>
 static void lx_finish_at(const LxJob* jobs, const LxPathBuf& P, LxCounters* C, int bounce0, uint32_t* spill,
                          float* accum, uint32_t grid, uint32_t chunk, const LxDevScene* us, hipStream_t s) {
     hipLaunchKernelGGL((k_finish<false, BLOCK, OCC, W, UNI>), dim3(grid), dim3(BLOCK), 0, s, jobs, P, C, bounce0,
                        spill, accum, chunk, us);
 }
-template <int W, bool UNI>
+template <int W, int UNI>
 static void lx_finish_w(const LxJob* jobs, const LxPathBuf& P, LxCounters* C, int bounce0, uint32_t* spill,
                         float* accum, uint32_t threads, int block, int occ, uint32_t chunk, const LxDevScene* us,
-                        hipStream_t s, bool count) {
-    if (count) {
-        hipLaunchKernelGGL((k_finish<true, 128, 0, W, UNI>), dim3(threads / 128), dim3(128), 0, s, jobs, P, C,
-                           bounce0, spill, accum, chunk, us);
-    } else if (occ >= 4) {
+                        hipStream_t s) {
+    if (occ >= 4) {
         if (block == 256 && threads >= 256)
             lx_finish_at<256, 4, W, UNI>(jobs, P, C, bounce0, spill, accum, threads / 256, chunk, us, s);
         else lx_finish_at<128, 4, W, UNI>(jobs, P, C, bounce0, spill, accum, threads / 128, chunk, us, s);
@@ -583,16 +581,26 @@ static void lx_finish_w(const LxJob* jobs, const LxPathBuf& P, LxCounters* C, in
 }
 hipError_t lx_launch_finish(const LxJob* jobs, const LxPathBuf& P, LxCounters* C, int bounce0, uint32_t* spill,
                             float* accum, uint32_t threads, int block, int occ, int width, int chunk,
-                            const LxDevScene* uscene, hipStream_t s, bool count) {
+                            const LxDevScene* uscene, bool single_job, hipStream_t s, bool count) {
     /* threads = grid * block lanes, the same for every shape, so one spill
      * allocation serves them all */
     uint32_t c = chunk < 1 ? 1u : (chunk > 64 ? 64u : (uint32_t)chunk);
-    if (uscene) {
-        if (width == 8) lx_finish_w<8, true>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, uscene, s, count);
-        else lx_finish_w<4, true>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, uscene, s, count);
+    dim3 g(threads / 128), b(128);
+    if (count) {
+        /* the counting replay: same paths and rays, per-lane scene and job */
+        if (width == 8)
+            hipLaunchKernelGGL((k_finish<true, 128, 0, 8, 0>), g, b, 0, s, jobs, P, C, bounce0, spill, accum, c, uscene);
+        else
+            hipLaunchKernelGGL((k_finish<true, 128, 0, 4, 0>), g, b, 0, s, jobs, P, C, bounce0, spill, accum, c, uscene);
+    } else if (uscene && single_job) {
+        if (width == 8) lx_finish_w<8, 2>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, uscene, s);
+        else lx_finish_w<4, 2>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, uscene, s);
+    } else if (uscene) {
+        if (width == 8) lx_finish_w<8, 1>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, uscene, s);
+        else lx_finish_w<4, 1>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, uscene, s);
     } else {
-        if (width == 8) lx_finish_w<8, false>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, nullptr, s, count);
-        else lx_finish_w<4, false>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, nullptr, s, count);
+        if (width == 8) lx_finish_w<8, 0>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, nullptr, s);
+        else lx_finish_w<4, 0>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, nullptr, s);
     }
     return hipGetLastError();
 }

@@ -103,14 +103,12 @@ void lx_enqueue_round(lx_context* ctx, LxJob* d_jobs, const int32_t* d_spp, uint
         std::swap(cur, nxt);
     }
     if (wf < max_depth) {
-        /* ev[2wf] -> ev[2wf+1] brackets the tail kernel; the remaining events
-         * mark empty intervals so the layout matches the all-wavefront round */
+        /* ev[2wf] -> ev[2wf+1] brackets the tail kernel */
         if (ev) LX_HIP(hipEventRecord(ev[2 * wf], s));
         LX_HIP(lx_launch_finish(d_jobs, *cur, d_cnt, wf, d_spill, d_accum, trace_grid * LX_TRACE_BLOCK,
                                 ctx->tuning.block_size, ctx->tuning.tail_min_waves, width, ctx->tuning.queue_chunk,
-                                uscene, s, count));
-        if (ev)
-            for (int k = 2 * wf + 1; k < 2 * max_depth; k++) LX_HIP(hipEventRecord(ev[k], s));
+                                uscene, njobs == 1, s, count));
+        if (ev && 2 * wf + 1 < 2 * max_depth) LX_HIP(hipEventRecord(ev[2 * wf + 1], s));
     }
     if (ev) LX_HIP(hipEventRecord(ev[2 * max_depth], s));
     LX_HIP(lx_launch_end_round(d_jobs, njobs, s));

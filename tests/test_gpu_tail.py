@@ -111,3 +111,16 @@ def test_autotune_picks_valid_tuning_same_image(tuned, tmp_path):
     tuned.set_tuning(wavefront_bounces=-1, queue_chunk=32)
     tuned.load_tuning(path)
     assert tuned.tuning() == best
+
+
+def test_event_timed_rounds_equal_graph_replay(tuned):
+    data = scenes.icosphere_hdri(96, 64, level=4, pieces=8, env_size=(128, 64))
+    sc = lx.Scene(tuned, data)
+    a = lx.Render(sc, emis=True)
+    a.iterate(1, 3)
+    b = lx.Render(sc, emis=True)
+    b.set_timing(events=True)
+    b.iterate(1, 3)
+    assert np.array_equal(a.accum()[0], b.accum()[0])
+    s = b.stats()
+    assert s["tail_launches"] == 3 and s["tail_ms"] > 0 and s["other_ms"] > 0
