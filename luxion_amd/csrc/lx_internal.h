@@ -118,14 +118,6 @@ struct DevBuf {
 };
 
 /* ---------------------------------------------------------- BVH build */
-/* Builder output node, up to eight children; packed into LxNode4 or LxNode8
- * by the scene once every tree's indices are final. */
-struct LxWideNode {
-    float lo[8][3], hi[8][3];
-    uint32_t child[8]; /* node index, leaf ref or LX_CHILD_EMPTY */
-    uint32_t n;
-};
-
 struct BvhBuildResult {
     std::vector<LxWideNode> nodes; /* local indices */
     std::vector<float4> tris;     /* leaf-ordered (v0|prim, e1, e2) */

@@ -44,6 +44,15 @@ struct alignas(256) LxNode8 {
 };
 static_assert(sizeof(LxNode4) == 128 && sizeof(LxNode8) == 256, "node layout");
 
+/* Builder output node, up to eight children (host and GPU builders); packed
+ * into LxNode4 or LxNode8 by the scene once every tree's indices are final.
+ * Child boxes are already padded. */
+struct LxWideNode {
+    float lo[8][3], hi[8][3];
+    uint32_t child[8]; /* node index, leaf ref or LX_CHILD_EMPTY */
+    uint32_t n;
+};
+
 struct alignas(16) LxInstance {
     float inv[12]; /* world -> object, row-major 3x4 */
     float m[12];   /* object -> world */
