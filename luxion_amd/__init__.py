@@ -79,7 +79,7 @@ class Context:
         check(self.lib.lx_tuning_set(self.h, C.byref(t)))
 
 
-def build_opts(bvh_width=0, leaf_size=4, builder=1, sah_bins=16, cdf_precision=F.LX_CDF_FP64):
+def build_opts(bvh_width=0, leaf_size=4, builder=2, sah_bins=16, cdf_precision=F.LX_CDF_FP64):
     o = F.lx_build_opts()
     o.bvh_width, o.leaf_size, o.builder, o.sah_bins, o.cdf_precision = (bvh_width, leaf_size, builder, sah_bins,
                                                                         cdf_precision)

@@ -172,6 +172,28 @@ __device__ __forceinline__ int count_of(int node, int n, const int2* range) {
     return r.y - r.x + 1;
 }
 
+/* SAH leaf test for a subtree of at most leaf_size triangles: it stays a
+ * leaf unless splitting it once (a box test, relative cost 0.5 of a
+ * triangle test, then each child's triangles weighted by its area share)
+ * is cheaper than testing all its triangles */
+__device__ __forceinline__ bool make_leaf(int c, int n, int leaf_size, const int2* __restrict__ child,
+                                          const int2* __restrict__ range, const float* __restrict__ nlo,
+                                          const float* __restrict__ nhi);
+
+__device__ __forceinline__ bool make_leaf(int c, int n, int leaf_size, const int2* __restrict__ child,
+                                          const int2* __restrict__ range, const float* __restrict__ nlo,
+                                          const float* __restrict__ nhi) {
+    if (c >= n - 1) return true;
+    int cnt = count_of(c, n, range);
+    if (cnt > leaf_size) return false;
+    float a = area3(nlo + 3 * c, nhi + 3 * c);
+    if (!(a > 0.0f)) return true;
+    int2 k = child[c];
+    float split = 0.5f + (area3(nlo + 3 * k.x, nhi + 3 * k.x) * (float)count_of(k.x, n, range) +
+                          area3(nlo + 3 * k.y, nhi + 3 * k.y) * (float)count_of(k.y, n, range)) / a;
+    return !(split < (float)cnt);
+}
+
 /* one level of the collapse: frontier entries (binary node, wide slot) */
 template <int W>
 __global__ void k_collapse(const int2* __restrict__ fin, uint32_t nin, int n, int leaf_size,
@@ -190,7 +212,7 @@ __global__ void k_collapse(const int2* __restrict__ fin, uint32_t nin, int n, in
         float ba = -1.0f;
         for (int k = 0; k < nk; k++) {
             int c = kids[k];
-            if (c >= n - 1 || count_of(c, n, range) <= leaf_size) continue; /* a leaf */
+            if (make_leaf(c, n, leaf_size, child, range, nlo, nhi)) continue;
             float a = area3(nlo + 3 * c, nhi + 3 * c);
             if (a > ba) {
                 ba = a;
@@ -220,7 +242,7 @@ __global__ void k_collapse(const int2* __restrict__ fin, uint32_t nin, int n, in
             w.hi[k][a] = h + pad;
         }
         int cnt = count_of(c, n, range);
-        if (c >= n - 1 || cnt <= leaf_size) {
+        if (make_leaf(c, n, leaf_size, child, range, nlo, nhi)) {
             int first = c >= n - 1 ? c - (n - 1) : range[c].x;
             w.child[k] = LX_LEAF_BIT | ((uint32_t)(cnt - 1) << 28) | (uint32_t)first;
         } else {
@@ -231,6 +253,33 @@ __global__ void k_collapse(const int2* __restrict__ fin, uint32_t nin, int n, in
         }
     }
     out[f.y] = w;
+}
+
+/* a cluster record: a maximal subtree of at most K triangles */
+struct LxCluster {
+    int node, count, first, pad;
+    float lo[3], hi[3];
+};
+
+__global__ void k_cluster_roots(int n, int K, const int2* __restrict__ range, const int* __restrict__ parent,
+                                const float* __restrict__ nlo, const float* __restrict__ nhi,
+                                LxCluster* __restrict__ out, uint32_t* __restrict__ counter) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= 2 * n - 1) return;
+    int c = count_of(v, n, range);
+    int p = parent[v];
+    if (c > K || (p >= 0 && count_of(p, n, range) <= K)) return;
+    uint32_t q = atomicAdd(counter, 1u);
+    LxCluster r;
+    r.node = v;
+    r.count = c;
+    r.first = v >= n - 1 ? v - (n - 1) : range[v].x;
+    r.pad = 0;
+    for (int a = 0; a < 3; a++) {
+        r.lo[a] = nlo[3 * v + a];
+        r.hi[a] = nhi[3 * v + a];
+    }
+    out[q] = r;
 }
 
 __global__ void k_reorder(const float* __restrict__ pos, const uint32_t* __restrict__ idx,
@@ -244,6 +293,182 @@ __global__ void k_reorder(const float* __restrict__ pos, const uint32_t* __restr
     tris[3 * k + 0] = make_float4(a[0], a[1], a[2], __uint_as_float(t));
     tris[3 * k + 1] = make_float4(b[0] - a[0], b[1] - a[1], b[2] - a[2], 0.0f);
     tris[3 * k + 2] = make_float4(c[0] - a[0], c[1] - a[1], c[2] - a[2], 0.0f);
+}
+
+} // namespace
+
+namespace {
+
+const uint32_t kClusterMark = 0x40000000u; /* child ref placeholder: frontier index */
+
+/* Top of the tree over clusters: binned SAH with triangle-count weights. */
+struct TopNode {
+    float lo[3], hi[3];
+    int left = -1, right = -1; /* top node ids; -1 for a cluster leaf */
+    int cluster = -1;
+};
+
+float top_area(const float* lo, const float* hi) {
+    float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    if (!(dx >= 0.0f) || !(dy >= 0.0f) || !(dz >= 0.0f)) return 0.0f;
+    return 2.0f * (dx * dy + dy * dz + dz * dx);
+}
+
+int build_top(std::vector<TopNode>& T, std::vector<LxCluster>& C, int b, int e, int bins) {
+    int me = (int)T.size();
+    T.emplace_back();
+    TopNode nd;
+    float clo[3], chi[3];
+    for (int a = 0; a < 3; a++) {
+        nd.lo[a] = clo[a] = 3.4e38f;
+        nd.hi[a] = chi[a] = -3.4e38f;
+    }
+    for (int i = b; i < e; i++)
+        for (int a = 0; a < 3; a++) {
+            nd.lo[a] = std::min(nd.lo[a], C[i].lo[a]);
+            nd.hi[a] = std::max(nd.hi[a], C[i].hi[a]);
+            float c = 0.5f * (C[i].lo[a] + C[i].hi[a]);
+            clo[a] = std::min(clo[a], c);
+            chi[a] = std::max(chi[a], c);
+        }
+    if (e - b == 1) {
+        nd.cluster = b;
+        T[me] = nd;
+        return me;
+    }
+    /* binned SAH: cost of a split = sum over sides of area * triangles */
+    double best = 1e300;
+    int bax = -1, bbin = -1;
+    std::vector<double> cnt(bins), racc(bins);
+    std::vector<float> blo(3 * bins), bhi(3 * bins);
+    for (int a = 0; a < 3; a++) {
+        float ext = chi[a] - clo[a];
+        if (!(ext > 0.0f)) continue;
+        std::fill(cnt.begin(), cnt.end(), 0.0);
+        for (int k = 0; k < bins; k++)
+            for (int q = 0; q < 3; q++) {
+                blo[3 * k + q] = 3.4e38f;
+                bhi[3 * k + q] = -3.4e38f;
+            }
+        for (int i = b; i < e; i++) {
+            float c = 0.5f * (C[i].lo[a] + C[i].hi[a]);
+            int k = std::min(bins - 1, std::max(0, (int)((c - clo[a]) / ext * bins)));
+            cnt[k] += C[i].count;
+            for (int q = 0; q < 3; q++) {
+                blo[3 * k + q] = std::min(blo[3 * k + q], C[i].lo[q]);
+                bhi[3 * k + q] = std::max(bhi[3 * k + q], C[i].hi[q]);
+            }
+        }
+        /* right-to-left sweep */
+        float rlo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, rhi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+        double rc = 0;
+        for (int k = bins - 1; k > 0; k--) {
+            for (int q = 0; q < 3; q++) {
+                rlo[q] = std::min(rlo[q], blo[3 * k + q]);
+                rhi[q] = std::max(rhi[q], bhi[3 * k + q]);
+            }
+            rc += cnt[k];
+            racc[k] = rc > 0 ? top_area(rlo, rhi) * rc : 0.0;
+        }
+        float llo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, lhi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+        double lc = 0;
+        for (int k = 0; k < bins - 1; k++) {
+            for (int q = 0; q < 3; q++) {
+                llo[q] = std::min(llo[q], blo[3 * k + q]);
+                lhi[q] = std::max(lhi[q], bhi[3 * k + q]);
+            }
+            lc += cnt[k];
+            if (lc == 0 || rc - lc + cnt[0] * 0 <= 0) continue;
+            double cost = top_area(llo, lhi) * lc + racc[k + 1];
+            double total_right = 0;
+            for (int q = k + 1; q < bins; q++) total_right += cnt[q];
+            if (total_right == 0) continue;
+            if (cost < best) {
+                best = cost;
+                bax = a;
+                bbin = k;
+            }
+        }
+    }
+    int mid;
+    if (bax < 0) {
+        mid = (b + e) / 2; /* all centroids equal: split the list */
+    } else {
+        float ext = chi[bax] - clo[bax];
+        auto it = std::partition(C.begin() + b, C.begin() + e, [&](const LxCluster& c) {
+            float cc = 0.5f * (c.lo[bax] + c.hi[bax]);
+            int k = std::min(bins - 1, std::max(0, (int)((cc - clo[bax]) / ext * bins)));
+            return k <= bbin;
+        });
+        mid = (int)(it - C.begin());
+        if (mid == b || mid == e) mid = (b + e) / 2;
+    }
+    nd.left = build_top(T, C, b, mid, bins);
+    nd.right = build_top(T, C, mid, e, bins);
+    T[me] = nd;
+    return me;
+}
+
+int top_count(const std::vector<TopNode>& T, const std::vector<LxCluster>& C, int t) {
+    return T[t].cluster >= 0 ? C[T[t].cluster].count : top_count(T, C, T[t].left) + top_count(T, C, T[t].right);
+}
+
+/* Collapse the top tree to W-wide nodes on the host.  Clusters are not
+ * expanded: a small one becomes a leaf, a larger one a wide node the device
+ * collapse fills in (frontier entry: Karras node, slot). */
+uint32_t collapse_top(const std::vector<TopNode>& T, const std::vector<LxCluster>& C, int t, int W, int leaf_size,
+                      std::vector<LxWideNode>& top, std::vector<int2>& frontier, int depth, int& max_depth) {
+    max_depth = std::max(max_depth, depth);
+    uint32_t me = (uint32_t)top.size();
+    top.emplace_back();
+    int kids[8] = {T[t].left, T[t].right, -1, -1, -1, -1, -1, -1};
+    int nk = 2;
+    while (nk < W) {
+        int best = -1;
+        float ba = -1.0f;
+        for (int k = 0; k < nk; k++) {
+            const TopNode& c = T[kids[k]];
+            if (c.cluster >= 0) continue;
+            float a = top_area(c.lo, c.hi);
+            if (a > ba) {
+                ba = a;
+                best = k;
+            }
+        }
+        if (best < 0) break;
+        int c = kids[best];
+        kids[best] = T[c].left;
+        kids[nk++] = T[c].right;
+    }
+    LxWideNode w;
+    std::memset(&w, 0, sizeof(w));
+    w.n = (uint32_t)nk;
+    for (int k = 0; k < 8; k++) {
+        if (k >= nk) {
+            w.child[k] = LX_CHILD_EMPTY;
+            continue;
+        }
+        const TopNode& c = T[kids[k]];
+        for (int a = 0; a < 3; a++) {
+            w.lo[k][a] = c.lo[a];
+            w.hi[k][a] = c.hi[a];
+        }
+        lx_pad_box(w.lo[k], w.hi[k]);
+        if (c.cluster < 0) {
+            w.child[k] = collapse_top(T, C, kids[k], W, leaf_size, top, frontier, depth + 1, max_depth);
+        } else {
+            const LxCluster& cl = C[c.cluster];
+            if (cl.count <= leaf_size) {
+                w.child[k] = LX_LEAF_BIT | ((uint32_t)(cl.count - 1) << 28) | (uint32_t)cl.first;
+            } else {
+                /* a cluster wide node: numbered after the top nodes later */
+                frontier.push_back(make_int2(cl.node, -1));
+                w.child[k] = kClusterMark | (uint32_t)(frontier.size() - 1);
+            }
+        }
+    }
+    top[me] = w;
+    return me;
 }
 
 } // namespace
@@ -322,15 +547,66 @@ bool lx_build_blas_gpu(lx_context* ctx, const float* pos, const uint32_t* idx, u
     d_front[0].alloc(ctx, (size_t)std::max(1, nint) * 8);
     d_front[1].alloc(ctx, (size_t)std::max(1, nint) * 8);
     d_cnt.alloc(ctx, 16);
-    int2 root = make_int2(0, 0);
-    LX_HIP(hipMemcpyAsync(d_front[0].ptr, &root, 8, hipMemcpyHostToDevice, s));
-    uint32_t cnt[4] = {1u, 0u, 0u, 0u};
-    LX_HIP(hipMemcpyAsync(d_cnt.ptr, cnt, 16, hipMemcpyHostToDevice, s));
-    uint32_t nin = 1;
+    /* binned-SAH refinement of the top: clusters = maximal subtrees of at
+     * most kCluster triangles; the tree above them is rebuilt with SAH */
+    const int kCluster = 64;
+    std::vector<LxWideNode> top;
+    std::vector<int2> frontier;
+    uint32_t next_slot = 1;
+    int top_depth = 0;
+    if (n > kCluster) {
+        DevBuf d_cl, d_ccnt;
+        d_cl.alloc(ctx, nall * sizeof(LxCluster));
+        static const uint32_t kZero4[4] = {0, 0, 0, 0};
+        d_ccnt.upload(ctx, kZero4, 16);
+        hipLaunchKernelGGL(k_cluster_roots, dim3((uint32_t)((nall + 255) / 256)), dim3(256), 0, s, n, kCluster,
+                           d_range.as<int2>(), d_parent.as<int>(), d_nlo.as<float>(), d_nhi.as<float>(),
+                           d_cl.as<LxCluster>(), d_ccnt.as<uint32_t>());
+        LX_HIP(hipGetLastError());
+        uint32_t ncl = 0;
+        LX_HIP(hipMemcpyAsync(&ncl, d_ccnt.ptr, 4, hipMemcpyDeviceToHost, s));
+        LX_HIP(hipStreamSynchronize(s));
+        std::vector<LxCluster> C(ncl);
+        LX_HIP(hipMemcpyAsync(C.data(), d_cl.ptr, (size_t)ncl * sizeof(LxCluster), hipMemcpyDeviceToHost, s));
+        LX_HIP(hipStreamSynchronize(s));
+        std::sort(C.begin(), C.end(), [](const LxCluster& a, const LxCluster& b) { return a.first < b.first; });
+        if (ncl >= 2) {
+            std::vector<TopNode> T;
+            T.reserve(2 * ncl);
+            int troot = build_top(T, C, 0, (int)ncl, 32);
+            collapse_top(T, C, troot, W, leaf_size, top, frontier, 0, top_depth);
+            /* top nodes take slots 0 .. top.size() - 1, cluster wide nodes
+             * the next frontier.size() slots, the device collapse the rest */
+            uint32_t base = (uint32_t)top.size();
+            for (size_t k = 0; k < frontier.size(); k++) frontier[k].y = (int)(base + k);
+            for (auto& w : top)
+                for (int k = 0; k < 8; k++) {
+                    uint32_t c = w.child[k];
+                    if (c == LX_CHILD_EMPTY || (c & LX_LEAF_BIT) || !(c & kClusterMark)) continue;
+                    w.child[k] = base + (c & ~kClusterMark);
+                }
+            next_slot = base + (uint32_t)frontier.size();
+        }
+    }
     int cur = 0, depth = 0;
+    uint32_t nin;
+    uint32_t cnt[4] = {1u, 0u, 0u, 0u};
+    if (top.empty()) {
+        static const int2 kRoot = make_int2(0, 0);
+        LX_HIP(hipMemcpyAsync(d_front[0].ptr, &kRoot, 8, hipMemcpyHostToDevice, s));
+        nin = 1;
+    } else {
+        if (!frontier.empty())
+            LX_HIP(hipMemcpyAsync(d_front[0].ptr, frontier.data(), frontier.size() * 8, hipMemcpyHostToDevice, s));
+        nin = (uint32_t)frontier.size();
+        cnt[0] = next_slot;
+        depth = top_depth + 1;
+    }
+    LX_HIP(hipMemcpyAsync(d_cnt.ptr, cnt, 16, hipMemcpyHostToDevice, s));
+    LX_HIP(hipStreamSynchronize(s));
+    static const uint32_t kZero = 0;
     while (nin > 0) {
-        uint32_t zero = 0;
-        LX_HIP(hipMemcpyAsync(d_cnt.as<uint32_t>() + 1, &zero, 4, hipMemcpyHostToDevice, s));
+        LX_HIP(hipMemcpyAsync(d_cnt.as<uint32_t>() + 1, &kZero, 4, hipMemcpyHostToDevice, s));
         if (W == 8)
             hipLaunchKernelGGL(k_collapse<8>, dim3((nin + 127) / 128), dim3(128), 0, s, d_front[cur].as<int2>(), nin, n,
                                leaf_size, d_child.as<int2>(), d_range.as<int2>(), d_nlo.as<float>(),
@@ -353,6 +629,7 @@ bool lx_build_blas_gpu(lx_context* ctx, const float* pos, const uint32_t* idx, u
     LX_HIP(hipMemcpyAsync(out.nodes.data(), d_nodes.ptr, (size_t)nodes * sizeof(LxWideNode), hipMemcpyDeviceToHost, s));
     LX_HIP(hipMemcpyAsync(out.tris.data(), d_tris.ptr, (size_t)ntri * 48, hipMemcpyDeviceToHost, s));
     LX_HIP(hipStreamSynchronize(s));
+    for (size_t k = 0; k < top.size(); k++) out.nodes[k] = top[k];
     out.root = 0;
     out.depth = depth;
     return true;

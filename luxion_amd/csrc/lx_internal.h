@@ -131,8 +131,11 @@ void lx_build_blas_host(const float* pos, const uint32_t* idx, uint32_t ntri, in
                         BvhBuildResult& out);
 /* TLAS over instance world boxes (one instance per leaf). */
 void lx_build_tlas_host(const std::vector<float>& boxes /* 6 per instance */, int width, BvhBuildResult& out);
-/* GPU LBVH + binned-SAH refinement (builder 0); returns false if the GPU
- * path is unavailable for this input. */
+/* GPU builder (builder 0; builder 2 uses it for meshes of at least
+ * LX_GPU_BUILD_MIN_TRIS triangles): linear BVH from Morton codes, binned SAH
+ * over clusters at the top, SAH leaf test at the bottom; returns false if the
+ * GPU path is unavailable for this input. */
+#define LX_GPU_BUILD_MIN_TRIS (1u << 20)
 bool lx_build_blas_gpu(lx_context* ctx, const float* pos, const uint32_t* idx, uint32_t ntri, int leaf_size, int bins,
                        int width, BvhBuildResult& out);
 void lx_pad_box(float lo[3], float hi[3]);

@@ -309,7 +309,7 @@ static std::vector<N> pack_nodes(const std::vector<LxWideNode>& in, uint32_t tla
 
 /* Precompute and upload everything; builds the BVHs. */
 void lx_scene_accel(lx_scene* s, const lx_build_opts* o) {
-    lx_build_opts opt{4, 4, 1, 16, LX_CDF_FP64};
+    lx_build_opts opt{0, 4, 2, 16, LX_CDF_FP64};
     if (o) opt = *o;
     if (opt.bvh_width == 0) opt.bvh_width = s->ctx->tuning.bvh_width;
     LX_REQUIRE(opt.bvh_width == 4 || opt.bvh_width == 8, LX_ERR_INVALID, "bvh_width must be 4 or 8 (0: the tuning's)");
@@ -361,8 +361,8 @@ void lx_scene_accel(lx_scene* s, const lx_build_opts* o) {
         BvhBuildResult r;
         uint32_t nt = (uint32_t)(m.idx.size() / 3);
         bool ok = false;
-        if (opt.builder == 0)
-            ok = lx_build_blas_gpu(ctx, m.pos.data(), m.idx.data(), nt, opt.leaf_size, opt.sah_bins, opt.bvh_width, r);
+        bool gpu = opt.builder == 0 || (opt.builder == 2 && nt >= LX_GPU_BUILD_MIN_TRIS);
+        if (gpu) ok = lx_build_blas_gpu(ctx, m.pos.data(), m.idx.data(), nt, opt.leaf_size, opt.sah_bins, opt.bvh_width, r);
         if (!ok) lx_build_blas_host(m.pos.data(), m.idx.data(), nt, opt.leaf_size, opt.sah_bins, opt.bvh_width, r);
         uint32_t noff = (uint32_t)nodes.size(), toff = (uint32_t)(tris.size() / 3);
         LX_REQUIRE((size_t)toff + nt < (1u << 28) - 1, LX_ERR_INVALID, "scene too large for the leaf encoding");
