@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define LX_ABI_VERSION 1
+#define LX_ABI_VERSION 2
 
 typedef enum lx_status {
     LX_OK = 0,
@@ -95,6 +95,20 @@ typedef struct lx_instance_desc {
     float transform[12];
 } lx_instance_desc;
 
+/* Legacy material classes (sceneStructs.h MaterialType, used when the
+ * reference is compiled with UBER_SHADER 0).  Only LX_SHADING_LEGACY renders
+ * read material_type; the uber shader treats every material as microfacet.
+ * lx_scene_load_glb assigns it with the reference loader's rule
+ * (src/scene/scene_load_gltf.cpp:211-228, include/scene/gltf_material_utils.h):
+ * emissive, else glass, else microfacet. */
+enum {
+    LX_MATERIAL_MICROFACET = 0, /* Cook-Torrance uber shader */
+    LX_MATERIAL_DIFFUSE = 1,    /* Lambert (src/shaders/lambert.cu) */
+    LX_MATERIAL_SPECULAR = 2,   /* perfect mirror (src/shaders/specular.cu) */
+    LX_MATERIAL_EMISSIVE = 3,   /* light: emits, Lambert-sampled */
+    LX_MATERIAL_GLASS = 4       /* smooth dielectric (src/shaders/glass.cu) */
+};
+
 /* The reference's uber-shader Material (sceneStructs.h Material). Texture
  * indices are -1 when absent. */
 typedef struct lx_material_desc {
@@ -116,6 +130,7 @@ typedef struct lx_material_desc {
     float alpha;
     int32_t alpha_mode;
     float alpha_cutoff;
+    int32_t material_type; /* LX_MATERIAL_*, read by LX_SHADING_LEGACY only */
 } lx_material_desc;
 
 /* Texel data, row-major, 4 channels.  RGBA8 / RGBA16 decode to float as
@@ -212,6 +227,25 @@ void lx_scene_destroy(lx_scene* scene);
 
 /* ----------------------------------------------------------- render jobs */
 enum { LX_TONEMAP_REINHARD = 0, LX_TONEMAP_AGX = 1, LX_TONEMAP_ACES = 2 };
+/* Material debug views (src/material_debug_render.cu:3-26; names from
+ * src/app/render_imgui.cpp:336).  A view traces one bounce (src/pathtrace.cu:107),
+ * writes the queried value as the pixel colour, and lx_render_read_ldr shows
+ * it without tonemap or gamma (src/kernels/image_kernels.cu:20). */
+enum {
+    LX_DEBUG_OFF = 0,
+    LX_DEBUG_ALBEDO = 1,
+    LX_DEBUG_WORLD_NORMAL = 2, /* shading normal * 0.5 + 0.5 */
+    LX_DEBUG_NORMAL_MAP = 3,   /* raw normal-map texel, (0.5, 0.5, 1) without one */
+    LX_DEBUG_ROUGHNESS = 4,
+    LX_DEBUG_METALLIC = 5,
+    LX_DEBUG_ALPHA = 6
+};
+/* LX_SHADING_UBER: every material is the Cook-Torrance uber shader (the
+ * reference's default build, include/config.h:23).  LX_SHADING_LEGACY: the
+ * per-material_type shaders of the UBER_SHADER 0 build
+ * (src/sample_materials.cu, src/update_throughput_materials.cu,
+ * src/kernels/shade_path.cu:44-92). */
+enum { LX_SHADING_UBER = 0, LX_SHADING_LEGACY = 1 };
 
 /* PathTracerOptions (include/pathtrace/pathtracer_options.h:3-21) plus the
  * CLI flags -dmis / -emis (src/main.cpp:117-124). */
@@ -223,6 +257,8 @@ typedef struct lx_render_opts {
     uint32_t seed;            /* 0 = the reference's RNG streams */
     int32_t color_mode;       /* LX_TONEMAP_* for lx_render_read_ldr */
     const lx_camera_desc* camera; /* NULL = scene camera */
+    int32_t material_debug_mode; /* LX_DEBUG_* (PathTracerOptions::material_debug_mode) */
+    int32_t shading_model;       /* LX_SHADING_* */
 } lx_render_opts;
 
 /* Replaces pathtraceInit(Scene*) (src/pathtrace/pathtrace_buffers.cu:21-140). */
@@ -343,6 +379,8 @@ typedef struct lx_job {
     uint32_t seed;        /* per-job RNG stream; 0 = reference stream */
     uint32_t first_iter;  /* 1-based iteration of the first sample */
     double deadline_ms;   /* served form: relative deadline, 0 = none */
+    int32_t material_debug_mode; /* LX_DEBUG_* */
+    int32_t shading_model;       /* LX_SHADING_* */
 } lx_job;
 
 typedef struct lx_job_result {

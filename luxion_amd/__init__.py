@@ -54,11 +54,6 @@ class Context:
         check(self.lib.lx_tuning_get(self.h, C.byref(t)))
         return {k: getattr(t, k) for k, _ in F.lx_tuning._fields_}
 
-    def tuning(self):
-        t = F.lx_tuning()
-        check(self.lib.lx_tuning_get(self.h, C.byref(t)))
-        return {k: getattr(t, k) for k, _ in F.lx_tuning._fields_}
-
     def autotune(self, jobs, json_path=None):
         """Time the batched form over `jobs` for each launch shape (and BVH
         width, when the jobs' scenes can be rebuilt); keep the fastest.
@@ -193,20 +188,23 @@ class Render:
     records, accumulation)."""
 
     def __init__(self, scene, max_depth=0, dmis=False, emis=False, envmap_intensity=1.0, seed=0, color_mode=2,
-                 camera=None):
+                 camera=None, material_debug_mode=0, shading_model=0):
+        """material_debug_mode: LX_DEBUG_* view (one bounce, no tonemap);
+        shading_model: LX_SHADING_UBER (default) or LX_SHADING_LEGACY."""
         self.scene = scene
         self.lib = scene.lib
         self.h = C.c_void_p()
         o = F.lx_render_opts()
         o.max_depth, o.dmis, o.emis = max_depth, int(dmis), int(emis)
         o.envmap_intensity, o.seed, o.color_mode = envmap_intensity, seed, color_mode
+        o.material_debug_mode, o.shading_model = material_debug_mode, shading_model
         self._cam = scenes.camera_desc(camera) if camera is not None else None
         o.camera = C.pointer(self._cam) if self._cam is not None else None
         check(self.lib.lx_render_create(scene.h, C.byref(o), C.byref(self.h)))
         cam, depth = scene.camera()
         self.width = camera["width"] if camera else cam["width"]
         self.height = camera["height"] if camera else cam["height"]
-        self.depth = max_depth if max_depth > 0 else depth
+        self.depth = 1 if material_debug_mode else (max_depth if max_depth > 0 else depth)
         self.next_iter = 1
 
     def iterate(self, first_iter, n):
@@ -333,12 +331,13 @@ def parse_glb(path, envmap=None, env_head=0):
 
 # ------------------------------------------------------------ batched form
 def make_job(scene, camera, spp, max_depth=0, dmis=False, emis=False, envmap_intensity=1.0, seed=0, first_iter=1,
-             deadline_ms=0.0):
+             deadline_ms=0.0, material_debug_mode=0, shading_model=0):
     j = F.lx_job()
     j.scene = scene.h.value
     j.camera = scenes.camera_desc(camera)
     j.spp, j.max_depth, j.dmis, j.emis = spp, max_depth, int(dmis), int(emis)
     j.envmap_intensity, j.seed, j.first_iter, j.deadline_ms = envmap_intensity, seed, first_iter, deadline_ms
+    j.material_debug_mode, j.shading_model = material_debug_mode, shading_model
     return j
 
 

@@ -20,6 +20,10 @@ LX_ALPHA_OPAQUE, LX_ALPHA_MASK, LX_ALPHA_BLEND = 0, 1, 2
 LX_CDF_FP64, LX_CDF_FP32_REFERENCE = 0, 1
 LX_TONEMAP_REINHARD, LX_TONEMAP_AGX, LX_TONEMAP_ACES = 0, 1, 2
 LX_TIMING_EVENTS, LX_TIMING_COUNT_WORK = 1, 2
+LX_MATERIAL_MICROFACET, LX_MATERIAL_DIFFUSE, LX_MATERIAL_SPECULAR, LX_MATERIAL_EMISSIVE, LX_MATERIAL_GLASS = range(5)
+LX_DEBUG_OFF, LX_DEBUG_ALBEDO, LX_DEBUG_WORLD_NORMAL, LX_DEBUG_NORMAL_MAP, LX_DEBUG_ROUGHNESS, LX_DEBUG_METALLIC, \
+    LX_DEBUG_ALPHA = range(7)
+LX_SHADING_UBER, LX_SHADING_LEGACY = 0, 1
 
 F2 = C.c_float * 2
 F3 = C.c_float * 3
@@ -46,7 +50,7 @@ class lx_material_desc(C.Structure):
                 ("normal_xf", lx_tex_transform), ("emission_color", F3), ("emission_strength", C.c_float),
                 ("emissive_tex", C.c_int32), ("emissive_xf", lx_tex_transform), ("ior", C.c_float),
                 ("transmission", C.c_float), ("alpha", C.c_float), ("alpha_mode", C.c_int32),
-                ("alpha_cutoff", C.c_float)]
+                ("alpha_cutoff", C.c_float), ("material_type", C.c_int32)]
 
 
 class lx_texture_desc(C.Structure):
@@ -83,7 +87,8 @@ class lx_scene_info(C.Structure):
 class lx_render_opts(C.Structure):
     _fields_ = [("max_depth", C.c_int32), ("dmis", C.c_int32), ("emis", C.c_int32),
                 ("envmap_intensity", C.c_float), ("seed", C.c_uint32), ("color_mode", C.c_int32),
-                ("camera", C.POINTER(lx_camera_desc))]
+                ("camera", C.POINTER(lx_camera_desc)), ("material_debug_mode", C.c_int32),
+                ("shading_model", C.c_int32)]
 
 
 class lx_render_stats(C.Structure):
@@ -111,7 +116,8 @@ class lx_hit(C.Structure):
 class lx_job(C.Structure):
     _fields_ = [("scene", C.c_void_p), ("camera", lx_camera_desc), ("spp", C.c_int32), ("max_depth", C.c_int32),
                 ("dmis", C.c_int32), ("emis", C.c_int32), ("envmap_intensity", C.c_float), ("seed", C.c_uint32),
-                ("first_iter", C.c_uint32), ("deadline_ms", C.c_double)]
+                ("first_iter", C.c_uint32), ("deadline_ms", C.c_double), ("material_debug_mode", C.c_int32),
+                ("shading_model", C.c_int32)]
 
 
 class lx_job_result(C.Structure):

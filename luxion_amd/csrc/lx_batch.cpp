@@ -19,6 +19,23 @@ static double now_ms() {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+void lx_validate_views(int32_t debug_mode, int32_t shading_model) {
+    LX_REQUIRE(debug_mode >= LX_DEBUG_OFF && debug_mode <= LX_DEBUG_ALPHA, LX_ERR_INVALID,
+               "material_debug_mode must be one of LX_DEBUG_*");
+    LX_REQUIRE(shading_model == LX_SHADING_UBER || shading_model == LX_SHADING_LEGACY, LX_ERR_INVALID,
+               "shading_model must be LX_SHADING_UBER or LX_SHADING_LEGACY");
+}
+
+int lx_job_depth(int32_t max_depth, int trace_depth, int32_t debug_mode) {
+    /* a material debug view traces one bounce (src/pathtrace.cu:107) */
+    if (debug_mode != LX_DEBUG_OFF) return 1;
+    return max_depth > 0 ? max_depth : trace_depth;
+}
+
+uint32_t lx_job_flags(int32_t debug_mode, int32_t shading_model) {
+    return ((uint32_t)debug_mode & LX_JF_DEBUG_MASK) | (shading_model == LX_SHADING_LEGACY ? LX_JF_LEGACY : 0u);
+}
+
 void lx_validate_job(const lx_job& j) {
     LX_REQUIRE(j.scene != nullptr, LX_ERR_INVALID, "job without scene");
     lx_scene* s = j.scene;
@@ -28,6 +45,7 @@ void lx_validate_job(const lx_job& j) {
     LX_REQUIRE(j.first_iter >= 1, LX_ERR_INVALID, "iterations are 1-based");
     int depth = j.max_depth > 0 ? j.max_depth : s->trace_depth;
     LX_REQUIRE(depth <= LX_MAX_DEPTH, LX_ERR_INVALID, "max_depth above the supported maximum");
+    lx_validate_views(j.material_debug_mode, j.shading_model);
     if (j.dmis)
         LX_REQUIRE(!s->emissive.empty() && !(s->total_area < 1e-5f), LX_ERR_INVALID,
                    "-dmis requires the scene to contain emissive geometry");
@@ -41,7 +59,8 @@ LxJob lx_make_dev_job(const lx_job& j, uint32_t accum_off) {
     J.cam = j.camera;
     J.num_pix = (uint32_t)j.camera.width * (uint32_t)j.camera.height;
     J.accum_off = accum_off;
-    J.depth = j.max_depth > 0 ? j.max_depth : j.scene->trace_depth;
+    J.depth = lx_job_depth(j.max_depth, j.scene->trace_depth, j.material_debug_mode);
+    J.flags = lx_job_flags(j.material_debug_mode, j.shading_model);
     J.dmis = j.dmis;
     J.emis = j.emis;
     J.intensity = j.envmap_intensity;

@@ -305,6 +305,7 @@ lx_status lx_render_create(lx_scene* s, const lx_render_opts* o, lx_render** out
     LX_REQUIRE(s->accel_built, LX_ERR_STATE, "call lx_scene_build_accel first");
     *out = nullptr;
     validate_flags(s, o->dmis, o->emis);
+    lx_validate_views(o->material_debug_mode, o->shading_model);
     lx_context* ctx = s->ctx;
     std::lock_guard<std::mutex> lk(ctx->mu);
     ctx->set_device();
@@ -315,7 +316,7 @@ lx_status lx_render_create(lx_scene* s, const lx_render_opts* o, lx_render** out
         r->opts = *o;
         r->cam = o->camera ? *o->camera : s->cam;
         LX_REQUIRE(r->cam.width > 0 && r->cam.height > 0, LX_ERR_INVALID, "camera resolution must be positive");
-        r->depth = o->max_depth > 0 ? o->max_depth : s->trace_depth;
+        r->depth = lx_job_depth(o->max_depth, s->trace_depth, o->material_debug_mode);
         LX_REQUIRE(r->depth <= LX_MAX_DEPTH, LX_ERR_INVALID, "max_depth above the supported maximum");
         r->npix = (uint32_t)r->cam.width * (uint32_t)r->cam.height;
         r->wb.alloc(ctx, r->npix, o->dmis != 0, o->emis != 0, scene_has_normal_maps(s));
@@ -335,6 +336,7 @@ lx_status lx_render_create(lx_scene* s, const lx_render_opts* o, lx_render** out
         J.num_pix = r->npix;
         J.accum_off = 0;
         J.depth = r->depth;
+        J.flags = lx_job_flags(o->material_debug_mode, o->shading_model);
         J.dmis = o->dmis;
         J.emis = o->emis;
         J.intensity = o->envmap_intensity;
@@ -466,7 +468,8 @@ lx_status lx_render_read_ldr(lx_render* r, uint8_t* rgba, int32_t tonemap) {
     std::lock_guard<std::mutex> lk(r->ctx->mu);
     r->ctx->set_device();
     if (!r->ldr.ptr) r->ldr.alloc(r->ctx, (size_t)r->npix * 4);
-    LX_HIP(lx_launch_tonemap(r->accum.as<float>(), r->npix, (float)r->spp, tonemap, r->ldr.as<uint8_t>(),
+    int mode = r->opts.material_debug_mode != LX_DEBUG_OFF ? -1 : tonemap;
+    LX_HIP(lx_launch_tonemap(r->accum.as<float>(), r->npix, (float)r->spp, mode, r->ldr.as<uint8_t>(),
                              r->ctx->stream));
     LX_HIP(hipMemcpyAsync(rgba, r->ldr.ptr, (size_t)r->npix * 4, hipMemcpyDeviceToHost, r->ctx->stream));
     LX_HIP(hipStreamSynchronize(r->ctx->stream));

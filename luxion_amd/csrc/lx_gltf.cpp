@@ -263,6 +263,25 @@ double ext_number(const LxJson& mat, const char* ext, const char* key, double df
     return x ? x->number(key, dflt) : dflt;
 }
 
+/* isGlass, include/scene/gltf_material_utils.h:9-47: a transmission or volume
+ * extension, an ior away from 1, a see-through MASK / BLEND base colour, or a
+ * see-through specular-glossiness diffuse factor. */
+bool legacy_is_glass(const LxJson& mat, double base_alpha) {
+    const LxJson* e = mat.get("extensions");
+    if (e && (e->has("KHR_materials_transmission") || e->has("KHR_materials_volume"))) return true;
+    if (const LxJson* x = e ? e->get("KHR_materials_ior") : nullptr) {
+        const LxJson* v = x->get("ior");
+        if (v && v->kind == LxJson::NUM && std::fabs(v->num - 1.0) > 0.01) return true;
+    }
+    std::string am = mat.string("alphaMode", "OPAQUE");
+    if ((am == "BLEND" || am == "MASK") && base_alpha < (double)0.99f) return true;
+    if (const LxJson* x = e ? e->get("KHR_materials_pbrSpecularGlossiness") : nullptr) {
+        const std::vector<LxJson>& df = x->array("diffuseFactor");
+        if (df.size() == 4 && df[3].kind == LxJson::NUM && df[3].num < 0.99) return true;
+    }
+    return false;
+}
+
 } // namespace
 
 /* The scene description built from a .glb file, owning every array the
@@ -349,6 +368,15 @@ void lx_gltf_build(const std::string& glb_path, const char* exr_path, LxGltfScen
         md.albedo_xf = tex_xf(pbr->get("baseColorTexture"));
         md.normal_tex = tex_index(m.get("normalTexture"));
         md.normal_xf = tex_xf(m.get("normalTexture"));
+        /* legacy material class (the UBER_SHADER 0 branch,
+         * src/scene/scene_load_gltf.cpp:211-228): emissive, else glass,
+         * else microfacet */
+        float ec = std::sqrt(md.emission_color[0] * md.emission_color[0] + md.emission_color[1] * md.emission_color[1] +
+                             md.emission_color[2] * md.emission_color[2]);
+        if (md.emission_strength * ec > 0.00001f) md.material_type = LX_MATERIAL_EMISSIVE;
+        else if (legacy_is_glass(m, bcf[3]) && md.metallic < 0.01f && md.roughness < 0.01f)
+            md.material_type = LX_MATERIAL_GLASS;
+        else md.material_type = LX_MATERIAL_MICROFACET;
         for (int t : {md.albedo_tex, md.metallic_roughness_tex, md.normal_tex, md.emissive_tex})
             if (t >= (int)S.texs.size()) bad("material names a missing texture");
         S.mats.push_back(md);

@@ -243,6 +243,12 @@ int lx_wavefront_bounces(const lx_context* ctx, int max_depth);
 /* ------------------------------------------------------ batched / served */
 void lx_tuning_check(const lx_tuning& t); /* throws LX_ERR_INVALID */
 void lx_validate_job(const lx_job& j);
+/* Rejects an unknown material debug view or shading model. */
+void lx_validate_views(int32_t debug_mode, int32_t shading_model);
+/* Bounces a job traces: one for a material debug view, else max_depth or the
+ * scene's trace depth. */
+int lx_job_depth(int32_t max_depth, int trace_depth, int32_t debug_mode);
+uint32_t lx_job_flags(int32_t debug_mode, int32_t shading_model);
 LxJob lx_make_dev_job(const lx_job& j, uint32_t accum_off);
 int lx_simulate_rounds(const std::vector<uint32_t>& npix, const std::vector<int32_t>& spp, uint32_t cap,
                        std::vector<int>* finish_round);

@@ -485,10 +485,12 @@ __global__ void k_tonemap(const float* __restrict__ accum, uint32_t npix, float 
     f3 p = mk3(accum[3 * i], accum[3 * i + 1], accum[3 * i + 2]);
     float inv = 1.0f / spp;
     p = p * inv;
-    if (mode == LX_TONEMAP_REINHARD) p = mk3(p.x / (p.x + 1.0f), p.y / (p.y + 1.0f), p.z / (p.z + 1.0f));
-    else if (mode == LX_TONEMAP_AGX) p = lx_agx(p);
-    else p = lx_aces(p);
-    p = mk3(lxm_pow(p.x, 0.45f), lxm_pow(p.y, 0.45f), lxm_pow(p.z, 0.45f));
+    if (mode >= 0) { /* mode < 0: a material debug view, shown as is (image_kernels.cu:20) */
+        if (mode == LX_TONEMAP_REINHARD) p = mk3(p.x / (p.x + 1.0f), p.y / (p.y + 1.0f), p.z / (p.z + 1.0f));
+        else if (mode == LX_TONEMAP_AGX) p = lx_agx(p);
+        else p = lx_aces(p);
+        p = mk3(lxm_pow(p.x, 0.45f), lxm_pow(p.y, 0.45f), lxm_pow(p.z, 0.45f));
+    }
     int r = (int)((double)p.x * 255.0), g = (int)((double)p.y * 255.0), b = (int)((double)p.z * 255.0);
     r = r < 0 ? 0 : (r > 255 ? 255 : r);
     g = g < 0 ? 0 : (g > 255 ? 255 : g);

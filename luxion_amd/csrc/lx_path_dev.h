@@ -194,98 +194,130 @@ LXD bool lx_shade_path(const LxJob& J, const LxDevScene& S, const LxRec& R, int 
         N = R.N;
         f2 uv = R.uv;
         const lx_material_desc& mat = S.mats[R.mat];
+        /* legacy shading (UBER_SHADER 0) dispatches on the material class; the
+         * uber shader treats every material as microfacet */
+        const bool legacy = (J.flags & LX_JF_LEGACY) != 0u;
+        const int mt = legacy ? mat.material_type : LX_MATERIAL_MICROFACET;
         f3 color = lx_get_albedo(S, mat, uv);
+        /* the legacy loader gives an emissive material its emission colour as
+         * base colour (src/scene/scene_load_gltf.cpp:211-213) */
+        if (mt == LX_MATERIAL_EMISSIVE && mat.albedo_tex < 0)
+            color = mk3(mat.emission_color[0], mat.emission_color[1], mat.emission_color[2]);
         f3 normal = N;
         if (mat.normal_tex >= 0) normal = lx_get_normal(S, mat, N, R.Tg, uv);
         float rough, metal;
         lx_get_mr(S, mat, uv, rough, metal);
         f3 emission = lx_get_emission(S, mat, uv);
-        bool is_specular = rough < 0.05f;
+        bool is_specular = legacy ? (mt == LX_MATERIAL_SPECULAR || mt == LX_MATERIAL_GLASS) : rough < 0.05f;
         f3 nf = dot(-d, normal) < 0.0f ? -normal : normal;
-        sample_dir = lx_sample_ct(-d, normal, rough, metal, rng, color, mat.ior, mat.transmission);
-        bool emissive = length(emission) > 0.00001f;
-        if (J.emis) {
-            if (emissive) {
-                float w = 1.0f;
-                if (depth > 1 && !spec) {
-                    float pb = last_pdf;
-                    float pe = lx_envmap_pdf(S, d);
-                    w = (pb * pb) / (pb * pb + pe * pe);
+        const uint32_t dbg = J.flags & LX_JF_DEBUG_MASK;
+        if (dbg != 0u) {
+            /* render_material_debug_mode replaces the colour and ends the path */
+            col = lx_debug_color(S, mat, dbg, color, normal, uv, rough, metal);
+            kill = true;
+        } else {
+            if (mt == LX_MATERIAL_MICROFACET)
+                sample_dir = lx_sample_ct(-d, normal, rough, metal, rng, color, mat.ior, mat.transmission);
+            else
+                sample_dir = lx_sample_legacy(mt, d, normal, mat.ior, rng);
+            bool emissive = legacy ? mt == LX_MATERIAL_EMISSIVE : length(emission) > 0.00001f;
+            if (J.emis) {
+                if (emissive) {
+                    float w = 1.0f;
+                    if (depth > 1 && !spec) {
+                        float pb = last_pdf;
+                        float pe = lx_envmap_pdf(S, d);
+                        w = (pb * pb) / (pb * pb + pe * pe);
+                    }
+                    col = col + thr * (w * emission);
+                    kill = true;
+                } else if ((DEFER || R.env_t < 0.0f) && !is_specular && has_exr) {
+                    f3 ed = R.ed;
+                    float pe = lx_envmap_pdf(S, ed);
+                    float pb = mt == LX_MATERIAL_MICROFACET
+                                   ? lx_pdf(-d, ed, normal, rough, metal, color, mat.ior, mat.transmission)
+                                   : lx_legacy_pdf(mt, ed, normal);
+                    float w = (pe * pe) / (pe * pe + pb * pb);
+                    f3 brdf = mt == LX_MATERIAL_MICROFACET
+                                  ? lx_brdf(-d, normal, ed, color, rough, metal, mat.ior, mat.transmission)
+                                  : lx_legacy_brdf(mt, color);
+                    f3 L = lx_env_lookup(S, normalize(ed), J.intensity);
+                    float cs = dot(nf, ed);
+                    if (cs > 0.0f && pe > 1e-8f) {
+                        f3 contrib = ((L * brdf) * cs) / pe;
+                        if (DEFER) {
+                            *pend_add = true;
+                            *pend_col = (thr * contrib) * w;
+                        } else {
+                            col = col + (thr * contrib) * w;
+                        }
+                    }
                 }
-                col = col + thr * (w * emission);
-                kill = true;
-            } else if ((DEFER || R.env_t < 0.0f) && !is_specular && has_exr) {
-                f3 ed = R.ed;
-                float pe = lx_envmap_pdf(S, ed);
-                float pb = lx_pdf(-d, ed, normal, rough, metal, color, mat.ior, mat.transmission);
-                float w = (pe * pe) / (pe * pe + pb * pb);
-                f3 brdf = lx_brdf(-d, normal, ed, color, rough, metal, mat.ior, mat.transmission);
-                f3 L = lx_env_lookup(S, normalize(ed), J.intensity);
-                float cs = dot(nf, ed);
-                if (cs > 0.0f && pe > 1e-8f) {
-                    f3 contrib = ((L * brdf) * cs) / pe;
-                    if (DEFER) {
-                        *pend_add = true;
-                        *pend_col = (thr * contrib) * w;
-                    } else {
+            } else if (J.dmis) {
+                if (emissive) {
+                    float w = 1.0f;
+                    if (depth > 1 && !spec) {
+                        float pb = last_pdf;
+                        float dist_sq = t * t;
+                        float cl = dot(N, -d);
+                        if (cl > 0.0001f) {
+                            float pa = 1.0f / S.total_area;
+                            float psa = pa * (dist_sq / cl);
+                            w = (pb * pb) / (pb * pb + psa * psa);
+                        } else {
+                            w = 0.0f;
+                        }
+                    }
+                    col = col + thr * (w * emission);
+                    kill = true;
+                } else if (R.dl_t > 0.0f && !is_specular) {
+                    f3 sp = (t * d) + o;
+                    f3 lv = R.lp - sp;
+                    float dsq = dot(lv, lv);
+                    float dist = lxm_sqrt(dsq);
+                    f3 ld = lv / dist;
+                    float cs = dot(nf, ld);
+                    float cl = dot(R.dlN, -ld);
+                    if (cs > 0.0f && cl > 0.0f) {
+                        float pa = 1.0f / S.total_area;
+                        float psa = pa * (dsq / cl);
+                        float pb = mt == LX_MATERIAL_MICROFACET
+                                       ? lx_pdf(-d, ld, normal, rough, metal, color, mat.ior, mat.transmission)
+                                       : lx_legacy_pdf(mt, ld, normal);
+                        float w = (psa * psa) / (psa * psa + pb * pb);
+                        f3 brdf = mt == LX_MATERIAL_MICROFACET
+                                      ? lx_brdf(-d, normal, ld, color, rough, metal, mat.ior, mat.transmission)
+                                      : lx_legacy_brdf(mt, color);
+                        f3 Le = lx_get_emission(S, S.mats[R.dlmat], R.dluv);
+                        f3 contrib = ((Le * brdf) * cs) / psa;
                         col = col + (thr * contrib) * w;
                     }
                 }
-            }
-        } else if (J.dmis) {
-            if (emissive) {
-                float w = 1.0f;
-                if (depth > 1 && !spec) {
-                    float pb = last_pdf;
-                    float dist_sq = t * t;
-                    float cl = dot(N, -d);
-                    if (cl > 0.0001f) {
-                        float pa = 1.0f / S.total_area;
-                        float psa = pa * (dist_sq / cl);
-                        w = (pb * pb) / (pb * pb + psa * psa);
-                    } else {
-                        w = 0.0f;
-                    }
-                }
-                col = col + thr * (w * emission);
+            } else if (emissive) {
+                col = col + thr * emission;
                 kill = true;
-            } else if (R.dl_t > 0.0f && !is_specular) {
-                f3 sp = (t * d) + o;
-                f3 lv = R.lp - sp;
-                float dsq = dot(lv, lv);
-                float dist = lxm_sqrt(dsq);
-                f3 ld = lv / dist;
-                float cs = dot(nf, ld);
-                float cl = dot(R.dlN, -ld);
-                if (cs > 0.0f && cl > 0.0f) {
-                    float pa = 1.0f / S.total_area;
-                    float psa = pa * (dsq / cl);
-                    float pb = lx_pdf(-d, ld, normal, rough, metal, color, mat.ior, mat.transmission);
-                    float w = (psa * psa) / (psa * psa + pb * pb);
-                    f3 brdf = lx_brdf(-d, normal, ld, color, rough, metal, mat.ior, mat.transmission);
-                    f3 Le = lx_get_emission(S, S.mats[R.dlmat], R.dluv);
-                    f3 contrib = ((Le * brdf) * cs) / psa;
-                    col = col + (thr * contrib) * w;
-                }
             }
-        } else if (emissive) {
-            col = col + thr * emission;
-            kill = true;
-        }
-        if (!kill) {
-            /* update_throughput_materials, src/update_throughput_materials.cu:8-67 */
-            bool transmitted = dot(-d, normal) * dot(sample_dir, normal) < 0.0f;
-            bool nspec = is_specular || transmitted;
-            f3 ct = lx_ct_eval_cos(-d, normal, sample_dir, color, rough, metal, mat.ior, mat.transmission);
-            float pdf = lx_pdf(-d, sample_dir, normal, rough, metal, color, mat.ior, mat.transmission);
-            thr = thr * (ct / pdf);
-            last_pdf = pdf;
-            spec = nspec;
-            if (depth >= 3) {
-                float mt = lxm_max(thr.x, lxm_max(thr.y, thr.z));
-                float sv = lxm_clamp(mt, 0.0f, 1.0f);
-                if (rng.u01() >= sv) kill = true;
-                else thr = thr / sv;
+            if (!kill) {
+                /* update_throughput_materials, src/update_throughput_materials.cu:8-67 */
+                if (mt == LX_MATERIAL_MICROFACET) {
+                    /* only the uber build counts a transmission as specular */
+                    bool transmitted = !legacy && dot(-d, normal) * dot(sample_dir, normal) < 0.0f;
+                    bool nspec = is_specular || transmitted;
+                    f3 ct = lx_ct_eval_cos(-d, normal, sample_dir, color, rough, metal, mat.ior, mat.transmission);
+                    float pdf = lx_pdf(-d, sample_dir, normal, rough, metal, color, mat.ior, mat.transmission);
+                    thr = thr * (ct / pdf);
+                    last_pdf = pdf;
+                    spec = nspec;
+                } else {
+                    last_pdf = lx_legacy_update(mt, d, normal, sample_dir, color, mat.ior, mat.transmission, thr, last_pdf);
+                    spec = is_specular;
+                }
+                if (depth >= 3) {
+                    float mt = lxm_max(thr.x, lxm_max(thr.y, thr.z));
+                    float sv = lxm_clamp(mt, 0.0f, 1.0f);
+                    if (rng.u01() >= sv) kill = true;
+                    else thr = thr / sv;
+                }
             }
         }
     } else if (has_exr) {

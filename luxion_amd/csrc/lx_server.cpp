@@ -340,7 +340,7 @@ lx_status lx_server_submit(lx_server* S, const lx_job* job, lx_job_result* res, 
     try {
         LX_REQUIRE(S && job && res && id, LX_ERR_INVALID, "null argument");
         lx_validate_job(*job);
-        int depth = job->max_depth > 0 ? job->max_depth : job->scene->trace_depth;
+        int depth = lx_job_depth(job->max_depth, job->scene->trace_depth, job->material_debug_mode);
         LX_REQUIRE(depth <= S->max_depth, LX_ERR_INVALID, "job depth above the server's maximum");
         LX_REQUIRE(job->scene->dev.bvh_width == S->width, LX_ERR_INVALID,
                    "job scene BVH width differs from the server's (the context tuning's bvh_width at start)");

@@ -498,4 +498,140 @@ LXD f3 lx_sample_env_dir(const LxDevScene& S, LxRng& rng) {
     return mk3(st * lxm_cos(phi), lxm_cos(theta), st * lxm_sin(phi));
 }
 
+/* get_alpha, src/material_queries.cu:119-137 */
+LXD float lx_get_alpha(const LxDevScene& S, const lx_material_desc& m, f2 uv) {
+    if (m.alpha_mode == LX_ALPHA_OPAQUE) return 1.0f;
+    float a = m.alpha;
+    if (m.albedo_tex >= 0) {
+        uv = lx_apply_xf(uv, m.albedo_xf);
+        float t[4];
+        lx_tex_sample(S.texs[m.albedo_tex], uv.x, uv.y, t);
+        a *= t[3];
+    }
+    if (m.alpha_mode == LX_ALPHA_MASK) return a >= m.alpha_cutoff ? 1.0f : 0.0f;
+    return a;
+}
+
+/* The raw normal-map texel get_normal reports for the debug view
+ * (material_queries.cu:36-39, 83): (0.5, 0.5, 1) without a normal map. */
+LXD f3 lx_normal_map_rgb(const LxDevScene& S, const lx_material_desc& m, f2 uv) {
+    if (m.normal_tex < 0) return mk3(0.5f, 0.5f, 1.0f);
+    uv = lx_apply_xf(uv, m.normal_xf);
+    float t[4];
+    lx_tex_sample(S.texs[m.normal_tex], uv.x, uv.y, t);
+    return mk3(t[0], t[1], t[2]);
+}
+
+/* render_material_debug_mode, src/material_debug_render.cu:3-26 */
+LXD f3 lx_debug_color(const LxDevScene& S, const lx_material_desc& m, uint32_t mode, f3 albedo, f3 normal, f2 uv,
+                      float rough, float metal) {
+    switch (mode) {
+    case LX_DEBUG_ALBEDO: return albedo;
+    case LX_DEBUG_WORLD_NORMAL: return mk3(normal.x * 0.5f + 0.5f, normal.y * 0.5f + 0.5f, normal.z * 0.5f + 0.5f);
+    case LX_DEBUG_NORMAL_MAP: return lx_normal_map_rgb(S, m, uv);
+    case LX_DEBUG_ROUGHNESS: return mk3(rough, rough, rough);
+    case LX_DEBUG_METALLIC: return mk3(metal, metal, metal);
+    case LX_DEBUG_ALPHA: {
+        float a = lx_get_alpha(S, m, uv);
+        return mk3(a, a, a);
+    }
+    default: return mk3(0.0f, 0.0f, 0.0f);
+    }
+}
+
+/* ------------------------------------------- legacy (UBER_SHADER 0) shaders */
+/* glm::reflect(I, N) = I - N * dot(N, I) * 2 */
+LXD f3 lx_reflect(f3 I, f3 N) { return I - ((N * dot(N, I)) * 2.0f); }
+
+/* Lambert::PDF, src/shaders/lambert.cu:9-12 */
+LXD float lx_lambert_pdf(f3 wi, f3 n) { return lxm_max(lxm_max(dot(wi, n), 0.0f) * LX_INV_PI_CT, 1e-6f); }
+
+/* TransmissiveGlass::FresnelDielectricEval, src/shaders/glass.cu:27-55 (the
+ * glass shader's own Fresnel, which takes the raw ior) */
+LXD float lx_glass_fresnel(float ci, float ior) {
+    float ei = 1.0f, et = ior;
+    ci = lxm_clamp(ci, -1.0f, 1.0f);
+    if (!(ci > 0.0f)) {
+        float tmp = ei;
+        ei = et;
+        et = tmp;
+        ci = lxm_abs(ci);
+    }
+    float si = lxm_sqrt(lxm_max(0.0f, 1.0f - ci * ci));
+    float st = (ei / et) * si;
+    if (st >= 1.0f) return 1.0f;
+    float ct = lxm_sqrt(lxm_max(0.0f, 1.0f - st * st));
+    float rpa = ((et * ci) - (ei * ct)) / ((et * ci) + (ei * ct));
+    float rpe = ((ei * ci) - (et * ct)) / ((ei * ci) + (et * ct));
+    return (rpa * rpa + rpe * rpe) / 2.0f;
+}
+
+/* TransmissiveGlass::sampleGlass with sampleSpecularRefl / sampleSpecularTrans
+ * (glass.cu:5-25, 57-75); glm::refract yields NaN under total internal
+ * reflection, which the shader turns into the mirror direction. */
+LXD f3 lx_sample_glass(f3 d, f3 n, float ior, LxRng& rng) {
+    f3 wo = -d;
+    float r = rng.u01();
+    float F = lxm_clamp(lx_glass_fresnel(dot(wo, n), ior), 0.0f, 1.0f);
+    bool enter = dot(wo, n) > 0.0f;
+    f3 ln = enter ? n : -n;
+    f3 I = -wo;
+    if (r < F) return lx_reflect(I, ln);
+    float eta = enter ? (1.0f / ior) : (ior / 1.0f);
+    float dv = dot(ln, I);
+    float k = 1.0f - eta * eta * (1.0f - dv * dv);
+    if (!(k >= 0.0f)) return lx_reflect(I, ln);
+    f3 wi = (eta * I) - ((eta * dv + lxm_sqrt(k)) * ln);
+    if (dot(wi, wi) < 1e-6f || lxm_isnan(wi.x) || lxm_isnan(wi.y) || lxm_isnan(wi.z)) return lx_reflect(I, ln);
+    return normalize(wi);
+}
+
+/* sample_materials legacy dispatch, src/sample_materials.cu:24-37 (the
+ * microfacet class takes the uber-shader sampler) */
+LXD f3 lx_sample_legacy(int type, f3 d, f3 normal, float ior, LxRng& rng) {
+    if (type == LX_MATERIAL_DIFFUSE || type == LX_MATERIAL_EMISSIVE) return lx_cos_hemisphere(normal, rng);
+    if (type == LX_MATERIAL_SPECULAR) return lx_reflect(-(-d), normal); /* sampleMirror, specular.cu:14-19 */
+    return lx_sample_glass(d, normal, ior, rng);                         /* LX_MATERIAL_GLASS */
+}
+
+/* get_pdf_bsdf / get_brdf legacy branches, src/kernels/shade_path.cu:60-92:
+ * Lambert for diffuse, zero for the delta and emissive classes. */
+LXD float lx_legacy_pdf(int type, f3 wi, f3 normal) {
+    return type == LX_MATERIAL_DIFFUSE ? lx_lambert_pdf(wi, normal) : 0.0f;
+}
+LXD f3 lx_legacy_brdf(int type, f3 color) {
+    return type == LX_MATERIAL_DIFFUSE ? color / LX_PI_F : mk3(0.0f, 0.0f, 0.0f);
+}
+
+/* update_throughput_materials legacy branches for the non-microfacet
+ * classes, src/update_throughput_materials.cu:36-62; returns last_pdf (the
+ * emissive class leaves the path and last_pdf as they are). */
+LXD float lx_legacy_update(int type, f3 d, f3 normal, f3 wi, f3 color, float ior, float tr, f3& thr, float last_pdf) {
+    if (type == LX_MATERIAL_DIFFUSE) {
+        /* Lambert::shadePathLambert / PDF, lambert.cu:5-30 */
+        f3 f = (color * LX_INV_PI_CT) * lxm_max(dot(wi, normal), 0.0f);
+        float pdf = lx_lambert_pdf(wi, normal);
+        thr = thr * (f / pdf);
+        return pdf;
+    }
+    if (type == LX_MATERIAL_SPECULAR) { /* shadePathSpecular, specular.cu:5-12 */
+        thr = thr * color;
+        return 1.0f;
+    }
+    if (type == LX_MATERIAL_GLASS) { /* shadePathGlass, glass.cu:77-95 */
+        f3 wo = -d;
+        float ci = dot(wo, normal);
+        bool enter = ci > 0.f;
+        float ei = enter ? 1.0f : ior, et = enter ? ior : 1.0f;
+        bool refr = dot(wo, normal) * dot(wi, normal) < 0.0f;
+        float at = refr ? (ei / et) : 1.0f;
+        at = at * at;
+        f3 tint = refr ? mk3(1.0f + tr * (color.x - 1.0f), 1.0f + tr * (color.y - 1.0f), 1.0f + tr * (color.z - 1.0f))
+                       : mk3(1.0f, 1.0f, 1.0f);
+        thr = thr * (at * tint);
+        return 1.0f;
+    }
+    return last_pdf;
+}
+
 #endif

@@ -109,8 +109,10 @@ struct LxJob {
     int32_t iter;   /* current 1-based iteration */
     int32_t active; /* takes part in this round */
     int32_t spp_done;
-    int32_t pad;
+    uint32_t flags; /* LX_JF_*: material debug view, legacy shading */
 };
+#define LX_JF_DEBUG_MASK 7u /* LX_DEBUG_* view, 0 = off */
+#define LX_JF_LEGACY 8u     /* LX_SHADING_LEGACY */
 
 /* Path state (struct of arrays, capacity N). */
 struct LxPathBuf {

@@ -38,11 +38,12 @@ def look_at_camera(eye, target, up, fovy_deg, width, height):
 def material(color=(0.8, 0.8, 0.8), roughness=1.0, metallic=0.0, emission=(0.0, 0.0, 0.0), strength=0.0,
              ior=1.5, transmission=0.0, alpha=1.0, alpha_mode=F.LX_ALPHA_OPAQUE, alpha_cutoff=0.5,
              albedo_tex=-1, mr_tex=-1, normal_tex=-1, emissive_tex=-1, uv_scale=(1.0, 1.0), uv_rotation=0.0,
-             uv_offset=(0.0, 0.0)):
+             uv_offset=(0.0, 0.0), material_type=F.LX_MATERIAL_MICROFACET):
+    """material_type (LX_MATERIAL_*) is read only by LX_SHADING_LEGACY renders."""
     return dict(color=color, roughness=roughness, metallic=metallic, emission=emission, strength=strength, ior=ior,
                 transmission=transmission, alpha=alpha, alpha_mode=alpha_mode, alpha_cutoff=alpha_cutoff,
                 albedo_tex=albedo_tex, mr_tex=mr_tex, normal_tex=normal_tex, emissive_tex=emissive_tex,
-                uv_scale=uv_scale, uv_rotation=uv_rotation, uv_offset=uv_offset)
+                uv_scale=uv_scale, uv_rotation=uv_rotation, uv_offset=uv_offset, material_type=material_type)
 
 
 def affine(translate=(0, 0, 0), rot_y=0.0, scale=(1, 1, 1), rot_x=0.0, rot_z=0.0):
@@ -207,6 +208,7 @@ def material_desc(m):
     d.emissive_tex, d.emissive_xf = m["emissive_tex"], xf
     d.ior, d.transmission, d.alpha = m["ior"], m["transmission"], m["alpha"]
     d.alpha_mode, d.alpha_cutoff = m["alpha_mode"], m["alpha_cutoff"]
+    d.material_type = m.get("material_type", F.LX_MATERIAL_MICROFACET)
     return d
 
 
@@ -420,6 +422,80 @@ def checker_textures(size=1024, squares=16):
     holes = ((x * squares * 2 // size) % 2 == 0) & ((y * squares * 2 // size) % 2 == 0)
     mask[..., 3] = np.where(holes, 0, 255).astype(np.uint8)
     return base, mask
+
+
+def sphere_uv(v):
+    """Spherical uv of unit-sphere points: u from the azimuth, v from the polar angle."""
+    r = np.linalg.norm(v, axis=1)
+    return np.stack([0.5 + np.arctan2(v[:, 2], v[:, 0]) / (2 * math.pi),
+                     np.arccos(np.clip(v[:, 1] / r, -1, 1)) / math.pi], 1)
+
+
+def bump_normal_map(size=256, freq=8):
+    """RGBA8 tangent-space normal map of a sinusoidal bump field."""
+    y, x = np.mgrid[0:size, 0:size] / size
+    dx = 0.35 * np.cos(2 * math.pi * freq * x)
+    dy = 0.35 * np.cos(2 * math.pi * freq * y)
+    n = np.stack([-dx, -dy, np.ones_like(dx)], -1)
+    n /= np.linalg.norm(n, axis=-1, keepdims=True)
+    out = np.empty((size, size, 4), np.uint8)
+    out[..., 0:3] = np.round((n * 0.5 + 0.5) * 255).astype(np.uint8)
+    out[..., 3] = 255
+    return out
+
+
+def material_zoo(width=320, height=180, level=3, env_size=(256, 128), trace_depth=8, flat=False):
+    """One sphere per legacy material class (diffuse, mirror, glass, microfacet
+    with metallic-roughness and normal maps, emissive) on a checker floor, an
+    alpha-MASK and an alpha-BLEND panel, a quad light and a synthetic HDRI.
+    Every material also renders under the uber shader.  flat: spheres
+    without vertex normals (faceted)."""
+    s = SceneData()
+    s.name = "material_zoo"
+    base, mask = checker_textures(128, 8)
+    nm = bump_normal_map(128, 6)
+    mr = np.empty((64, 64, 4), np.uint8)
+    yy, xx = np.mgrid[0:64, 0:64]
+    mr[..., 0] = 0
+    mr[..., 1] = (40 + 3 * xx).astype(np.uint8)   # roughness scale
+    mr[..., 2] = (255 - 3 * yy).astype(np.uint8)  # metallic scale
+    mr[..., 3] = 255
+    s.textures += [base, mask, nm, mr]
+    T_BASE, T_MASK, T_NRM, T_MR = 0, 1, 2, 3
+    m_floor = s.add_material(material((1, 1, 1), roughness=0.8, albedo_tex=T_BASE, uv_scale=(4.0, 4.0),
+                                      material_type=F.LX_MATERIAL_DIFFUSE))
+    m_diff = s.add_material(material((0.7, 0.3, 0.2), roughness=1.0, material_type=F.LX_MATERIAL_DIFFUSE))
+    m_mirror = s.add_material(material((0.9, 0.9, 0.95), roughness=0.0, metallic=1.0,
+                                       material_type=F.LX_MATERIAL_SPECULAR))
+    m_glass = s.add_material(material((0.8, 1.0, 0.9), roughness=0.0, transmission=0.6, ior=1.45,
+                                      material_type=F.LX_MATERIAL_GLASS))
+    m_micro = s.add_material(material((0.9, 0.7, 0.3), roughness=0.6, metallic=0.8, mr_tex=T_MR, normal_tex=T_NRM,
+                                      uv_scale=(2.0, 1.0), uv_rotation=0.3, material_type=F.LX_MATERIAL_MICROFACET))
+    m_emit = s.add_material(material((0.2, 0.2, 0.2), emission=(1.0, 0.85, 0.6), strength=4.0,
+                                     material_type=F.LX_MATERIAL_EMISSIVE))
+    m_mask = s.add_material(material((1, 1, 1), roughness=0.5, albedo_tex=T_MASK, alpha_mode=F.LX_ALPHA_MASK,
+                                     alpha_cutoff=0.5))
+    m_blend = s.add_material(material((0.3, 0.5, 0.9), roughness=0.4, alpha=0.45, alpha_mode=F.LX_ALPHA_BLEND))
+    m_light = s.add_material(material((1, 1, 1), emission=(1.0, 1.0, 1.0), strength=12.0,
+                                      material_type=F.LX_MATERIAL_EMISSIVE))
+    p, idx = _quad((-4, 0, -4), (-4, 0, 4), (4, 0, 4), (4, 0, -4), np.array((0, 1, 0), np.float32))
+    uv = (p[:, [0, 2]] + 4.0) / 8.0
+    s.add_instance(s.add_mesh(p, idx, None, uv), m_floor)
+    v, f = icosphere(level)
+    sph = s.add_mesh(v, f, None if flat else v, sphere_uv(v))
+    for k, m in enumerate((m_diff, m_mirror, m_glass, m_micro, m_emit)):
+        s.add_instance(sph, m, affine((-2.4 + 1.2 * k, 0.5, 0.0), 0.4 * k, (0.5, 0.5, 0.5)))
+    pp, pi = _quad((-0.6, 0, 0), (0.6, 0, 0), (0.6, 1.2, 0), (-0.6, 1.2, 0), np.array((0, 0, 1), np.float32))
+    puv = np.stack([(pp[:, 0] + 0.6) / 1.2, pp[:, 1] / 1.2], 1)
+    panel = s.add_mesh(pp, pi, None, puv)
+    s.add_instance(panel, m_mask, affine((-1.0, 0.0, 1.3), 0.2))
+    s.add_instance(panel, m_blend, affine((1.0, 0.0, 1.3), -0.2))
+    lp, li = _quad((-0.5, 0, -0.5), (0.5, 0, -0.5), (0.5, 0, 0.5), (-0.5, 0, 0.5), np.array((0, -1, 0), np.float32))
+    s.add_instance(s.add_mesh(lp, li), m_light, affine((0.0, 3.0, 0.5)))
+    s.env = synthetic_hdri(env_size[0], env_size[1], sun_radiance=50.0, sun_diameter_deg=4.0)
+    s.camera = look_at_camera((0.0, 1.6, 5.5), (0.0, 0.5, 0.0), (0, 1, 0), 45.0, width, height)
+    s.trace_depth = trace_depth
+    return s
 
 
 def instanced_interior(width=1920, height=1080, n_meshes=64, n_instances=10000, tri_range=(10000, 500000),

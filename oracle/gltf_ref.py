@@ -123,7 +123,31 @@ def load(path):
     mats = [dict(base_color=(1.0, 0.0, 0.0), roughness=0.0, metallic=0.0, ior=0.0, transmission=0.0,
                  emission_color=(0.0, 0.0, 0.0), emission_strength=0.0, alpha=1.0, alpha_mode=0, alpha_cutoff=0.5,
                  albedo_tex=-1, metallic_roughness_tex=-1, normal_tex=-1, emissive_tex=-1, albedo_xf=xf(None),
-                 normal_xf=xf(None), emissive_xf=xf(None), metallic_roughness_xf=xf(None))]
+                 normal_xf=xf(None), emissive_xf=xf(None), metallic_roughness_xf=xf(None), material_type=0)]
+
+    def is_glass(m, base_alpha):
+        """isGlass, include/scene/gltf_material_utils.h:9-47."""
+        ext = m.get("extensions", {})
+        if "KHR_materials_transmission" in ext or "KHR_materials_volume" in ext:
+            return True
+        if "ior" in ext.get("KHR_materials_ior", {}) and abs(float(ext["KHR_materials_ior"]["ior"]) - 1.0) > 0.01:
+            return True
+        if m.get("alphaMode", "OPAQUE") in ("BLEND", "MASK") and base_alpha < float(np.float32(0.99)):
+            return True
+        df = ext.get("KHR_materials_pbrSpecularGlossiness", {}).get("diffuseFactor")
+        return isinstance(df, list) and len(df) == 4 and df[3] < 0.99
+
+    def legacy_type(d, m, base_alpha):
+        """Material class of the UBER_SHADER 0 loader branch
+        (src/scene/scene_load_gltf.cpp:211-228): 3 emissive, 4 glass, 0 microfacet."""
+        e = np.asarray(d["emission_color"], np.float32)
+        ln = np.sqrt(np.float32(np.float32(e[0] * e[0] + e[1] * e[1]) + e[2] * e[2]))
+        if np.float32(np.float32(d["emission_strength"]) * ln) > np.float32(1e-5):
+            return 3
+        if is_glass(m, base_alpha) and np.float32(d["metallic"]) < np.float32(0.01) and \
+                np.float32(d["roughness"]) < np.float32(0.01):
+            return 4
+        return 0
     for m in js.get("materials", []):
         pbr = m.get("pbrMetallicRoughness", {})
         bc = pbr.get("baseColorFactor", [1.0, 1.0, 1.0, 1.0])
@@ -144,6 +168,7 @@ def load(path):
             albedo_xf=xf(pbr.get("baseColorTexture")), normal_xf=xf(m.get("normalTexture")),
             emissive_xf=xf(m.get("emissiveTexture")),
             metallic_roughness_xf=xf(pbr.get("metallicRoughnessTexture"))))
+        mats[-1]["material_type"] = legacy_type(mats[-1], m, bc[3])
 
     meshes, insts, mesh_of = [], [], {}
     cam = []

@@ -38,6 +38,7 @@
 #include <numeric>
 #include <thread>
 #include <vector>
+#include <limits>
 
 namespace {
 
@@ -428,6 +429,7 @@ struct Mat {
     float ior, transmission, alpha;
     int alpha_mode;
     float alpha_cutoff;
+    int type; /* LX_MATERIAL_*: legacy shading class */
 };
 
 struct Inst {
@@ -957,6 +959,43 @@ static V3 get_emission(const lxo_scene& s, const Mat& m, V2 uv) {
     return e;
 }
 
+/* get_alpha, material_queries.cu:119-137 */
+static float get_alpha(const lxo_scene& s, const Mat& m, V2 uv) {
+    if (m.alpha_mode == LX_ALPHA_OPAQUE) return 1.0f;
+    float alpha = m.alpha;
+    if (m.albedo_tex >= 0) {
+        V2 tuv = apply_xf(uv, m.albedo_xf);
+        float t[4];
+        tex_sample(s.texs[m.albedo_tex], tuv.x, tuv.y, t);
+        alpha = alpha * t[3];
+    }
+    if (m.alpha_mode == LX_ALPHA_MASK) return alpha >= m.alpha_cutoff ? 1.0f : 0.0f;
+    return alpha;
+}
+
+/* render_material_debug_mode, src/material_debug_render.cu:3-26, with the
+ * normal-map texel get_normal writes to out_normal_map
+ * (material_queries.cu:36-39 and 83). */
+static V3 debug_view(const lxo_scene& s, const Mat& m, int mode, V3 albedo, V3 normal, V2 uv, float rough,
+                     float metal) {
+    if (mode == LX_DEBUG_ALBEDO) return albedo;
+    if (mode == LX_DEBUG_WORLD_NORMAL) return add(muls(normal, 0.5f), v3(0.5f, 0.5f, 0.5f));
+    if (mode == LX_DEBUG_NORMAL_MAP) {
+        if (m.normal_tex < 0) return v3(0.5f, 0.5f, 1.0f);
+        V2 tuv = apply_xf(uv, m.normal_xf);
+        float t[4];
+        tex_sample(s.texs[m.normal_tex], tuv.x, tuv.y, t);
+        return v3(t[0], t[1], t[2]);
+    }
+    if (mode == LX_DEBUG_ROUGHNESS) return v3(rough, rough, rough);
+    if (mode == LX_DEBUG_METALLIC) return v3(metal, metal, metal);
+    if (mode == LX_DEBUG_ALPHA) {
+        float a = get_alpha(s, m, uv);
+        return v3(a, a, a);
+    }
+    return v3(0.0f, 0.0f, 0.0f);
+}
+
 /* ------------------------------------------------------ Cook-Torrance */
 /* cook_torrance.cu:8-25 */
 static float D_TR(V3 h, V3 n, float alpha) {
@@ -1157,6 +1196,101 @@ static V3 ct_eval_cos(V3 wo, V3 normal, V3 wi, V3 albedo, float roughness, float
     return muls(f, lxm_abs(dot(wi, normal)));
 }
 
+/* ---------------------------------------- legacy shaders (UBER_SHADER 0) */
+/* glm::reflect: I - N * dot(N, I) * 2 */
+static V3 glm_reflect(V3 I, V3 N) { return sub(I, muls(muls(N, dot(N, I)), 2.0f)); }
+
+/* Lambert::PDF, src/shaders/lambert.cu:9-12: max(1e-6, max(0, cos) / pi) */
+static float lambert_pdf(V3 wi, V3 n) {
+    float c = dot(wi, n);
+    if (!(c > 0.0f)) c = 0.0f;
+    float p = c * INV_PI_CT;
+    return p > 1e-6f ? p : 1e-6f;
+}
+
+/* TransmissiveGlass::FresnelDielectricEval, src/shaders/glass.cu:27-55 */
+static float glass_fresnel(float cos_i, float ior) {
+    float eta_i = 1.0f, eta_t = ior;
+    cos_i = lxm_clamp(cos_i, -1.0f, 1.0f);
+    if (!(cos_i > 0.0f)) {
+        eta_i = ior;
+        eta_t = 1.0f;
+        cos_i = lxm_abs(cos_i);
+    }
+    float sin_i = lxm_sqrt(lxm_max(0.0f, 1.0f - cos_i * cos_i));
+    float sin_t = (eta_i / eta_t) * sin_i;
+    if (sin_t >= 1.0f) return 1.0f; /* total internal reflection */
+    float cos_t = lxm_sqrt(lxm_max(0.0f, 1.0f - sin_t * sin_t));
+    float par = ((eta_t * cos_i) - (eta_i * cos_t)) / ((eta_t * cos_i) + (eta_i * cos_t));
+    float per = ((eta_i * cos_i) - (eta_t * cos_t)) / ((eta_i * cos_i) + (eta_t * cos_t));
+    return (par * par + per * per) / 2.0f;
+}
+
+/* glm::refract(I, N, eta) as the vector overload computes it: NaN when k < 0
+ * (sqrt of a negative times 0). */
+static V3 glm_refract(V3 I, V3 N, float eta) {
+    float dv = dot(N, I);
+    float k = 1.0f - eta * eta * (1.0f - dv * dv);
+    float mask = k >= 0.0f ? 1.0f : 0.0f;
+    float root = k >= 0.0f ? lxm_sqrt(k) : std::numeric_limits<float>::quiet_NaN();
+    return muls(sub(smul(eta, I), smul(eta * dv + root, N)), mask);
+}
+
+/* sampleGlass (glass.cu:57-75) with sampleSpecularRefl / sampleSpecularTrans
+ * (glass.cu:5-25) */
+static V3 sample_glass(V3 dir, V3 n, float ior, Rng& rng) {
+    V3 wo = neg(dir);
+    float r = rng.u01();
+    float F = lxm_clamp(glass_fresnel(dot(wo, n), ior), 0.0f, 1.0f);
+    bool enter = dot(wo, n) > 0.0f;
+    V3 loc = enter ? n : neg(n);
+    if (r < F) return glm_reflect(neg(wo), loc);
+    V3 wi = glm_refract(neg(wo), loc, enter ? (1.0f / ior) : (ior / 1.0f));
+    if (dot(wi, wi) < 1e-6f || lxm_isnan(wi.x) || lxm_isnan(wi.y) || lxm_isnan(wi.z)) return glm_reflect(neg(wo), loc);
+    return normalize(wi);
+}
+
+/* sample_materials, src/sample_materials.cu:24-37, non-microfacet classes */
+static V3 legacy_sample(int type, V3 dir, V3 n, float ior, Rng& rng) {
+    switch (type) {
+    case LX_MATERIAL_DIFFUSE:
+    case LX_MATERIAL_EMISSIVE: return cosine_hemisphere(n, rng); /* Lambert::sampleHemisphere */
+    case LX_MATERIAL_SPECULAR: return glm_reflect(neg(neg(dir)), n); /* PerfectSpecular::sampleMirror */
+    default: return sample_glass(dir, n, ior, rng);
+    }
+}
+
+/* update_throughput_materials, src/update_throughput_materials.cu:36-62,
+ * non-microfacet classes; returns the new last_pdf. */
+static float legacy_update(int type, V3 dir, V3 n, V3 wi, V3 color, float ior, float transmission, V3& thr,
+                           float last_pdf) {
+    if (type == LX_MATERIAL_DIFFUSE) {
+        float c = dot(wi, n);
+        V3 f = muls(muls(color, INV_PI_CT), c > 0.0f ? c : 0.0f);
+        float pdf = lambert_pdf(wi, n);
+        thr = mul(thr, divs(f, pdf));
+        return pdf;
+    }
+    if (type == LX_MATERIAL_SPECULAR) {
+        thr = mul(thr, color);
+        return 1.0f;
+    }
+    if (type == LX_MATERIAL_GLASS) {
+        V3 wo = neg(dir);
+        bool enter = dot(wo, n) > 0.f;
+        float eta_i = enter ? 1.0f : ior;
+        float eta_t = enter ? ior : 1.0f;
+        bool refracted = dot(wo, n) * dot(wi, n) < 0.0f;
+        float att = refracted ? (eta_i / eta_t) : 1.0f;
+        att = att * att;
+        V3 one = v3(1.0f, 1.0f, 1.0f);
+        V3 tint = refracted ? add(one, smul(transmission, sub(color, one))) : one; /* glm::mix(1, color, T) */
+        thr = mul(thr, smul(att, tint));
+        return 1.0f;
+    }
+    return last_pdf;
+}
+
 /* ------------------------------------------------------- CDF sampling */
 /* cudaUtils::select_from_cdf, src/common.cu:3-15 */
 static int select_from_cdf(const float* cdf, int size, float xi) {
@@ -1257,6 +1391,8 @@ struct BounceRecords {
 struct RenderCfg {
     int W, H, depth;
     bool dmis, emis;
+    int debug_mode; /* LX_DEBUG_* */
+    bool legacy;    /* LX_SHADING_LEGACY */
     float intensity;
     uint32_t seed;
     lx_camera_desc cam;
@@ -1291,15 +1427,35 @@ static bool shade_one(const lxo_scene& s, const RenderCfg& cfg, Path& p, const B
     V3 dir = p.d;
     if (is.t > 0.0f) {
         const Mat& mat = s.mats[is.mat];
+        /* UBER_SHADER 1 treats every material as microfacet; the legacy build
+         * dispatches on the class (shade_path.cu:44-92) */
+        int type = cfg.legacy ? mat.type : LX_MATERIAL_MICROFACET;
         V3 color = get_albedo(s, mat, is.uv);
+        if (type == LX_MATERIAL_EMISSIVE && mat.albedo_tex < 0) color = mat.emission_color; /* gltf.cpp:211-213 */
         V3 normal = get_normal(s, mat, is);
         float rough, metal;
         get_mr(s, mat, is.uv, rough, metal);
         V3 emission = get_emission(s, mat, is.uv);
-        bool is_specular = rough < 0.05f;
+        bool is_specular = cfg.legacy ? (type == LX_MATERIAL_SPECULAR || type == LX_MATERIAL_GLASS) : rough < 0.05f;
         V3 nf = dot(neg(dir), normal) < 0.0f ? neg(normal) : normal;
-        p.sample_dir = sample_ct(neg(dir), normal, rough, metal, rng, color, mat.ior, mat.transmission);
-        bool emissive = length(emission) > 0.00001f;
+        if (cfg.debug_mode != 0) {
+            /* render_material_debug_mode: the view replaces the colour, the path ends */
+            p.color = debug_view(s, mat, cfg.debug_mode, color, normal, is.uv, rough, metal);
+            p.kill = true;
+        } else {
+        bool micro = type == LX_MATERIAL_MICROFACET;
+        if (micro) p.sample_dir = sample_ct(neg(dir), normal, rough, metal, rng, color, mat.ior, mat.transmission);
+        else p.sample_dir = legacy_sample(type, dir, normal, mat.ior, rng);
+        bool emissive = cfg.legacy ? type == LX_MATERIAL_EMISSIVE : length(emission) > 0.00001f;
+        /* get_pdf_bsdf / get_brdf, shade_path.cu:60-92 */
+        auto pdf_bsdf = [&](V3 wi) {
+            return micro ? PDF(neg(dir), wi, normal, rough, metal, color, mat.ior, mat.transmission)
+                         : (type == LX_MATERIAL_DIFFUSE ? lambert_pdf(wi, normal) : 0.0f);
+        };
+        auto brdf_of = [&](V3 wi) {
+            return micro ? BRDF(neg(dir), normal, wi, color, rough, metal, mat.ior, mat.transmission)
+                         : (type == LX_MATERIAL_DIFFUSE ? divs(color, PI_F) : v3(0.0f, 0.0f, 0.0f));
+        };
         if (cfg.emis) {
             if (emissive) {
                 float w = 1.0f;
@@ -1313,9 +1469,9 @@ static bool shade_one(const lxo_scene& s, const RenderCfg& cfg, Path& p, const B
             } else if (br.env_t < 0.0f && !is_specular && !p.kill && has_exr) {
                 V3 ed = p.env_sample;
                 float pe = envmap_pdf(s, ed);
-                float pb = PDF(neg(dir), ed, normal, rough, metal, color, mat.ior, mat.transmission);
+                float pb = pdf_bsdf(ed);
                 float w = (pe * pe) / (pe * pe + pb * pb);
-                V3 brdf = BRDF(neg(dir), normal, ed, color, rough, metal, mat.ior, mat.transmission);
+                V3 brdf = brdf_of(ed);
                 V3 L = env_lookup(s, normalize(ed), cfg.intensity);
                 float cs = dot(nf, ed);
                 if (cs > 0.0f && pe > 1e-8f) {
@@ -1351,9 +1507,9 @@ static bool shade_one(const lxo_scene& s, const RenderCfg& cfg, Path& p, const B
                 if (cs > 0.0f && cl > 0.0f) {
                     float pa = 1.0f / s.total_area;
                     float psa = pa * (dsq / cl);
-                    float pb = PDF(neg(dir), ld, normal, rough, metal, color, mat.ior, mat.transmission);
+                    float pb = pdf_bsdf(ld);
                     float w = (psa * psa) / (psa * psa + pb * pb);
-                    V3 brdf = BRDF(neg(dir), normal, ld, color, rough, metal, mat.ior, mat.transmission);
+                    V3 brdf = brdf_of(ld);
                     V3 Le = get_emission(s, s.mats[br.dl.mat], br.dl.uv);
                     V3 contrib = divs(muls(mul(Le, brdf), cs), psa);
                     p.color = add(p.color, muls(mul(p.thr, contrib), w));
@@ -1365,19 +1521,26 @@ static bool shade_one(const lxo_scene& s, const RenderCfg& cfg, Path& p, const B
         }
         if (!p.kill) {
             /* update_throughput_materials, update_throughput_materials.cu:8-67 */
-            bool transmitted = dot(neg(dir), normal) * dot(p.sample_dir, normal) < 0.0f;
-            bool spec = is_specular || transmitted;
-            V3 ct = ct_eval_cos(neg(dir), normal, p.sample_dir, color, rough, metal, mat.ior, mat.transmission);
-            float pdf = PDF(neg(dir), p.sample_dir, normal, rough, metal, color, mat.ior, mat.transmission);
-            p.thr = mul(p.thr, divs(ct, pdf));
-            p.last_pdf = pdf;
-            p.spec = spec;
+            if (micro) {
+                bool transmitted = !cfg.legacy && dot(neg(dir), normal) * dot(p.sample_dir, normal) < 0.0f;
+                bool spec = is_specular || transmitted;
+                V3 ct = ct_eval_cos(neg(dir), normal, p.sample_dir, color, rough, metal, mat.ior, mat.transmission);
+                float pdf = PDF(neg(dir), p.sample_dir, normal, rough, metal, color, mat.ior, mat.transmission);
+                p.thr = mul(p.thr, divs(ct, pdf));
+                p.last_pdf = pdf;
+                p.spec = spec;
+            } else {
+                p.last_pdf = legacy_update(type, dir, normal, p.sample_dir, color, mat.ior, mat.transmission, p.thr,
+                                           p.last_pdf);
+                p.spec = is_specular;
+            }
             if (depth >= 3) {
                 float mt = lxm_max(p.thr.x, lxm_max(p.thr.y, p.thr.z));
                 float sv = lxm_clamp(mt, 0.0f, 1.0f);
                 if (rng.u01() >= sv) p.kill = true;
                 else p.thr = divs(p.thr, sv);
             }
+        }
         }
     } else if (!p.kill && has_exr) {
         V3 d = normalize(dir);
@@ -1526,6 +1689,9 @@ static RenderCfg make_cfg(const lxo_scene& s, const lxo_render_opts* o) {
     c.cam.width = c.W;
     c.cam.height = c.H;
     c.depth = o->max_depth > 0 ? o->max_depth : s.trace_depth;
+    c.debug_mode = o->material_debug_mode;
+    if (c.debug_mode != 0) c.depth = 1; /* pathtrace.cu:107 */
+    c.legacy = o->shading_model == LX_SHADING_LEGACY;
     c.dmis = o->dmis != 0;
     c.emis = o->emis != 0;
     c.intensity = o->envmap_intensity;
@@ -1601,6 +1767,7 @@ lxo_scene* lxo_scene_create(const lx_scene_desc* d, int cdf_mode, int build_bvh)
         m.alpha = a.alpha;
         m.alpha_mode = a.alpha_mode;
         m.alpha_cutoff = a.alpha_cutoff;
+        m.type = a.material_type;
     }
     s->texs.resize(d->num_textures);
     for (uint32_t i = 0; i < d->num_textures; i++) decode_texture(d->textures[i], s->texs[i]);
@@ -1801,6 +1968,35 @@ void lxo_ct_btdf(const float* wo, const float* n, const float* wi, const float* 
 float lxo_ct_fresnel_dielectric(float c, float ei, float et) { return F_diel(c, ei, et); }
 float lxo_ct_d(const float* h, const float* n, float alpha) { return D_TR(ld3(h), ld3(n), alpha); }
 float lxo_ct_smith_g1(const float* w, const float* n, float alpha) { return G1(ld3(w), ld3(n), alpha); }
+
+float lxo_glass_fresnel(float cos_i, float ior) { return glass_fresnel(cos_i, ior); }
+float lxo_lambert_pdf(const float* wi, const float* n) { return lambert_pdf(ld3(wi), ld3(n)); }
+
+int lxo_legacy_sample(int type, const float* dir, const float* n, float ior, int iter, int index, int depth,
+                      uint32_t seed, float* dir3) {
+    Rng rng = make_rng(iter, index, depth, seed);
+    Rng probe = rng;
+    V3 d = legacy_sample(type, ld3(dir), ld3(n), ior, rng);
+    dir3[0] = d.x;
+    dir3[1] = d.y;
+    dir3[2] = d.z;
+    int k = 0;
+    while (probe.x != rng.x && k < 16) {
+        probe.u01();
+        k++;
+    }
+    return k;
+}
+
+float lxo_legacy_update(int type, const float* dir, const float* n, const float* wi, const float* color, float ior,
+                        float transmission, float* thr3) {
+    V3 thr = ld3(thr3);
+    float pdf = legacy_update(type, ld3(dir), ld3(n), ld3(wi), ld3(color), ior, transmission, thr, -1.0f);
+    thr3[0] = thr.x;
+    thr3[1] = thr.y;
+    thr3[2] = thr.z;
+    return pdf;
+}
 
 int lxo_ct_sample(const float* wo, const float* n, float r, float m, const float* c, float ior, float tr, int iter,
                   int index, int depth, uint32_t seed, float* dir3) {

@@ -46,6 +46,8 @@ typedef struct lxo_render_opts {
     lx_camera_desc camera;
     int32_t traverse_mode;
     int32_t threads;
+    int32_t material_debug_mode; /* LX_DEBUG_* */
+    int32_t shading_model;       /* LX_SHADING_* */
 } lxo_render_opts;
 
 typedef struct lxo_stats {
@@ -77,6 +79,15 @@ void lxo_ct_brdf(const float* v, const float* n, const float* l, const float* al
 void lxo_ct_btdf(const float* wo, const float* n, const float* wi, const float* albedo, float roughness,
                  float metallic, float ior, float transmission, float* out3);
 float lxo_ct_fresnel_dielectric(float cos_i, float eta_i, float eta_t);
+/* Legacy (UBER_SHADER 0) shaders: the glass shader's Fresnel; one sample of
+ * class `type` from RNG(iter, index, depth) (returns the u01 draws used);
+ * the throughput factor and last_pdf of update_throughput_materials. */
+float lxo_glass_fresnel(float cos_i, float ior);
+int lxo_legacy_sample(int type, const float* dir, const float* n, float ior, int iter, int index, int depth,
+                      uint32_t seed, float* dir3);
+float lxo_legacy_update(int type, const float* dir, const float* n, const float* wi, const float* color, float ior,
+                        float transmission, float* thr3);
+float lxo_lambert_pdf(const float* wi, const float* n);
 float lxo_ct_d(const float* h, const float* n, float alpha);
 float lxo_ct_smith_g1(const float* w, const float* n, float alpha);
 /* Sample the uber-shader lobe from RNG(iter, index, depth); returns the

@@ -38,14 +38,14 @@ def test_ffi_signature_table_covers_header():
 
 def test_abi_version_and_clean_error_state():
     lib = F.load()
-    assert lib.lx_abi_version() == 1
+    assert lib.lx_abi_version() == 2
     assert lib.lx_last_error() == b""
 
 
 def test_struct_sizes_match_c_layout():
     # sizes from the header's field lists (x86-64 SysV alignment)
     assert C.sizeof(F.lx_tex_transform) == 20
-    assert C.sizeof(F.lx_material_desc) == 4 * (3 + 1 + 5 + 2 + 1 + 5 + 1 + 5 + 3 + 1 + 1 + 5 + 5)
+    assert C.sizeof(F.lx_material_desc) == 4 * (3 + 1 + 5 + 2 + 1 + 5 + 1 + 5 + 3 + 1 + 1 + 5 + 5 + 1)
     assert C.sizeof(F.lx_camera_desc) == 4 * (2 + 12 + 2)
     assert C.sizeof(F.lx_ray) == 32
     assert C.sizeof(F.lx_hit) == 20
