@@ -61,6 +61,10 @@ def main():
     ap.add_argument("--served-jobs", type=int, default=48, help="jobs per arrival rate")
     ap.add_argument("--wavefront-bounces", type=int, default=None,
                     help="bounces launched one kernel each before the tail kernel (default: the context's)")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="run the library's autotuner on this workload (8-spp job) before timing")
+    ap.add_argument("--set", action="append", default=[], metavar="KNOB=VALUE",
+                    help="set a tuning knob (lx_tuning field) after autotuning, e.g. --set sort_mode=1")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run the plain 16-byte copy kernel over this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -93,8 +97,15 @@ def main():
         f"build {info['build_ms']:.0f} ms")
     depth = data.trace_depth
 
+    if args.autotune:
+        t0 = time.time()
+        best = ctx.autotune([lx.make_job(sc, data.camera, 8, emis=True)])
+        log(f"autotuned in {time.time() - t0:.1f} s: {best}")
     if args.wavefront_bounces is not None:
         ctx.set_tuning(wavefront_bounces=args.wavefront_bounces)
+    for kv in args.set:
+        k, v = kv.split("=", 1)
+        ctx.set_tuning(**{k: int(v)})
     tuning = ctx.tuning()
     r = lx.Render(sc, emis=True)
     r.iterate(1, args.warmup)

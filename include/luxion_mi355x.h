@@ -404,12 +404,29 @@ typedef struct lx_tuning {
                                   negative = every bounce its own launch */
     int32_t tail_min_waves;    /* 0, or 4: the tail kernel is compiled for at least
                                   4 waves per SIMD (fewer registers, some spills) */
+    int32_t sort_mode;         /* LX_SORT_* bits: queue-reordering stages (same bits) */
+    int32_t pixel_tile;        /* camera paths enter the queue tile x tile pixel
+                                  square by square (1, 8, 16 or 32; 1 = row-major;
+                                  same bits) */
+    int32_t tile_band;         /* squares follow each other in bands of this many
+                                  square rows, column by column (0 = row by row;
+                                  0..64; same bits) */
 } lx_tuning;
+
+/* Coherence stages (the reference's RAY_SORTING and MATERIAL_SORTING,
+ * src/pathtrace.cu:157-185 and 266-276, include/config.h:3-4): LX_SORT_RAYS
+ * orders the live-path queue by a Morton code of each ray before every traced
+ * bounce (the per-bounce traces and the tail kernel's first one);
+ * LX_SORT_MATERIALS orders the per-bounce shading by hit material.  Both only
+ * permute the queue, so the rendered bits do not change. */
+enum { LX_SORT_NONE = 0, LX_SORT_RAYS = 1, LX_SORT_MATERIALS = 2 };
 
 lx_status lx_tuning_get(const lx_context* ctx, lx_tuning* out);
 lx_status lx_tuning_set(lx_context* ctx, const lx_tuning* t);
-/* Autotuner: sweeps block size, BVH width and queue granularity on the given
- * job set, keeps the fastest, and writes it as JSON to json_path (if non-NULL). */
+/* Autotuner: sweeps the launch knobs (per-bounce / tail split, occupancy,
+ * block size, grid size, queue granularity, sort stages and, for scenes it
+ * may rebuild, BVH width) on the given job set, keeps the fastest, and writes
+ * it as JSON to json_path (if non-NULL). */
 lx_status lx_autotune(lx_context* ctx, const lx_job* jobs, uint32_t n, const char* json_path, lx_tuning* best);
 lx_status lx_tuning_load(lx_context* ctx, const char* json_path);
 

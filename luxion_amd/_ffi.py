@@ -24,6 +24,7 @@ LX_MATERIAL_MICROFACET, LX_MATERIAL_DIFFUSE, LX_MATERIAL_SPECULAR, LX_MATERIAL_E
 LX_DEBUG_OFF, LX_DEBUG_ALBEDO, LX_DEBUG_WORLD_NORMAL, LX_DEBUG_NORMAL_MAP, LX_DEBUG_ROUGHNESS, LX_DEBUG_METALLIC, \
     LX_DEBUG_ALPHA = range(7)
 LX_SHADING_UBER, LX_SHADING_LEGACY = 0, 1
+LX_SORT_NONE, LX_SORT_RAYS, LX_SORT_MATERIALS = 0, 1, 2
 
 F2 = C.c_float * 2
 F3 = C.c_float * 3
@@ -128,7 +129,8 @@ class lx_job_result(C.Structure):
 class lx_tuning(C.Structure):
     _fields_ = [("block_size", C.c_int32), ("bvh_width", C.c_int32), ("queue_chunk", C.c_int32),
                 ("waves_per_cu", C.c_int32), ("use_graph", C.c_int32), ("max_paths_per_round", C.c_uint32),
-                ("wavefront_bounces", C.c_int32), ("tail_min_waves", C.c_int32)]
+                ("wavefront_bounces", C.c_int32), ("tail_min_waves", C.c_int32), ("sort_mode", C.c_int32),
+                ("pixel_tile", C.c_int32), ("tile_band", C.c_int32)]
 
 
 class lx_server_opts(C.Structure):

@@ -39,11 +39,15 @@ std::vector<Knob> knobs() {
         {"block_size", &lx_tuning::block_size, {128, 256}},
         {"waves_per_cu", &lx_tuning::waves_per_cu, {8, 12, 16, 24}},
         {"queue_chunk", &lx_tuning::queue_chunk, {1, 8, 32}},
+        {"sort_mode", &lx_tuning::sort_mode, {0, 1, 2, 3}},
+        {"pixel_tile", &lx_tuning::pixel_tile, {1, 8, 16}},
+        {"tile_band", &lx_tuning::tile_band, {0, 4, 8}},
     };
 }
 
 const char* kFields[] = {"block_size", "bvh_width", "queue_chunk", "waves_per_cu", "use_graph",
-                         "max_paths_per_round", "wavefront_bounces", "tail_min_waves"};
+                         "max_paths_per_round", "wavefront_bounces", "tail_min_waves", "sort_mode",
+                         "pixel_tile", "tile_band"};
 
 int32_t* field_ptr(lx_tuning& t, const std::string& k) {
     if (k == "block_size") return &t.block_size;
@@ -54,6 +58,9 @@ int32_t* field_ptr(lx_tuning& t, const std::string& k) {
     if (k == "max_paths_per_round") return (int32_t*)&t.max_paths_per_round;
     if (k == "wavefront_bounces") return &t.wavefront_bounces;
     if (k == "tail_min_waves") return &t.tail_min_waves;
+    if (k == "sort_mode") return &t.sort_mode;
+    if (k == "pixel_tile") return &t.pixel_tile;
+    if (k == "tile_band") return &t.tile_band;
     return nullptr;
 }
 
@@ -110,6 +117,11 @@ void check_tuning(const lx_tuning& t) {
     LX_REQUIRE(t.block_size == 128 || t.block_size == 256, LX_ERR_INVALID, "block_size must be 128 or 256");
     LX_REQUIRE(t.tail_min_waves == 0 || t.tail_min_waves == 4, LX_ERR_INVALID, "tail_min_waves must be 0 or 4");
     LX_REQUIRE(t.queue_chunk >= 1 && t.queue_chunk <= 64, LX_ERR_INVALID, "queue_chunk must be 1..64");
+    LX_REQUIRE(t.sort_mode >= 0 && t.sort_mode <= (LX_SORT_RAYS | LX_SORT_MATERIALS), LX_ERR_INVALID,
+               "sort_mode must be a combination of LX_SORT_RAYS and LX_SORT_MATERIALS");
+    LX_REQUIRE(t.pixel_tile == 1 || t.pixel_tile == 8 || t.pixel_tile == 16 || t.pixel_tile == 32, LX_ERR_INVALID,
+               "pixel_tile must be 1, 8, 16 or 32");
+    LX_REQUIRE(t.tile_band >= 0 && t.tile_band <= 64, LX_ERR_INVALID, "tile_band must be 0..64");
 }
 
 } // namespace

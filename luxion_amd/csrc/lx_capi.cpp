@@ -70,6 +70,9 @@ lx_status lx_context_create(int device, uint64_t budget, lx_context** out) {
         c->tuning.use_graph = 1;
         c->tuning.wavefront_bounces = LX_DEFAULT_WAVEFRONT_BOUNCES;
         c->tuning.tail_min_waves = 4;
+        c->tuning.sort_mode = LX_SORT_NONE;
+        c->tuning.pixel_tile = 8;
+        c->tuning.tile_band = 0;
     } catch (...) {
         if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
@@ -419,7 +422,16 @@ lx_status lx_render_iterate(lx_render* r, uint32_t first_iter, uint32_t n) {
             for (auto& e : ev) (void)hipEventDestroy(e);
         }
     } else if (ctx->tuning.use_graph) {
+        if (r->graph && std::memcmp(&r->graph_tuning, &ctx->tuning, sizeof(lx_tuning)) != 0) {
+            /* the launch shapes changed since the capture */
+            LX_HIP(hipStreamSynchronize(ctx->stream));
+            (void)hipGraphExecDestroy(r->graph);
+            (void)hipGraphDestroy(r->graph_src);
+            r->graph = nullptr;
+            r->graph_src = nullptr;
+        }
         if (!r->graph) {
+            r->graph_tuning = ctx->tuning;
             LX_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
             try {
                 enqueue(r, nullptr);
@@ -561,7 +573,8 @@ lx_status lx_debug_bounce(lx_render* r, uint32_t iter, uint32_t* survivors, uint
     LxJob* dj = r->jobs.as<LxJob>();
     LxCounters* dc = r->counters.as<LxCounters>();
     LX_HIP(lx_launch_begin_round(dj, r->job_spp.as<int32_t>(), 1, dc, r->sched.as<LxSched>(), r->npix, s));
-    LX_HIP(lx_launch_camera(dj, r->sched.as<LxSched>(), dc, r->wb.A, r->cam_grid, s));
+    LX_HIP(lx_launch_camera(dj, r->sched.as<LxSched>(), dc, r->wb.A, r->cam_grid, s,
+                            (uint32_t)r->ctx->tuning.pixel_tile, (uint32_t)r->ctx->tuning.tile_band));
     LxPathBuf* cur = &r->wb.A;
     LxPathBuf* nxt = &r->wb.B;
     size_t N = r->npix;

@@ -193,9 +193,10 @@ void lx_affine_inverse(const float* m, float* out);
 
 /* ------------------------------------------------------------- render */
 struct WaveBufs {
-    DevBuf pathA, pathB, rec;
+    DevBuf pathA, pathB, rec, sort;
     LxPathBuf A{}, B{};
     LxRecBuf R{};
+    uint32_t *keys = nullptr, *order = nullptr, *hist = nullptr; /* queue sort stage */
     uint32_t cap = 0;
     void alloc(lx_context* ctx, uint32_t cap, bool dmis, bool emis, bool tangents);
 };
@@ -219,6 +220,7 @@ struct lx_render {
     uint32_t trace_launches = 0, shade_launches = 0, tail_launches = 0;
     hipGraphExec_t graph = nullptr;
     hipGraph_t graph_src = nullptr;
+    lx_tuning graph_tuning{}; /* the tuning the graph was captured with */
 
 };
 

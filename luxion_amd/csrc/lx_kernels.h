@@ -29,20 +29,33 @@ struct LxSched {
 hipError_t lx_launch_begin_round(LxJob* jobs, const int32_t* job_spp, uint32_t njobs, LxCounters* C, LxSched* sched,
                                  uint32_t cap, hipStream_t s);
 hipError_t lx_launch_end_round(LxJob* jobs, uint32_t njobs, hipStream_t s);
+/* tile: camera paths of a job cover tile x tile pixel squares in turn (1 =
+ * row-major); band: squares in bands of this many square rows, walked
+ * column by column (0 = row by row) */
 hipError_t lx_launch_camera(const LxJob* jobs, const LxSched* sched, const LxCounters* C, const LxPathBuf& P,
-                            uint32_t grid, hipStream_t s);
+                            uint32_t grid, hipStream_t s, uint32_t tile = 1, uint32_t band = 0);
 /* width: the BVH width (4 or 8) of every scene the jobs reference */
+/* order: NULL, or the queue permutation from lx_launch_sort (thread q takes
+ * the path in slot order[q]) */
 hipError_t lx_launch_trace(const LxJob* jobs, const LxPathBuf& P, const LxRecBuf& R, LxCounters* C, int bounce,
-                           uint32_t* spill, uint32_t grid, int width, hipStream_t s, bool count = false);
+                           uint32_t* spill, uint32_t grid, int width, hipStream_t s, bool count = false,
+                           const uint32_t* order = nullptr);
 hipError_t lx_launch_shade(const LxJob* jobs, const LxPathBuf& P, const LxPathBuf& Q, const LxRecBuf& R, LxCounters* C,
-                           int bounce, float* accum, uint32_t grid, hipStream_t s);
+                           int bounce, float* accum, uint32_t grid, hipStream_t s, const uint32_t* order = nullptr);
+/* Queue permutation of the count[bounce] live paths by ray cell
+ * (LX_SORT_RAYS) or hit material (LX_SORT_MATERIALS) into order;
+ * keys: path capacity entries, hist: LX_SORT_HIST entries. */
+#define LX_SORT_HIST 4097
+hipError_t lx_launch_sort(const LxJob* jobs, const LxPathBuf& P, const LxRecBuf& R, const LxCounters* C, int bounce,
+                          int mode, uint32_t* keys, uint32_t* hist, uint32_t* order, uint32_t grid, hipStream_t s);
 /* threads: total lanes (a multiple of 128); block: 128 or 256; occ: 0 or 4
  * minimum waves per SIMD; chunk: a wave refills once this many of its lanes
  * (1..64) are idle */
 hipError_t lx_launch_finish(const LxJob* jobs, const LxPathBuf& P, LxCounters* C, int bounce0, uint32_t* spill,
                             float* accum, uint32_t threads, int block, int occ, int width, int chunk,
                             const LxDevScene* uscene /* the scene of every job, or NULL */,
-                            bool single_job /* njobs == 1 */, hipStream_t s, bool count = false);
+                            bool single_job /* njobs == 1 */, hipStream_t s, bool count = false,
+                            const uint32_t* order = nullptr);
 hipError_t lx_launch_trace_rays(const LxDevScene* S, const lx_ray* rays, uint32_t n, int any, int alpha,
                                 uint32_t iteration, lx_hit* hits, uint8_t* occ, uint32_t* spill, uint32_t* overflow,
                                 uint32_t grid, int width, hipStream_t s);

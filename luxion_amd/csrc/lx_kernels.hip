@@ -74,9 +74,56 @@ __global__ void k_end_round(LxJob* __restrict__ jobs, uint32_t njobs) {
     if (j < njobs && jobs[j].active) jobs[j].spp_done += 1;
 }
 
+/* The pixel of a job's q-th camera path.  tile <= 1: row-major.  Otherwise
+ * the image's whole tile x tile squares come first, with pixels in Z (Morton)
+ * order inside a square -- an 8x8 square is one wave64, so a wave's camera
+ * rays and their first hits are compact -- then the right strip and the
+ * bottom strip the squares leave, row-major.  Squares follow each other
+ * row by row (band 0), or, with band > 0, in horizontal bands of `band`
+ * square rows walked column by column, so that the waves running at one
+ * time cover a compact region of the image.  A bijection of [0, W*H) onto
+ * the pixels for any W, H. */
+LXD uint32_t lx_even_bits(uint32_t v) {
+    v &= 0x55555555u;
+    v = (v | (v >> 1)) & 0x33333333u;
+    v = (v | (v >> 2)) & 0x0f0f0f0fu;
+    v = (v | (v >> 4)) & 0x00ff00ffu;
+    v = (v | (v >> 8)) & 0x0000ffffu;
+    return v;
+}
+LXD uint32_t lx_tile_pixel(uint32_t q, uint32_t W, uint32_t H, uint32_t tile, uint32_t band) {
+    if (tile <= 1u) return q;
+    uint32_t Wt = W / tile * tile, Ht = H / tile * tile;
+    uint32_t nt = Wt * Ht;
+    if (q < nt) {
+        uint32_t per = tile * tile;
+        uint32_t t = q / per, w = q % per;
+        uint32_t tiles_x = Wt / tile, tiles_y = Ht / tile;
+        uint32_t tx, ty;
+        if (band == 0u) {
+            tx = t % tiles_x;
+            ty = t / tiles_x;
+        } else {
+            uint32_t b = t / (tiles_x * band); /* every band but the last is full */
+            uint32_t rows = min(band, tiles_y - b * band);
+            uint32_t u = t - b * tiles_x * band;
+            tx = u / rows;
+            ty = b * band + u % rows;
+        }
+        uint32_t x = tx * tile + lx_even_bits(w), y = ty * tile + lx_even_bits(w >> 1);
+        return y * W + x;
+    }
+    uint32_t r = q - nt;
+    uint32_t rw = W - Wt, nr = rw * Ht; /* right strip: columns Wt.., rows ..Ht */
+    if (r < nr) return (r / rw) * W + Wt + r % rw;
+    r -= nr; /* bottom strip: rows Ht.., every column */
+    return (Ht + r / W) * W + r % W;
+}
+
 /* generateRayFromCamera, src/kernels/generate_ray_from_camera.cu:8-39 */
 __global__ __launch_bounds__(256) void k_camera(const LxJob* __restrict__ jobs, const LxSched* __restrict__ sched,
-                                                const LxCounters* __restrict__ C, LxPathBuf P) {
+                                                const LxCounters* __restrict__ C, LxPathBuf P, uint32_t tile,
+                                                uint32_t band) {
     __shared__ uint32_t s_off[LX_SCHED_MAX];
     __shared__ uint32_t s_idx[LX_SCHED_MAX];
     uint32_t nact = sched->n_active;
@@ -96,8 +143,8 @@ __global__ __launch_bounds__(256) void k_camera(const LxJob* __restrict__ jobs, 
         }
         uint32_t j = s_idx[lo];
         const LxJob& J = jobs[j];
-        uint32_t index = i - J.path_off;
         int W = J.cam.width;
+        uint32_t index = lx_tile_pixel(i - J.path_off, (uint32_t)W, (uint32_t)J.cam.height, tile, band);
         int x = (int)(index % (uint32_t)W), y = (int)(index / (uint32_t)W);
         LxRng rng = lx_make_rng(J.iter, (int)index, J.depth, J.seed);
         float x1 = rng.u01() - 0.5f;
@@ -194,7 +241,8 @@ struct LxRecStore {
 template <bool CNT, int W>
 __global__ __launch_bounds__(LX_TRACE_BLOCK) void k_trace(const LxJob* __restrict__ jobs, LxPathBuf P, LxRecBuf R,
                                                            LxCounters* __restrict__ C, int bounce,
-                                                           uint32_t* __restrict__ spill) {
+                                                           uint32_t* __restrict__ spill,
+                                                           const uint32_t* __restrict__ order) {
     __shared__ uint32_t s_stack[LX_STACK * LX_TRACE_BLOCK];
     uint32_t n = C->count[bounce];
     uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -206,7 +254,8 @@ __global__ __launch_bounds__(LX_TRACE_BLOCK) void k_trace(const LxJob* __restric
     st.sstride = gsz;
     LxRayCount rc = {0, 0, 0};
     LxWork wc = {0, 0, 0}, wa = {0, 0, 0};
-    for (uint32_t i = gtid; i < n; i += gsz) {
+    for (uint32_t q = gtid; q < n; q += gsz) {
+        uint32_t i = order ? order[q] : q; /* slot of the q-th queued path */
         const LxJob& J = jobs[P.job[i]];
         const LxDevScene& S = *J.scene;
         uint32_t pix = P.pix[i] & 0x7fffffffu;
@@ -249,7 +298,8 @@ LXD void lx_accumulate(float* __restrict__ accum, const LxJob& J, uint32_t pix, 
 
 __global__ __launch_bounds__(LX_SHADE_BLOCK) void k_shade(const LxJob* __restrict__ jobs, LxPathBuf P, LxPathBuf Q,
                                                            LxRecBuf R, LxCounters* __restrict__ C, int bounce,
-                                                           float* __restrict__ accum) {
+                                                           float* __restrict__ accum,
+                                                           const uint32_t* __restrict__ order) {
     uint32_t n = C->count[bounce];
     uint32_t wave_base = blockIdx.x * blockDim.x + (threadIdx.x & ~(LX_WAVE - 1));
     uint32_t gsz = gridDim.x * blockDim.x;
@@ -257,8 +307,9 @@ __global__ __launch_bounds__(LX_SHADE_BLOCK) void k_shade(const LxJob* __restric
     int depth = bounce + 1;
     uint32_t nsh = 0;
     for (uint32_t base = wave_base; base < n; base += gsz) {
-        uint32_t i = base + lane;
-        bool active = i < n;
+        uint32_t q = base + lane;
+        bool active = q < n;
+        uint32_t i = active && order ? order[q] : q; /* slot of the q-th queued path */
         bool survive = false;
         f3 o, d, thr, col;
         float pdf = 0.0f;
@@ -325,7 +376,7 @@ template <bool CNT, int BLOCK, int OCC, int W, int UNI>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1)))
 void k_finish(const LxJob* __restrict__ jobs, LxPathBuf P, LxCounters* __restrict__ C, int bounce0,
               uint32_t* __restrict__ spill, float* __restrict__ accum, uint32_t chunk,
-              const LxDevScene* __restrict__ uscene) {
+              const LxDevScene* __restrict__ uscene, const uint32_t* __restrict__ order) {
     __shared__ uint32_t s_stack[LX_STACK * BLOCK];
     uint32_t n = C->count[bounce0];
     uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -361,6 +412,7 @@ void k_finish(const LxJob* __restrict__ jobs, LxPathBuf P, LxCounters* __restric
             if (!have && !flush) {
                 uint32_t my = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
                 if (my < n) {
+                    if (order) my = order[my]; /* slot of the my-th queued path */
                     job = P.job[my];
                     uint32_t pw = P.pix[my];
                     spec = (pw >> 31) != 0u;
@@ -414,6 +466,141 @@ void k_finish(const LxJob* __restrict__ jobs, LxPathBuf P, LxCounters* __restric
     }
     lx_flush_counts<CNT>(C, &C->work[1], rc, nsh, wc, wa);
     lx_wave_add(&C->tail_bounces, nsh);
+}
+
+/* ------------------------------------------------------------ sort stages */
+/* Queue reordering for coherence (the reference sorts path indices with
+ * thrust::sort_by_key: Morton codes of the rays, src/pathtrace.cu:157-185,
+ * src/kernels/intersection_precompute.cu; material ids, pathtrace.cu:266-276).
+ * Keys here have 12 bits -- a 16^3 Morton cell of a point one reach along the
+ * ray inside the scene box, or the hit material (misses in one more bucket)
+ * -- so a bucket sort (histogram, scan, scatter) gives the permutation in
+ * three small launches with no temporary storage.  The order inside a bucket
+ * is the atomics' order; it only decides which lane takes which path. */
+#define LX_SORT_BINS 4097u
+
+LXD uint32_t lx_cell_key(uint32_t x, uint32_t y, uint32_t z) {
+    uint32_t k = 0;
+    for (int b = 3; b >= 0; b--) k = (k << 3) | (((x >> b) & 1u) << 2) | (((y >> b) & 1u) << 1) | ((z >> b) & 1u);
+    return k;
+}
+LXD uint32_t lx_cell(float p, float lo, float inv) {
+    float c = (p - lo) * inv * 16.0f;
+    c = fminf(fmaxf(c, 0.0f), 15.0f);
+    return (uint32_t)c;
+}
+
+/* Lanes of a wave that share a key add to its counter with one atomic: the
+ * keys of neighbouring paths mostly agree, and same-address atomics from one
+ * wave would otherwise serialise.  Returns this lane's offset from the
+ * counter's old value. */
+template <class Add>
+LXD uint32_t lx_wave_bucket_add(uint32_t key, bool active, Add add) {
+    uint32_t lane = lx_lane();
+    uint32_t mine = 0;
+    uint64_t todo = __ballot(active);
+    while (todo) {
+        uint32_t leader = (uint32_t)__builtin_ctzll(todo);
+        uint32_t lkey = __shfl(key, (int)leader, LX_WAVE);
+        uint64_t same = __ballot(active && key == lkey) & todo;
+        uint32_t base = 0;
+        if (lane == leader) base = add(lkey, (uint32_t)__popcll(same));
+        base = __shfl(base, (int)leader, LX_WAVE);
+        if ((same >> lane) & 1ull) mine = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+        todo &= ~same;
+    }
+    return mine;
+}
+
+__global__ __launch_bounds__(256) void k_sort_keys(const LxJob* __restrict__ jobs, LxPathBuf P, LxRecBuf R,
+                                                   const LxCounters* __restrict__ C, int bounce, int mode,
+                                                   uint32_t* __restrict__ keys, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t lh[LX_SORT_BINS];
+    for (uint32_t k = threadIdx.x; k < LX_SORT_BINS; k += blockDim.x) lh[k] = 0u;
+    __syncthreads();
+    uint32_t n = C->count[bounce];
+    uint32_t gsz = gridDim.x * blockDim.x;
+    /* whole waves iterate together so the wave-wide aggregation sees every lane */
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~(LX_WAVE - 1u)); base < n; base += gsz) {
+        uint32_t i = base + lx_lane();
+        bool active = i < n;
+        uint32_t key = 0;
+        if (!active) {
+        } else if (mode == LX_SORT_MATERIALS) {
+            key = R.t[i] > 0.0f ? min((uint32_t)R.mat[i], 4095u) : 4096u;
+        } else {
+            const LxDevScene& S = *jobs[P.job[i]].scene;
+            float r = S.sort_reach;
+            key = lx_cell_key(lx_cell(P.ox[i] + P.dx[i] * r, S.box_lo[0], S.box_inv[0]),
+                              lx_cell(P.oy[i] + P.dy[i] * r, S.box_lo[1], S.box_inv[1]),
+                              lx_cell(P.oz[i] + P.dz[i] * r, S.box_lo[2], S.box_inv[2]));
+        }
+        if (active) keys[i] = key;
+        lx_wave_bucket_add(key, active, [&](uint32_t k, uint32_t c) { return atomicAdd(&lh[k], c); });
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < LX_SORT_BINS; k += blockDim.x)
+        if (lh[k]) atomicAdd(&hist[k], lh[k]);
+}
+
+/* exclusive prefix sum of the histogram, in place (one workgroup) */
+__global__ __launch_bounds__(256) void k_sort_scan(uint32_t* __restrict__ hist) {
+    __shared__ uint32_t part[256];
+    const uint32_t per = (LX_SORT_BINS + 255u) / 256u;
+    uint32_t b0 = threadIdx.x * per, b1 = min(b0 + per, LX_SORT_BINS);
+    uint32_t sum = 0;
+    for (uint32_t k = b0; k < b1; k++) sum += hist[k];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < 256u; off <<= 1) {
+        uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[threadIdx.x] - sum;
+    for (uint32_t k = b0; k < b1; k++) {
+        uint32_t h = hist[k];
+        hist[k] = run;
+        run += h;
+    }
+}
+
+/* Each workgroup places a chunk of 16 keys per thread: ranks within the chunk
+ * from an LDS histogram, then one global atomic per distinct key of the
+ * chunk reserves its range -- neighbouring paths mostly share keys, and
+ * per-wave atomics on the same few counters would serialise. */
+#define LX_SORT_ITEMS 16
+__global__ __launch_bounds__(256) void k_sort_scatter(const LxCounters* __restrict__ C, int bounce,
+                                                      const uint32_t* __restrict__ keys, uint32_t* __restrict__ offs,
+                                                      uint32_t* __restrict__ order) {
+    __shared__ uint32_t lcount[LX_SORT_BINS];
+    __shared__ uint32_t lbase[LX_SORT_BINS];
+    const uint32_t n = C->count[bounce];
+    const uint32_t chunk = 256u * LX_SORT_ITEMS;
+    for (uint32_t c0 = blockIdx.x * chunk; c0 < n; c0 += gridDim.x * chunk) {
+        for (uint32_t b = threadIdx.x; b < LX_SORT_BINS; b += 256u) lcount[b] = 0u;
+        __syncthreads();
+        uint32_t key[LX_SORT_ITEMS], rank[LX_SORT_ITEMS];
+#pragma unroll
+        for (int j = 0; j < LX_SORT_ITEMS; j++) {
+            uint32_t i = c0 + (uint32_t)j * 256u + threadIdx.x;
+            bool active = i < n;
+            key[j] = active ? keys[i] : 0u;
+            rank[j] = lx_wave_bucket_add(key[j], active,
+                                         [&](uint32_t k, uint32_t c) { return atomicAdd(&lcount[k], c); });
+        }
+        __syncthreads();
+        for (uint32_t b = threadIdx.x; b < LX_SORT_BINS; b += 256u)
+            if (lcount[b]) lbase[b] = atomicAdd(&offs[b], lcount[b]);
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < LX_SORT_ITEMS; j++) {
+            uint32_t i = c0 + (uint32_t)j * 256u + threadIdx.x;
+            if (i < n) order[lbase[key[j]] + rank[j]] = i;
+        }
+        __syncthreads();
+    }
 }
 
 /* ------------------------------------------------------------ test hooks */
@@ -542,48 +729,59 @@ hipError_t lx_launch_end_round(LxJob* jobs, uint32_t njobs, hipStream_t s) {
     return hipGetLastError();
 }
 hipError_t lx_launch_camera(const LxJob* jobs, const LxSched* sched, const LxCounters* C, const LxPathBuf& P,
-                            uint32_t grid, hipStream_t s) {
-    hipLaunchKernelGGL(k_camera, dim3(grid), dim3(256), 0, s, jobs, sched, C, P);
+                            uint32_t grid, hipStream_t s, uint32_t tile, uint32_t band) {
+    hipLaunchKernelGGL(k_camera, dim3(grid), dim3(256), 0, s, jobs, sched, C, P, tile, band);
     return hipGetLastError();
 }
 hipError_t lx_launch_trace(const LxJob* jobs, const LxPathBuf& P, const LxRecBuf& R, LxCounters* C, int bounce,
-                           uint32_t* spill, uint32_t grid, int width, hipStream_t s, bool count) {
+                           uint32_t* spill, uint32_t grid, int width, hipStream_t s, bool count,
+                           const uint32_t* order) {
     dim3 g(grid), b(LX_TRACE_BLOCK);
     if (width == 8) {
-        if (count) hipLaunchKernelGGL((k_trace<true, 8>), g, b, 0, s, jobs, P, R, C, bounce, spill);
-        else hipLaunchKernelGGL((k_trace<false, 8>), g, b, 0, s, jobs, P, R, C, bounce, spill);
+        if (count) hipLaunchKernelGGL((k_trace<true, 8>), g, b, 0, s, jobs, P, R, C, bounce, spill, order);
+        else hipLaunchKernelGGL((k_trace<false, 8>), g, b, 0, s, jobs, P, R, C, bounce, spill, order);
     } else {
-        if (count) hipLaunchKernelGGL((k_trace<true, 4>), g, b, 0, s, jobs, P, R, C, bounce, spill);
-        else hipLaunchKernelGGL((k_trace<false, 4>), g, b, 0, s, jobs, P, R, C, bounce, spill);
+        if (count) hipLaunchKernelGGL((k_trace<true, 4>), g, b, 0, s, jobs, P, R, C, bounce, spill, order);
+        else hipLaunchKernelGGL((k_trace<false, 4>), g, b, 0, s, jobs, P, R, C, bounce, spill, order);
     }
     return hipGetLastError();
 }
 hipError_t lx_launch_shade(const LxJob* jobs, const LxPathBuf& P, const LxPathBuf& Q, const LxRecBuf& R, LxCounters* C,
-                           int bounce, float* accum, uint32_t grid, hipStream_t s) {
-    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(LX_SHADE_BLOCK), 0, s, jobs, P, Q, R, C, bounce, accum);
+                           int bounce, float* accum, uint32_t grid, hipStream_t s, const uint32_t* order) {
+    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(LX_SHADE_BLOCK), 0, s, jobs, P, Q, R, C, bounce, accum, order);
+    return hipGetLastError();
+}
+hipError_t lx_launch_sort(const LxJob* jobs, const LxPathBuf& P, const LxRecBuf& R, const LxCounters* C, int bounce,
+                          int mode, uint32_t* keys, uint32_t* hist, uint32_t* order, uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_u32, dim3((LX_SORT_BINS + 255) / 256), dim3(256), 0, s, hist, (size_t)LX_SORT_BINS, 0u);
+    hipLaunchKernelGGL(k_sort_keys, dim3(grid), dim3(256), 0, s, jobs, P, R, C, bounce, mode, keys, hist);
+    hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(256), 0, s, hist);
+    hipLaunchKernelGGL(k_sort_scatter, dim3(grid), dim3(256), 0, s, C, bounce, keys, hist, order);
     return hipGetLastError();
 }
 template <int BLOCK, int OCC, int W, int UNI>
 static void lx_finish_at(const LxJob* jobs, const LxPathBuf& P, LxCounters* C, int bounce0, uint32_t* spill,
-                         float* accum, uint32_t grid, uint32_t chunk, const LxDevScene* us, hipStream_t s) {
+                         float* accum, uint32_t grid, uint32_t chunk, const LxDevScene* us, const uint32_t* order,
+                         hipStream_t s) {
     hipLaunchKernelGGL((k_finish<false, BLOCK, OCC, W, UNI>), dim3(grid), dim3(BLOCK), 0, s, jobs, P, C, bounce0,
-                       spill, accum, chunk, us);
+                       spill, accum, chunk, us, order);
 }
 template <int W, int UNI>
 static void lx_finish_w(const LxJob* jobs, const LxPathBuf& P, LxCounters* C, int bounce0, uint32_t* spill,
                         float* accum, uint32_t threads, int block, int occ, uint32_t chunk, const LxDevScene* us,
-                        hipStream_t s) {
+                        const uint32_t* order, hipStream_t s) {
     if (occ >= 4) {
         if (block == 256 && threads >= 256)
-            lx_finish_at<256, 4, W, UNI>(jobs, P, C, bounce0, spill, accum, threads / 256, chunk, us, s);
-        else lx_finish_at<128, 4, W, UNI>(jobs, P, C, bounce0, spill, accum, threads / 128, chunk, us, s);
+            lx_finish_at<256, 4, W, UNI>(jobs, P, C, bounce0, spill, accum, threads / 256, chunk, us, order, s);
+        else lx_finish_at<128, 4, W, UNI>(jobs, P, C, bounce0, spill, accum, threads / 128, chunk, us, order, s);
     } else {
-        lx_finish_at<128, 0, W, UNI>(jobs, P, C, bounce0, spill, accum, threads / 128, chunk, us, s);
+        lx_finish_at<128, 0, W, UNI>(jobs, P, C, bounce0, spill, accum, threads / 128, chunk, us, order, s);
     }
 }
 hipError_t lx_launch_finish(const LxJob* jobs, const LxPathBuf& P, LxCounters* C, int bounce0, uint32_t* spill,
                             float* accum, uint32_t threads, int block, int occ, int width, int chunk,
-                            const LxDevScene* uscene, bool single_job, hipStream_t s, bool count) {
+                            const LxDevScene* uscene, bool single_job, hipStream_t s, bool count,
+                            const uint32_t* order) {
     /* threads = grid * block lanes, the same for every shape, so one spill
      * allocation serves them all */
     uint32_t c = chunk < 1 ? 1u : (chunk > 64 ? 64u : (uint32_t)chunk);
@@ -591,18 +789,20 @@ hipError_t lx_launch_finish(const LxJob* jobs, const LxPathBuf& P, LxCounters* C
     if (count) {
         /* the counting replay: same paths and rays, per-lane scene and job */
         if (width == 8)
-            hipLaunchKernelGGL((k_finish<true, 128, 0, 8, 0>), g, b, 0, s, jobs, P, C, bounce0, spill, accum, c, uscene);
+            hipLaunchKernelGGL((k_finish<true, 128, 0, 8, 0>), g, b, 0, s, jobs, P, C, bounce0, spill, accum, c, uscene,
+                               order);
         else
-            hipLaunchKernelGGL((k_finish<true, 128, 0, 4, 0>), g, b, 0, s, jobs, P, C, bounce0, spill, accum, c, uscene);
+            hipLaunchKernelGGL((k_finish<true, 128, 0, 4, 0>), g, b, 0, s, jobs, P, C, bounce0, spill, accum, c, uscene,
+                               order);
     } else if (uscene && single_job) {
-        if (width == 8) lx_finish_w<8, 2>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, uscene, s);
-        else lx_finish_w<4, 2>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, uscene, s);
+        if (width == 8) lx_finish_w<8, 2>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, uscene, order, s);
+        else lx_finish_w<4, 2>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, uscene, order, s);
     } else if (uscene) {
-        if (width == 8) lx_finish_w<8, 1>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, uscene, s);
-        else lx_finish_w<4, 1>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, uscene, s);
+        if (width == 8) lx_finish_w<8, 1>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, uscene, order, s);
+        else lx_finish_w<4, 1>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, uscene, order, s);
     } else {
-        if (width == 8) lx_finish_w<8, 0>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, nullptr, s);
-        else lx_finish_w<4, 0>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, nullptr, s);
+        if (width == 8) lx_finish_w<8, 0>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, nullptr, order, s);
+        else lx_finish_w<4, 0>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, nullptr, order, s);
     }
     return hipGetLastError();
 }

@@ -50,6 +50,11 @@ void WaveBufs::alloc(lx_context* ctx, uint32_t c, bool dmis, bool emis, bool tan
         carve(p, (float**)&P.pix, c);
         carve(p, (float**)&P.job, c);
     }
+    /* sort stage: keys and permutation per path, one histogram */
+    sort.alloc(ctx, 2 * arr + ((size_t)LX_SORT_HIST * 4 + 255) / 256 * 256);
+    keys = (uint32_t*)sort.ptr;
+    order = (uint32_t*)((uint8_t*)sort.ptr + arr);
+    hist = (uint32_t*)((uint8_t*)sort.ptr + 2 * arr);
     /* records: 7 always, 3 tangent, 4 environment, 10 direct light */
     size_t nrec = 7 + 4 + (tangents ? 3 : 0) + (dmis ? 10 : 0);
     rec.alloc(ctx, nrec * arr);
@@ -92,22 +97,35 @@ void lx_enqueue_round(lx_context* ctx, LxJob* d_jobs, const int32_t* d_spp, uint
     hipStream_t s = ctx->stream;
     int wf = lx_wavefront_bounces(ctx, max_depth);
     LX_HIP(lx_launch_begin_round(d_jobs, d_spp, njobs, d_cnt, d_sched, path_cap, s));
-    LX_HIP(lx_launch_camera(d_jobs, d_sched, d_cnt, wb.A, cam_grid, s));
+    LX_HIP(lx_launch_camera(d_jobs, d_sched, d_cnt, wb.A, cam_grid, s, (uint32_t)ctx->tuning.pixel_tile,
+                            (uint32_t)ctx->tuning.tile_band));
     LxPathBuf* cur = &wb.A;
     LxPathBuf* nxt = &wb.B;
+    const int sm = ctx->tuning.sort_mode;
+    /* the sort stages write wb.order and the next launch reads it; a
+     * per-bounce stage sits inside the timed bracket of the launch it serves */
+    const uint32_t sort_grid = std::min(shade_grid, 2u * (uint32_t)std::max(1, ctx->num_cus));
+    auto sort = [&](int bounce, int mode) -> const uint32_t* {
+        LX_HIP(lx_launch_sort(d_jobs, *cur, wb.R, d_cnt, bounce, mode, wb.keys, wb.hist, wb.order, sort_grid, s));
+        return wb.order;
+    };
     for (int b = 0; b < wf; b++) {
         if (ev) LX_HIP(hipEventRecord(ev[2 * b], s));
-        LX_HIP(lx_launch_trace(d_jobs, *cur, wb.R, d_cnt, b, d_spill, trace_grid, width, s, count));
+        const uint32_t* ot = (sm & LX_SORT_RAYS) ? sort(b, LX_SORT_RAYS) : nullptr;
+        LX_HIP(lx_launch_trace(d_jobs, *cur, wb.R, d_cnt, b, d_spill, trace_grid, width, s, count, ot));
         if (ev) LX_HIP(hipEventRecord(ev[2 * b + 1], s));
-        LX_HIP(lx_launch_shade(d_jobs, *cur, *nxt, wb.R, d_cnt, b, d_accum, shade_grid, s));
+        const uint32_t* os = (sm & LX_SORT_MATERIALS) ? sort(b, LX_SORT_MATERIALS) : nullptr;
+        LX_HIP(lx_launch_shade(d_jobs, *cur, *nxt, wb.R, d_cnt, b, d_accum, shade_grid, s, os));
         std::swap(cur, nxt);
     }
     if (wf < max_depth) {
-        /* ev[2wf] -> ev[2wf+1] brackets the tail kernel */
+        /* ev[2wf] -> ev[2wf+1] brackets the tail kernel alone (its queue sort
+         * runs before the bracket) */
+        const uint32_t* of = (sm & LX_SORT_RAYS) ? sort(wf, LX_SORT_RAYS) : nullptr;
         if (ev) LX_HIP(hipEventRecord(ev[2 * wf], s));
         LX_HIP(lx_launch_finish(d_jobs, *cur, d_cnt, wf, d_spill, d_accum, trace_grid * LX_TRACE_BLOCK,
                                 ctx->tuning.block_size, ctx->tuning.tail_min_waves, width, ctx->tuning.queue_chunk,
-                                uscene, njobs == 1, s, count));
+                                uscene, njobs == 1, s, count, of));
         if (ev && 2 * wf + 1 < 2 * max_depth) LX_HIP(hipEventRecord(ev[2 * wf + 1], s));
     }
     if (ev) LX_HIP(hipEventRecord(ev[2 * max_depth], s));

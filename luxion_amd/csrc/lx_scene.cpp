@@ -431,6 +431,13 @@ void lx_scene_accel(lx_scene* s, const lx_build_opts* o) {
         I.num_tris = (uint32_t)(s->meshes[g.mesh].idx.size() / 3);
         I.has_n_uv = (s->meshes[g.mesh].has_n ? 1u : 0u) | (s->meshes[g.mesh].has_uv ? 2u : 0u);
     }
+    /* world box for the ray-sort keys */
+    float wlo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, whi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (size_t i = 0; i < s->insts.size(); i++)
+        for (int r = 0; r < 3; r++) {
+            wlo[r] = std::min(wlo[r], wboxes[6 * i + r]);
+            whi[r] = std::max(whi[r], wboxes[6 * i + 3 + r]);
+        }
     BvhBuildResult tl;
     lx_build_tlas_host(wboxes, opt.bvh_width, tl);
     uint32_t noff = (uint32_t)nodes.size();
@@ -488,6 +495,14 @@ void lx_scene_accel(lx_scene* s, const lx_build_opts* o) {
     D.total_area = s->total_area;
     D.has_env = s->has_env ? 1 : 0;
     D.bvh_width = opt.bvh_width;
+    double diag2 = 0.0;
+    for (int r = 0; r < 3; r++) {
+        double e = s->insts.empty() ? 0.0 : (double)whi[r] - (double)wlo[r];
+        D.box_lo[r] = s->insts.empty() ? 0.0f : wlo[r];
+        D.box_inv[r] = e > 0.0 ? (float)(1.0 / e) : 0.0f;
+        diag2 += e * e;
+    }
+    D.sort_reach = (float)(0.25 * std::sqrt(diag2));
     s->d_scene.upload(ctx, &D, sizeof(LxDevScene));
     LX_HIP(hipStreamSynchronize(ctx->stream));
     s->accel_built = true;

@@ -92,7 +92,10 @@ struct LxDevScene {
     float total_area;
     int32_t has_env;
     int32_t bvh_width;
-    int32_t pad[2];
+    /* ray sort keys: world box of all instances and a reach along the ray */
+    float box_lo[3], box_inv[3];
+    float sort_reach;
+    int32_t pad;
 };
 
 /* One render job (single renders are a batch of one). */
