@@ -248,9 +248,9 @@ __global__ __launch_bounds__(LX_TRACE_BLOCK) void k_trace(const LxJob* __restric
     uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t gsz = gridDim.x * blockDim.x;
     LxStack st;
-    st.lds = s_stack + threadIdx.x;
+    st.lds = (LX_LDS uint32_t*)(s_stack + threadIdx.x);
     st.stride = LX_TRACE_BLOCK;
-    st.spill = spill + gtid;
+    st.spill = (LX_GLOBAL uint32_t*)(spill + gtid);
     st.sstride = gsz;
     LxRayCount rc = {0, 0, 0};
     LxWork wc = {0, 0, 0}, wa = {0, 0, 0};
@@ -383,9 +383,9 @@ void k_finish(const LxJob* __restrict__ jobs, LxPathBuf P, LxCounters* __restric
     uint32_t gsz = gridDim.x * blockDim.x;
     uint32_t lane = lx_lane();
     LxStack st;
-    st.lds = s_stack + threadIdx.x;
+    st.lds = (LX_LDS uint32_t*)(s_stack + threadIdx.x);
     st.stride = BLOCK;
-    st.spill = spill + gtid;
+    st.spill = (LX_GLOBAL uint32_t*)(spill + gtid);
     st.sstride = gsz;
     LxRayCount rc = {0, 0, 0};
     LxWork wc = {0, 0, 0}, wa = {0, 0, 0};
@@ -615,9 +615,9 @@ __global__ __launch_bounds__(LX_TRACE_BLOCK) void k_trace_rays(const LxDevScene*
     uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t gsz = gridDim.x * blockDim.x;
     LxStack st;
-    st.lds = s_stack + threadIdx.x;
+    st.lds = (LX_LDS uint32_t*)(s_stack + threadIdx.x);
     st.stride = LX_TRACE_BLOCK;
-    st.spill = spill + gtid;
+    st.spill = (LX_GLOBAL uint32_t*)(spill + gtid);
     st.sstride = gsz;
     LxDevScene S = *Sp;
     uint32_t amask = alpha ? LX_INST_ALPHA : 0u;

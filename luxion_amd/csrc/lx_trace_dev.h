@@ -115,10 +115,29 @@ LXD bool lx_better(float t, uint32_t inst, uint32_t prim, const LxHitD& h) {
 #define LX_FAR_INFLATE 1.00000048f  /* 1 + 2^-21 */
 #define LX_CULL_MARGIN 1.00001526f  /* 1 + 2^-16 */
 
+/* Explicit address spaces: node, triangle, instance and spill accesses
+ * become global (not flat) memory instructions and the stack column LDS
+ * instructions, instead of flat accesses that also wait on the LDS counter
+ * and carry 64-bit addresses per lane. */
+#define LX_GLOBAL __attribute__((address_space(1)))
+#define LX_LDS __attribute__((address_space(3)))
+template <class T>
+LXD const LX_GLOBAL T* lx_gptr(const T* p) {
+    return (const LX_GLOBAL T*)p;
+}
+/* clang vector types: HIP's float4 / int4 cannot be copied from an
+ * address-space-qualified reference */
+typedef float lx_v4f __attribute__((ext_vector_type(4)));
+typedef int lx_v4i __attribute__((ext_vector_type(4)));
+LXD float4 lx_ld4(const LX_GLOBAL lx_v4f* p) {
+    lx_v4f v = *p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
 struct LxStack {
-    uint32_t* lds;     /* this lane's column: entry k at lds[k * stride] */
+    LX_LDS uint32_t* lds;     /* this lane's column: entry k at lds[k * stride] */
     uint32_t stride;
-    uint32_t* spill;   /* this lane's spill column: entry k at spill[k * sstride] */
+    LX_GLOBAL uint32_t* spill; /* this lane's spill column: entry k at spill[k * sstride] */
     uint32_t sstride;
     int sp;
     LXD void push(uint32_t v) {
@@ -151,10 +170,10 @@ LXD uint32_t lx_boxw(const typename LxNodeT<W>::type* __restrict__ nd, f3 o, f3 
     float lxs[W], hxs[W], lys[W], hys[W], lzs[W], hzs[W];
 #pragma unroll
     for (int g = 0; g < W / 4; g++) {
-        const float4* L = (const float4*)&nd->lox;
-        float4 a = L[0 * (W / 4) + g], b = L[1 * (W / 4) + g], c = L[2 * (W / 4) + g];
-        float4 d = L[3 * (W / 4) + g], e = L[4 * (W / 4) + g], f = L[5 * (W / 4) + g];
-        int4 ch = ((const int4*)&nd->child)[g];
+        const LX_GLOBAL lx_v4f* L = (const LX_GLOBAL lx_v4f*)lx_gptr((const float4*)&nd->lox);
+        lx_v4f a = L[0 * (W / 4) + g], b = L[1 * (W / 4) + g], c = L[2 * (W / 4) + g];
+        lx_v4f d = L[3 * (W / 4) + g], e = L[4 * (W / 4) + g], f = L[5 * (W / 4) + g];
+        lx_v4i ch = ((const LX_GLOBAL lx_v4i*)lx_gptr((const int4*)&nd->child))[g];
         lxs[4 * g + 0] = a.x, lxs[4 * g + 1] = a.y, lxs[4 * g + 2] = a.z, lxs[4 * g + 3] = a.w;
         hxs[4 * g + 0] = b.x, hxs[4 * g + 1] = b.y, hxs[4 * g + 2] = b.z, hxs[4 * g + 3] = b.w;
         lys[4 * g + 0] = c.x, lys[4 * g + 1] = c.y, lys[4 * g + 2] = c.z, lys[4 * g + 3] = c.w;
@@ -180,6 +199,74 @@ LXD uint32_t lx_boxw(const typename LxNodeT<W>::type* __restrict__ nd, f3 o, f3 
         mask |= h ? (1u << k) : 0u;
     }
     return mask;
+}
+
+/* After a node test with hit mask m != 0: returns the nearest hit child (the
+ * next node) and pushes the other hit children far to near, so they pop
+ * near to far.  BVH4 orders by a 5-exchange sorting network (branch-free);
+ * wider nodes by repeated selection.  The order only decides which boxes
+ * are culled by a closer hit found first, never which hit is returned. */
+template <int W>
+LXD uint32_t lx_order_children(const float* tn, const uint32_t* cs, uint32_t m, LxStack& st, uint32_t* overflow) {
+    if (W == 4) {
+        float t0 = (m & 1u) ? tn[0] : __builtin_inff(), t1 = (m & 2u) ? tn[1] : __builtin_inff();
+        float t2 = (m & 4u) ? tn[2] : __builtin_inff(), t3 = (m & 8u) ? tn[3] : __builtin_inff();
+        uint32_t c0 = cs[0], c1 = cs[1], c2 = cs[2], c3 = cs[3];
+        bool h0 = m & 1u, h1 = (m >> 1) & 1u, h2 = (m >> 2) & 1u, h3 = (m >> 3) & 1u;
+#define LX_CX(a, b)                                                                                                    \
+    {                                                                                                                  \
+        bool sw = t##b < t##a;                                                                                         \
+        float tt = sw ? t##b : t##a;                                                                                   \
+        t##b = sw ? t##a : t##b;                                                                                       \
+        t##a = tt;                                                                                                     \
+        uint32_t cc = sw ? c##b : c##a;                                                                                \
+        c##b = sw ? c##a : c##b;                                                                                       \
+        c##a = cc;                                                                                                     \
+        bool hh = sw ? h##b : h##a;                                                                                    \
+        h##b = sw ? h##a : h##b;                                                                                       \
+        h##a = hh;                                                                                                     \
+    }
+        LX_CX(0, 1) LX_CX(2, 3) LX_CX(0, 2) LX_CX(1, 3) LX_CX(1, 2)
+#undef LX_CX
+        int n = (int)h1 + (int)h2 + (int)h3;
+        if (st.sp + n > LX_STACK + LX_SPILL) {
+            atomicOr(overflow, 1u);
+            return c0;
+        }
+        if (h3) st.push(c3);
+        if (h2) st.push(c2);
+        if (h1) st.push(c1);
+        return c0;
+    } else {
+        float bt = __builtin_inff();
+        int bk = -1;
+#pragma unroll
+        for (int k = 0; k < W; k++)
+            if ((m >> k) & 1u)
+                if (tn[k] < bt) {
+                    bt = tn[k];
+                    bk = k;
+                }
+        m &= ~(1u << bk);
+        while (m) {
+            float ft = -1.0f;
+            int fk = -1;
+#pragma unroll
+            for (int k = 0; k < W; k++)
+                if ((m >> k) & 1u)
+                    if (tn[k] > ft) {
+                        ft = tn[k];
+                        fk = k;
+                    }
+            m &= ~(1u << fk);
+            if (st.sp >= LX_STACK + LX_SPILL) {
+                atomicOr(overflow, 1u);
+                continue;
+            }
+            st.push(cs[fk]);
+        }
+        return cs[bk];
+    }
 }
 
 /* Closest (ANY = false) or any (ANY = true) accepted hit along the ray. */
@@ -222,42 +309,14 @@ LXD LxHitD lx_trace(const LxDevScene& S, f3 wo, f3 wd, float tmax, uint32_t key,
             if (m == 0u) {
                 do_pop = true;
             } else {
-                /* nearest child next; the rest pushed far to near */
-                float bt = __builtin_inff();
-                int bk = -1;
-#pragma unroll
-                for (int k = 0; k < W; k++)
-                    if ((m >> k) & 1u)
-                        if (tn[k] < bt) {
-                            bt = tn[k];
-                            bk = k;
-                        }
-                m &= ~(1u << bk);
-                while (m) {
-                    float ft = -1.0f;
-                    int fk = -1;
-#pragma unroll
-                    for (int k = 0; k < W; k++)
-                        if ((m >> k) & 1u)
-                            if (tn[k] > ft) {
-                                ft = tn[k];
-                                fk = k;
-                            }
-                    m &= ~(1u << fk);
-                    if (st.sp >= LX_STACK + LX_SPILL) {
-                        atomicOr(overflow, 1u);
-                        continue;
-                    }
-                    st.push(cs[fk]);
-                }
-                node = cs[bk];
+                node = lx_order_children<W>(tn, cs, m, st, overflow);
                 continue;
             }
         } else if (!in_blas) {
             cur_inst = node & LX_LEAF_FIRST_MASK;
-            const LxInstance& I = S.inst[cur_inst];
-            o = xf_point(I.inv, wo);
-            d = xf_vector(I.inv, wd);
+            const LX_GLOBAL LxInstance& I = lx_gptr(S.inst)[cur_inst];
+            o = xf_point((const float*)I.inv, wo);
+            d = xf_vector((const float*)I.inv, wd);
             ix = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
             inst_flags = I.flags & alpha_mask;
             in_blas = true;
@@ -274,8 +333,8 @@ LXD LxHitD lx_trace(const LxDevScene& S, f3 wo, f3 wd, float tmax, uint32_t key,
             uint32_t first = node & LX_LEAF_FIRST_MASK;
             uint32_t cnt = ((node >> 28) & 7u) + 1u;
             for (uint32_t k = 0; k < cnt; k++) {
-                const float4* tp = S.tris + 3u * (size_t)(first + k);
-                float4 A = tp[0], E1 = tp[1], E2 = tp[2];
+                const LX_GLOBAL lx_v4f* tp = (const LX_GLOBAL lx_v4f*)lx_gptr(S.tris) + 3u * (size_t)(first + k);
+                float4 A = lx_ld4(tp), E1 = lx_ld4(tp + 1), E2 = lx_ld4(tp + 2);
                 float t, u, v;
                 if (CNT) work->tris++;
                 if (!lx_mt(o, d, A, E1, E2, tmax, t, u, v)) continue;
@@ -355,41 +414,14 @@ LXD LxHitD lx_trace_pair(const LxDevScene& S, bool has0, f3 o0, f3 d0, float tma
             if (m == 0u) {
                 do_pop = true;
             } else {
-                float bt = __builtin_inff();
-                int bk = -1;
-#pragma unroll
-                for (int k = 0; k < W; k++)
-                    if ((m >> k) & 1u)
-                        if (tn[k] < bt) {
-                            bt = tn[k];
-                            bk = k;
-                        }
-                m &= ~(1u << bk);
-                while (m) {
-                    float ft = -1.0f;
-                    int fk = -1;
-#pragma unroll
-                    for (int k = 0; k < W; k++)
-                        if ((m >> k) & 1u)
-                            if (tn[k] > ft) {
-                                ft = tn[k];
-                                fk = k;
-                            }
-                    m &= ~(1u << fk);
-                    if (st.sp >= LX_STACK + LX_SPILL) {
-                        atomicOr(overflow, 1u);
-                        continue;
-                    }
-                    st.push(cs[fk]);
-                }
-                node = cs[bk];
+                node = lx_order_children<W>(tn, cs, m, st, overflow);
                 continue;
             }
         } else if (!in_blas) {
             cur_inst = node & LX_LEAF_FIRST_MASK;
-            const LxInstance& I = S.inst[cur_inst];
-            o = xf_point(I.inv, wo);
-            d = xf_vector(I.inv, wd);
+            const LX_GLOBAL LxInstance& I = lx_gptr(S.inst)[cur_inst];
+            o = xf_point((const float*)I.inv, wo);
+            d = xf_vector((const float*)I.inv, wd);
             ix = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
             inst_flags = I.flags & alpha_mask;
             in_blas = true;
@@ -406,8 +438,8 @@ LXD LxHitD lx_trace_pair(const LxDevScene& S, bool has0, f3 o0, f3 d0, float tma
             uint32_t first = node & LX_LEAF_FIRST_MASK;
             uint32_t cnt = ((node >> 28) & 7u) + 1u;
             for (uint32_t k = 0; k < cnt; k++) {
-                const float4* tp = S.tris + 3u * (size_t)(first + k);
-                float4 A = tp[0], E1 = tp[1], E2 = tp[2];
+                const LX_GLOBAL lx_v4f* tp = (const LX_GLOBAL lx_v4f*)lx_gptr(S.tris) + 3u * (size_t)(first + k);
+                float4 A = lx_ld4(tp), E1 = lx_ld4(tp + 1), E2 = lx_ld4(tp + 2);
                 float t, u, v;
                 if (CNT) work->tris++;
                 if (!lx_mt(o, d, A, E1, E2, tmax, t, u, v)) continue;
