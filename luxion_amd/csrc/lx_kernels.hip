@@ -290,7 +290,7 @@ LXD void lx_load_rec(const LxJob& J, const LxDevScene& S, const LxRecBuf& R, uin
 
 LXD void lx_accumulate(float* __restrict__ accum, const LxJob& J, uint32_t pix, f3 col) {
     /* finalGather: the path's colour is final */
-    float* a = accum + 3u * (size_t)(J.accum_off + pix);
+    LX_GLOBAL float* a = (LX_GLOBAL float*)(accum + 3u * (size_t)(J.accum_off + pix));
     a[0] += col.x;
     a[1] += col.y;
     a[2] += col.z;
@@ -412,16 +412,16 @@ void k_finish(const LxJob* __restrict__ jobs, LxPathBuf P, LxCounters* __restric
             if (!have && !flush) {
                 uint32_t my = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
                 if (my < n) {
-                    if (order) my = order[my]; /* slot of the my-th queued path */
-                    job = P.job[my];
-                    uint32_t pw = P.pix[my];
+                    if (order) my = lx_gptr(order)[my]; /* slot of the my-th queued path */
+                    job = lx_gptr(P.job)[my];
+                    uint32_t pw = lx_gptr(P.pix)[my];
                     spec = (pw >> 31) != 0u;
                     pix = pw & 0x7fffffffu;
-                    o = mk3(P.ox[my], P.oy[my], P.oz[my]);
-                    d = mk3(P.dx[my], P.dy[my], P.dz[my]);
-                    thr = mk3(P.tx[my], P.ty[my], P.tz[my]);
-                    col = mk3(P.cx[my], P.cy[my], P.cz[my]);
-                    pdf = P.pdf[my];
+                    o = mk3(lx_gptr(P.ox)[my], lx_gptr(P.oy)[my], lx_gptr(P.oz)[my]);
+                    d = mk3(lx_gptr(P.dx)[my], lx_gptr(P.dy)[my], lx_gptr(P.dz)[my]);
+                    thr = mk3(lx_gptr(P.tx)[my], lx_gptr(P.ty)[my], lx_gptr(P.tz)[my]);
+                    col = mk3(lx_gptr(P.cx)[my], lx_gptr(P.cy)[my], lx_gptr(P.cz)[my]);
+                    pdf = lx_gptr(P.pdf)[my];
                     b = bounce0;
                     have = true;
                 }

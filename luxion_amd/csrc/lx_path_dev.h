@@ -75,7 +75,7 @@ LXD void lx_trace_path(const LxJob& J, const LxDevScene& S, f3 o, f3 d, uint32_t
     if (J.dmis) out.light(lp);
     if (J.emis) out.env_dir(ed);
     if (h.inst != 0xffffffffu) {
-        bool want_t = S.mats[S.inst[h.inst].material].normal_tex >= 0;
+        bool want_t = lx_gptr(S.mats)[lx_gptr(S.inst)[h.inst].material].normal_tex >= 0;
         f3 N, Tg;
         f2 uv;
         int mat;
@@ -139,7 +139,7 @@ LXD void lx_trace_path_d(const LxJob& J, const LxDevScene& S, f3 o, f3 d, uint32
     if (J.dmis) out.light(lp);
     if (J.emis) out.env_dir(ed);
     if (h.inst != 0xffffffffu) {
-        bool want_t = S.mats[S.inst[h.inst].material].normal_tex >= 0;
+        bool want_t = lx_gptr(S.mats)[lx_gptr(S.inst)[h.inst].material].normal_tex >= 0;
         f3 N, Tg;
         f2 uv;
         int mat;

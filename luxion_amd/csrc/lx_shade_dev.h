@@ -17,6 +17,25 @@
 
 #define LXD __device__ __forceinline__
 
+/* Explicit address spaces: scene data read through pointers cast to the
+ * global address space compiles to global (not flat) memory instructions,
+ * which neither wait on the LDS counter nor carry 64-bit addresses per lane;
+ * the traversal stack column is typed LDS. */
+#define LX_GLOBAL __attribute__((address_space(1)))
+#define LX_LDS __attribute__((address_space(3)))
+template <class T>
+LXD const LX_GLOBAL T* lx_gptr(const T* p) {
+    return (const LX_GLOBAL T*)p;
+}
+/* clang vector types: HIP's float4 / int4 cannot be copied from an
+ * address-space-qualified reference */
+typedef float lx_v4f __attribute__((ext_vector_type(4)));
+typedef int lx_v4i __attribute__((ext_vector_type(4)));
+LXD float4 lx_ld4(const LX_GLOBAL lx_v4f* p) {
+    lx_v4f v = *p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
 struct f3 {
     float x, y, z;
 };
@@ -91,19 +110,19 @@ LXD LxRng lx_make_rng(int iter, int index, int depth, uint32_t seed) {
 LXD void lx_texel(const LxTexDesc& t, int i, int j, float* o) {
     size_t k = ((size_t)j * (size_t)t.w + (size_t)i);
     if (t.fmt == LX_TEX_RGBA8) {
-        uint32_t p = ((const uint32_t*)t.data)[k];
+        uint32_t p = lx_gptr((const uint32_t*)t.data)[k];
         o[0] = ((float)(p & 0xffu)) / 255.0f;
         o[1] = ((float)((p >> 8) & 0xffu)) / 255.0f;
         o[2] = ((float)((p >> 16) & 0xffu)) / 255.0f;
         o[3] = ((float)(p >> 24)) / 255.0f;
     } else if (t.fmt == LX_TEX_RGBA16) {
-        const uint16_t* s = (const uint16_t*)t.data + 4 * k;
+        const LX_GLOBAL uint16_t* s = lx_gptr((const uint16_t*)t.data) + 4 * k;
         o[0] = ((float)s[0]) / 65535.0f;
         o[1] = ((float)s[1]) / 65535.0f;
         o[2] = ((float)s[2]) / 65535.0f;
         o[3] = ((float)s[3]) / 65535.0f;
     } else {
-        float4 v = ((const float4*)t.data)[k];
+        float4 v = lx_ld4((const LX_GLOBAL lx_v4f*)t.data + k);
         o[0] = v.x;
         o[1] = v.y;
         o[2] = v.z;
@@ -412,7 +431,8 @@ LXD f3 lx_ct_eval_cos(f3 wo, f3 n, f3 wi, f3 albedo, float rough, float metal, f
 
 /* ---------------------------------------------------------- CDF sampling */
 /* cudaUtils::select_from_cdf, src/common.cu:3-15 */
-LXD int lx_select_cdf(const float* cdf, int size, float xi) {
+LXD int lx_select_cdf(const float* cdf_, int size, float xi) {
+    const LX_GLOBAL float* cdf = lx_gptr(cdf_);
     int lo = 0, hi = size - 1;
     while (lo < hi) {
         int mid = lo + (hi - lo) / 2;
@@ -433,9 +453,11 @@ LXD float lx_envmap_pdf(const LxDevScene& S, f3 d) {
     col = col < 0 ? 0 : (col > W - 1 ? W - 1 : col);
     int row = (int)(v * H);
     row = row < 0 ? 0 : (row > H - 1 ? H - 1 : row);
-    float pm = (S.marginal[row] - (row == 0 ? 0.0f : S.marginal[row - 1])) * H;
+    const LX_GLOBAL float* marg = lx_gptr(S.marginal);
+    const LX_GLOBAL float* cond = lx_gptr(S.conditional);
+    float pm = (marg[row] - (row == 0 ? 0.0f : marg[row - 1])) * H;
     size_t ci = (size_t)row * W + col;
-    float pc = (S.conditional[ci] - (col == 0 ? 0.0f : S.conditional[ci - 1])) * W;
+    float pc = (cond[ci] - (col == 0 ? 0.0f : cond[ci - 1])) * W;
     float st = lxm_sin(theta);
     return (st > 1e-6f) ? (pm * pc) / (2.0f * LX_PI_F * LX_PI_F * st) : 0.0f;
 }
@@ -457,17 +479,18 @@ LXD f3 lx_env_lookup(const LxDevScene& S, f3 d, float intensity) {
 LXD f3 lx_sample_light_point(const LxDevScene& S, LxRng& rng) {
     float rnd = rng.u01();
     int sel = lx_select_cdf(S.emissive_prefix, (int)S.num_emissive, rnd);
-    const LxInstance& g = S.inst[S.emissive_inst[sel]];
-    float lower = (sel == 0) ? 0.0f : S.emissive_prefix[sel - 1];
-    float upper = S.emissive_prefix[sel];
+    const LX_GLOBAL LxInstance& g = lx_gptr(S.inst)[lx_gptr(S.emissive_inst)[sel]];
+    const LX_GLOBAL float* prefix = lx_gptr(S.emissive_prefix);
+    float lower = (sel == 0) ? 0.0f : prefix[sel - 1];
+    float upper = prefix[sel];
     float den = upper - lower;
     float to = (den > 1e-10f) ? (rnd - lower) / den : 0.0f;
     to = lxm_clamp(to, 0.0f, 1.0f);
-    uint32_t co = S.tri_cdf_off[sel];
-    int ntri = (int)(S.tri_cdf_off[sel + 1] - co);
+    uint32_t co = lx_gptr(S.tri_cdf_off)[sel];
+    int ntri = (int)(lx_gptr(S.tri_cdf_off)[sel + 1] - co);
     int tri = lx_select_cdf(S.tri_cdf + co, ntri, to);
-    const uint32_t* ix = S.idx + g.idx_off + 3u * (uint32_t)tri;
-    const float* P = S.pos + 3u * (size_t)g.vtx_off;
+    const LX_GLOBAL uint32_t* ix = lx_gptr(S.idx) + g.idx_off + 3u * (uint32_t)tri;
+    const LX_GLOBAL float* P = lx_gptr(S.pos) + 3u * (size_t)g.vtx_off;
     f3 v0 = mk3(P[3 * ix[0]], P[3 * ix[0] + 1], P[3 * ix[0] + 2]);
     f3 v1 = mk3(P[3 * ix[1]], P[3 * ix[1] + 1], P[3 * ix[1] + 2]);
     f3 v2 = mk3(P[3 * ix[2]], P[3 * ix[2] + 1], P[3 * ix[2] + 2]);
@@ -476,7 +499,7 @@ LXD f3 lx_sample_light_point(const LxDevScene& S, LxRng& rng) {
     float u = 1.0f - r1;
     float v = r2 * r1;
     f3 local = ((u * v0) + (v * v1)) + ((1.0f - u - v) * v2);
-    return xf_point(g.m, local);
+    return xf_point((const float*)g.m, local);
 }
 
 /* sampleDirectLight environment branch, sample_direct_light.cu:68-100 */
@@ -484,8 +507,9 @@ LXD f3 lx_sample_env_dir(const LxDevScene& S, LxRng& rng) {
     int W = S.env.w, H = S.env.h;
     float rnd = rng.u01();
     int mg = lx_select_cdf(S.marginal, H, rnd);
-    float lower = (mg == 0) ? 0.0f : S.marginal[mg - 1];
-    float upper = S.marginal[mg];
+    const LX_GLOBAL float* marg = lx_gptr(S.marginal);
+    float lower = (mg == 0) ? 0.0f : marg[mg - 1];
+    float upper = marg[mg];
     float den = upper - lower;
     float off = (den > 1e-10f) ? (rnd - lower) / den : 0.0f;
     off = lxm_clamp(off, 0.0f, 1.0f);

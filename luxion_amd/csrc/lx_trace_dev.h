@@ -45,7 +45,7 @@ LXD uint32_t lx_hash_u32(uint32_t x) {
 
 /* __anyhit__ms_all, optix_triangle.cu:317-366.  Keyed on the pixel index
  * (deterministic) instead of the compacted launch index. */
-LXD bool lx_any_hit_accept(const LxDevScene& S, const LxInstance& I, uint32_t inst, uint32_t prim, float bu,
+LXD bool lx_any_hit_accept(const LxDevScene& S, const LX_GLOBAL LxInstance& I, uint32_t inst, uint32_t prim, float bu,
                            float bv, f3 wdir, uint32_t key, uint32_t iteration) {
     const lx_material_desc& mat = S.mats[I.material];
     if (mat.alpha_mode == LX_ALPHA_OPAQUE) return true;
@@ -53,8 +53,8 @@ LXD bool lx_any_hit_accept(const LxDevScene& S, const LxInstance& I, uint32_t in
     if (mat.albedo_tex >= 0) {
         f2 uv = mk2(0.0f, 0.0f);
         if (I.has_n_uv & 2u) {
-            const uint32_t* ix = S.idx + I.idx_off + 3u * prim;
-            const float* UV = S.uv + 2u * (size_t)I.vtx_off;
+            const LX_GLOBAL uint32_t* ix = lx_gptr(S.idx) + I.idx_off + 3u * prim;
+            const LX_GLOBAL float* UV = lx_gptr(S.uv) + 2u * (size_t)I.vtx_off;
             f2 A = mk2(UV[2 * ix[0]], UV[2 * ix[0] + 1]);
             f2 B = mk2(UV[2 * ix[1]], UV[2 * ix[1] + 1]);
             f2 C = mk2(UV[2 * ix[2]], UV[2 * ix[2] + 1]);
@@ -114,25 +114,6 @@ LXD bool lx_better(float t, uint32_t inst, uint32_t prim, const LxHitD& h) {
 
 #define LX_FAR_INFLATE 1.00000048f  /* 1 + 2^-21 */
 #define LX_CULL_MARGIN 1.00001526f  /* 1 + 2^-16 */
-
-/* Explicit address spaces: node, triangle, instance and spill accesses
- * become global (not flat) memory instructions and the stack column LDS
- * instructions, instead of flat accesses that also wait on the LDS counter
- * and carry 64-bit addresses per lane. */
-#define LX_GLOBAL __attribute__((address_space(1)))
-#define LX_LDS __attribute__((address_space(3)))
-template <class T>
-LXD const LX_GLOBAL T* lx_gptr(const T* p) {
-    return (const LX_GLOBAL T*)p;
-}
-/* clang vector types: HIP's float4 / int4 cannot be copied from an
- * address-space-qualified reference */
-typedef float lx_v4f __attribute__((ext_vector_type(4)));
-typedef int lx_v4i __attribute__((ext_vector_type(4)));
-LXD float4 lx_ld4(const LX_GLOBAL lx_v4f* p) {
-    lx_v4f v = *p;
-    return make_float4(v.x, v.y, v.z, v.w);
-}
 
 struct LxStack {
     LX_LDS uint32_t* lds;     /* this lane's column: entry k at lds[k * stride] */
@@ -341,7 +322,7 @@ LXD LxHitD lx_trace(const LxDevScene& S, f3 wo, f3 wd, float tmax, uint32_t key,
                 uint32_t prim = lxm_f2u(A.w);
                 if (!ANY && !lx_better(t, cur_inst, prim, best)) continue;
                 if ((inst_flags & LX_INST_ALPHA) &&
-                    !lx_any_hit_accept(S, S.inst[cur_inst], cur_inst, prim, u, v, wd, key, iteration))
+                    !lx_any_hit_accept(S, lx_gptr(S.inst)[cur_inst], cur_inst, prim, u, v, wd, key, iteration))
                     continue;
                 best.t = t;
                 best.inst = cur_inst;
@@ -446,7 +427,7 @@ LXD LxHitD lx_trace_pair(const LxDevScene& S, bool has0, f3 o0, f3 d0, float tma
                 uint32_t prim = lxm_f2u(A.w);
                 if (!any && !lx_better(t, cur_inst, prim, best)) continue;
                 if ((inst_flags & LX_INST_ALPHA) &&
-                    !lx_any_hit_accept(S, S.inst[cur_inst], cur_inst, prim, u, v, wd, key, iteration))
+                    !lx_any_hit_accept(S, lx_gptr(S.inst)[cur_inst], cur_inst, prim, u, v, wd, key, iteration))
                     continue;
                 best.t = t;
                 best.inst = cur_inst;
@@ -481,10 +462,10 @@ LXD LxHitD lx_trace_pair(const LxDevScene& S, bool has0, f3 o0, f3 d0, float tma
 /* __closesthit__ch, optix_triangle.cu:156-243: interpolated normal (not
  * flipped toward the ray), uv, uv-derivative tangent, object -> world. */
 LXD void lx_hit_attrs(const LxDevScene& S, const LxHitD& h, bool want_tangent, f3& N, f3& Tg, f2& uv, int& mat) {
-    const LxInstance& I = S.inst[h.inst];
-    const uint32_t* ix = S.idx + I.idx_off + 3u * h.prim;
+    const LX_GLOBAL LxInstance& I = lx_gptr(S.inst)[h.inst];
+    const LX_GLOBAL uint32_t* ix = lx_gptr(S.idx) + I.idx_off + 3u * h.prim;
     uint32_t i0 = ix[0], i1 = ix[1], i2 = ix[2];
-    const float* P = S.pos + 3u * (size_t)I.vtx_off;
+    const LX_GLOBAL float* P = lx_gptr(S.pos) + 3u * (size_t)I.vtx_off;
     f3 A = mk3(P[3 * i0], P[3 * i0 + 1], P[3 * i0 + 2]);
     f3 B = mk3(P[3 * i1], P[3 * i1 + 1], P[3 * i1 + 2]);
     f3 C = mk3(P[3 * i2], P[3 * i2 + 1], P[3 * i2 + 2]);
@@ -492,7 +473,7 @@ LXD void lx_hit_attrs(const LxDevScene& S, const LxHitD& h, bool want_tangent, f
     f3 normal;
     float bA = 1.0f - h.u - h.v;
     if (I.has_n_uv & 1u) {
-        const float* Nn = S.nrm + 3u * (size_t)I.vtx_off;
+        const LX_GLOBAL float* Nn = lx_gptr(S.nrm) + 3u * (size_t)I.vtx_off;
         f3 nA = mk3(Nn[3 * i0], Nn[3 * i0 + 1], Nn[3 * i0 + 2]);
         f3 nB = mk3(Nn[3 * i1], Nn[3 * i1 + 1], Nn[3 * i1 + 2]);
         f3 nC = mk3(Nn[3 * i2], Nn[3 * i2 + 1], Nn[3 * i2 + 2]);
@@ -503,7 +484,7 @@ LXD void lx_hit_attrs(const LxDevScene& S, const LxHitD& h, bool want_tangent, f
     f3 tangent = mk3(1.0f, 0.0f, 0.0f);
     uv = mk2(0.0f, 0.0f);
     if (I.has_n_uv & 2u) {
-        const float* U = S.uv + 2u * (size_t)I.vtx_off;
+        const LX_GLOBAL float* U = lx_gptr(S.uv) + 2u * (size_t)I.vtx_off;
         f2 a = mk2(U[2 * i0], U[2 * i0 + 1]), b = mk2(U[2 * i1], U[2 * i1 + 1]), c = mk2(U[2 * i2], U[2 * i2 + 1]);
         uv = mk2((bA * a.x + h.u * b.x) + h.v * c.x, (bA * a.y + h.u * b.y) + h.v * c.y);
         if (want_tangent) {
@@ -515,9 +496,9 @@ LXD void lx_hit_attrs(const LxDevScene& S, const LxHitD& h, bool want_tangent, f
                               f * (d2.y * e1.z - d1.y * e2.z));
         }
     }
-    normal = normalize(xf_normal(I.inv, normal));
+    normal = normalize(xf_normal((const float*)I.inv, normal));
     if (want_tangent) {
-        tangent = normalize(xf_vector(I.m, tangent));
+        tangent = normalize(xf_vector((const float*)I.m, tangent));
         f3 T = tangent - dot(tangent, normal) * normal;
         float len2 = dot(T, T);
         if (len2 > 1e-10f) {
@@ -534,19 +515,19 @@ LXD void lx_hit_attrs(const LxDevScene& S, const LxHitD& h, bool want_tangent, f
 
 /* __closesthit__ch_direct_light, optix_triangle.cu:259-305 */
 LXD void lx_dl_attrs(const LxDevScene& S, const LxHitD& h, f3& N, f2& uv, int& mat) {
-    const LxInstance& I = S.inst[h.inst];
-    const uint32_t* ix = S.idx + I.idx_off + 3u * h.prim;
+    const LX_GLOBAL LxInstance& I = lx_gptr(S.inst)[h.inst];
+    const LX_GLOBAL uint32_t* ix = lx_gptr(S.idx) + I.idx_off + 3u * h.prim;
     uint32_t i0 = ix[0], i1 = ix[1], i2 = ix[2];
     float bA = 1.0f - h.u - h.v;
     f3 normal;
     if (I.has_n_uv & 1u) {
-        const float* Nn = S.nrm + 3u * (size_t)I.vtx_off;
+        const LX_GLOBAL float* Nn = lx_gptr(S.nrm) + 3u * (size_t)I.vtx_off;
         f3 nA = mk3(Nn[3 * i0], Nn[3 * i0 + 1], Nn[3 * i0 + 2]);
         f3 nB = mk3(Nn[3 * i1], Nn[3 * i1 + 1], Nn[3 * i1 + 2]);
         f3 nC = mk3(Nn[3 * i2], Nn[3 * i2 + 1], Nn[3 * i2 + 2]);
         normal = normalize(((bA * nA) + (h.u * nB)) + (h.v * nC));
     } else {
-        const float* P = S.pos + 3u * (size_t)I.vtx_off;
+        const LX_GLOBAL float* P = lx_gptr(S.pos) + 3u * (size_t)I.vtx_off;
         f3 A = mk3(P[3 * i0], P[3 * i0 + 1], P[3 * i0 + 2]);
         f3 B = mk3(P[3 * i1], P[3 * i1 + 1], P[3 * i1 + 2]);
         f3 C = mk3(P[3 * i2], P[3 * i2 + 1], P[3 * i2 + 2]);
@@ -554,11 +535,11 @@ LXD void lx_dl_attrs(const LxDevScene& S, const LxHitD& h, f3& N, f2& uv, int& m
     }
     uv = mk2(0.0f, 0.0f);
     if (I.has_n_uv & 2u) {
-        const float* U = S.uv + 2u * (size_t)I.vtx_off;
+        const LX_GLOBAL float* U = lx_gptr(S.uv) + 2u * (size_t)I.vtx_off;
         f2 a = mk2(U[2 * i0], U[2 * i0 + 1]), b = mk2(U[2 * i1], U[2 * i1 + 1]), c = mk2(U[2 * i2], U[2 * i2 + 1]);
         uv = mk2((bA * a.x + h.u * b.x) + h.v * c.x, (bA * a.y + h.u * b.y) + h.v * c.y);
     }
-    N = normalize(xf_normal(I.inv, normal));
+    N = normalize(xf_normal((const float*)I.inv, normal));
     mat = (int)I.material;
 }
 
