@@ -315,7 +315,7 @@ void lx_build_tlas_host(const std::vector<float>& boxes, int width, BvhBuildResu
         }
         p.id = i;
     }
-    Builder B(prims, 1, 16);
+    Builder B(prims, 1, 64); /* instances are few: finer bins are cheap */
     int32_t root = B.alloc();
     B.build(root, 0, n, 0);
     B.nodes.resize(B.next.load());

@@ -353,7 +353,11 @@ void lx_scene_accel(lx_scene* s, const lx_build_opts* o) {
     std::vector<LxWideNode> nodes;
     std::vector<float4> tris;
     std::vector<uint32_t> blas_root(s->meshes.size());
-    std::vector<std::array<float, 6>> mesh_box(s->meshes.size());
+    /* object-space boxes bounding each mesh: its BLAS's boxes two levels
+     * below the root (at most 64), or the mesh box; a rotated instance's
+     * world box is the union of these boxes' transformed corners, much
+     * tighter than the transformed box of the whole mesh */
+    std::vector<std::vector<std::array<float, 6>>> mesh_boxes(s->meshes.size());
     int max_depth = 0;
     double sah = 0.0;
     for (size_t i = 0; i < s->meshes.size(); i++) {
@@ -376,6 +380,22 @@ void lx_scene_accel(lx_scene* s, const lx_build_opts* o) {
             }
             nodes.push_back(n);
         }
+        if (!(r.root & LX_LEAF_BIT) && r.root != LX_CHILD_EMPTY) {
+            const LxWideNode& rn = r.nodes[r.root];
+            for (int k = 0; k < 8; k++) {
+                uint32_t c = rn.child[k];
+                if (c == LX_CHILD_EMPTY) continue;
+                if (c & LX_LEAF_BIT) {
+                    mesh_boxes[i].push_back({rn.lo[k][0], rn.lo[k][1], rn.lo[k][2], rn.hi[k][0], rn.hi[k][1], rn.hi[k][2]});
+                    continue;
+                }
+                const LxWideNode& cn = r.nodes[c];
+                for (int q = 0; q < 8; q++)
+                    if (cn.child[q] != LX_CHILD_EMPTY)
+                        mesh_boxes[i].push_back(
+                            {cn.lo[q][0], cn.lo[q][1], cn.lo[q][2], cn.hi[q][0], cn.hi[q][1], cn.hi[q][2]});
+            }
+        }
         uint32_t rr = r.root;
         if (rr & LX_LEAF_BIT) rr = (rr & ~LX_LEAF_FIRST_MASK) | ((rr & LX_LEAF_FIRST_MASK) + toff);
         else rr += noff;
@@ -390,7 +410,7 @@ void lx_scene_accel(lx_scene* s, const lx_build_opts* o) {
                 b[a] = std::min(b[a], m.pos[3 * v + a]);
                 b[3 + a] = std::max(b[3 + a], m.pos[3 * v + a]);
             }
-        mesh_box[i] = b;
+        if (mesh_boxes[i].empty()) mesh_boxes[i].push_back(b);
     }
 
     /* TLAS over conservative world boxes of the instances */
@@ -398,17 +418,17 @@ void lx_scene_accel(lx_scene* s, const lx_build_opts* o) {
     std::vector<LxInstance> dinst(s->insts.size());
     for (size_t i = 0; i < s->insts.size(); i++) {
         const lx_instance_desc& g = s->insts[i];
-        const std::array<float, 6>& b = mesh_box[g.mesh];
         double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-        for (int c = 0; c < 8; c++) {
-            double p[3] = {(c & 1) ? b[3] : b[0], (c & 2) ? b[4] : b[1], (c & 4) ? b[5] : b[2]};
-            for (int r = 0; r < 3; r++) {
-                double w = (double)g.transform[4 * r] * p[0] + (double)g.transform[4 * r + 1] * p[1] +
-                           (double)g.transform[4 * r + 2] * p[2] + (double)g.transform[4 * r + 3];
-                lo[r] = std::min(lo[r], w);
-                hi[r] = std::max(hi[r], w);
+        for (const std::array<float, 6>& b : mesh_boxes[g.mesh])
+            for (int c = 0; c < 8; c++) {
+                double p[3] = {(c & 1) ? b[3] : b[0], (c & 2) ? b[4] : b[1], (c & 4) ? b[5] : b[2]};
+                for (int r = 0; r < 3; r++) {
+                    double w = (double)g.transform[4 * r] * p[0] + (double)g.transform[4 * r + 1] * p[1] +
+                               (double)g.transform[4 * r + 2] * p[2] + (double)g.transform[4 * r + 3];
+                    lo[r] = std::min(lo[r], w);
+                    hi[r] = std::max(hi[r], w);
+                }
             }
-        }
         float flo[3], fhi[3];
         for (int r = 0; r < 3; r++) {
             flo[r] = std::nextafter((float)lo[r], -FLT_MAX);

@@ -256,6 +256,79 @@ struct LxWork {
     uint32_t nodes, tris, insts;
 };
 
+/* Next entry of the traversal stack; a sentinel (leaving an instance)
+ * restores the world-space ray and is consumed in the same step, so an
+ * iteration of the traversal loops never spends itself on a sentinel.
+ * Returns false when the stack is empty (the ray is done). */
+LXD bool lx_pop_next(LxStack& st, uint32_t& node, bool& in_blas, f3& o, f3& d, f3& ix, f3 wo, f3 wd, f3 iwd) {
+    if (st.sp == 0) return false;
+    node = st.pop();
+    if (node == LX_SENTINEL) {
+        o = wo;
+        d = wd;
+        ix = iwd;
+        in_blas = false;
+        if (st.sp == 0) return false;
+        node = st.pop();
+    }
+    return true;
+}
+
+/* Enters the instance of TLAS leaf `node`: ray to object space, sentinel
+ * pushed, node = the BLAS root (tested in the same iteration).  Returns
+ * false if the stack is full (the instance is skipped and flagged). */
+template <bool CNT>
+LXD bool lx_enter_instance(const LxDevScene& S, uint32_t& node, uint32_t& cur_inst, uint32_t& inst_flags,
+                           bool& in_blas, f3& o, f3& d, f3& ix, f3 wo, f3 wd, uint32_t alpha_mask, LxStack& st,
+                           uint32_t* overflow, LxWork* work) {
+    if (st.sp >= LX_STACK + LX_SPILL) {
+        atomicOr(overflow, 1u);
+        return false;
+    }
+    cur_inst = node & LX_LEAF_FIRST_MASK;
+    const LX_GLOBAL LxInstance& I = lx_gptr(S.inst)[cur_inst];
+    o = xf_point((const float*)I.inv, wo);
+    d = xf_vector((const float*)I.inv, wd);
+    ix = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    inst_flags = I.flags & alpha_mask;
+    in_blas = true;
+    st.push(LX_SENTINEL);
+    if (CNT) work->insts++;
+    node = I.blas_root;
+    return true;
+}
+
+/* Triangle leaf of the current instance; returns true if an any-hit ray
+ * found its hit. */
+template <bool ANY, bool CNT>
+LXD bool lx_leaf(const LxDevScene& S, uint32_t node, uint32_t cur_inst, uint32_t inst_flags, f3 o, f3 d, f3 wd,
+                 float tmax, uint32_t key, uint32_t iteration, LxHitD& best, LxWork* work) {
+    uint32_t first = node & LX_LEAF_FIRST_MASK;
+    uint32_t cnt = ((node >> 28) & 7u) + 1u;
+    for (uint32_t k = 0; k < cnt; k++) {
+        const LX_GLOBAL lx_v4f* tp = (const LX_GLOBAL lx_v4f*)lx_gptr(S.tris) + 3u * (size_t)(first + k);
+        float4 A = lx_ld4(tp), E1 = lx_ld4(tp + 1), E2 = lx_ld4(tp + 2);
+        float t, u, v;
+        if (CNT) work->tris++;
+        if (!lx_mt(o, d, A, E1, E2, tmax, t, u, v)) continue;
+        uint32_t prim = lxm_f2u(A.w);
+        if (!ANY && !lx_better(t, cur_inst, prim, best)) continue;
+        if ((inst_flags & LX_INST_ALPHA) &&
+            !lx_any_hit_accept(S, lx_gptr(S.inst)[cur_inst], cur_inst, prim, u, v, wd, key, iteration))
+            continue;
+        best.t = t;
+        best.inst = cur_inst;
+        best.prim = prim;
+        best.u = u;
+        best.v = v;
+        if (ANY) return true;
+    }
+    return false;
+}
+
+/* Closest (ANY = false) or any (ANY = true) accepted hit along the ray.
+ * One loop iteration = (instance entry, then) one triangle leaf or one
+ * node test. */
 template <bool ANY, bool CNT = false, int W = 4>
 LXD LxHitD lx_trace(const LxDevScene& S, f3 wo, f3 wd, float tmax, uint32_t key, uint32_t iteration,
                     uint32_t alpha_mask, LxStack& st, uint32_t* overflow, LxWork* work = nullptr) {
@@ -274,68 +347,30 @@ LXD LxHitD lx_trace(const LxDevScene& S, f3 wo, f3 wd, float tmax, uint32_t key,
     st.sp = 0;
     uint32_t node = S.tlas_root;
     while (true) {
-        bool do_pop = false;
-        if (node == LX_SENTINEL) {
-            o = wo;
-            d = wd;
-            ix = iwd;
-            in_blas = false;
-            do_pop = true;
-        } else if (!(node & LX_LEAF_BIT)) {
+        bool more = true;
+        if ((node & LX_LEAF_BIT) && !in_blas) {
+            if (!lx_enter_instance<CNT>(S, node, cur_inst, inst_flags, in_blas, o, d, ix, wo, wd, alpha_mask, st,
+                                        overflow, work))
+                more = lx_pop_next(st, node, in_blas, o, d, ix, wo, wd, iwd);
+        }
+        if (!more) break;
+        if (node & LX_LEAF_BIT) {
+            /* a TLAS leaf here (popped after a skipped instance) waits for the next iteration */
+            if (!in_blas) continue;
+            if (lx_leaf<ANY, CNT>(S, node, cur_inst, inst_flags, o, d, wd, tmax, key, iteration, best, work))
+                return best;
+            if (!lx_pop_next(st, node, in_blas, o, d, ix, wo, wd, iwd)) break;
+        } else {
             float tcut = (best.inst == 0xffffffffu ? tmax : best.t) * LX_CULL_MARGIN;
             float tn[W];
             uint32_t cs[W];
             uint32_t m = lx_boxw<W>(nodes + node, o, ix, tcut, tn, cs);
             if (CNT) work->nodes++;
             if (m == 0u) {
-                do_pop = true;
+                if (!lx_pop_next(st, node, in_blas, o, d, ix, wo, wd, iwd)) break;
             } else {
                 node = lx_order_children<W>(tn, cs, m, st, overflow);
-                continue;
             }
-        } else if (!in_blas) {
-            cur_inst = node & LX_LEAF_FIRST_MASK;
-            const LX_GLOBAL LxInstance& I = lx_gptr(S.inst)[cur_inst];
-            o = xf_point((const float*)I.inv, wo);
-            d = xf_vector((const float*)I.inv, wd);
-            ix = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-            inst_flags = I.flags & alpha_mask;
-            in_blas = true;
-            if (st.sp >= LX_STACK + LX_SPILL) {
-                atomicOr(overflow, 1u);
-                do_pop = true;
-            } else {
-                st.push(LX_SENTINEL);
-                if (CNT) work->insts++;
-                node = I.blas_root;
-                continue;
-            }
-        } else {
-            uint32_t first = node & LX_LEAF_FIRST_MASK;
-            uint32_t cnt = ((node >> 28) & 7u) + 1u;
-            for (uint32_t k = 0; k < cnt; k++) {
-                const LX_GLOBAL lx_v4f* tp = (const LX_GLOBAL lx_v4f*)lx_gptr(S.tris) + 3u * (size_t)(first + k);
-                float4 A = lx_ld4(tp), E1 = lx_ld4(tp + 1), E2 = lx_ld4(tp + 2);
-                float t, u, v;
-                if (CNT) work->tris++;
-                if (!lx_mt(o, d, A, E1, E2, tmax, t, u, v)) continue;
-                uint32_t prim = lxm_f2u(A.w);
-                if (!ANY && !lx_better(t, cur_inst, prim, best)) continue;
-                if ((inst_flags & LX_INST_ALPHA) &&
-                    !lx_any_hit_accept(S, lx_gptr(S.inst)[cur_inst], cur_inst, prim, u, v, wd, key, iteration))
-                    continue;
-                best.t = t;
-                best.inst = cur_inst;
-                best.prim = prim;
-                best.u = u;
-                best.v = v;
-                if (ANY) return best;
-            }
-            do_pop = true;
-        }
-        if (do_pop) {
-            if (st.sp == 0) break;
-            node = st.pop();
         }
     }
     return best;
@@ -379,70 +414,32 @@ LXD LxHitD lx_trace_pair(const LxDevScene& S, bool has0, f3 o0, f3 d0, float tma
     begin();
     while (true) {
         LxWork* work = any ? wany : wclosest;
-        bool do_pop = false, found_any = false;
-        if (node == LX_SENTINEL) {
-            o = wo;
-            d = wd;
-            ix = iwd;
-            in_blas = false;
-            do_pop = true;
-        } else if (!(node & LX_LEAF_BIT)) {
-            float tcut = (best.inst == 0xffffffffu ? tmax : best.t) * LX_CULL_MARGIN;
-            float tn[W];
-            uint32_t cs[W];
-            uint32_t m = lx_boxw<W>(nodes + node, o, ix, tcut, tn, cs);
-            if (CNT) work->nodes++;
-            if (m == 0u) {
-                do_pop = true;
-            } else {
-                node = lx_order_children<W>(tn, cs, m, st, overflow);
-                continue;
-            }
-        } else if (!in_blas) {
-            cur_inst = node & LX_LEAF_FIRST_MASK;
-            const LX_GLOBAL LxInstance& I = lx_gptr(S.inst)[cur_inst];
-            o = xf_point((const float*)I.inv, wo);
-            d = xf_vector((const float*)I.inv, wd);
-            ix = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-            inst_flags = I.flags & alpha_mask;
-            in_blas = true;
-            if (st.sp >= LX_STACK + LX_SPILL) {
-                atomicOr(overflow, 1u);
-                do_pop = true;
-            } else {
-                st.push(LX_SENTINEL);
-                if (CNT) work->insts++;
-                node = I.blas_root;
-                continue;
-            }
-        } else {
-            uint32_t first = node & LX_LEAF_FIRST_MASK;
-            uint32_t cnt = ((node >> 28) & 7u) + 1u;
-            for (uint32_t k = 0; k < cnt; k++) {
-                const LX_GLOBAL lx_v4f* tp = (const LX_GLOBAL lx_v4f*)lx_gptr(S.tris) + 3u * (size_t)(first + k);
-                float4 A = lx_ld4(tp), E1 = lx_ld4(tp + 1), E2 = lx_ld4(tp + 2);
-                float t, u, v;
-                if (CNT) work->tris++;
-                if (!lx_mt(o, d, A, E1, E2, tmax, t, u, v)) continue;
-                uint32_t prim = lxm_f2u(A.w);
-                if (!any && !lx_better(t, cur_inst, prim, best)) continue;
-                if ((inst_flags & LX_INST_ALPHA) &&
-                    !lx_any_hit_accept(S, lx_gptr(S.inst)[cur_inst], cur_inst, prim, u, v, wd, key, iteration))
-                    continue;
-                best.t = t;
-                best.inst = cur_inst;
-                best.prim = prim;
-                best.u = u;
-                best.v = v;
-                if (any) {
-                    found_any = true;
-                    break;
-                }
-            }
-            do_pop = true;
+        bool more = true, found_any = false;
+        if ((node & LX_LEAF_BIT) && !in_blas) {
+            if (!lx_enter_instance<CNT>(S, node, cur_inst, inst_flags, in_blas, o, d, ix, wo, wd, alpha_mask, st,
+                                        overflow, work))
+                more = lx_pop_next(st, node, in_blas, o, d, ix, wo, wd, iwd);
         }
-        bool ray_done = found_any || (do_pop && st.sp == 0);
-        if (ray_done) {
+        if (more) {
+            if (node & LX_LEAF_BIT) {
+                /* a TLAS leaf here (popped after a skipped instance) waits for the next iteration */
+                if (in_blas)
+                found_any = any ? lx_leaf<true, CNT>(S, node, cur_inst, inst_flags, o, d, wd, tmax, key, iteration,
+                                                     best, work)
+                                : lx_leaf<false, CNT>(S, node, cur_inst, inst_flags, o, d, wd, tmax, key, iteration,
+                                                      best, work);
+                if (in_blas && !found_any) more = lx_pop_next(st, node, in_blas, o, d, ix, wo, wd, iwd);
+            } else {
+                float tcut = (best.inst == 0xffffffffu ? tmax : best.t) * LX_CULL_MARGIN;
+                float tn[W];
+                uint32_t cs[W];
+                uint32_t m = lx_boxw<W>(nodes + node, o, ix, tcut, tn, cs);
+                if (CNT) work->nodes++;
+                if (m == 0u) more = lx_pop_next(st, node, in_blas, o, d, ix, wo, wd, iwd);
+                else node = lx_order_children<W>(tn, cs, m, st, overflow);
+            }
+        }
+        if (found_any || !more) {
             if (!any) break;
             /* first ray finished: record it and start the second */
             occluded0 = best.inst != 0xffffffffu;
@@ -452,9 +449,7 @@ LXD LxHitD lx_trace_pair(const LxDevScene& S, bool has0, f3 o0, f3 d0, float tma
             wd = d1;
             tmax = tmax1;
             begin();
-            continue;
         }
-        if (do_pop) node = st.pop();
     }
     return best;
 }
