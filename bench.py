@@ -195,6 +195,32 @@ def main():
                "sample": f"{iters} full 1920x1080 frame(s) at 1 spp of the same workload (iterations "
                          f"{first}..{first + iters - 1}), oracle/lxo_oracle.cpp restatement with its "
                          f"reference-style binary SAH BVH, {threads} threads; BVH build excluded"}
+        # one thread on 200,000 random pixels of the same frame
+        pix = np.sort(rng.choice(n_pix, 200000, replace=False)).astype(np.uint32)
+        _, c1 = o.render(first, 1, pixels=pix, emis=True, threads=1)
+        cpu["single_thread_value"] = (c1["radiance_rays"] + c1["dl_shadow_rays"] + c1["env_shadow_rays"]) \
+            / (c1["wall_ms"] * 1e-3) / 1e6
+        cpu["single_thread_sample"] = "200,000 random pixels of one frame (iteration %d), 1 thread" % first
+        # the CPU-runnable Cornell workload, GPU and CPU on the same job
+        cd = scenes.cornell(256, 256)
+        oc = lxo.OracleScene(cd)
+        _, cc = oc.render(1, 16, dmis=True, threads=threads)
+        csc = lx.Scene(ctx, cd)
+        rg = lx.Render(csc, dmis=True)
+        rg.iterate(1, 2)
+        rg.sync()
+        rg.reset()
+        t0 = time.perf_counter()
+        rg.iterate(1, 16)
+        rg.sync()
+        tg = time.perf_counter() - t0
+        gs = rg.stats()
+        rg.close()
+        cpu["cornell_256_16spp_dmis"] = {
+            "cpu_mrays_per_s": (cc["radiance_rays"] + cc["dl_shadow_rays"]) / (cc["wall_ms"] * 1e-3) / 1e6,
+            "cpu_threads": threads, "cpu_ms": cc["wall_ms"],
+            "gpu_mrays_per_s": (gs["radiance_rays"] + gs["dl_shadow_rays"]) / tg / 1e6, "gpu_ms": tg * 1e3}
+        log(f"cpu baseline: one thread {cpu['single_thread_value']:.3f} Mrays/s; cornell {cpu['cornell_256_16spp_dmis']}")
 
     batched = served = None
     if args.batched:
