@@ -139,8 +139,12 @@ lx_status lx_autotune(lx_context* ctx, const lx_job* jobs, uint32_t n, const cha
     bool can_rebuild = true;
     for (lx_scene* s : scenes) can_rebuild &= s->refs.load() == 1;
     std::vector<lx_job_result> res(n);
-    auto measure = [&](const lx_tuning& t) {
+    auto set_tuning = [&](const lx_tuning& t) {
+        std::lock_guard<std::mutex> lk(ctx->mu);
         ctx->tuning = t;
+    };
+    auto measure = [&](const lx_tuning& t) {
+        set_tuning(t);
         double best = 1e300;
         for (int rep = 0; rep < 2; rep++) {
             for (auto& r : res) std::memset(&r, 0, sizeof(r));
@@ -160,7 +164,11 @@ lx_status lx_autotune(lx_context* ctx, const lx_job* jobs, uint32_t n, const cha
             lx_scene_accel(s, &o);
         }
     };
-    lx_tuning start = ctx->tuning;
+    lx_tuning start;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        start = ctx->tuning;
+    }
     lx_tuning cur = start;
     cur.bvh_width = jobs[0].scene->dev.bvh_width;
     try {
@@ -194,7 +202,7 @@ lx_status lx_autotune(lx_context* ctx, const lx_job* jobs, uint32_t n, const cha
                 }
             }
         }
-        ctx->tuning = cur;
+        set_tuning(cur);
         if (best_out) *best_out = cur;
         if (json_path) {
             std::ofstream f(json_path);
@@ -202,7 +210,7 @@ lx_status lx_autotune(lx_context* ctx, const lx_job* jobs, uint32_t n, const cha
             f << to_json(cur, cur_ms);
         }
     } catch (...) {
-        ctx->tuning = start;
+        set_tuning(start);
         throw;
     }
     LX_CATCH
@@ -215,6 +223,7 @@ lx_status lx_tuning_load(lx_context* ctx, const char* json_path) {
     LX_REQUIRE(f.good(), LX_ERR_IO, std::string("cannot read ") + json_path);
     std::stringstream ss;
     ss << f.rdbuf();
+    std::lock_guard<std::mutex> lk(ctx->mu);
     lx_tuning t = ctx->tuning;
     parse_tuning_json(ss.str(), t);
     check_tuning(t);

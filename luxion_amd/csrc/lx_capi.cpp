@@ -109,9 +109,12 @@ void lx_context_destroy(lx_context* c) { lx_context_release(c); }
 
 uint64_t lx_context_bytes_in_use(const lx_context* c) { return c ? c->in_use.load() : 0; }
 
+/* The tuning is read under the context mutex by every launch sequence (a
+ * server's request thread included), so reads and writes take it too. */
 lx_status lx_tuning_get(const lx_context* c, lx_tuning* out) {
     LX_TRY
     LX_REQUIRE(c && out, LX_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(const_cast<lx_context*>(c)->mu);
     *out = c->tuning;
     LX_CATCH
 }
@@ -120,6 +123,7 @@ lx_status lx_tuning_set(lx_context* c, const lx_tuning* t) {
     LX_TRY
     LX_REQUIRE(c && t, LX_ERR_INVALID, "null argument");
     lx_tuning_check(*t);
+    std::lock_guard<std::mutex> lk(c->mu);
     c->tuning = *t;
     LX_CATCH
 }

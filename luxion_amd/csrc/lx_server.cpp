@@ -277,8 +277,8 @@ lx_status lx_server_start(lx_context* ctx, const lx_server_opts* o, lx_server** 
         S->opts = opts;
         S->slots = (uint32_t)opts.max_jobs_in_flight;
         S->max_depth = opts.max_depth;
-        S->width = ctx->tuning.bvh_width == 8 ? 8 : 4;
         std::lock_guard<std::mutex> dl(ctx->mu);
+        S->width = ctx->tuning.bvh_width == 8 ? 8 : 4;
         ctx->set_device();
         uint64_t avail = ctx->budget > ctx->in_use.load() ? ctx->budget - ctx->in_use.load() : 0;
         uint64_t budget = opts.hbm_budget_bytes ? std::min<uint64_t>(opts.hbm_budget_bytes, avail) : avail;
