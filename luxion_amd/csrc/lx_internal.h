@@ -131,11 +131,13 @@ void lx_build_blas_host(const float* pos, const uint32_t* idx, uint32_t ntri, in
                         BvhBuildResult& out);
 /* TLAS over instance world boxes (one instance per leaf). */
 void lx_build_tlas_host(const std::vector<float>& boxes /* 6 per instance */, int width, BvhBuildResult& out);
-/* GPU builder (builder 0; builder 2 uses it for meshes of at least
- * LX_GPU_BUILD_MIN_TRIS triangles): linear BVH from Morton codes, binned SAH
- * over clusters at the top, SAH leaf test at the bottom; returns false if the
- * GPU path is unavailable for this input. */
-#define LX_GPU_BUILD_MIN_TRIS (1u << 20)
+/* GPU builder (builder 0; builder 2 uses it for BLASes of at least
+ * LX_GPU_BUILD_MIN_TRIS triangles, where the host build would take seconds):
+ * linear BVH from Morton codes, binned SAH over clusters at the top, SAH leaf
+ * test at the bottom; returns false if the GPU path is unavailable for this
+ * input.  Its trees are built faster and traversed slower than the host
+ * builder's. */
+#define LX_GPU_BUILD_MIN_TRIS (1u << 22)
 bool lx_build_blas_gpu(lx_context* ctx, const float* pos, const uint32_t* idx, uint32_t ntri, int leaf_size, int bins,
                        int width, BvhBuildResult& out);
 void lx_pad_box(float lo[3], float hi[3]);
@@ -176,7 +178,7 @@ struct lx_scene {
     float total_area = 0.0f;
 
     /* device */
-    DevBuf d_nodes, d_tris, d_inst, d_idx, d_pos, d_nrm, d_uv, d_mats, d_texdesc, d_texels, d_env, d_marg, d_cond,
+    DevBuf d_nodes, d_tris, d_inst, d_groups, d_idx, d_pos, d_nrm, d_uv, d_mats, d_texdesc, d_texels, d_env, d_marg, d_cond,
         d_emis, d_emis_prefix, d_tri_cdf, d_tri_cdf_off, d_scene;
     std::vector<DevBuf> d_tex;
     LxDevScene dev{};

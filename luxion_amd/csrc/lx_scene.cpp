@@ -14,6 +14,8 @@
 #include "lx_internal.h"
 
 #include <algorithm>
+#include <array>
+#include <map>
 #include <chrono>
 #include <cmath>
 #include <cfloat>
@@ -349,25 +351,60 @@ void lx_scene_accel(lx_scene* s, const lx_build_opts* o) {
         }
     }
 
-    /* BLAS per mesh */
+    /* traversal groups: instances with one bitwise-identical transform (so
+     * one world -> object transform of the ray, with the same bits for all of
+     * them) whose meshes have no other user share a BLAS over all their
+     * triangles, each triangle record tagged with its instance; a triangle
+     * test there gives the bits it gives under its own instance.  Any other
+     * instance is a group of its own over its mesh's BLAS. */
+    std::vector<uint32_t> mesh_users(s->meshes.size(), 0);
+    for (const lx_instance_desc& g : s->insts) mesh_users[g.mesh]++;
+    std::vector<std::vector<uint32_t>> groups;
+    {
+        std::map<std::array<uint32_t, 12>, size_t> by_xf;
+        for (size_t i = 0; i < s->insts.size(); i++) {
+            const lx_instance_desc& g = s->insts[i];
+            if (mesh_users[g.mesh] == 1) {
+                std::array<uint32_t, 12> key;
+                std::memcpy(key.data(), g.transform, 48);
+                auto it = by_xf.find(key);
+                if (it != by_xf.end()) {
+                    groups[it->second].push_back((uint32_t)i);
+                    continue;
+                }
+                by_xf[key] = groups.size();
+            }
+            groups.push_back({(uint32_t)i});
+        }
+    }
+
     std::vector<LxWideNode> nodes;
     std::vector<float4> tris;
-    std::vector<uint32_t> blas_root(s->meshes.size());
-    /* object-space boxes bounding each mesh: its BLAS's boxes two levels
-     * below the root (at most 64), or the mesh box; a rotated instance's
-     * world box is the union of these boxes' transformed corners, much
-     * tighter than the transformed box of the whole mesh */
-    std::vector<std::vector<std::array<float, 6>>> mesh_boxes(s->meshes.size());
     int max_depth = 0;
     double sah = 0.0;
-    for (size_t i = 0; i < s->meshes.size(); i++) {
-        const HostMesh& m = s->meshes[i];
+    /* Builds a BLAS into the shared node / triangle arrays; returns its root
+     * and, in `boxes`, object-space boxes bounding it: the boxes two levels
+     * below the root (at most 64), or the box of all its vertices.  A rotated
+     * instance's world box is the union of these boxes' transformed corners,
+     * much tighter than the transformed box of the whole mesh. */
+    auto add_blas = [&](const float* pos, size_t nv, const uint32_t* idx, uint32_t nt, const uint32_t* tag_off,
+                        const uint32_t* tag_inst, size_t ntag, std::vector<std::array<float, 6>>& boxes) {
         BvhBuildResult r;
-        uint32_t nt = (uint32_t)(m.idx.size() / 3);
         bool ok = false;
         bool gpu = opt.builder == 0 || (opt.builder == 2 && nt >= LX_GPU_BUILD_MIN_TRIS);
-        if (gpu) ok = lx_build_blas_gpu(ctx, m.pos.data(), m.idx.data(), nt, opt.leaf_size, opt.sah_bins, opt.bvh_width, r);
-        if (!ok) lx_build_blas_host(m.pos.data(), m.idx.data(), nt, opt.leaf_size, opt.sah_bins, opt.bvh_width, r);
+        if (gpu) ok = lx_build_blas_gpu(ctx, pos, idx, nt, opt.leaf_size, opt.sah_bins, opt.bvh_width, r);
+        if (!ok) lx_build_blas_host(pos, idx, nt, opt.leaf_size, opt.sah_bins, opt.bvh_width, r);
+        if (ntag) {
+            /* group-wide triangle index -> (instance, triangle of its mesh) */
+            for (size_t k = 0; k < r.tris.size() / 3; k++) {
+                uint32_t t;
+                std::memcpy(&t, &r.tris[3 * k].w, 4);
+                size_t m = (size_t)(std::upper_bound(tag_off, tag_off + ntag, t) - tag_off) - 1;
+                uint32_t local = t - tag_off[m];
+                std::memcpy(&r.tris[3 * k].w, &local, 4);
+                std::memcpy(&r.tris[3 * k + 1].w, &tag_inst[m], 4);
+            }
+        }
         uint32_t noff = (uint32_t)nodes.size(), toff = (uint32_t)(tris.size() / 3);
         LX_REQUIRE((size_t)toff + nt < (1u << 28) - 1, LX_ERR_INVALID, "scene too large for the leaf encoding");
         for (LxWideNode n : r.nodes) {
@@ -386,45 +423,81 @@ void lx_scene_accel(lx_scene* s, const lx_build_opts* o) {
                 uint32_t c = rn.child[k];
                 if (c == LX_CHILD_EMPTY) continue;
                 if (c & LX_LEAF_BIT) {
-                    mesh_boxes[i].push_back({rn.lo[k][0], rn.lo[k][1], rn.lo[k][2], rn.hi[k][0], rn.hi[k][1], rn.hi[k][2]});
+                    boxes.push_back({rn.lo[k][0], rn.lo[k][1], rn.lo[k][2], rn.hi[k][0], rn.hi[k][1], rn.hi[k][2]});
                     continue;
                 }
                 const LxWideNode& cn = r.nodes[c];
                 for (int q = 0; q < 8; q++)
                     if (cn.child[q] != LX_CHILD_EMPTY)
-                        mesh_boxes[i].push_back(
-                            {cn.lo[q][0], cn.lo[q][1], cn.lo[q][2], cn.hi[q][0], cn.hi[q][1], cn.hi[q][2]});
+                        boxes.push_back({cn.lo[q][0], cn.lo[q][1], cn.lo[q][2], cn.hi[q][0], cn.hi[q][1], cn.hi[q][2]});
             }
         }
         uint32_t rr = r.root;
         if (rr & LX_LEAF_BIT) rr = (rr & ~LX_LEAF_FIRST_MASK) | ((rr & LX_LEAF_FIRST_MASK) + toff);
         else rr += noff;
-        blas_root[i] = rr;
         tris.insert(tris.end(), r.tris.begin(), r.tris.end());
         max_depth = std::max(max_depth, r.depth);
         sah += r.sah;
-        /* object-space box of the mesh */
-        std::array<float, 6> b = {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX};
-        for (size_t v = 0; v < m.pos.size() / 3; v++)
-            for (int a = 0; a < 3; a++) {
-                b[a] = std::min(b[a], m.pos[3 * v + a]);
-                b[3 + a] = std::max(b[3 + a], m.pos[3 * v + a]);
-            }
-        if (mesh_boxes[i].empty()) mesh_boxes[i].push_back(b);
+        if (boxes.empty()) {
+            std::array<float, 6> b = {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX};
+            for (size_t v = 0; v < nv; v++)
+                for (int a = 0; a < 3; a++) {
+                    b[a] = std::min(b[a], pos[3 * v + a]);
+                    b[3 + a] = std::max(b[3 + a], pos[3 * v + a]);
+                }
+            boxes.push_back(b);
+        }
+        return rr;
+    };
+
+    /* BLAS per mesh of an untagged group (shared by all its instances) */
+    const uint32_t no_root = 0xffffffffu;
+    std::vector<uint32_t> mesh_root(s->meshes.size(), no_root);
+    std::vector<std::vector<std::array<float, 6>>> mesh_boxes(s->meshes.size());
+    for (const std::vector<uint32_t>& G : groups) {
+        if (G.size() != 1) continue;
+        uint32_t mi = s->insts[G[0]].mesh;
+        if (mesh_root[mi] != no_root) continue;
+        const HostMesh& m = s->meshes[mi];
+        mesh_root[mi] = add_blas(m.pos.data(), m.pos.size() / 3, m.idx.data(), (uint32_t)(m.idx.size() / 3), nullptr,
+                                 nullptr, 0, mesh_boxes[mi]);
     }
 
-    /* TLAS over conservative world boxes of the instances */
-    std::vector<float> wboxes(6 * s->insts.size());
-    std::vector<LxInstance> dinst(s->insts.size());
-    for (size_t i = 0; i < s->insts.size(); i++) {
-        const lx_instance_desc& g = s->insts[i];
+    /* group records, their BLASes and conservative world boxes */
+    std::vector<LxGroup> dgroups(groups.size());
+    std::vector<uint32_t> inst_root(s->insts.size());
+    std::vector<float> wboxes(6 * groups.size());
+    for (size_t gi = 0; gi < groups.size(); gi++) {
+        const std::vector<uint32_t>& G = groups[gi];
+        const lx_instance_desc& g0 = s->insts[G[0]];
+        std::vector<std::array<float, 6>> tagged_boxes;
+        const std::vector<std::array<float, 6>>* boxes;
+        uint32_t root;
+        if (G.size() == 1) {
+            root = mesh_root[g0.mesh];
+            boxes = &mesh_boxes[g0.mesh];
+        } else {
+            std::vector<float> gpos;
+            std::vector<uint32_t> gidx, off, who;
+            for (uint32_t ii : G) {
+                const HostMesh& m = s->meshes[s->insts[ii].mesh];
+                uint32_t base = (uint32_t)(gpos.size() / 3);
+                off.push_back((uint32_t)(gidx.size() / 3));
+                who.push_back(ii);
+                gpos.insert(gpos.end(), m.pos.begin(), m.pos.end());
+                for (uint32_t v : m.idx) gidx.push_back(v + base);
+            }
+            root = add_blas(gpos.data(), gpos.size() / 3, gidx.data(), (uint32_t)(gidx.size() / 3), off.data(),
+                            who.data(), off.size(), tagged_boxes);
+            boxes = &tagged_boxes;
+        }
         double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-        for (const std::array<float, 6>& b : mesh_boxes[g.mesh])
+        for (const std::array<float, 6>& b : *boxes)
             for (int c = 0; c < 8; c++) {
                 double p[3] = {(c & 1) ? b[3] : b[0], (c & 2) ? b[4] : b[1], (c & 4) ? b[5] : b[2]};
                 for (int r = 0; r < 3; r++) {
-                    double w = (double)g.transform[4 * r] * p[0] + (double)g.transform[4 * r + 1] * p[1] +
-                               (double)g.transform[4 * r + 2] * p[2] + (double)g.transform[4 * r + 3];
+                    double w = (double)g0.transform[4 * r] * p[0] + (double)g0.transform[4 * r + 1] * p[1] +
+                               (double)g0.transform[4 * r + 2] * p[2] + (double)g0.transform[4 * r + 3];
                     lo[r] = std::min(lo[r], w);
                     hi[r] = std::max(hi[r], w);
                 }
@@ -436,15 +509,31 @@ void lx_scene_accel(lx_scene* s, const lx_build_opts* o) {
         }
         lx_pad_box(flo, fhi);
         for (int r = 0; r < 3; r++) {
-            wboxes[6 * i + r] = flo[r];
-            wboxes[6 * i + 3 + r] = fhi[r];
+            wboxes[6 * gi + r] = flo[r];
+            wboxes[6 * gi + 3 + r] = fhi[r];
         }
+        LxGroup& D = dgroups[gi];
+        std::memcpy(D.inv, &s->inv[12 * (size_t)G[0]], 48);
+        D.blas_root = root;
+        D.flags = 0u;
+        for (uint32_t ii : G) {
+            if (s->mats[s->insts[ii].material].alpha_mode != LX_ALPHA_OPAQUE) D.flags |= LX_INST_ALPHA;
+            inst_root[ii] = root;
+        }
+        D.inst = G.size() == 1 ? G[0] : LX_GROUP_TAGGED;
+        D.pad = 0u;
+    }
+
+    /* per-instance records for hit attributes and emitter sampling */
+    std::vector<LxInstance> dinst(s->insts.size());
+    for (size_t i = 0; i < s->insts.size(); i++) {
+        const lx_instance_desc& g = s->insts[i];
         LxInstance& I = dinst[i];
         std::memcpy(I.inv, &s->inv[12 * i], 48);
         std::memcpy(I.m, g.transform, 48);
         I.mesh = g.mesh;
         I.material = g.material;
-        I.blas_root = blas_root[g.mesh];
+        I.blas_root = inst_root[i];
         I.flags = s->mats[g.material].alpha_mode != LX_ALPHA_OPAQUE ? LX_INST_ALPHA : 0u;
         I.idx_off = idx_off[g.mesh];
         I.vtx_off = vtx_off[g.mesh];
@@ -453,11 +542,12 @@ void lx_scene_accel(lx_scene* s, const lx_build_opts* o) {
     }
     /* world box for the ray-sort keys */
     float wlo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, whi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-    for (size_t i = 0; i < s->insts.size(); i++)
+    for (size_t i = 0; i < groups.size(); i++)
         for (int r = 0; r < 3; r++) {
             wlo[r] = std::min(wlo[r], wboxes[6 * i + r]);
             whi[r] = std::max(whi[r], wboxes[6 * i + 3 + r]);
         }
+    /* TLAS over the groups */
     BvhBuildResult tl;
     lx_build_tlas_host(wboxes, opt.bvh_width, tl);
     uint32_t noff = (uint32_t)nodes.size();
@@ -481,6 +571,7 @@ void lx_scene_accel(lx_scene* s, const lx_build_opts* o) {
     }
     s->d_tris.upload(ctx, tris.data(), tris.size() * sizeof(float4));
     s->d_inst.upload(ctx, dinst.data(), dinst.size() * sizeof(LxInstance));
+    s->d_groups.upload(ctx, dgroups.data(), dgroups.size() * sizeof(LxGroup));
     s->d_idx.upload(ctx, all_idx.data(), all_idx.size() * 4);
     s->d_pos.upload(ctx, all_pos.data(), all_pos.size() * 4);
     if (any_n) s->d_nrm.upload(ctx, all_nrm.data(), all_nrm.size() * 4);
@@ -497,6 +588,7 @@ void lx_scene_accel(lx_scene* s, const lx_build_opts* o) {
     D.nodes = s->d_nodes.ptr;
     D.tris = s->d_tris.as<float4>();
     D.inst = s->d_inst.as<LxInstance>();
+    D.groups = s->d_groups.as<LxGroup>();
     D.idx = s->d_idx.as<uint32_t>();
     D.pos = s->d_pos.as<float>();
     D.nrm = s->d_nrm.as<float>();
@@ -511,14 +603,15 @@ void lx_scene_accel(lx_scene* s, const lx_build_opts* o) {
     D.tri_cdf_off = s->d_tri_cdf_off.as<uint32_t>();
     D.tlas_root = tlas_root;
     D.num_inst = (uint32_t)s->insts.size();
+    D.num_groups = (uint32_t)groups.size();
     D.num_emissive = (uint32_t)s->emissive.size();
     D.total_area = s->total_area;
     D.has_env = s->has_env ? 1 : 0;
     D.bvh_width = opt.bvh_width;
     double diag2 = 0.0;
     for (int r = 0; r < 3; r++) {
-        double e = s->insts.empty() ? 0.0 : (double)whi[r] - (double)wlo[r];
-        D.box_lo[r] = s->insts.empty() ? 0.0f : wlo[r];
+        double e = groups.empty() ? 0.0 : (double)whi[r] - (double)wlo[r];
+        D.box_lo[r] = groups.empty() ? 0.0f : wlo[r];
         D.box_inv[r] = e > 0.0 ? (float)(1.0 / e) : 0.0f;
         diag2 += e * e;
     }
@@ -535,7 +628,7 @@ void lx_scene_accel(lx_scene* s, const lx_build_opts* o) {
     s->info.build_ms = now_ms() - t0;
     s->info.sah_cost = sah;
     uint64_t bytes = 0;
-    DevBuf* bufs[] = {&s->d_nodes, &s->d_tris, &s->d_inst,  &s->d_idx,  &s->d_pos,         &s->d_nrm,
+    DevBuf* bufs[] = {&s->d_nodes, &s->d_tris, &s->d_inst, &s->d_groups, &s->d_idx,  &s->d_pos,         &s->d_nrm,
                       &s->d_uv,    &s->d_mats, &s->d_texdesc, &s->d_env, &s->d_marg, &s->d_cond,
                       &s->d_emis,  &s->d_emis_prefix, &s->d_tri_cdf, &s->d_tri_cdf_off, &s->d_scene};
     for (DevBuf* b : bufs) bytes += b->bytes;

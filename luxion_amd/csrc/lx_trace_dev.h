@@ -274,9 +274,11 @@ LXD bool lx_pop_next(LxStack& st, uint32_t& node, bool& in_blas, f3& o, f3& d, f
     return true;
 }
 
-/* Enters the instance of TLAS leaf `node`: ray to object space, sentinel
- * pushed, node = the BLAS root (tested in the same iteration).  Returns
- * false if the stack is full (the instance is skipped and flagged). */
+/* Enters the group of TLAS leaf `node` (one instance, or instances sharing
+ * a transform and a tagged BLAS): ray to object space, sentinel pushed, node
+ * = the BLAS root (tested in the same iteration); cur_inst = the instance,
+ * or LX_GROUP_TAGGED when each triangle names its own.  Returns false if the
+ * stack is full (the group is skipped and flagged). */
 template <bool CNT>
 LXD bool lx_enter_instance(const LxDevScene& S, uint32_t& node, uint32_t& cur_inst, uint32_t& inst_flags,
                            bool& in_blas, f3& o, f3& d, f3& ix, f3 wo, f3 wd, uint32_t alpha_mask, LxStack& st,
@@ -285,21 +287,21 @@ LXD bool lx_enter_instance(const LxDevScene& S, uint32_t& node, uint32_t& cur_in
         atomicOr(overflow, 1u);
         return false;
     }
-    cur_inst = node & LX_LEAF_FIRST_MASK;
-    const LX_GLOBAL LxInstance& I = lx_gptr(S.inst)[cur_inst];
-    o = xf_point((const float*)I.inv, wo);
-    d = xf_vector((const float*)I.inv, wd);
+    const LX_GLOBAL LxGroup& G = lx_gptr(S.groups)[node & LX_LEAF_FIRST_MASK];
+    o = xf_point((const float*)G.inv, wo);
+    d = xf_vector((const float*)G.inv, wd);
     ix = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    inst_flags = I.flags & alpha_mask;
+    inst_flags = G.flags & alpha_mask;
+    cur_inst = G.inst;
     in_blas = true;
     st.push(LX_SENTINEL);
     if (CNT) work->insts++;
-    node = I.blas_root;
+    node = G.blas_root;
     return true;
 }
 
-/* Triangle leaf of the current instance; returns true if an any-hit ray
- * found its hit. */
+/* Triangle leaf of the current group; returns true if an any-hit ray found
+ * its hit. */
 template <bool ANY, bool CNT>
 LXD bool lx_leaf(const LxDevScene& S, uint32_t node, uint32_t cur_inst, uint32_t inst_flags, f3 o, f3 d, f3 wd,
                  float tmax, uint32_t key, uint32_t iteration, LxHitD& best, LxWork* work) {
@@ -312,12 +314,14 @@ LXD bool lx_leaf(const LxDevScene& S, uint32_t node, uint32_t cur_inst, uint32_t
         if (CNT) work->tris++;
         if (!lx_mt(o, d, A, E1, E2, tmax, t, u, v)) continue;
         uint32_t prim = lxm_f2u(A.w);
-        if (!ANY && !lx_better(t, cur_inst, prim, best)) continue;
-        if ((inst_flags & LX_INST_ALPHA) &&
-            !lx_any_hit_accept(S, lx_gptr(S.inst)[cur_inst], cur_inst, prim, u, v, wd, key, iteration))
-            continue;
+        uint32_t inst = cur_inst == LX_GROUP_TAGGED ? lxm_f2u(E1.w) : cur_inst;
+        if (!ANY && !lx_better(t, inst, prim, best)) continue;
+        if (inst_flags & LX_INST_ALPHA) {
+            const LX_GLOBAL LxInstance& I = lx_gptr(S.inst)[inst];
+            if ((I.flags & LX_INST_ALPHA) && !lx_any_hit_accept(S, I, inst, prim, u, v, wd, key, iteration)) continue;
+        }
         best.t = t;
-        best.inst = cur_inst;
+        best.inst = inst;
         best.prim = prim;
         best.u = u;
         best.v = v;

@@ -64,6 +64,20 @@ struct alignas(16) LxInstance {
 };
 #define LX_INST_ALPHA 1u /* material needs the any-hit alpha test */
 
+/* A TLAS leaf: instances that the traversal enters together.  Instances with
+ * one bitwise-identical transform whose meshes have no other user share one
+ * BLAS over all their triangles (a tagged group: each triangle record carries
+ * its instance in e1.w); any other instance is a group of its own over its
+ * mesh's BLAS. */
+struct alignas(16) LxGroup {
+    float inv[12];      /* world -> object of every member */
+    uint32_t blas_root;
+    uint32_t flags;     /* LX_INST_ALPHA if any member needs the any-hit alpha test */
+    uint32_t inst;      /* the member instance, or LX_GROUP_TAGGED */
+    uint32_t pad;
+};
+#define LX_GROUP_TAGGED 0xffffffffu
+
 struct LxTexDesc {
     const void* data;
     int32_t w, h, fmt, pad;
@@ -73,6 +87,7 @@ struct LxDevScene {
     const void* nodes; /* LxNode4* or LxNode8* */
     const float4* tris;
     const LxInstance* inst;
+    const LxGroup* groups; /* TLAS leaves */
     const uint32_t* idx;
     const float* pos;
     const float* nrm;
@@ -95,7 +110,7 @@ struct LxDevScene {
     /* ray sort keys: world box of all instances and a reach along the ray */
     float box_lo[3], box_inv[3];
     float sort_reach;
-    int32_t pad;
+    uint32_t num_groups;
 };
 
 /* One render job (single renders are a batch of one). */
