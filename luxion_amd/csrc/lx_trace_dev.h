@@ -190,10 +190,10 @@ LXD uint32_t lx_boxw(const typename LxNodeT<W>::type* __restrict__ nd, f3 o, f3 
 template <int W>
 LXD uint32_t lx_order_children(const float* tn, const uint32_t* cs, uint32_t m, LxStack& st, uint32_t* overflow) {
     if (W == 4) {
-        float t0 = (m & 1u) ? tn[0] : __builtin_inff(), t1 = (m & 2u) ? tn[1] : __builtin_inff();
-        float t2 = (m & 4u) ? tn[2] : __builtin_inff(), t3 = (m & 8u) ? tn[3] : __builtin_inff();
+        const float inf = __builtin_inff();
+        float t0 = (m & 1u) ? tn[0] : inf, t1 = (m & 2u) ? tn[1] : inf;
+        float t2 = (m & 4u) ? tn[2] : inf, t3 = (m & 8u) ? tn[3] : inf;
         uint32_t c0 = cs[0], c1 = cs[1], c2 = cs[2], c3 = cs[3];
-        bool h0 = m & 1u, h1 = (m >> 1) & 1u, h2 = (m >> 2) & 1u, h3 = (m >> 3) & 1u;
 #define LX_CX(a, b)                                                                                                    \
     {                                                                                                                  \
         bool sw = t##b < t##a;                                                                                         \
@@ -203,20 +203,28 @@ LXD uint32_t lx_order_children(const float* tn, const uint32_t* cs, uint32_t m, 
         uint32_t cc = sw ? c##b : c##a;                                                                                \
         c##b = sw ? c##a : c##b;                                                                                       \
         c##a = cc;                                                                                                     \
-        bool hh = sw ? h##b : h##a;                                                                                    \
-        h##b = sw ? h##a : h##b;                                                                                       \
-        h##a = hh;                                                                                                     \
     }
         LX_CX(0, 1) LX_CX(2, 3) LX_CX(0, 2) LX_CX(1, 3) LX_CX(1, 2)
 #undef LX_CX
+        /* hit children have finite entry distances and sort first */
+        bool h1 = t1 != inf, h2 = t2 != inf, h3 = t3 != inf;
         int n = (int)h1 + (int)h2 + (int)h3;
-        if (st.sp + n > LX_STACK + LX_SPILL) {
+        int sp = st.sp;
+        if (sp + n > LX_STACK + LX_SPILL) {
             atomicOr(overflow, 1u);
             return c0;
         }
-        if (h3) st.push(c3);
-        if (h2) st.push(c2);
-        if (h1) st.push(c1);
+        if (sp + 3 <= LX_STACK) {
+            /* the n hit children far to near at sp.., the others above the new top */
+            st.lds[(sp + (h1 ? n - 1 : 0)) * st.stride] = c1;
+            st.lds[(sp + (h2 ? n - 2 : 1)) * st.stride] = c2;
+            st.lds[(sp + (h3 ? 0 : 2)) * st.stride] = c3;
+            st.sp = sp + n;
+        } else {
+            if (h3) st.push(c3);
+            if (h2) st.push(c2);
+            if (h1) st.push(c1);
+        }
         return c0;
     } else {
         float bt = __builtin_inff();
