@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define LX_ABI_VERSION 2
+#define LX_ABI_VERSION 3
 
 typedef enum lx_status {
     LX_OK = 0,
@@ -411,6 +411,14 @@ typedef struct lx_tuning {
     int32_t tile_band;         /* squares follow each other in bands of this many
                                   square rows, column by column (0 = row by row;
                                   0..64; same bits) */
+    int32_t overlap_rounds;    /* 1 (default): lx_render_iterate of two or more
+                                  samples without timing runs consecutive rounds on
+                                  two streams, so one round's last long paths
+                                  overlap the next round's start; each round's
+                                  colours go to a buffer of their own, added to
+                                  the accumulation in round order (same bits).
+                                  0: one round after another on the render's
+                                  stream (captured in a hipGraph with use_graph) */
 } lx_tuning;
 
 /* Coherence stages (the reference's RAY_SORTING and MATERIAL_SORTING,

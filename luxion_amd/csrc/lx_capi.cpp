@@ -73,6 +73,7 @@ lx_status lx_context_create(int device, uint64_t budget, lx_context** out) {
         c->tuning.sort_mode = LX_SORT_NONE;
         c->tuning.pixel_tile = 8;
         c->tuning.tile_band = 0;
+        c->tuning.overlap_rounds = 1;
     } catch (...) {
         if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
@@ -374,6 +375,97 @@ static void enqueue(lx_render* r, hipEvent_t* ev) {
                      r->scene->d_scene.as<LxDevScene>(), ev, r->counting);
 }
 
+/* Buffers and streams of the overlapped form, made on first use. */
+static void overlap_setup(lx_render* r) {
+    if (r->overlap_ready) return;
+    lx_context* ctx = r->ctx;
+    auto& O = r->odd;
+    O.wb.alloc(ctx, r->npix, r->opts.dmis != 0, r->opts.emis != 0, scene_has_normal_maps(r->scene));
+    O.jobs.alloc(ctx, sizeof(LxJob));
+    O.job_spp.alloc(ctx, 16);
+    O.counters.alloc(ctx, sizeof(LxCounters));
+    LX_HIP(lx_launch_fill_u32(O.counters.as<uint32_t>(), sizeof(LxCounters) / 4, 0u, ctx->stream));
+    O.sched.alloc(ctx, sizeof(LxSched));
+    O.spill.alloc(ctx, r->spill.bytes);
+    for (int p = 0; p < 2; p++) {
+        r->color[p].alloc(ctx, (size_t)r->npix * 12);
+        if (!r->pstream[p]) LX_HIP(hipStreamCreateWithFlags(&r->pstream[p], hipStreamNonBlocking));
+        if (!r->ev_done[p]) LX_HIP(hipEventCreateWithFlags(&r->ev_done[p], hipEventDisableTiming));
+        if (!r->ev_free[p]) LX_HIP(hipEventCreateWithFlags(&r->ev_free[p], hipEventDisableTiming));
+    }
+    if (!r->ev_up) LX_HIP(hipEventCreateWithFlags(&r->ev_up, hipEventDisableTiming));
+    r->overlap_ready = true;
+}
+
+/* n >= 2 rounds, round k on stream k % 2 with that parity's buffers; the job
+ * copy of parity p starts at iteration first_iter + p and steps by 2.  Round
+ * k's colours are added to the accumulation on the context's stream after
+ * round k - 1's, and round k + 2 waits for that add before it reuses the
+ * colour buffer. */
+static void iterate_overlapped(lx_render* r, uint32_t first_iter, uint32_t n) {
+    overlap_setup(r);
+    lx_context* ctx = r->ctx;
+    hipStream_t ms = ctx->stream;
+    LxJob* djob[2] = {r->jobs.as<LxJob>(), r->odd.jobs.as<LxJob>()};
+    int32_t* dspp[2] = {r->job_spp.as<int32_t>(), r->odd.job_spp.as<int32_t>()};
+    LxCounters* dcnt[2] = {r->counters.as<LxCounters>(), r->odd.counters.as<LxCounters>()};
+    LxSched* dsch[2] = {r->sched.as<LxSched>(), r->odd.sched.as<LxSched>()};
+    uint32_t* dspill[2] = {r->spill.as<uint32_t>(), r->odd.spill.as<uint32_t>()};
+    WaveBufs* wb[2] = {&r->wb, &r->odd.wb};
+    for (int p = 0; p < 2; p++) {
+        /* sources of async copies: members of the render, untouched until the next call */
+        LxJob& J = r->ov_job[p];
+        J = r->job;
+        J.iter = (int32_t)first_iter - 2 + p;
+        J.spp_done = 0;
+        J.active = 0;
+        J.flags |= LX_JF_ITER2;
+        r->ov_spp[p] = (int32_t)((n + 1 - (uint32_t)p) / 2); /* rounds of this parity */
+        LX_HIP(hipMemcpyAsync(djob[p], &J, sizeof(LxJob), hipMemcpyHostToDevice, ms));
+        LX_HIP(hipMemcpyAsync(dspp[p], &r->ov_spp[p], 4, hipMemcpyHostToDevice, ms));
+    }
+    LX_HIP(hipEventRecord(r->ev_up, ms));
+    for (int p = 0; p < 2; p++) LX_HIP(hipStreamWaitEvent(r->pstream[p], r->ev_up, 0));
+    const size_t nf = (size_t)r->npix * 3;
+    for (uint32_t k = 0; k < n; k++) {
+        int p = (int)(k & 1u);
+        hipStream_t s = r->pstream[p];
+        if (k >= 2) LX_HIP(hipStreamWaitEvent(s, r->ev_free[p], 0));
+        LX_HIP(lx_launch_fill_u32(r->color[p].as<uint32_t>(), nf, 0x80000000u, s));
+        lx_enqueue_round(ctx, djob[p], dspp[p], 1, dcnt[p], dsch[p], r->npix, *wb[p], r->color[p].as<float>(),
+                         dspill[p], r->depth, r->trace_grid, r->shade_grid, r->cam_grid, r->scene->dev.bvh_width,
+                         r->scene->d_scene.as<LxDevScene>(), nullptr, false, s);
+        LX_HIP(hipEventRecord(r->ev_done[p], s));
+        LX_HIP(hipStreamWaitEvent(ms, r->ev_done[p], 0));
+        LX_HIP(lx_launch_add_f32(r->accum.as<float>(), r->color[p].as<float>(), nf, ms));
+        LX_HIP(hipEventRecord(r->ev_free[p], ms));
+    }
+}
+
+/* Both parities' counters (the odd set's only once it exists). */
+static void read_counters(lx_render* r, LxCounters& c) {
+    LX_HIP(hipMemcpy(&c, r->counters.ptr, sizeof(c), hipMemcpyDeviceToHost));
+    if (!r->overlap_ready) return;
+    LxCounters o;
+    LX_HIP(hipMemcpy(&o, r->odd.counters.ptr, sizeof(o), hipMemcpyDeviceToHost));
+    c.rad_rays += o.rad_rays;
+    c.dl_rays += o.dl_rays;
+    c.env_rays += o.env_rays;
+    c.bounces += o.bounces;
+    c.tail_bounces += o.tail_bounces;
+    for (int w = 0; w < 2; w++) {
+        c.work[w].closest_rays += o.work[w].closest_rays;
+        c.work[w].any_rays += o.work[w].any_rays;
+        c.work[w].closest_nodes += o.work[w].closest_nodes;
+        c.work[w].closest_tris += o.work[w].closest_tris;
+        c.work[w].closest_insts += o.work[w].closest_insts;
+        c.work[w].any_nodes += o.work[w].any_nodes;
+        c.work[w].any_tris += o.work[w].any_tris;
+        c.work[w].any_insts += o.work[w].any_insts;
+    }
+    c.stack_overflow |= o.stack_overflow;
+}
+
 lx_status lx_render_iterate(lx_render* r, uint32_t first_iter, uint32_t n) {
     LX_TRY
     LX_REQUIRE(r, LX_ERR_INVALID, "null render");
@@ -382,6 +474,11 @@ lx_status lx_render_iterate(lx_render* r, uint32_t first_iter, uint32_t n) {
     lx_context* ctx = r->ctx;
     std::lock_guard<std::mutex> lk(ctx->mu);
     ctx->set_device();
+    if (n >= 2 && ctx->tuning.overlap_rounds && !r->timing && !r->counting) {
+        iterate_overlapped(r, first_iter, n);
+        r->spp += n;
+        return LX_OK;
+    }
     upload_job(r, first_iter, n);
     if (r->timing || r->counting) {
         /* eager launches; with timing, an event before and after each
@@ -460,9 +557,12 @@ lx_status lx_render_sync(lx_render* r) {
     LX_REQUIRE(r, LX_ERR_INVALID, "null render");
     r->ctx->set_device();
     LX_HIP(hipStreamSynchronize(r->ctx->stream));
-    uint32_t ov = 0;
+    uint32_t ov = 0, ov1 = 0;
     LX_HIP(hipMemcpy(&ov, (uint8_t*)r->counters.ptr + offsetof(LxCounters, stack_overflow), 4, hipMemcpyDeviceToHost));
-    LX_REQUIRE(ov == 0, LX_ERR_STATE, "traversal stack overflow (BVH deeper than the supported stack)");
+    if (r->overlap_ready)
+        LX_HIP(hipMemcpy(&ov1, (uint8_t*)r->odd.counters.ptr + offsetof(LxCounters, stack_overflow), 4,
+                         hipMemcpyDeviceToHost));
+    LX_REQUIRE((ov | ov1) == 0, LX_ERR_STATE, "traversal stack overflow (BVH deeper than the supported stack)");
     LX_CATCH
 }
 
@@ -499,6 +599,8 @@ lx_status lx_render_reset(lx_render* r) {
     r->ctx->set_device();
     LX_HIP(lx_launch_fill_u32(r->accum.as<uint32_t>(), (size_t)r->npix * 3, 0u, r->ctx->stream));
     LX_HIP(lx_launch_fill_u32(r->counters.as<uint32_t>(), sizeof(LxCounters) / 4, 0u, r->ctx->stream));
+    if (r->overlap_ready)
+        LX_HIP(lx_launch_fill_u32(r->odd.counters.as<uint32_t>(), sizeof(LxCounters) / 4, 0u, r->ctx->stream));
     LX_HIP(hipStreamSynchronize(r->ctx->stream));
     r->spp = 0;
     r->trace_ms = r->shade_ms = r->other_ms = r->tail_ms = 0;
@@ -520,7 +622,7 @@ lx_status lx_render_stats_get(lx_render* r, lx_render_stats* out) {
     lx_status st = lx_render_sync(r);
     if (st != LX_OK) return st;
     LxCounters c;
-    LX_HIP(hipMemcpy(&c, r->counters.ptr, sizeof(c), hipMemcpyDeviceToHost));
+    read_counters(r, c);
     std::memset(out, 0, sizeof(*out));
     out->radiance_rays = c.rad_rays;
     out->dl_shadow_rays = c.dl_rays;
@@ -559,6 +661,12 @@ void lx_render_destroy(lx_render* r) {
     (void)hipStreamSynchronize(r->ctx->stream);
     if (r->graph) (void)hipGraphExecDestroy(r->graph);
     if (r->graph_src) (void)hipGraphDestroy(r->graph_src);
+    for (int p = 0; p < 2; p++) {
+        if (r->pstream[p]) (void)hipStreamDestroy(r->pstream[p]);
+        if (r->ev_done[p]) (void)hipEventDestroy(r->ev_done[p]);
+        if (r->ev_free[p]) (void)hipEventDestroy(r->ev_free[p]);
+    }
+    if (r->ev_up) (void)hipEventDestroy(r->ev_up);
     lx_scene* s = r->scene;
     delete r;
     lx_scene_release(s);

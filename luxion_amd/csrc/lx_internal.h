@@ -224,6 +224,21 @@ struct lx_render {
     hipGraph_t graph_src = nullptr;
     lx_tuning graph_tuning{}; /* the tuning the graph was captured with */
 
+    /* overlapped rounds (tuning.overlap_rounds): rounds alternate between two
+     * streams; round parity 0 uses the buffers above, parity 1 `odd`; each
+     * parity's colours go to color[p] (filled with -0.0, so the one add per
+     * pixel copies the path's colour bits), added to accum in round order on
+     * the context's stream */
+    struct OddSet {
+        WaveBufs wb;
+        DevBuf jobs, job_spp, counters, sched, spill;
+    } odd;
+    LxJob ov_job[2]{};
+    int32_t ov_spp[2] = {0, 0};
+    DevBuf color[2];
+    hipStream_t pstream[2] = {nullptr, nullptr};
+    hipEvent_t ev_up = nullptr, ev_done[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
+    bool overlap_ready = false;
 };
 
 /* Records one round (begin, camera, bounces, end) on the stream. */
@@ -235,7 +250,7 @@ void lx_enqueue_round(lx_context* ctx, LxJob* d_jobs, const int32_t* d_spp, uint
                       hipEvent_t* ev /* 2*max_depth+1 slots or NULL: [2b], [2b+1] bracket the trace and
                                         shade of bounce b < wavefront_bounces, [2w], [2w+1] the tail kernel,
                                         [2*max_depth] the round's end; other slots are not recorded */,
-                      bool count = false);
+                      bool count = false, hipStream_t stream = nullptr /* NULL: the context's stream */);
 uint32_t lx_spill_words(uint32_t trace_grid);
 /* bounces launched one kernel each before the tail kernel takes over
  * (tuning.wavefront_bounces; negative or above max_depth = all of them,

@@ -63,5 +63,6 @@ hipError_t lx_launch_tonemap(const float* accum, uint32_t npix, float spp, int m
 hipError_t lx_launch_math_eval(int fn, const float* x, const float* y, float* out, uint32_t n, hipStream_t s);
 hipError_t lx_launch_copy16(const void* src, void* dst, size_t n16, uint32_t grid, hipStream_t s);
 hipError_t lx_launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
+hipError_t lx_launch_add_f32(float* dst, const float* src, size_t n, hipStream_t s);
 
 #endif

@@ -52,7 +52,7 @@ __global__ void k_begin_round(LxJob* __restrict__ jobs, const int32_t* __restric
             if (want && na < LX_SCHED_MAX && off + J.num_pix <= path_cap) {
                 J.active = 1;
                 J.path_off = off;
-                J.iter = J.iter + 1;
+                J.iter = J.iter + ((J.flags & LX_JF_ITER2) ? 2 : 1);
                 sched->idx[na] = j;
                 sched->off[na] = off;
                 na++;
@@ -718,6 +718,12 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
+/* dst += src, element by element (one round's colours into the accumulation) */
+__global__ void k_add_f32(float* __restrict__ dst, const float* __restrict__ src, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = dst[i] + src[i];
+}
+
 /* ------------------------------------------------------------ launchers */
 hipError_t lx_launch_begin_round(LxJob* jobs, const int32_t* job_spp, uint32_t njobs, LxCounters* C, LxSched* sched,
                                  uint32_t cap, hipStream_t s) {
@@ -834,5 +840,12 @@ hipError_t lx_launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) 
     if (blocks > 65535) blocks = 65535;
     if (blocks == 0) blocks = 1;
     hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)blocks), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+hipError_t lx_launch_add_f32(float* dst, const float* src, size_t n, hipStream_t s) {
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 65535) blocks = 65535;
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(k_add_f32, dim3((uint32_t)blocks), dim3(256), 0, s, dst, src, n);
     return hipGetLastError();
 }

@@ -93,8 +93,8 @@ void WaveBufs::alloc(lx_context* ctx, uint32_t c, bool dmis, bool emis, bool tan
 void lx_enqueue_round(lx_context* ctx, LxJob* d_jobs, const int32_t* d_spp, uint32_t njobs, LxCounters* d_cnt,
                       LxSched* d_sched, uint32_t path_cap, WaveBufs& wb, float* d_accum, uint32_t* d_spill,
                       int max_depth, uint32_t trace_grid, uint32_t shade_grid, uint32_t cam_grid, int width,
-                      const LxDevScene* uscene, hipEvent_t* ev, bool count) {
-    hipStream_t s = ctx->stream;
+                      const LxDevScene* uscene, hipEvent_t* ev, bool count, hipStream_t stream) {
+    hipStream_t s = stream ? stream : ctx->stream;
     int wf = lx_wavefront_bounces(ctx, max_depth);
     LX_HIP(lx_launch_begin_round(d_jobs, d_spp, njobs, d_cnt, d_sched, path_cap, s));
     LX_HIP(lx_launch_camera(d_jobs, d_sched, d_cnt, wb.A, cam_grid, s, (uint32_t)ctx->tuning.pixel_tile,

@@ -131,6 +131,7 @@ struct LxJob {
 };
 #define LX_JF_DEBUG_MASK 7u /* LX_DEBUG_* view, 0 = off */
 #define LX_JF_LEGACY 8u     /* LX_SHADING_LEGACY */
+#define LX_JF_ITER2 16u     /* rounds of this job alternate with another copy of it: iter += 2 */
 
 /* Path state (struct of arrays, capacity N). */
 struct LxPathBuf {

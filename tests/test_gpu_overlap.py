@@ -1,0 +1,92 @@
+"""Overlapped rounds (tuning overlap_rounds = 1): consecutive rounds of one
+render run on two streams, each round's colours in a buffer of its own,
+added to the accumulation in round order.  The image bits, ray counts and
+sample counts equal one-round-at-a-time rendering, for odd and even sample
+counts, across calls, resets and mode switches, and against the oracle."""
+import numpy as np
+import pytest
+
+import luxion_amd as lx
+from luxion_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture
+def tuned(ctx):
+    saved = ctx.tuning()
+    yield ctx
+    ctx.set_tuning(**saved)
+
+
+def _render(ctx, sc, kw, calls, overlap, use_graph=1, wf=0):
+    ctx.set_tuning(overlap_rounds=overlap, use_graph=use_graph, wavefront_bounces=wf)
+    r = lx.Render(sc, **kw)
+    for first, n in calls:
+        r.iterate(first, n)
+    img, spp = r.accum()
+    st = r.stats()
+    r.close()
+    return img, spp, {k: st[k] for k in ("radiance_rays", "dl_shadow_rays", "env_shadow_rays", "path_bounces")}
+
+
+@pytest.mark.parametrize("calls", [[(1, 2)], [(1, 5)], [(1, 3), (4, 4)], [(1, 1), (2, 2), (4, 3)]])
+def test_overlapped_equals_sequential(tuned, calls):
+    data = scenes.material_zoo(96, 64, level=3, env_size=(128, 64))
+    sc = lx.Scene(tuned, data)
+    for kw in (dict(emis=True), dict(dmis=True)):
+        a = _render(tuned, sc, kw, calls, overlap=1)
+        b = _render(tuned, sc, kw, calls, overlap=0)
+        assert np.array_equal(a[0], b[0]), (calls, kw)
+        assert a[1] == b[1] == sum(n for _, n in calls)
+        assert a[2] == b[2], (calls, kw)
+
+
+def test_overlapped_with_per_bounce_launches_and_eager(tuned):
+    data = scenes.cornell(80, 72)
+    sc = lx.Scene(tuned, data)
+    want = _render(tuned, sc, dict(dmis=True), [(1, 6)], overlap=0, use_graph=0, wf=-1)
+    for wf in (-1, 2, 0):
+        got = _render(tuned, sc, dict(dmis=True), [(1, 6)], overlap=1, wf=wf)
+        assert np.array_equal(got[0], want[0]), wf
+        assert got[2] == want[2], wf
+
+
+def test_overlapped_renders_oracle_bits(tuned, oracle):
+    data = scenes.icosphere_hdri(128, 72, level=5, pieces=16, env_size=(256, 128))
+    sc = lx.Scene(tuned, data)
+    tuned.set_tuning(overlap_rounds=1)
+    r = lx.Render(sc, emis=True)
+    r.iterate(1, 3)
+    img, spp = r.accum()
+    want, _ = oracle.OracleScene(data).render(1, 3, emis=True, threads=8)
+    assert spp == 3
+    assert np.array_equal(img.reshape(-1, 3), want)
+
+
+def test_reset_between_overlapped_calls_and_mode_switch(tuned):
+    data = scenes.cornell(64, 64)
+    sc = lx.Scene(tuned, data)
+    tuned.set_tuning(overlap_rounds=1)
+    r = lx.Render(sc, dmis=True)
+    r.iterate(1, 4)
+    r.reset()
+    r.iterate(1, 3)
+    tuned.set_tuning(overlap_rounds=0)
+    r.iterate(4, 2)  # one round after another, continuing the same image
+    tuned.set_tuning(overlap_rounds=1)
+    r.iterate(6, 2)
+    img, spp = r.accum()
+    st = r.stats()
+    ref = lx.Render(sc, dmis=True)
+    tuned.set_tuning(overlap_rounds=0)
+    ref.iterate(1, 7)
+    assert spp == 7 and ref.accum()[1] == 7
+    assert np.array_equal(img, ref.accum()[0])
+    rs = ref.stats()
+    assert st["radiance_rays"] == rs["radiance_rays"] and st["path_bounces"] == rs["path_bounces"]
+
+
+def test_overlap_rounds_validated(tuned):
+    with pytest.raises(lx.LuxionError, match="overlap_rounds"):
+        tuned.set_tuning(overlap_rounds=2)

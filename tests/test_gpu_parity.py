@@ -195,8 +195,13 @@ def test_render_full_cfg1_subset_matches_oracle(ctx, oracle):
 def test_graph_replay_equals_eager(ctx, oracle):
     data = scenes.cornell(96, 96)
     sc = lx.Scene(ctx, data)
-    a = lx.Render(sc, dmis=True)
-    a.iterate(1, 6)  # captured round replayed 6 times
+    saved = ctx.tuning()
+    try:
+        ctx.set_tuning(overlap_rounds=0, use_graph=1)
+        a = lx.Render(sc, dmis=True)
+        a.iterate(1, 6)  # captured round replayed 6 times
+    finally:
+        ctx.set_tuning(**saved)
     b = lx.Render(sc, dmis=True)
     b.set_timing(True)  # eager launches with event timing
     for it in range(1, 7):

@@ -65,7 +65,7 @@ def test_sorted_batch_of_two_scenes_equals_unsorted(tuned):
 def test_graph_recaptured_after_tuning_change(tuned):
     data = scenes.cornell(80, 80)
     sc = lx.Scene(tuned, data)
-    tuned.set_tuning(wavefront_bounces=-1, sort_mode=0, use_graph=1)
+    tuned.set_tuning(wavefront_bounces=-1, sort_mode=0, use_graph=1, overlap_rounds=0)
     r = lx.Render(sc, dmis=True)
     r.iterate(1, 2)
     tuned.set_tuning(wavefront_bounces=1, sort_mode=3)

@@ -113,9 +113,13 @@ def test_autotune_picks_valid_tuning_same_image(tuned, tmp_path):
     assert tuned.tuning() == best
 
 
-def test_event_timed_rounds_equal_graph_replay(tuned):
+@pytest.mark.parametrize("overlap", [0, 1])
+def test_event_timed_rounds_equal_graph_replay(tuned, overlap):
+    """Event-timed eager rounds give the bits of the graph replay (overlap 0)
+    and of the overlapped rounds (overlap 1)."""
     data = scenes.icosphere_hdri(96, 64, level=4, pieces=8, env_size=(128, 64))
     sc = lx.Scene(tuned, data)
+    tuned.set_tuning(overlap_rounds=overlap, use_graph=1)
     a = lx.Render(sc, emis=True)
     a.iterate(1, 3)
     b = lx.Render(sc, emis=True)
