@@ -162,10 +162,49 @@ extern "C" lx_status lx_batch_run(lx_context* ctx, const lx_job* jobs, uint32_t 
         LX_HIP(lx_launch_fill_u32(d_cnt.as<uint32_t>(), sizeof(LxCounters) / 4, 0u, s));
         std::vector<int> fin;
         int rounds = lx_simulate_rounds(npix, hspp, cap, &fin);
+        /* overlapped rounds (tuning.overlap_rounds) need every job in every
+         * round until it is done, so that round k is sample k of each job in
+         * both forms and the per-pixel sums keep their order */
+        bool overlap = ctx->tuning.overlap_rounds != 0 && rounds >= 2 && n <= LX_SCHED_MAX && (uint64_t)cap >= accum_pix;
+        WaveBufs wb1;
+        DevBuf d_jobs1, d_spp1, d_cnt1, d_sched1, d_spill1, d_col[2];
+        std::vector<LxJob> hjp[2];
+        std::vector<int32_t> hsp[2];
+        if (overlap) {
+            try {
+                wb1.alloc(ctx, cap, any_dmis, true, any_tan);
+                d_spill1.alloc(ctx, d_spill.bytes);
+                d_cnt1.alloc(ctx, sizeof(LxCounters));
+                d_sched1.alloc(ctx, sizeof(LxSched));
+                d_jobs1.alloc(ctx, n * sizeof(LxJob));
+                d_spp1.alloc(ctx, n * 4);
+                for (DevBuf& b : d_col) b.alloc(ctx, accum_pix * 12);
+            } catch (const LxError& e) {
+                if (e.status != LX_ERR_BUDGET) throw;
+                overlap = false; /* one round after another in what the budget leaves */
+            }
+        }
+        if (overlap) {
+            /* parity p's copy of job j: iterations first + p, first + p + 2, ... */
+            for (int p = 0; p < 2; p++) {
+                hjp[p] = hj;
+                hsp[p].resize(n);
+                for (uint32_t i = 0; i < n; i++) {
+                    hjp[p][i].iter = hj[i].iter - 1 + p;
+                    hjp[p][i].flags |= LX_JF_ITER2;
+                    hsp[p][i] = (hspp[i] + 1 - p) / 2;
+                }
+            }
+            LX_HIP(hipMemcpyAsync(d_jobs.ptr, hjp[0].data(), n * sizeof(LxJob), hipMemcpyHostToDevice, s));
+            LX_HIP(hipMemcpyAsync(d_spp.ptr, hsp[0].data(), n * 4, hipMemcpyHostToDevice, s));
+            LX_HIP(hipMemcpyAsync(d_jobs1.ptr, hjp[1].data(), n * sizeof(LxJob), hipMemcpyHostToDevice, s));
+            LX_HIP(hipMemcpyAsync(d_spp1.ptr, hsp[1].data(), n * 4, hipMemcpyHostToDevice, s));
+            LX_HIP(lx_launch_fill_u32(d_cnt1.as<uint32_t>(), sizeof(LxCounters) / 4, 0u, s));
+        }
         /* one captured round, replayed; an event after every round dates job completion */
         hipGraph_t g = nullptr;
         hipGraphExec_t ge = nullptr;
-        bool use_graph = ctx->tuning.use_graph != 0;
+        bool use_graph = ctx->tuning.use_graph != 0 && !overlap;
         if (use_graph) {
             LX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             try {
@@ -182,20 +221,70 @@ extern "C" lx_status lx_batch_run(lx_context* ctx, const lx_job* jobs, uint32_t 
         }
         std::vector<hipEvent_t> ev(rounds + 1);
         for (auto& e : ev) LX_HIP(hipEventCreate(&e));
+        /* overlapped form: round k on stream k % 2 into colour pool k % 2
+         * (filled with -0.0: the one add per pixel copies the colour's bits),
+         * added into the accumulation on the context's stream in round order */
+        hipStream_t ps[2] = {nullptr, nullptr};
+        hipEvent_t e_up = nullptr, e_done[2] = {nullptr, nullptr}, e_free[2] = {nullptr, nullptr};
+        struct Cleanup {
+            hipStream_t* ps;
+            hipEvent_t *up, *done, *free;
+            ~Cleanup() {
+                for (int p = 0; p < 2; p++) {
+                    if (ps[p]) (void)hipStreamDestroy(ps[p]);
+                    if (done[p]) (void)hipEventDestroy(done[p]);
+                    if (free[p]) (void)hipEventDestroy(free[p]);
+                }
+                if (*up) (void)hipEventDestroy(*up);
+            }
+        } cleanup{ps, &e_up, e_done, e_free};
+        if (overlap) {
+            for (int p = 0; p < 2; p++) {
+                LX_HIP(hipStreamCreateWithFlags(&ps[p], hipStreamNonBlocking));
+                LX_HIP(hipEventCreateWithFlags(&e_done[p], hipEventDisableTiming));
+                LX_HIP(hipEventCreateWithFlags(&e_free[p], hipEventDisableTiming));
+            }
+            LX_HIP(hipEventCreateWithFlags(&e_up, hipEventDisableTiming));
+            LX_HIP(hipEventRecord(e_up, s));
+            for (int p = 0; p < 2; p++) LX_HIP(hipStreamWaitEvent(ps[p], e_up, 0));
+        }
+        LxJob* pj[2] = {d_jobs.as<LxJob>(), d_jobs1.as<LxJob>()};
+        int32_t* pspp[2] = {d_spp.as<int32_t>(), d_spp1.as<int32_t>()};
+        LxCounters* pcnt[2] = {d_cnt.as<LxCounters>(), d_cnt1.as<LxCounters>()};
+        LxSched* psch[2] = {d_sched.as<LxSched>(), d_sched1.as<LxSched>()};
+        uint32_t* pspill[2] = {d_spill.as<uint32_t>(), d_spill1.as<uint32_t>()};
+        WaveBufs* pwb[2] = {&wb, &wb1};
         LX_HIP(hipEventRecord(ev[0], s));
         double t_start = now_ms();
         for (int k = 0; k < rounds; k++) {
-            if (use_graph) LX_HIP(hipGraphLaunch(ge, s));
-            else
+            if (overlap) {
+                int p = k & 1;
+                if (k >= 2) LX_HIP(hipStreamWaitEvent(ps[p], e_free[p], 0));
+                LX_HIP(lx_launch_fill_u32(d_col[p].as<uint32_t>(), accum_pix * 3, 0x80000000u, ps[p]));
+                lx_enqueue_round(ctx, pj[p], pspp[p], n, pcnt[p], psch[p], cap, *pwb[p], d_col[p].as<float>(),
+                                 pspill[p], max_depth, tg, sg, cg, width, uscene, nullptr, false, ps[p]);
+                LX_HIP(hipEventRecord(e_done[p], ps[p]));
+                LX_HIP(hipStreamWaitEvent(s, e_done[p], 0));
+                LX_HIP(lx_launch_add_f32(d_accum.as<float>(), d_col[p].as<float>(), accum_pix * 3, s));
+                LX_HIP(hipEventRecord(e_free[p], s));
+            } else if (use_graph) {
+                LX_HIP(hipGraphLaunch(ge, s));
+            } else {
                 lx_enqueue_round(ctx, d_jobs.as<LxJob>(), d_spp.as<int32_t>(), n, d_cnt.as<LxCounters>(),
                                  d_sched.as<LxSched>(), cap, wb, d_accum.as<float>(), d_spill.as<uint32_t>(), max_depth,
                                  tg, sg, cg, width, uscene, nullptr);
+            }
             LX_HIP(hipEventRecord(ev[k + 1], s));
         }
         LX_HIP(hipStreamSynchronize(s));
         double t_end = now_ms();
         LxCounters c;
         LX_HIP(hipMemcpy(&c, d_cnt.ptr, sizeof(c), hipMemcpyDeviceToHost));
+        if (overlap) {
+            LxCounters c1;
+            LX_HIP(hipMemcpy(&c1, d_cnt1.ptr, sizeof(c1), hipMemcpyDeviceToHost));
+            lx_counters_add(c, c1);
+        }
         std::vector<float> round_ms(rounds + 1, 0.0f);
         for (int k = 1; k <= rounds; k++) LX_HIP(hipEventElapsedTime(&round_ms[k], ev[0], ev[k]));
         for (auto& e : ev) (void)hipEventDestroy(e);
@@ -204,6 +293,11 @@ extern "C" lx_status lx_batch_run(lx_context* ctx, const lx_job* jobs, uint32_t 
         LX_REQUIRE(c.stack_overflow == 0, LX_ERR_STATE, "traversal stack overflow");
         std::vector<LxJob> done(n);
         LX_HIP(hipMemcpy(done.data(), d_jobs.ptr, n * sizeof(LxJob), hipMemcpyDeviceToHost));
+        if (overlap) {
+            std::vector<LxJob> done1(n);
+            LX_HIP(hipMemcpy(done1.data(), d_jobs1.ptr, n * sizeof(LxJob), hipMemcpyDeviceToHost));
+            for (uint32_t i = 0; i < n; i++) done[i].spp_done += done1[i].spp_done;
+        }
         uint64_t tot_rays = c.rad_rays + c.dl_rays + c.env_rays;
         for (uint32_t i = 0; i < n; i++) {
             lx_job_result& r = results[i];

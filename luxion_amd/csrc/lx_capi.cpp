@@ -448,22 +448,7 @@ static void read_counters(lx_render* r, LxCounters& c) {
     if (!r->overlap_ready) return;
     LxCounters o;
     LX_HIP(hipMemcpy(&o, r->odd.counters.ptr, sizeof(o), hipMemcpyDeviceToHost));
-    c.rad_rays += o.rad_rays;
-    c.dl_rays += o.dl_rays;
-    c.env_rays += o.env_rays;
-    c.bounces += o.bounces;
-    c.tail_bounces += o.tail_bounces;
-    for (int w = 0; w < 2; w++) {
-        c.work[w].closest_rays += o.work[w].closest_rays;
-        c.work[w].any_rays += o.work[w].any_rays;
-        c.work[w].closest_nodes += o.work[w].closest_nodes;
-        c.work[w].closest_tris += o.work[w].closest_tris;
-        c.work[w].closest_insts += o.work[w].closest_insts;
-        c.work[w].any_nodes += o.work[w].any_nodes;
-        c.work[w].any_tris += o.work[w].any_tris;
-        c.work[w].any_insts += o.work[w].any_insts;
-    }
-    c.stack_overflow |= o.stack_overflow;
+    lx_counters_add(c, o);
 }
 
 lx_status lx_render_iterate(lx_render* r, uint32_t first_iter, uint32_t n) {

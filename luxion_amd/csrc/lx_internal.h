@@ -252,6 +252,8 @@ void lx_enqueue_round(lx_context* ctx, LxJob* d_jobs, const int32_t* d_spp, uint
                                         [2*max_depth] the round's end; other slots are not recorded */,
                       bool count = false, hipStream_t stream = nullptr /* NULL: the context's stream */);
 uint32_t lx_spill_words(uint32_t trace_grid);
+/* a += b for the cumulative fields (ray, bounce and work counts, overflow flag) */
+void lx_counters_add(LxCounters& a, const LxCounters& b);
 /* bounces launched one kernel each before the tail kernel takes over
  * (tuning.wavefront_bounces; negative or above max_depth = all of them,
  * 0 = the tail kernel runs every bounce) */

@@ -132,6 +132,25 @@ void lx_enqueue_round(lx_context* ctx, LxJob* d_jobs, const int32_t* d_spp, uint
     LX_HIP(lx_launch_end_round(d_jobs, njobs, s));
 }
 
+void lx_counters_add(LxCounters& a, const LxCounters& b) {
+    a.rad_rays += b.rad_rays;
+    a.dl_rays += b.dl_rays;
+    a.env_rays += b.env_rays;
+    a.bounces += b.bounces;
+    a.tail_bounces += b.tail_bounces;
+    for (int w = 0; w < 2; w++) {
+        a.work[w].closest_rays += b.work[w].closest_rays;
+        a.work[w].any_rays += b.work[w].any_rays;
+        a.work[w].closest_nodes += b.work[w].closest_nodes;
+        a.work[w].closest_tris += b.work[w].closest_tris;
+        a.work[w].closest_insts += b.work[w].closest_insts;
+        a.work[w].any_nodes += b.work[w].any_nodes;
+        a.work[w].any_tris += b.work[w].any_tris;
+        a.work[w].any_insts += b.work[w].any_insts;
+    }
+    a.stack_overflow |= b.stack_overflow;
+}
+
 int lx_wavefront_bounces(const lx_context* ctx, int max_depth) {
     int wf = ctx->tuning.wavefront_bounces;
     if (wf < 0 || wf > max_depth) wf = max_depth;

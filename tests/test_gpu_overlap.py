@@ -90,3 +90,42 @@ def test_reset_between_overlapped_calls_and_mode_switch(tuned):
 def test_overlap_rounds_validated(tuned):
     with pytest.raises(lx.LuxionError, match="overlap_rounds"):
         tuned.set_tuning(overlap_rounds=2)
+
+
+def _batch(ctx, jobs, **tuning):
+    ctx.set_tuning(**tuning)
+    return lx.batch_run(ctx, jobs())
+
+
+def test_batch_overlapped_equals_sequential(tuned):
+    """Two scenes (per-lane scene in the tail kernel), sample counts 1..5 and
+    a later first iteration: images, sample counts and status agree."""
+    a = lx.Scene(tuned, scenes.cornell(64, 64))
+    zd = scenes.material_zoo(80, 48, level=2, env_size=(64, 32))
+    b = lx.Scene(tuned, zd)
+    jobs = lambda: [lx.make_job(a, scenes.cornell(64, 64).camera, 5, dmis=True, seed=5),
+                    lx.make_job(b, zd.camera, 2, emis=True, seed=9),
+                    lx.make_job(a, scenes.cornell(48, 80).camera, 1, dmis=True, seed=11),
+                    lx.make_job(b, zd.camera, 4, emis=True, first_iter=7),
+                    lx.make_job(a, scenes.cornell(33, 17).camera, 3, dmis=True)]
+    seq = _batch(tuned, jobs, overlap_rounds=0)
+    ovl = _batch(tuned, jobs, overlap_rounds=1)
+    for x, y in zip(seq, ovl):
+        assert x["status"] == y["status"] == 0
+        assert x["spp"] == y["spp"]
+        assert np.array_equal(x["image"], y["image"])
+    one = _batch(tuned, lambda: jobs()[:1], overlap_rounds=1)  # a single job: scalar job and scene
+    assert np.array_equal(one[0]["image"], seq[0]["image"])
+
+
+def test_batch_with_limited_paths_runs_rounds_in_sequence(tuned):
+    """With fewer path slots than the jobs' pixels some jobs wait for later
+    rounds; the overlapped form then falls back to rounds one after another
+    and gives the same images."""
+    sc = lx.Scene(tuned, scenes.cornell(64, 64))
+    jobs = lambda: [lx.make_job(sc, scenes.cornell(64, 64).camera, 3, dmis=True, seed=k) for k in range(4)]
+    want = _batch(tuned, jobs, overlap_rounds=0)
+    got = _batch(tuned, jobs, overlap_rounds=1, max_paths_per_round=2 * 64 * 64)
+    for x, y in zip(want, got):
+        assert x["spp"] == y["spp"] == 3
+        assert np.array_equal(x["image"], y["image"])
