@@ -318,7 +318,12 @@ def main():
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
                      "bytes_per_launch": int(bytes_per_launch), "avg_launch_ms": round(launch_ms, 4),
-                     "per_ray": {k: round(v, 2) for k, v in per_ray.items()}},
+                     "per_ray": {k: round(v, 2) for k, v in per_ray.items()},
+                     "overlapped_frame": {
+                         "achieved": round(kbytes / args.steps / (t_timed / args.steps) / 1e9, 1),
+                         "frac": round(kbytes / args.steps / (t_timed / args.steps) / 1e9 / HBM_PEAK_GBPS, 4),
+                         "note": "the same algorithmic bytes per frame over the timed region's frame time "
+                                 "(consecutive rounds overlapped)"}},
         "tuning": tuning,
         "batched": batched,
         "served": served,
