@@ -718,6 +718,19 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
+/* Pixel ranges of an RGB pool: range y is (first pixel r[2y], count r[2y+1]);
+ * fill sets its floats' bits to v, add does dst += src over them. */
+__global__ void k_fill_ranges(uint32_t* __restrict__ p, const uint32_t* __restrict__ r, uint32_t v) {
+    size_t base = 3ull * r[2 * blockIdx.y], n = 3ull * r[2 * blockIdx.y + 1];
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[base + i] = v;
+}
+__global__ void k_add_ranges(float* __restrict__ dst, const float* __restrict__ src, const uint32_t* __restrict__ r) {
+    size_t base = 3ull * r[2 * blockIdx.y], n = 3ull * r[2 * blockIdx.y + 1];
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[base + i] = dst[base + i] + src[base + i];
+}
+
 /* dst += src, element by element (one round's colours into the accumulation) */
 __global__ void k_add_f32(float* __restrict__ dst, const float* __restrict__ src, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -840,6 +853,16 @@ hipError_t lx_launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) 
     if (blocks > 65535) blocks = 65535;
     if (blocks == 0) blocks = 1;
     hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)blocks), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+hipError_t lx_launch_fill_ranges(uint32_t* p, const uint32_t* ranges, uint32_t nranges, uint32_t v, hipStream_t s) {
+    if (nranges == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_ranges, dim3(64, nranges), dim3(256), 0, s, p, ranges, v);
+    return hipGetLastError();
+}
+hipError_t lx_launch_add_ranges(float* dst, const float* src, const uint32_t* ranges, uint32_t nranges, hipStream_t s) {
+    if (nranges == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_add_ranges, dim3(64, nranges), dim3(256), 0, s, dst, src, ranges);
     return hipGetLastError();
 }
 hipError_t lx_launch_add_f32(float* dst, const float* src, size_t n, hipStream_t s) {

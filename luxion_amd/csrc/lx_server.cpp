@@ -39,6 +39,10 @@ struct SrvJob {
     uint32_t npix = 0, accum_off = 0;
     bool cancel = false;
     lx_status status = LX_OK;
+    /* overlapped rounds */
+    uint32_t launched = 0, merged = 0; /* samples launched / added into the pool */
+    int64_t last_round = -1;           /* round of the last launched sample */
+    bool stopped = false;              /* finished (or failed): no more samples */
 };
 
 /* First-fit allocator over pixel ranges of the accumulation pool. */
@@ -103,12 +107,36 @@ struct lx_server {
     uint32_t tg = 0, sg = 0, cg = 0;
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
+    /* overlapped rounds (tuning.overlap_rounds at start): round r runs on
+     * par[r % 2]'s stream with its own job table, counters, paths and colour
+     * pool; only the sampled jobs' pixel ranges of the pool are filled with
+     * -0.0 and then added into d_accum, round after round, on the context's
+     * stream */
+    bool overlap = false;
+    struct Par {
+        DevBuf jobs, spp, cnt, sched, spill, col, rng;
+        WaveBufs wb;
+        std::vector<LxJob> hj;      /* async-copy sources: untouched until this parity's next round */
+        std::vector<int32_t> hs;
+        std::vector<uint32_t> hr;
+        std::vector<SrvJob*> in_round;
+        uint32_t nr = 0;
+        int64_t round = -1;
+        LxCounters last{};          /* counters as of this parity's last processed round */
+        hipStream_t s = nullptr;
+        hipEvent_t done = nullptr, merged = nullptr;
+    } par[2];
     /* stats */
     std::vector<double> lat;
     lx_server_stats st{};
     uint64_t rays_prev = 0;
 
     void loop();
+    void loop_overlapped();
+    void release_slot(SrvJob* j) {
+        ranges.release(j->accum_off, j->npix);
+        slot_job[j->slot] = nullptr;
+    }
 };
 
 static void finish_job(lx_server* S, SrvJob* j, lx_status status, double t) {
@@ -260,6 +288,174 @@ void lx_server::loop() {
 
 extern "C" {
 
+/* The request loop with overlapped rounds: round r is launched, then round
+ * r - 1 (on the other stream) is waited for and its jobs' samples counted,
+ * results copied and slots released, so the host work and round r - 1's last
+ * long paths both overlap round r.  A job's samples are added into its pool
+ * range in round order (one sample per round), so its image is the bits of
+ * the loop without overlap. */
+void lx_server::loop_overlapped() {
+    int64_t r = 0;  /* next round */
+    int pend = -1;  /* parity of the launched round not yet processed, or -1 */
+    while (true) {
+        std::vector<int> fresh; /* slots admitted now: their pool range starts at zero */
+        bool work = false;
+        {
+            std::unique_lock<std::mutex> lk(m);
+            double t = now_ms();
+            const int64_t oldest = pend >= 0 ? par[pend].round : r; /* first round not yet processed */
+            /* cancellations and deadlines of active jobs: no more samples; the slot
+             * is free once no launched round holds the job */
+            for (uint32_t k = 0; k < slots; k++) {
+                SrvJob* j = slot_job[k];
+                if (!j || j->stopped) continue;
+                bool expired = j->job.deadline_ms > 0 && t - j->res->submit_ms > j->job.deadline_ms;
+                if (j->cancel || expired) {
+                    finish_job(this, j, j->cancel ? LX_ERR_CANCELLED : LX_ERR_TIMEOUT, t);
+                    j->stopped = true;
+                    if (j->last_round < oldest) release_slot(j);
+                }
+            }
+            /* admissions, FIFO */
+            while (!queue.empty()) {
+                SrvJob* j = queue.front();
+                if (j->cancel) {
+                    queue.pop_front();
+                    finish_job(this, j, LX_ERR_CANCELLED, t);
+                    continue;
+                }
+                if (inject_fail > 0 || j->npix > pool_pix || j->npix > cap) {
+                    if (inject_fail > 0) inject_fail--;
+                    queue.pop_front();
+                    finish_job(this, j, LX_ERR_BUDGET, t);
+                    continue;
+                }
+                int free_slot = -1;
+                for (uint32_t k = 0; k < slots; k++)
+                    if (!slot_job[k]) {
+                        free_slot = (int)k;
+                        break;
+                    }
+                uint32_t off = 0;
+                if (free_slot < 0 || !ranges.alloc(j->npix, off)) break; /* wait for space */
+                queue.pop_front();
+                j->slot = free_slot;
+                j->accum_off = off;
+                j->state = 1;
+                j->res->start_ms = t;
+                j->launched = j->merged = 0;
+                j->last_round = -1;
+                slot_job[free_slot] = j;
+                fresh.push_back(free_slot);
+            }
+            uint32_t nact = 0;
+            for (uint32_t k = 0; k < slots; k++) {
+                SrvJob* j = slot_job[k];
+                if (!j) continue;
+                nact++;
+                if (!j->stopped && j->launched < (uint32_t)j->job.spp) work = true;
+            }
+            st.peak_jobs_in_flight = std::max(st.peak_jobs_in_flight, nact);
+            if (!work && pend < 0) {
+                if (stop && queue.empty()) break;
+                cv.wait_for(lk, std::chrono::milliseconds(50));
+                continue;
+            }
+        }
+        try {
+            std::lock_guard<std::mutex> dl(ctx->mu);
+            ctx->set_device();
+            hipStream_t ms = ctx->stream;
+            double t_launch = now_ms();
+            if (work) {
+                int p = (int)(r & 1);
+                Par& P = par[p];
+                for (int k : fresh) {
+                    SrvJob* j = slot_job[k];
+                    LX_HIP(lx_launch_fill_u32(d_accum.as<uint32_t>() + 3 * (size_t)j->accum_off, 3 * (size_t)j->npix,
+                                              0u, ms));
+                }
+                P.in_round.clear();
+                P.nr = 0;
+                for (uint32_t k = 0; k < slots; k++) {
+                    SrvJob* j = slot_job[k];
+                    if (j && !j->stopped && j->launched < (uint32_t)j->job.spp) {
+                        P.hj[k] = lx_make_dev_job(j->job, j->accum_off);
+                        P.hj[k].iter = (int32_t)j->job.first_iter - 1 + (int32_t)j->launched; /* +1 at the round's start */
+                        P.hs[k] = 1;
+                        P.hr[2 * P.nr] = j->accum_off;
+                        P.hr[2 * P.nr + 1] = j->npix;
+                        P.nr++;
+                        P.in_round.push_back(j);
+                        j->launched++;
+                        j->last_round = r;
+                    } else {
+                        P.hs[k] = 0; /* not scheduled this round */
+                    }
+                }
+                LX_HIP(hipMemcpyAsync(P.jobs.ptr, P.hj.data(), slots * sizeof(LxJob), hipMemcpyHostToDevice, P.s));
+                LX_HIP(hipMemcpyAsync(P.spp.ptr, P.hs.data(), slots * 4, hipMemcpyHostToDevice, P.s));
+                LX_HIP(hipMemcpyAsync(P.rng.ptr, P.hr.data(), (size_t)P.nr * 8, hipMemcpyHostToDevice, P.s));
+                LX_HIP(lx_launch_fill_ranges(P.col.as<uint32_t>(), P.rng.as<uint32_t>(), P.nr, 0x80000000u, P.s));
+                lx_enqueue_round(ctx, P.jobs.as<LxJob>(), P.spp.as<int32_t>(), slots, P.cnt.as<LxCounters>(),
+                                 P.sched.as<LxSched>(), cap, P.wb, P.col.as<float>(), P.spill.as<uint32_t>(),
+                                 max_depth, tg, sg, cg, width, nullptr, nullptr, false, P.s);
+                LX_HIP(hipEventRecord(P.done, P.s));
+                LX_HIP(hipStreamWaitEvent(ms, P.done, 0));
+                LX_HIP(lx_launch_add_ranges(d_accum.as<float>(), P.col.as<float>(), P.rng.as<uint32_t>(), P.nr, ms));
+                LX_HIP(hipEventRecord(P.merged, ms));
+                P.round = r;
+                r++;
+            }
+            if (pend >= 0) {
+                Par& Q = par[pend];
+                LX_HIP(hipEventSynchronize(Q.merged));
+                LX_HIP(hipMemcpy(&Q.last, Q.cnt.ptr, sizeof(LxCounters), hipMemcpyDeviceToHost));
+                LX_REQUIRE(Q.last.stack_overflow == 0, LX_ERR_STATE, "traversal stack overflow");
+                std::lock_guard<std::mutex> lk(m);
+                st.rays = par[0].last.rad_rays + par[0].last.dl_rays + par[0].last.env_rays + par[1].last.rad_rays +
+                          par[1].last.dl_rays + par[1].last.env_rays;
+                st.rounds++;
+                double t = now_ms();
+                st.busy_ms += t - t_launch;
+                for (SrvJob* j : Q.in_round) {
+                    j->merged++;
+                    if (!j->stopped) j->res->spp_done = j->merged;
+                    if (!j->stopped && j->merged == (uint32_t)j->job.spp) {
+                        /* every sample of the job is in its range (later rounds do not touch it) */
+                        if (j->res->rgb_sum)
+                            LX_HIP(hipMemcpy(j->res->rgb_sum, (uint8_t*)d_accum.ptr + (size_t)j->accum_off * 12,
+                                             (size_t)j->npix * 12, hipMemcpyDeviceToHost));
+                        j->res->rays = 0;
+                        finish_job(this, j, LX_OK, t);
+                        j->stopped = true;
+                        release_slot(j);
+                    } else if (j->stopped && j->last_round == Q.round) {
+                        release_slot(j); /* cancelled or expired while this round ran */
+                    }
+                }
+                Q.in_round.clear();
+                pend = -1;
+            }
+            if (work) pend = (int)((r - 1) & 1);
+        } catch (const LxError& e) {
+            /* a device failure ends every in-flight job with its status */
+            (void)hipDeviceSynchronize();
+            std::lock_guard<std::mutex> lk(m);
+            double t = now_ms();
+            for (uint32_t k = 0; k < slots; k++)
+                if (SrvJob* j = slot_job[k]) {
+                    if (!j->stopped) finish_job(this, j, e.status, t);
+                    slot_job[k] = nullptr;
+                }
+            pend = -1;
+            stop = true;
+        }
+        done_cv.notify_all();
+    }
+    done_cv.notify_all();
+}
+
 lx_status lx_server_start(lx_context* ctx, const lx_server_opts* o, lx_server** out) {
     try {
         LX_REQUIRE(ctx && out, LX_ERR_INVALID, "null argument");
@@ -279,15 +475,18 @@ lx_status lx_server_start(lx_context* ctx, const lx_server_opts* o, lx_server** 
         S->max_depth = opts.max_depth;
         std::lock_guard<std::mutex> dl(ctx->mu);
         S->width = ctx->tuning.bvh_width == 8 ? 8 : 4;
+        S->overlap = ctx->tuning.overlap_rounds != 0;
         ctx->set_device();
         uint64_t avail = ctx->budget > ctx->in_use.load() ? ctx->budget - ctx->in_use.load() : 0;
         uint64_t budget = opts.hbm_budget_bytes ? std::min<uint64_t>(opts.hbm_budget_bytes, avail) : avail;
-        /* split the budget: path capacity and accumulation pool */
+        /* split the budget: path capacity and accumulation pool (overlapped
+         * rounds: two sets of path buffers, the pool and two colour pools) */
+        const uint64_t sets = S->overlap ? 2 : 1, pools = S->overlap ? 3 : 1;
         uint64_t per_path = 2 * 15 * 4 + 24 * 4;
-        uint64_t cap = opts.max_paths_per_round ? opts.max_paths_per_round : std::min<uint64_t>(budget / 2 / per_path,
-                                                                                                    16u << 20);
+        uint64_t cap = opts.max_paths_per_round ? opts.max_paths_per_round
+                                                : std::min<uint64_t>(budget / 2 / per_path / sets, 16u << 20);
         cap = std::max<uint64_t>(cap, 4096);
-        uint64_t pool = (budget - std::min(budget, cap * per_path)) / 12 / 2;
+        uint64_t pool = (budget - std::min(budget, cap * per_path * sets)) / 12 / 2 / pools;
         pool = std::min<uint64_t>(pool, (1ull << 31) - 1);
         LX_REQUIRE(pool >= 4096, LX_ERR_BUDGET, "HBM budget too small for the served form");
         S->cap = (uint32_t)cap;
@@ -310,6 +509,32 @@ lx_status lx_server_start(lx_context* ctx, const lx_server_opts* o, lx_server** 
         S->d_spill.alloc(ctx, (size_t)lx_spill_words(S->tg) * 4);
         hipStream_t s = ctx->stream;
         LX_HIP(lx_launch_fill_u32(S->d_cnt.as<uint32_t>(), sizeof(LxCounters) / 4, 0u, s));
+        if (S->overlap) {
+            for (auto& P : S->par) {
+                P.wb.alloc(ctx, S->cap, true, true, true);
+                P.jobs.alloc(ctx, S->slots * sizeof(LxJob));
+                P.spp.alloc(ctx, S->slots * 4);
+                P.cnt.alloc(ctx, sizeof(LxCounters));
+                P.sched.alloc(ctx, sizeof(LxSched));
+                P.spill.alloc(ctx, S->d_spill.bytes);
+                P.col.alloc(ctx, (size_t)S->pool_pix * 12);
+                P.rng.alloc(ctx, (size_t)S->slots * 8);
+                P.hj.assign(S->slots, LxJob{});
+                P.hs.assign(S->slots, 0);
+                P.hr.assign(2 * (size_t)S->slots, 0u);
+                LX_HIP(lx_launch_fill_u32(P.cnt.as<uint32_t>(), sizeof(LxCounters) / 4, 0u, s));
+                LX_HIP(hipStreamCreateWithFlags(&P.s, hipStreamNonBlocking));
+                LX_HIP(hipEventCreateWithFlags(&P.done, hipEventDisableTiming));
+                LX_HIP(hipEventCreateWithFlags(&P.merged, hipEventDisableTiming));
+            }
+            LX_HIP(hipStreamSynchronize(s));
+            ctx->refs++;
+            lx_server* raw = S.release();
+            raw->th = std::thread([raw]() { raw->loop_overlapped(); });
+            *out = raw;
+            lx_set_error("");
+            return LX_OK;
+        }
         LX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         try {
             lx_enqueue_round(ctx, S->d_jobs.as<LxJob>(), S->d_spp.as<int32_t>(), S->slots, S->d_cnt.as<LxCounters>(),
@@ -446,6 +671,14 @@ lx_status lx_server_stop(lx_server* S) {
         (void)hipStreamSynchronize(ctx->stream);
         if (S->ge) (void)hipGraphExecDestroy(S->ge);
         if (S->g) (void)hipGraphDestroy(S->g);
+        for (auto& P : S->par) {
+            if (P.s) {
+                (void)hipStreamSynchronize(P.s);
+                (void)hipStreamDestroy(P.s);
+            }
+            if (P.done) (void)hipEventDestroy(P.done);
+            if (P.merged) (void)hipEventDestroy(P.merged);
+        }
     }
     for (auto& kv : S->jobs) {
         lx_scene_release(kv.second->job.scene);

@@ -39,6 +39,15 @@ def _jobs(corn, ico):
     return jobs
 
 
+@pytest.fixture(params=[1, 0], ids=["overlapped", "sequential"])
+def server_mode(ctx, request):
+    """Servers started with overlapped rounds (the default) and without."""
+    saved = ctx.tuning()
+    ctx.set_tuning(overlap_rounds=request.param)
+    yield request.param
+    ctx.set_tuning(**saved)
+
+
 def _alone(sc, cam, spp, dmis, emis, seed, first_iter):
     r = lx.Render(sc, dmis=dmis, emis=emis, seed=seed, camera=cam)
     r.iterate(first_iter, spp)
@@ -72,7 +81,7 @@ def test_batch_with_small_path_capacity_schedules_subsets(ctx, scenes_pair):
     assert max(r["finish_ms"] for r in part) >= min(r["finish_ms"] for r in part)
 
 
-def test_server_result_same_alone_and_in_mix(ctx, scenes_pair):
+def test_server_result_same_alone_and_in_mix(ctx, scenes_pair, server_mode):
     corn, ico = scenes_pair
     spec = _jobs(corn, ico)
     srv = lx.Server(ctx, max_jobs=8, max_depth=8)
@@ -98,7 +107,7 @@ def test_server_result_same_alone_and_in_mix(ctx, scenes_pair):
         srv.stop()
 
 
-def test_server_failure_handling_isolated(ctx, scenes_pair):
+def test_server_failure_handling_isolated(ctx, scenes_pair, server_mode):
     corn, ico = scenes_pair
     srv = lx.Server(ctx, max_jobs=8, max_depth=8)
     try:
@@ -130,3 +139,25 @@ def test_workload_drivers_small(ctx, scenes_pair):
     s = workloads.run_served(ctx, workloads.mixed_jobs(specs, 6, seed=6, side=(16, 48), spp=(1, 4)), 500.0,
                              max_depth=8)
     assert s["p99_ms"] >= s["p50_ms"] > 0 and s["jobs"] == 6
+
+
+def test_server_many_small_jobs_overlapped_equal_alone(ctx, scenes_pair, server_mode):
+    """More jobs than slots, with pool ranges freed and reused while rounds
+    are in flight: every image still equals the job rendered alone."""
+    corn, ico = scenes_pair
+    srv = lx.Server(ctx, max_jobs=3, max_depth=8)
+    try:
+        spec = []
+        for k in range(10):
+            w, h = 16 + 8 * (k % 4), 16 + 4 * (k % 3)
+            spec.append((corn, orbit_camera(k, w, h), dict(spp=1 + k % 4, dmis=True, seed=k, first_iter=1 + k)))
+        ids = [srv.submit(lx.make_job(sc, cam, **kw)) for sc, cam, kw in spec]
+        for i in ids:
+            assert srv.poll(i, 60000)
+        for (sc, cam, kw), i in zip(spec, ids):
+            r = srv.result(i)
+            assert r["status"] == 0 and r["spp"] == kw["spp"]
+            assert np.array_equal(r["image"], _alone(sc, cam, kw["spp"], True, False, kw["seed"], kw["first_iter"]))
+        assert srv.stats()["jobs_done"] == 10
+    finally:
+        srv.stop()
