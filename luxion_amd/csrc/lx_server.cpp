@@ -377,9 +377,15 @@ void lx_server::loop_overlapped() {
                 }
                 P.in_round.clear();
                 P.nr = 0;
-                for (uint32_t k = 0; k < slots; k++) {
+                /* the round's jobs, chosen here so that they fit the path capacity
+                 * together (the round then schedules every one of them); the first
+                 * slot rotates so that no job waits behind the same others */
+                uint64_t used = 0;
+                for (uint32_t q = 0; q < slots; q++) {
+                    uint32_t k = (uint32_t)(((uint64_t)r + q) % slots);
                     SrvJob* j = slot_job[k];
-                    if (j && !j->stopped && j->launched < (uint32_t)j->job.spp) {
+                    if (j && !j->stopped && j->launched < (uint32_t)j->job.spp && used + j->npix <= cap) {
+                        used += j->npix;
                         P.hj[k] = lx_make_dev_job(j->job, j->accum_off);
                         P.hj[k].iter = (int32_t)j->job.first_iter - 1 + (int32_t)j->launched; /* +1 at the round's start */
                         P.hs[k] = 1;

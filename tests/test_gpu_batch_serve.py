@@ -161,3 +161,22 @@ def test_server_many_small_jobs_overlapped_equal_alone(ctx, scenes_pair, server_
         assert srv.stats()["jobs_done"] == 10
     finally:
         srv.stop()
+
+
+def test_server_jobs_beyond_path_capacity_take_turns(ctx, scenes_pair, server_mode):
+    """Path capacity for one 64x64 job per round, several jobs in flight: the
+    jobs take turns, and each image still equals the job rendered alone."""
+    corn, ico = scenes_pair
+    srv = lx.Server(ctx, max_jobs=4, max_depth=8, max_paths=4096)
+    try:
+        spec = [(corn, orbit_camera(k, 64 if k % 2 == 0 else 32, 64 if k % 2 == 0 else 32),
+                 dict(spp=2 + k % 3, dmis=True, seed=3 * k)) for k in range(6)]
+        ids = [srv.submit(lx.make_job(sc, cam, **kw)) for sc, cam, kw in spec]
+        for i in ids:
+            assert srv.poll(i, 60000)
+        for (sc, cam, kw), i in zip(spec, ids):
+            r = srv.result(i)
+            assert r["status"] == 0 and r["spp"] == kw["spp"]
+            assert np.array_equal(r["image"], _alone(sc, cam, kw["spp"], True, False, kw["seed"], 1))
+    finally:
+        srv.stop()
