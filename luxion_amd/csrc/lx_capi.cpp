@@ -375,26 +375,41 @@ static void enqueue(lx_render* r, hipEvent_t* ev) {
                      r->scene->d_scene.as<LxDevScene>(), ev, r->counting);
 }
 
-/* Buffers and streams of the overlapped form, made on first use. */
-static void overlap_setup(lx_render* r) {
-    if (r->overlap_ready) return;
+/* Buffers and streams of the overlapped form, made on first use.  Returns
+ * false (nothing kept) when the HBM budget cannot hold them: the render then
+ * runs its rounds one after another. */
+static bool overlap_setup(lx_render* r) {
+    if (r->overlap_ready) return true;
     lx_context* ctx = r->ctx;
     auto& O = r->odd;
-    O.wb.alloc(ctx, r->npix, r->opts.dmis != 0, r->opts.emis != 0, scene_has_normal_maps(r->scene));
-    O.jobs.alloc(ctx, sizeof(LxJob));
-    O.job_spp.alloc(ctx, 16);
-    O.counters.alloc(ctx, sizeof(LxCounters));
+    try {
+        O.wb.alloc(ctx, r->npix, r->opts.dmis != 0, r->opts.emis != 0, scene_has_normal_maps(r->scene));
+        O.jobs.alloc(ctx, sizeof(LxJob));
+        O.job_spp.alloc(ctx, 16);
+        O.counters.alloc(ctx, sizeof(LxCounters));
+        O.sched.alloc(ctx, sizeof(LxSched));
+        O.spill.alloc(ctx, r->spill.bytes);
+        for (int p = 0; p < 2; p++) r->color[p].alloc(ctx, (size_t)r->npix * 12);
+    } catch (const LxError& e) {
+        if (e.status != LX_ERR_BUDGET) throw;
+        O.wb = WaveBufs();
+        O.jobs.release();
+        O.job_spp.release();
+        O.counters.release();
+        O.sched.release();
+        O.spill.release();
+        for (int p = 0; p < 2; p++) r->color[p].release();
+        return false;
+    }
     LX_HIP(lx_launch_fill_u32(O.counters.as<uint32_t>(), sizeof(LxCounters) / 4, 0u, ctx->stream));
-    O.sched.alloc(ctx, sizeof(LxSched));
-    O.spill.alloc(ctx, r->spill.bytes);
     for (int p = 0; p < 2; p++) {
-        r->color[p].alloc(ctx, (size_t)r->npix * 12);
         if (!r->pstream[p]) LX_HIP(hipStreamCreateWithFlags(&r->pstream[p], hipStreamNonBlocking));
         if (!r->ev_done[p]) LX_HIP(hipEventCreateWithFlags(&r->ev_done[p], hipEventDisableTiming));
         if (!r->ev_free[p]) LX_HIP(hipEventCreateWithFlags(&r->ev_free[p], hipEventDisableTiming));
     }
     if (!r->ev_up) LX_HIP(hipEventCreateWithFlags(&r->ev_up, hipEventDisableTiming));
     r->overlap_ready = true;
+    return true;
 }
 
 /* n >= 2 rounds, round k on stream k % 2 with that parity's buffers; the job
@@ -403,7 +418,6 @@ static void overlap_setup(lx_render* r) {
  * round k - 1's, and round k + 2 waits for that add before it reuses the
  * colour buffer. */
 static void iterate_overlapped(lx_render* r, uint32_t first_iter, uint32_t n) {
-    overlap_setup(r);
     lx_context* ctx = r->ctx;
     hipStream_t ms = ctx->stream;
     LxJob* djob[2] = {r->jobs.as<LxJob>(), r->odd.jobs.as<LxJob>()};
@@ -459,7 +473,7 @@ lx_status lx_render_iterate(lx_render* r, uint32_t first_iter, uint32_t n) {
     lx_context* ctx = r->ctx;
     std::lock_guard<std::mutex> lk(ctx->mu);
     ctx->set_device();
-    if (n >= 2 && ctx->tuning.overlap_rounds && !r->timing && !r->counting) {
+    if (n >= 2 && ctx->tuning.overlap_rounds && !r->timing && !r->counting && overlap_setup(r)) {
         iterate_overlapped(r, first_iter, n);
         r->spp += n;
         return LX_OK;

@@ -129,3 +129,41 @@ def test_batch_with_limited_paths_runs_rounds_in_sequence(tuned):
     for x, y in zip(want, got):
         assert x["spp"] == y["spp"] == 3
         assert np.array_equal(x["image"], y["image"])
+
+
+def test_overlap_falls_back_when_the_budget_is_tight():
+    """In a context whose HBM budget holds the render but not a second set of
+    round buffers, overlapped iterate runs its rounds one after another."""
+    data = scenes.cornell(128, 128)
+    want = None
+    with_room = lx.Context(0)
+    try:
+        sc = lx.Scene(with_room, data)
+        r = lx.Render(sc, dmis=True)
+        with_room.set_tuning(overlap_rounds=0)
+        r.iterate(1, 3)
+        want = r.accum()[0]
+        r.close()
+    finally:
+        with_room.close()
+    for mib in range(4, 400, 4):
+        c = lx.Context(0, hbm_budget_bytes=mib << 20)
+        try:
+            sc = lx.Scene(c, data)
+            r = lx.Render(sc, dmis=True)
+        except lx.LuxionError:
+            c.close()
+            continue
+        try:
+            # the budget is tight: a second render (about the size of a second
+            # set of round buffers) does not fit
+            with pytest.raises(lx.LuxionError, match="budget"):
+                lx.Render(sc, dmis=True)
+            r.iterate(1, 3)  # overlap_rounds is on by default
+            img, spp = r.accum()
+            assert spp == 3 and np.array_equal(img, want), mib
+            r.close()
+        finally:
+            c.close()
+        return
+    pytest.fail("no budget fitted the render")
