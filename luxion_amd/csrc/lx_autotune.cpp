@@ -173,6 +173,9 @@ lx_status lx_autotune(lx_context* ctx, const lx_job* jobs, uint32_t n, const cha
     }
     lx_tuning cur = start;
     cur.bvh_width = jobs[0].scene->dev.bvh_width;
+    /* a candidate replaces the current choice only if it is at least 1 %
+     * faster: closer shapes differ by less than the run-to-run noise */
+    constexpr double kSwitch = 0.99;
     try {
         if (can_rebuild) rebuild(cur.bvh_width);
         double cur_ms = measure(cur);
@@ -184,7 +187,7 @@ lx_status lx_autotune(lx_context* ctx, const lx_job* jobs, uint32_t n, const cha
                     lx_tuning c = cur;
                     c.bvh_width = w;
                     double ms = measure(c);
-                    if (ms < cur_ms) {
+                    if (ms < cur_ms * kSwitch) {
                         cur_ms = ms;
                         cur = c;
                     }
@@ -197,7 +200,7 @@ lx_status lx_autotune(lx_context* ctx, const lx_job* jobs, uint32_t n, const cha
                     lx_tuning c = cur;
                     c.*(k.field) = v;
                     double ms = measure(c);
-                    if (ms < cur_ms) {
+                    if (ms < cur_ms * kSwitch) {
                         cur_ms = ms;
                         cur = c;
                     }
