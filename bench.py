@@ -28,6 +28,13 @@ RAY_BYTES, HIT_BYTES, OCC_BYTES = 32, 16, 4
 # index triples, normals and UVs, the material, the next ray and two shadow
 # rays, one queue word (texels not counted)
 SHADE_BYTES = 2 * 48 + 3 * 16 + 12 + 36 + 24 + 64 + 32 + 2 * 32 + 4
+# launch shapes for this workload, applied unless --autotune 1: the fastest
+# measured for the 1080p frame with overlapped rounds (tail kernel for every
+# bounce, 256-lane blocks, 16 waves per CU, ray-cell queue sort, 16x16 camera
+# squares in bands of 8 square rows); fixed so that runs, and the profiles of
+# the same command, time the same kernels
+WORKLOAD_TUNING = dict(wavefront_bounces=0, block_size=256, waves_per_cu=16, queue_chunk=32, tail_min_waves=4,
+                       bvh_width=4, sort_mode=1, pixel_tile=16, tile_band=8, overlap_rounds=1)
 
 
 def measured_traffic(kernel):
@@ -65,8 +72,9 @@ def main():
     ap.add_argument("--served-jobs", type=int, default=48, help="jobs per arrival rate")
     ap.add_argument("--wavefront-bounces", type=int, default=None,
                     help="bounces launched one kernel each before the tail kernel (default: the context's)")
-    ap.add_argument("--autotune", type=int, default=1,
-                    help="run the library's autotuner on this workload (8-spp job) before timing")
+    ap.add_argument("--autotune", type=int, default=0,
+                    help="1: run the library's autotuner on this workload (8-spp job) before timing instead "
+                         "of applying WORKLOAD_TUNING (its picks vary between shapes within the noise)")
     ap.add_argument("--set", action="append", default=[], metavar="KNOB=VALUE",
                     help="set a tuning knob (lx_tuning field) after autotuning, e.g. --set sort_mode=1")
     ap.add_argument("--copy-calibration", type=int, default=0,
@@ -105,6 +113,8 @@ def main():
         t0 = time.time()
         best = ctx.autotune([lx.make_job(sc, data.camera, 8, emis=True)])
         log(f"autotuned in {time.time() - t0:.1f} s: {best}")
+    else:
+        ctx.set_tuning(**WORKLOAD_TUNING)
     if args.wavefront_bounces is not None:
         ctx.set_tuning(wavefront_bounces=args.wavefront_bounces)
     for kv in args.set:
