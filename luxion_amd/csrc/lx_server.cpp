@@ -108,10 +108,13 @@ struct lx_server {
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
     /* overlapped rounds (tuning.overlap_rounds at start): round r runs on
-     * par[r % 2]'s stream with its own job table, counters, paths and colour
-     * pool; only the sampled jobs' pixel ranges of the pool are filled with
-     * -0.0 and then added into d_accum, round after round, on the context's
-     * stream */
+     * par[r % kDepth]'s stream with its own job table, counters, paths and
+     * colour pool; only the sampled jobs' pixel ranges of the pool are filled
+     * with -0.0 and then added into d_accum, round after round, on the
+     * context's stream.  Up to kDepth rounds are in flight: a served round is
+     * small and latency-bound (a few jobs' pixels), so more than two rounds
+     * share the GPU at low load */
+    static constexpr int kDepth = 3;
     bool overlap = false;
     struct Par {
         DevBuf jobs, spp, cnt, sched, spill, col, rng;
@@ -122,10 +125,11 @@ struct lx_server {
         std::vector<SrvJob*> in_round;
         uint32_t nr = 0;
         int64_t round = -1;
-        LxCounters last{};          /* counters as of this parity's last processed round */
+        double t_launch = 0.0;
+        LxCounters last{};          /* counters as of this set's last processed round */
         hipStream_t s = nullptr;
         hipEvent_t done = nullptr, merged = nullptr;
-    } par[2];
+    } par[kDepth];
     /* stats */
     std::vector<double> lat;
     lx_server_stats st{};
@@ -288,22 +292,24 @@ void lx_server::loop() {
 
 extern "C" {
 
-/* The request loop with overlapped rounds: round r is launched, then round
- * r - 1 (on the other stream) is waited for and its jobs' samples counted,
- * results copied and slots released, so the host work and round r - 1's last
- * long paths both overlap round r.  A job's samples are added into its pool
- * range in round order (one sample per round), so its image is the bits of
- * the loop without overlap. */
+/* The request loop with overlapped rounds: round r is launched while up to
+ * kDepth - 1 earlier rounds (on the other streams) still run; once kDepth
+ * rounds are in flight, or nothing is left to launch, the oldest is waited
+ * for and its jobs' samples counted, results copied and slots released, so
+ * the host work and the earlier rounds' last long paths overlap the newer
+ * rounds.  A job's samples are added into its pool range in round order (one
+ * sample per round), so its image is the bits of the loop without overlap. */
 void lx_server::loop_overlapped() {
-    int64_t r = 0;  /* next round */
-    int pend = -1;  /* parity of the launched round not yet processed, or -1 */
+    int64_t r = 0;                /* next round */
+    std::deque<int64_t> inflight; /* launched rounds not yet processed, oldest first */
+    double t_done = 0.0;          /* when the last processed round was found complete */
     while (true) {
         std::vector<int> fresh; /* slots admitted now: their pool range starts at zero */
         bool work = false;
         {
             std::unique_lock<std::mutex> lk(m);
             double t = now_ms();
-            const int64_t oldest = pend >= 0 ? par[pend].round : r; /* first round not yet processed */
+            const int64_t oldest = inflight.empty() ? r : inflight.front(); /* first round not yet processed */
             /* cancellations and deadlines of active jobs: no more samples; the slot
              * is free once no launched round holds the job */
             for (uint32_t k = 0; k < slots; k++) {
@@ -356,7 +362,7 @@ void lx_server::loop_overlapped() {
                 if (!j->stopped && j->launched < (uint32_t)j->job.spp) work = true;
             }
             st.peak_jobs_in_flight = std::max(st.peak_jobs_in_flight, nact);
-            if (!work && pend < 0) {
+            if (!work && inflight.empty()) {
                 if (stop && queue.empty()) break;
                 cv.wait_for(lk, std::chrono::milliseconds(50));
                 continue;
@@ -366,10 +372,11 @@ void lx_server::loop_overlapped() {
             std::lock_guard<std::mutex> dl(ctx->mu);
             ctx->set_device();
             hipStream_t ms = ctx->stream;
-            double t_launch = now_ms();
-            if (work) {
-                int p = (int)(r & 1);
-                Par& P = par[p];
+            bool launched = false;
+            if (work && inflight.size() < (size_t)kDepth) {
+                /* par[r % kDepth] is free: its previous round (r - kDepth) was processed */
+                Par& P = par[r % kDepth];
+                P.t_launch = now_ms();
                 for (int k : fresh) {
                     SrvJob* j = slot_job[k];
                     LX_HIP(lx_launch_fill_u32(d_accum.as<uint32_t>() + 3 * (size_t)j->accum_off, 3 * (size_t)j->npix,
@@ -411,19 +418,22 @@ void lx_server::loop_overlapped() {
                 LX_HIP(lx_launch_add_ranges(d_accum.as<float>(), P.col.as<float>(), P.rng.as<uint32_t>(), P.nr, ms));
                 LX_HIP(hipEventRecord(P.merged, ms));
                 P.round = r;
+                inflight.push_back(r);
                 r++;
+                launched = true;
             }
-            if (pend >= 0) {
-                Par& Q = par[pend];
+            if (!inflight.empty() && (inflight.size() >= (size_t)kDepth || !launched)) {
+                Par& Q = par[inflight.front() % kDepth];
                 LX_HIP(hipEventSynchronize(Q.merged));
                 LX_HIP(hipMemcpy(&Q.last, Q.cnt.ptr, sizeof(LxCounters), hipMemcpyDeviceToHost));
                 LX_REQUIRE(Q.last.stack_overflow == 0, LX_ERR_STATE, "traversal stack overflow");
                 std::lock_guard<std::mutex> lk(m);
-                st.rays = par[0].last.rad_rays + par[0].last.dl_rays + par[0].last.env_rays + par[1].last.rad_rays +
-                          par[1].last.dl_rays + par[1].last.env_rays;
+                st.rays = 0;
+                for (const Par& P : par) st.rays += P.last.rad_rays + P.last.dl_rays + P.last.env_rays;
                 st.rounds++;
                 double t = now_ms();
-                st.busy_ms += t - t_launch;
+                st.busy_ms += t - std::max(Q.t_launch, t_done); /* device time not counted twice */
+                t_done = t;
                 for (SrvJob* j : Q.in_round) {
                     j->merged++;
                     if (!j->stopped) j->res->spp_done = j->merged;
@@ -441,9 +451,8 @@ void lx_server::loop_overlapped() {
                     }
                 }
                 Q.in_round.clear();
-                pend = -1;
+                inflight.pop_front();
             }
-            if (work) pend = (int)((r - 1) & 1);
         } catch (const LxError& e) {
             /* a device failure ends every in-flight job with its status */
             (void)hipDeviceSynchronize();
@@ -454,7 +463,7 @@ void lx_server::loop_overlapped() {
                     if (!j->stopped) finish_job(this, j, e.status, t);
                     slot_job[k] = nullptr;
                 }
-            pend = -1;
+            inflight.clear();
             stop = true;
         }
         done_cv.notify_all();
@@ -486,8 +495,9 @@ lx_status lx_server_start(lx_context* ctx, const lx_server_opts* o, lx_server** 
         uint64_t avail = ctx->budget > ctx->in_use.load() ? ctx->budget - ctx->in_use.load() : 0;
         uint64_t budget = opts.hbm_budget_bytes ? std::min<uint64_t>(opts.hbm_budget_bytes, avail) : avail;
         /* split the budget: path capacity and accumulation pool (overlapped
-         * rounds: two sets of path buffers, the pool and two colour pools) */
-        const uint64_t sets = S->overlap ? 2 : 1, pools = S->overlap ? 3 : 1;
+         * rounds: a set of path buffers and a colour pool per round in flight,
+         * besides the accumulation pool) */
+        const uint64_t sets = S->overlap ? lx_server::kDepth : 1, pools = S->overlap ? lx_server::kDepth + 1 : 1;
         uint64_t per_path = 2 * 15 * 4 + 24 * 4;
         uint64_t cap = opts.max_paths_per_round ? opts.max_paths_per_round
                                                 : std::min<uint64_t>(budget / 2 / per_path / sets, 16u << 20);
