@@ -114,7 +114,7 @@ struct lx_server {
      * context's stream.  Up to kDepth rounds are in flight: a served round is
      * small and latency-bound (a few jobs' pixels), so more than two rounds
      * share the GPU at low load */
-    static constexpr int kDepth = 3;
+    static constexpr int kDepth = 4;
     bool overlap = false;
     struct Par {
         DevBuf jobs, spp, cnt, sched, spill, col, rng;
