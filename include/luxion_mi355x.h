@@ -411,12 +411,15 @@ typedef struct lx_tuning {
     int32_t tile_band;         /* squares follow each other in bands of this many
                                   square rows, column by column (0 = row by row;
                                   0..64; same bits) */
-    int32_t overlap_rounds;    /* 1 (default): lx_render_iterate of two or more
-                                  samples without timing runs consecutive rounds on
-                                  two streams, so one round's last long paths
-                                  overlap the next round's start; each round's
-                                  colours go to a buffer of their own, added to
-                                  the accumulation in round order (same bits).
+    int32_t overlap_rounds;    /* 1..3 (default 1): lx_render_iterate of two or more
+                                  samples without timing keeps this many rounds in
+                                  flight besides the current one, each on a stream
+                                  of its own, so one round's last long paths
+                                  overlap the next rounds; each round's colours go
+                                  to a buffer of their own, added to the
+                                  accumulation in round order (same bits).  The
+                                  batched form overlaps pairs of rounds and a
+                                  server four rounds whenever this is nonzero.
                                   0: one round after another on the render's
                                   stream (captured in a hipGraph with use_graph) */
 } lx_tuning;

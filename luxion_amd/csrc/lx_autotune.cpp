@@ -123,7 +123,7 @@ void check_tuning(const lx_tuning& t) {
     LX_REQUIRE(t.pixel_tile == 1 || t.pixel_tile == 8 || t.pixel_tile == 16 || t.pixel_tile == 32, LX_ERR_INVALID,
                "pixel_tile must be 1, 8, 16 or 32");
     LX_REQUIRE(t.tile_band >= 0 && t.tile_band <= 64, LX_ERR_INVALID, "tile_band must be 0..64");
-    LX_REQUIRE(t.overlap_rounds == 0 || t.overlap_rounds == 1, LX_ERR_INVALID, "overlap_rounds must be 0 or 1");
+    LX_REQUIRE(t.overlap_rounds >= 0 && t.overlap_rounds <= 3, LX_ERR_INVALID, "overlap_rounds must be 0..3");
 }
 
 } // namespace

@@ -191,7 +191,7 @@ extern "C" lx_status lx_batch_run(lx_context* ctx, const lx_job* jobs, uint32_t 
                 hsp[p].resize(n);
                 for (uint32_t i = 0; i < n; i++) {
                     hjp[p][i].iter = hj[i].iter - 1 + p;
-                    hjp[p][i].flags |= LX_JF_ITER2;
+                    hjp[p][i].flags |= 2u << LX_JF_STEP_SHIFT;
                     hsp[p][i] = (hspp[i] + 1 - p) / 2;
                 }
             }

@@ -375,79 +375,84 @@ static void enqueue(lx_render* r, hipEvent_t* ev) {
                      r->scene->d_scene.as<LxDevScene>(), ev, r->counting);
 }
 
-/* Buffers and streams of the overlapped form, made on first use.  Returns
- * false (nothing kept) when the HBM budget cannot hold them: the render then
- * runs its rounds one after another. */
-static bool overlap_setup(lx_render* r) {
-    if (r->overlap_ready) return true;
+/* Buffers and streams for `rounds` rounds in flight, made on first use.
+ * Returns false (nothing new kept) when the HBM budget cannot hold them: the
+ * render then runs its rounds one after another. */
+static bool overlap_setup(lx_render* r, int rounds) {
     lx_context* ctx = r->ctx;
-    auto& O = r->odd;
-    try {
-        O.wb.alloc(ctx, r->npix, r->opts.dmis != 0, r->opts.emis != 0, scene_has_normal_maps(r->scene));
-        O.jobs.alloc(ctx, sizeof(LxJob));
-        O.job_spp.alloc(ctx, 16);
-        O.counters.alloc(ctx, sizeof(LxCounters));
-        O.sched.alloc(ctx, sizeof(LxSched));
-        O.spill.alloc(ctx, r->spill.bytes);
-        for (int p = 0; p < 2; p++) r->color[p].alloc(ctx, (size_t)r->npix * 12);
-    } catch (const LxError& e) {
-        if (e.status != LX_ERR_BUDGET) throw;
-        O.wb = WaveBufs();
-        O.jobs.release();
-        O.job_spp.release();
-        O.counters.release();
-        O.sched.release();
-        O.spill.release();
-        for (int p = 0; p < 2; p++) r->color[p].release();
-        return false;
+    int have = r->sets_ready;
+    if (have < rounds) {
+        try {
+            for (int p = have; p < rounds; p++) {
+                auto& O = r->extra[p - 1];
+                O.wb.alloc(ctx, r->npix, r->opts.dmis != 0, r->opts.emis != 0, scene_has_normal_maps(r->scene));
+                O.jobs.alloc(ctx, sizeof(LxJob));
+                O.job_spp.alloc(ctx, 16);
+                O.counters.alloc(ctx, sizeof(LxCounters));
+                O.sched.alloc(ctx, sizeof(LxSched));
+                O.spill.alloc(ctx, r->spill.bytes);
+            }
+            for (int p = 0; p < rounds; p++)
+                if (!r->color[p].ptr) r->color[p].alloc(ctx, (size_t)r->npix * 12);
+        } catch (const LxError& e) {
+            if (e.status != LX_ERR_BUDGET) throw;
+            for (int p = have; p < rounds; p++) {
+                r->extra[p - 1] = lx_render::RoundSet();
+                r->color[p].release();
+            }
+            if (have == 1) r->color[0].release();
+            return false;
+        }
+        for (int p = have; p < rounds; p++)
+            LX_HIP(lx_launch_fill_u32(r->extra[p - 1].counters.as<uint32_t>(), sizeof(LxCounters) / 4, 0u,
+                                      ctx->stream));
+        r->sets_ready = rounds;
     }
-    LX_HIP(lx_launch_fill_u32(O.counters.as<uint32_t>(), sizeof(LxCounters) / 4, 0u, ctx->stream));
-    for (int p = 0; p < 2; p++) {
+    for (int p = 0; p < rounds; p++) {
         if (!r->pstream[p]) LX_HIP(hipStreamCreateWithFlags(&r->pstream[p], hipStreamNonBlocking));
         if (!r->ev_done[p]) LX_HIP(hipEventCreateWithFlags(&r->ev_done[p], hipEventDisableTiming));
         if (!r->ev_free[p]) LX_HIP(hipEventCreateWithFlags(&r->ev_free[p], hipEventDisableTiming));
     }
     if (!r->ev_up) LX_HIP(hipEventCreateWithFlags(&r->ev_up, hipEventDisableTiming));
-    r->overlap_ready = true;
     return true;
 }
 
-/* n >= 2 rounds, round k on stream k % 2 with that parity's buffers; the job
- * copy of parity p starts at iteration first_iter + p and steps by 2.  Round
- * k's colours are added to the accumulation on the context's stream after
- * round k - 1's, and round k + 2 waits for that add before it reuses the
- * colour buffer. */
-static void iterate_overlapped(lx_render* r, uint32_t first_iter, uint32_t n) {
+/* n >= 2 rounds, D = min(rounds, n) in flight: round k on stream k % D with
+ * set k % D; the job copy of set p starts at iteration first_iter + p and
+ * steps by D.  Round k's colours are added to the accumulation on the
+ * context's stream after round k - 1's, and round k + D waits for that add
+ * before it reuses the colour buffer. */
+static void iterate_overlapped(lx_render* r, uint32_t first_iter, uint32_t n, int rounds) {
     lx_context* ctx = r->ctx;
     hipStream_t ms = ctx->stream;
-    LxJob* djob[2] = {r->jobs.as<LxJob>(), r->odd.jobs.as<LxJob>()};
-    int32_t* dspp[2] = {r->job_spp.as<int32_t>(), r->odd.job_spp.as<int32_t>()};
-    LxCounters* dcnt[2] = {r->counters.as<LxCounters>(), r->odd.counters.as<LxCounters>()};
-    LxSched* dsch[2] = {r->sched.as<LxSched>(), r->odd.sched.as<LxSched>()};
-    uint32_t* dspill[2] = {r->spill.as<uint32_t>(), r->odd.spill.as<uint32_t>()};
-    WaveBufs* wb[2] = {&r->wb, &r->odd.wb};
-    for (int p = 0; p < 2; p++) {
+    const int D = (int)std::min<uint32_t>((uint32_t)rounds, n);
+    auto set_of = [&](int p) -> lx_render::RoundSet* { return p == 0 ? nullptr : &r->extra[p - 1]; };
+    for (int p = 0; p < D; p++) {
+        lx_render::RoundSet* O = set_of(p);
         /* sources of async copies: members of the render, untouched until the next call */
         LxJob& J = r->ov_job[p];
         J = r->job;
-        J.iter = (int32_t)first_iter - 2 + p;
+        J.iter = (int32_t)first_iter - D + p;
         J.spp_done = 0;
         J.active = 0;
-        J.flags |= LX_JF_ITER2;
-        r->ov_spp[p] = (int32_t)((n + 1 - (uint32_t)p) / 2); /* rounds of this parity */
-        LX_HIP(hipMemcpyAsync(djob[p], &J, sizeof(LxJob), hipMemcpyHostToDevice, ms));
-        LX_HIP(hipMemcpyAsync(dspp[p], &r->ov_spp[p], 4, hipMemcpyHostToDevice, ms));
+        J.flags = (J.flags & ~LX_JF_STEP_MASK) | ((uint32_t)D << LX_JF_STEP_SHIFT);
+        r->ov_spp[p] = (int32_t)((n - (uint32_t)p + (uint32_t)D - 1) / (uint32_t)D); /* rounds of this set */
+        LX_HIP(hipMemcpyAsync(O ? O->jobs.ptr : r->jobs.ptr, &J, sizeof(LxJob), hipMemcpyHostToDevice, ms));
+        LX_HIP(hipMemcpyAsync(O ? O->job_spp.ptr : r->job_spp.ptr, &r->ov_spp[p], 4, hipMemcpyHostToDevice, ms));
     }
     LX_HIP(hipEventRecord(r->ev_up, ms));
-    for (int p = 0; p < 2; p++) LX_HIP(hipStreamWaitEvent(r->pstream[p], r->ev_up, 0));
+    for (int p = 0; p < D; p++) LX_HIP(hipStreamWaitEvent(r->pstream[p], r->ev_up, 0));
     const size_t nf = (size_t)r->npix * 3;
     for (uint32_t k = 0; k < n; k++) {
-        int p = (int)(k & 1u);
+        int p = (int)(k % (uint32_t)D);
+        lx_render::RoundSet* O = set_of(p);
         hipStream_t s = r->pstream[p];
-        if (k >= 2) LX_HIP(hipStreamWaitEvent(s, r->ev_free[p], 0));
+        if (k >= (uint32_t)D) LX_HIP(hipStreamWaitEvent(s, r->ev_free[p], 0));
         LX_HIP(lx_launch_fill_u32(r->color[p].as<uint32_t>(), nf, 0x80000000u, s));
-        lx_enqueue_round(ctx, djob[p], dspp[p], 1, dcnt[p], dsch[p], r->npix, *wb[p], r->color[p].as<float>(),
-                         dspill[p], r->depth, r->trace_grid, r->shade_grid, r->cam_grid, r->scene->dev.bvh_width,
+        lx_enqueue_round(ctx, (O ? O->jobs : r->jobs).as<LxJob>(), (O ? O->job_spp : r->job_spp).as<int32_t>(), 1,
+                         (O ? O->counters : r->counters).as<LxCounters>(), (O ? O->sched : r->sched).as<LxSched>(),
+                         r->npix, O ? O->wb : r->wb, r->color[p].as<float>(), (O ? O->spill : r->spill).as<uint32_t>(),
+                         r->depth, r->trace_grid, r->shade_grid, r->cam_grid, r->scene->dev.bvh_width,
                          r->scene->d_scene.as<LxDevScene>(), nullptr, false, s);
         LX_HIP(hipEventRecord(r->ev_done[p], s));
         LX_HIP(hipStreamWaitEvent(ms, r->ev_done[p], 0));
@@ -456,13 +461,14 @@ static void iterate_overlapped(lx_render* r, uint32_t first_iter, uint32_t n) {
     }
 }
 
-/* Both parities' counters (the odd set's only once it exists). */
+/* Every round set's counters. */
 static void read_counters(lx_render* r, LxCounters& c) {
     LX_HIP(hipMemcpy(&c, r->counters.ptr, sizeof(c), hipMemcpyDeviceToHost));
-    if (!r->overlap_ready) return;
-    LxCounters o;
-    LX_HIP(hipMemcpy(&o, r->odd.counters.ptr, sizeof(o), hipMemcpyDeviceToHost));
-    lx_counters_add(c, o);
+    for (int p = 1; p < r->sets_ready; p++) {
+        LxCounters o;
+        LX_HIP(hipMemcpy(&o, r->extra[p - 1].counters.ptr, sizeof(o), hipMemcpyDeviceToHost));
+        lx_counters_add(c, o);
+    }
 }
 
 lx_status lx_render_iterate(lx_render* r, uint32_t first_iter, uint32_t n) {
@@ -473,8 +479,9 @@ lx_status lx_render_iterate(lx_render* r, uint32_t first_iter, uint32_t n) {
     lx_context* ctx = r->ctx;
     std::lock_guard<std::mutex> lk(ctx->mu);
     ctx->set_device();
-    if (n >= 2 && ctx->tuning.overlap_rounds && !r->timing && !r->counting && overlap_setup(r)) {
-        iterate_overlapped(r, first_iter, n);
+    const int rounds = 1 + ctx->tuning.overlap_rounds;
+    if (n >= 2 && rounds >= 2 && !r->timing && !r->counting && overlap_setup(r, rounds)) {
+        iterate_overlapped(r, first_iter, n, rounds);
         r->spp += n;
         return LX_OK;
     }
@@ -556,12 +563,15 @@ lx_status lx_render_sync(lx_render* r) {
     LX_REQUIRE(r, LX_ERR_INVALID, "null render");
     r->ctx->set_device();
     LX_HIP(hipStreamSynchronize(r->ctx->stream));
-    uint32_t ov = 0, ov1 = 0;
+    uint32_t ov = 0;
     LX_HIP(hipMemcpy(&ov, (uint8_t*)r->counters.ptr + offsetof(LxCounters, stack_overflow), 4, hipMemcpyDeviceToHost));
-    if (r->overlap_ready)
-        LX_HIP(hipMemcpy(&ov1, (uint8_t*)r->odd.counters.ptr + offsetof(LxCounters, stack_overflow), 4,
+    for (int p = 1; p < r->sets_ready; p++) {
+        uint32_t o = 0;
+        LX_HIP(hipMemcpy(&o, (uint8_t*)r->extra[p - 1].counters.ptr + offsetof(LxCounters, stack_overflow), 4,
                          hipMemcpyDeviceToHost));
-    LX_REQUIRE((ov | ov1) == 0, LX_ERR_STATE, "traversal stack overflow (BVH deeper than the supported stack)");
+        ov |= o;
+    }
+    LX_REQUIRE(ov == 0, LX_ERR_STATE, "traversal stack overflow (BVH deeper than the supported stack)");
     LX_CATCH
 }
 
@@ -598,8 +608,9 @@ lx_status lx_render_reset(lx_render* r) {
     r->ctx->set_device();
     LX_HIP(lx_launch_fill_u32(r->accum.as<uint32_t>(), (size_t)r->npix * 3, 0u, r->ctx->stream));
     LX_HIP(lx_launch_fill_u32(r->counters.as<uint32_t>(), sizeof(LxCounters) / 4, 0u, r->ctx->stream));
-    if (r->overlap_ready)
-        LX_HIP(lx_launch_fill_u32(r->odd.counters.as<uint32_t>(), sizeof(LxCounters) / 4, 0u, r->ctx->stream));
+    for (int p = 1; p < r->sets_ready; p++)
+        LX_HIP(lx_launch_fill_u32(r->extra[p - 1].counters.as<uint32_t>(), sizeof(LxCounters) / 4, 0u,
+                                  r->ctx->stream));
     LX_HIP(hipStreamSynchronize(r->ctx->stream));
     r->spp = 0;
     r->trace_ms = r->shade_ms = r->other_ms = r->tail_ms = 0;
@@ -660,7 +671,7 @@ void lx_render_destroy(lx_render* r) {
     (void)hipStreamSynchronize(r->ctx->stream);
     if (r->graph) (void)hipGraphExecDestroy(r->graph);
     if (r->graph_src) (void)hipGraphDestroy(r->graph_src);
-    for (int p = 0; p < 2; p++) {
+    for (int p = 0; p < lx_render::kMaxRounds; p++) {
         if (r->pstream[p]) (void)hipStreamDestroy(r->pstream[p]);
         if (r->ev_done[p]) (void)hipEventDestroy(r->ev_done[p]);
         if (r->ev_free[p]) (void)hipEventDestroy(r->ev_free[p]);

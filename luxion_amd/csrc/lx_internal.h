@@ -224,21 +224,23 @@ struct lx_render {
     hipGraph_t graph_src = nullptr;
     lx_tuning graph_tuning{}; /* the tuning the graph was captured with */
 
-    /* overlapped rounds (tuning.overlap_rounds): rounds alternate between two
-     * streams; round parity 0 uses the buffers above, parity 1 `odd`; each
-     * parity's colours go to color[p] (filled with -0.0, so the one add per
-     * pixel copies the path's colour bits), added to accum in round order on
-     * the context's stream */
-    struct OddSet {
+    /* overlapped rounds (tuning.overlap_rounds + 1 rounds in flight, at most
+     * kMaxRounds): round k runs on pstream[k % D] with set k % D -- set 0 is
+     * the buffers above, set p > 0 is extra[p - 1]; its colours go to
+     * color[k % D] (filled with -0.0, so the one add per pixel copies the
+     * path's colour bits), added to accum in round order on the context's
+     * stream */
+    static constexpr int kMaxRounds = 4;
+    struct RoundSet {
         WaveBufs wb;
         DevBuf jobs, job_spp, counters, sched, spill;
-    } odd;
-    LxJob ov_job[2]{};
-    int32_t ov_spp[2] = {0, 0};
-    DevBuf color[2];
-    hipStream_t pstream[2] = {nullptr, nullptr};
-    hipEvent_t ev_up = nullptr, ev_done[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
-    bool overlap_ready = false;
+    } extra[kMaxRounds - 1];
+    int sets_ready = 1; /* round sets allocated (set 0 always) */
+    LxJob ov_job[kMaxRounds]{};
+    int32_t ov_spp[kMaxRounds] = {};
+    DevBuf color[kMaxRounds];
+    hipStream_t pstream[kMaxRounds] = {};
+    hipEvent_t ev_up = nullptr, ev_done[kMaxRounds] = {}, ev_free[kMaxRounds] = {};
 };
 
 /* Records one round (begin, camera, bounces, end) on the stream. */

@@ -52,7 +52,8 @@ __global__ void k_begin_round(LxJob* __restrict__ jobs, const int32_t* __restric
             if (want && na < LX_SCHED_MAX && off + J.num_pix <= path_cap) {
                 J.active = 1;
                 J.path_off = off;
-                J.iter = J.iter + ((J.flags & LX_JF_ITER2) ? 2 : 1);
+                uint32_t step = (J.flags & LX_JF_STEP_MASK) >> LX_JF_STEP_SHIFT;
+                J.iter = J.iter + (step ? (int32_t)step : 1);
                 sched->idx[na] = j;
                 sched->off[na] = off;
                 na++;

@@ -131,7 +131,10 @@ struct LxJob {
 };
 #define LX_JF_DEBUG_MASK 7u /* LX_DEBUG_* view, 0 = off */
 #define LX_JF_LEGACY 8u     /* LX_SHADING_LEGACY */
-#define LX_JF_ITER2 16u     /* rounds of this job alternate with another copy of it: iter += 2 */
+/* bits 8..15: the job's rounds alternate with this many copies of it in all
+ * (rounds in flight); each of its rounds then advances iter by that step */
+#define LX_JF_STEP_SHIFT 8u
+#define LX_JF_STEP_MASK 0xff00u
 
 /* Path state (struct of arrays, capacity N). */
 struct LxPathBuf {

@@ -30,16 +30,36 @@ def _render(ctx, sc, kw, calls, overlap, use_graph=1, wf=0):
     return img, spp, {k: st[k] for k in ("radiance_rays", "dl_shadow_rays", "env_shadow_rays", "path_bounces")}
 
 
-@pytest.mark.parametrize("calls", [[(1, 2)], [(1, 5)], [(1, 3), (4, 4)], [(1, 1), (2, 2), (4, 3)]])
+@pytest.mark.parametrize("calls", [[(1, 2)], [(1, 5)], [(1, 3), (4, 4)], [(1, 1), (2, 2), (4, 3)], [(1, 7), (8, 2)]])
 def test_overlapped_equals_sequential(tuned, calls):
     data = scenes.material_zoo(96, 64, level=3, env_size=(128, 64))
     sc = lx.Scene(tuned, data)
     for kw in (dict(emis=True), dict(dmis=True)):
-        a = _render(tuned, sc, kw, calls, overlap=1)
         b = _render(tuned, sc, kw, calls, overlap=0)
-        assert np.array_equal(a[0], b[0]), (calls, kw)
-        assert a[1] == b[1] == sum(n for _, n in calls)
-        assert a[2] == b[2], (calls, kw)
+        for depth in (1, 2, 3):
+            a = _render(tuned, sc, kw, calls, overlap=depth)
+            assert np.array_equal(a[0], b[0]), (calls, kw, depth)
+            assert a[1] == b[1] == sum(n for _, n in calls)
+            assert a[2] == b[2], (calls, kw, depth)
+
+
+def test_changing_depth_between_calls(tuned):
+    """A render continues one image across calls made with different numbers
+    of rounds in flight (round sets are added as the depth grows)."""
+    data = scenes.cornell(72, 64)
+    sc = lx.Scene(tuned, data)
+    r = lx.Render(sc, dmis=True)
+    first = 1
+    for depth, n in ((1, 3), (3, 5), (2, 2), (0, 1), (3, 4)):
+        tuned.set_tuning(overlap_rounds=depth)
+        r.iterate(first, n)
+        first += n
+    tuned.set_tuning(overlap_rounds=0)
+    ref = lx.Render(sc, dmis=True)
+    ref.iterate(1, first - 1)
+    assert r.accum()[1] == first - 1
+    assert np.array_equal(r.accum()[0], ref.accum()[0])
+    assert r.stats()["radiance_rays"] == ref.stats()["radiance_rays"]
 
 
 def test_overlapped_with_per_bounce_launches_and_eager(tuned):
@@ -89,7 +109,7 @@ def test_reset_between_overlapped_calls_and_mode_switch(tuned):
 
 def test_overlap_rounds_validated(tuned):
     with pytest.raises(lx.LuxionError, match="overlap_rounds"):
-        tuned.set_tuning(overlap_rounds=2)
+        tuned.set_tuning(overlap_rounds=4)
 
 
 def _batch(ctx, jobs, **tuning):
