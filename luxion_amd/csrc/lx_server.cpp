@@ -5,15 +5,18 @@
  * accumulation pool sized from the HBM budget.  Jobs submitted from any
  * thread are admitted FIFO whenever a slot and enough accumulation space are
  * free; every round takes one sample of every admitted job in one shared
- * launch per bounce (the round is a captured hipGraph over the whole slot
- * table; empty slots have a zero sample target and are never scheduled).
- * Finished jobs leave between rounds and their results are copied out, so
- * jobs that arrive while others run overlap with them on the device.
+ * launch per bounce.  With overlapped rounds (the default) up to four rounds
+ * are in flight on streams of their own, each adding its colours into the
+ * jobs' pool ranges in round order; otherwise the round is a captured hipGraph
+ * over the whole slot table (empty slots have a zero sample target and are
+ * never scheduled).  Finished jobs leave between rounds and their results are
+ * copied out, so jobs that arrive while others run overlap with them on the
+ * device.
  * A job's pixels use only its own RNG streams, so its image does not depend
  * on which other jobs share its rounds.
  *
- * The reference has no batch or serve mode (SURVEY: run_best_scenes.ps1 runs
- * one process per job, serially).
+ * The reference has no batch or serve mode (run_best_scenes.ps1 runs one
+ * process per job, serially).
  */
 #include "lx_internal.h"
 

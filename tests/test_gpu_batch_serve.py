@@ -129,7 +129,7 @@ def test_server_failure_handling_isolated(ctx, scenes_pair, server_mode):
         srv.stop()
 
 
-def test_workload_drivers_small(ctx, scenes_pair):
+def test_workload_runners_small(ctx, scenes_pair):
     from luxion_amd import workloads
     corn, ico = scenes_pair
     specs = [(ico, dict(emis=True), (0.0, 0.0, 0.0), 3.0), (corn, dict(dmis=True), (0.0, 1.0, 0.0), 3.4)]
