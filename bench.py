@@ -52,6 +52,24 @@ def measured_traffic(kernel):
     return None
 
 
+def measured_issue(kernel):
+    """VALU busy and lane-utilisation fractions of `kernel` on this workload,
+    from the same committed summary (lines 'VALU busy = ... = X %' and 'VALU
+    lane utilisation = ... = Y %'), or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_cfg1_1080p_emis.md")
+    out = {}
+    try:
+        with open(path) as f:
+            for line in f:
+                s = line.strip()
+                for key, tag in (("valu_busy", "VALU busy ="), ("lane_utilisation", "VALU lane utilisation =")):
+                    if tag in s and s.endswith("%") and key not in out:
+                        out[key] = round(float(s.rsplit("=", 1)[1].strip().rstrip("%").strip()) / 100.0, 3)
+    except (OSError, ValueError):
+        return None
+    return out or None
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -328,6 +346,9 @@ def main():
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
                      "bytes_per_launch": int(bytes_per_launch), "avg_launch_ms": round(launch_ms, 4),
+                     "issue": measured_issue(kname),
+                     "limiter": "VALU issue with lanes idle behind each wave's longest ray (traversal "
+                                "divergence), not HBM bandwidth: HBM traffic is under half the algorithmic bytes",
                      "per_ray": {k: round(v, 2) for k, v in per_ray.items()},
                      "overlapped_frame": {
                          "achieved": round(kbytes / args.steps / (t_timed / args.steps) / 1e9, 1),
