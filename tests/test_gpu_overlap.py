@@ -1,6 +1,6 @@
-"""Overlapped rounds (tuning overlap_rounds = 1): consecutive rounds of one
-render run on two streams, each round's colours in a buffer of its own,
-added to the accumulation in round order.  The image bits, ray counts and
+"""Overlapped rounds (tuning overlap_rounds = 1..3): consecutive rounds of one
+render run on 1 + overlap_rounds streams, each round's colours in a buffer of
+its own, added to the accumulation in round order.  The image bits, ray counts and
 sample counts equal one-round-at-a-time rendering, for odd and even sample
 counts, across calls, resets and mode switches, and against the oracle."""
 import numpy as np
