@@ -81,6 +81,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--accuracy", type=int, default=1,
+                    help="1: compare a 16-spp render of the workload with the fp32 oracle (bits) and the FP64 "
+                         "oracle (RMSE) on 4,096 seeded pixels")
     ap.add_argument("--level", type=int, default=8)
     ap.add_argument("--batched", type=int, default=256,
                     help="jobs in the batched-form mix on the same scene (0 = skip)")
@@ -229,6 +232,7 @@ def main():
     spp_s = W * H * args.steps / t_timed
 
     cpu = None
+    o = None
     if args.cpu_baseline:
         from oracle import lxo
         threads = min(16, os.cpu_count() or 1)
@@ -273,6 +277,38 @@ def main():
             "cpu_threads": threads, "cpu_ms": cc["wall_ms"],
             "gpu_mrays_per_s": (gs["radiance_rays"] + gs["dl_shadow_rays"]) / tg / 1e6, "gpu_ms": tg * 1e3}
         log(f"cpu baseline: one thread {cpu['single_thread_value']:.3f} Mrays/s; cornell {cpu['cornell_256_16spp_dmis']}")
+
+    accuracy = None
+    if args.accuracy:
+        # Same seeds: the GPU's 16-spp sums on 4,096 seeded pixels against the
+        # fp32 oracle (bit for bit) and the FP64 oracle (same random streams,
+        # double arithmetic); converged: both 16-spp means against an FP64
+        # estimate from 256 later, independent samples per pixel.
+        from oracle import lxo
+        threads = min(16, os.cpu_count() or 1)
+        if o is None:
+            o = lxo.OracleScene(data)
+        o64 = lxo.OracleScene(data, fp64=True)
+        spp, pix_n = 16, 4096
+        pix = np.sort(np.random.default_rng(1).choice(W * H, pix_n, replace=False)).astype(np.uint32)
+        ra = lx.Render(sc, emis=True)
+        ra.iterate(1, spp)
+        g = ra.accum()[0].reshape(-1, 3)[pix]
+        ra.close()
+        f32, _ = o.render(1, spp, pixels=pix, emis=True, threads=threads)
+        f64, _ = o64.render(1, spp, pixels=pix, emis=True, threads=threads)
+        conv, _ = o64.render(1 + spp, 16 * spp, pixels=pix, emis=True, threads=threads)
+        g64, f64d, convd = g.astype(np.float64) / spp, f64.astype(np.float64) / spp, conv.astype(np.float64) / (16 * spp)
+        mean = float(convd.mean())
+        rmse = lambda a, b: float(np.sqrt(np.mean((a - b) ** 2)))
+        accuracy = {
+            "pixels": pix_n, "spp": spp, "seeds": "reference streams (seed 0), iterations 1..16",
+            "bitexact_vs_fp32_oracle": bool(np.array_equal(g.view(np.uint32), f32.view(np.uint32))),
+            "rel_rmse_vs_fp64_oracle_same_seeds": rmse(g64, f64d) / mean,
+            "rel_rmse_vs_converged_fp64": rmse(g64, convd) / mean,
+            "fp64_oracle_rel_rmse_vs_converged_fp64": rmse(f64d, convd) / mean,
+            "converged_reference": "FP64 oracle, 256 spp per pixel (iterations 17..272)"}
+        log(f"accuracy: {accuracy}")
 
     batched = served = None
     if args.batched:
@@ -359,6 +395,7 @@ def main():
         "batched": batched,
         "served": served,
         "cpu_baseline": cpu,
+        "accuracy": accuracy,
     }
     print(json.dumps(out), flush=True)
 

@@ -1,7 +1,11 @@
-"""Python view of the CPU oracle (oracle/_build/liblxoracle.so).
+"""Python view of the CPU oracle (oracle/_build/liblxoracle.so) and of its
+double-precision build (oracle/_build/liblxoracle64.so: the same restatement
+and random streams with double arithmetic, the FP64 reference for
+converged-image error; it exports scenes, tracing and rendering).
 
 Test infrastructure only: imported by tests/, __graft_entry__.smoke() and the
-CPU-baseline leg of bench.py, as the checker.  The product never imports it.
+CPU-baseline and accuracy legs of bench.py, as the checker.  The product never
+imports it.
 """
 import ctypes as C
 import os
@@ -13,6 +17,7 @@ from luxion_amd import _ffi as F
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "_build", "liblxoracle.so")
+LIB64 = os.path.join(HERE, "_build", "liblxoracle64.so")
 
 
 def build():
@@ -32,7 +37,31 @@ class lxo_stats(C.Structure):
 
 
 _lib = None
+_lib64 = None
 P = C.c_void_p
+
+
+def _scene_api(L):
+    L.lxo_scene_create.restype = P
+    L.lxo_scene_create.argtypes = [C.POINTER(F.lx_scene_desc), C.c_int, C.c_int]
+    L.lxo_scene_destroy.argtypes = [P]
+    L.lxo_read_env_cdfs.argtypes = [P, P, P]
+    L.lxo_read_emitter_cdfs.argtypes = [P, P, P, C.POINTER(C.c_uint32), C.POINTER(C.c_float)]
+    L.lxo_read_instance_tri_cdf.argtypes = [P, C.c_uint32, P, C.POINTER(C.c_uint32)]
+    L.lxo_trace.argtypes = [P, P, C.c_uint32, C.c_int, C.c_int, C.c_uint32, P, P, P]
+    L.lxo_render.argtypes = [P, C.POINTER(lxo_render_opts), C.c_uint32, C.c_uint32, P, C.c_uint32, P,
+                             C.POINTER(lxo_stats)]
+    return L
+
+
+def lib64():
+    """The double-precision build (scenes, tracing, rendering)."""
+    global _lib64
+    if _lib64 is None:
+        if not os.path.exists(LIB64):
+            build()
+        _lib64 = _scene_api(C.CDLL(LIB64))
+    return _lib64
 
 
 def lib():
@@ -40,21 +69,12 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB):
             build()
-        L = C.CDLL(LIB)
-        L.lxo_scene_create.restype = P
-        L.lxo_scene_create.argtypes = [C.POINTER(F.lx_scene_desc), C.c_int, C.c_int]
-        L.lxo_scene_destroy.argtypes = [P]
+        L = _scene_api(C.CDLL(LIB))
         L.lxo_scene_bvh_build_ms.restype = C.c_double
         L.lxo_scene_bvh_build_ms.argtypes = [P]
         L.lxo_scene_bvh_nodes.restype = C.c_uint32
         L.lxo_scene_bvh_nodes.argtypes = [P, C.c_uint32]
         L.lxo_scene_bvh_depth.argtypes = [P, C.c_uint32]
-        L.lxo_read_env_cdfs.argtypes = [P, P, P]
-        L.lxo_read_emitter_cdfs.argtypes = [P, P, P, C.POINTER(C.c_uint32), C.POINTER(C.c_float)]
-        L.lxo_read_instance_tri_cdf.argtypes = [P, C.c_uint32, P, C.POINTER(C.c_uint32)]
-        L.lxo_trace.argtypes = [P, P, C.c_uint32, C.c_int, C.c_int, C.c_uint32, P, P, P]
-        L.lxo_render.argtypes = [P, C.POINTER(lxo_render_opts), C.c_uint32, C.c_uint32, P, C.c_uint32, P,
-                                 C.POINTER(lxo_stats)]
         L.lxo_debug_bounce.argtypes = [P, C.POINTER(lxo_render_opts), C.c_uint32, P, C.c_uint32, P, P, P, P, P]
         L.lxo_utilhash.restype = C.c_uint32
         L.lxo_utilhash.argtypes = [C.c_uint32]
@@ -109,37 +129,40 @@ def math_eval(fn, x, y=None):
 
 
 class OracleScene:
-    def __init__(self, data, cdf_mode=F.LX_CDF_FP64, build_bvh=True):
+    """A scene in the oracle; fp64=True uses the double-precision build."""
+
+    def __init__(self, data, cdf_mode=F.LX_CDF_FP64, build_bvh=True, fp64=False):
         self.data = data
+        self.L = lib64() if fp64 else lib()
         desc, keep = data.to_desc()
-        self.h = lib().lxo_scene_create(C.byref(desc), cdf_mode, int(build_bvh))
+        self.h = self.L.lxo_scene_create(C.byref(desc), cdf_mode, int(build_bvh))
         del keep
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().lxo_scene_destroy(self.h)
+            self.L.lxo_scene_destroy(self.h)
             self.h = None
 
     def env_cdfs(self):
         H, W = self.data.env.shape[:2]
         m = np.empty(H, np.float32)
         c = np.empty(H * W, np.float32)
-        lib().lxo_read_env_cdfs(self.h, m.ctypes.data, c.ctypes.data)
+        self.L.lxo_read_env_cdfs(self.h, m.ctypes.data, c.ctypes.data)
         return m, c
 
     def emitter_cdfs(self):
         n = C.c_uint32()
         tot = C.c_float()
-        lib().lxo_read_emitter_cdfs(self.h, None, None, C.byref(n), C.byref(tot))
+        self.L.lxo_read_emitter_cdfs(self.h, None, None, C.byref(n), C.byref(tot))
         inst = np.empty(n.value, np.uint32)
         pre = np.empty(n.value, np.float32)
-        lib().lxo_read_emitter_cdfs(self.h, inst.ctypes.data, pre.ctypes.data, C.byref(n), C.byref(tot))
+        self.L.lxo_read_emitter_cdfs(self.h, inst.ctypes.data, pre.ctypes.data, C.byref(n), C.byref(tot))
         tris = []
         for k in range(n.value):
             m = C.c_uint32()
-            lib().lxo_read_instance_tri_cdf(self.h, k, None, C.byref(m))
+            self.L.lxo_read_instance_tri_cdf(self.h, k, None, C.byref(m))
             t = np.empty(m.value, np.float32)
-            lib().lxo_read_instance_tri_cdf(self.h, k, t.ctypes.data, C.byref(m))
+            self.L.lxo_read_instance_tri_cdf(self.h, k, t.ctypes.data, C.byref(m))
             tris.append(t)
         return inst, pre, tris, tot.value
 
@@ -149,11 +172,11 @@ class OracleScene:
         pix = None if pixel is None else np.ascontiguousarray(pixel, np.uint32)
         if occluded:
             out = np.zeros(n, np.uint8)
-            lib().lxo_trace(self.h, rays.ctypes.data, n, mode, int(alpha), iteration,
+            self.L.lxo_trace(self.h, rays.ctypes.data, n, mode, int(alpha), iteration,
                             None if pix is None else pix.ctypes.data, None, out.ctypes.data)
             return out
         out = np.zeros(n, dtype=[("t", "<f4"), ("instance", "<u4"), ("prim", "<u4"), ("u", "<f4"), ("v", "<f4")])
-        lib().lxo_trace(self.h, rays.ctypes.data, n, mode, int(alpha), iteration,
+        self.L.lxo_trace(self.h, rays.ctypes.data, n, mode, int(alpha), iteration,
                         None if pix is None else pix.ctypes.data, out.ctypes.data, None)
         return out
 
@@ -176,7 +199,7 @@ class OracleScene:
         pix = None if pixels is None else np.ascontiguousarray(pixels, np.uint32)
         out = np.zeros((n, 3), np.float32)
         st = lxo_stats()
-        lib().lxo_render(self.h, C.byref(o), first_iter, n_iters, None if pix is None else pix.ctypes.data, n,
+        self.L.lxo_render(self.h, C.byref(o), first_iter, n_iters, None if pix is None else pix.ctypes.data, n,
                          out.ctypes.data, C.byref(st))
         return out, {k: getattr(st, k) for k, _ in lxo_stats._fields_}
 
@@ -190,7 +213,7 @@ class OracleScene:
         rec = np.zeros(D * n * 8, np.float32)
         rc = np.zeros(D, np.uint32)
         rgb = np.zeros((n, 3), np.float32)
-        lib().lxo_debug_bounce(self.h, C.byref(o), iteration, None if pix is None else pix.ctypes.data, n,
+        self.L.lxo_debug_bounce(self.h, C.byref(o), iteration, None if pix is None else pix.ctypes.data, n,
                                surv.ctypes.data, sc.ctypes.data, rec.ctypes.data, rc.ctypes.data, rgb.ctypes.data)
         survivors = [surv[d * n:d * n + sc[d]].copy() for d in range(D)]
         records = [rec[d * n * 8:(d * n + rc[d]) * 8].reshape(-1, 8).copy() for d in range(D)]

@@ -39,53 +39,96 @@
 #include <thread>
 #include <vector>
 #include <limits>
+#include <type_traits>
+
+/* Arithmetic type of the restatement: float (the oracle proper, bit-exact
+ * with the reference's fp32 sequences) or, built with -DLXO_FP64, double: the
+ * same algorithm, random streams and constants in double precision, with
+ * libm's double transcendentals, as the FP64 reference for converged-image
+ * error.  The random draws stay float (the reference's u01). */
+#ifdef LXO_FP64
+typedef double lxr;
+#define LXO_D2(name, expr)                                                                                             \
+    template <class A, class B>                                                                                        \
+    static inline typename std::enable_if<std::is_same<A, double>::value || std::is_same<B, double>::value,            \
+                                          double>::type                                                                \
+    name(A a_, B b_) {                                                                                                 \
+        double x = (double)a_, y = (double)b_;                                                                         \
+        return expr;                                                                                                   \
+    }
+LXO_D2(lxm_min, x < y ? x : y)
+LXO_D2(lxm_max, x > y ? x : y)
+LXO_D2(lxm_atan2, std::atan2(x, y))
+LXO_D2(lxm_pow, std::pow(x, y))
+#undef LXO_D2
+template <class A, class B, class C>
+static inline typename std::enable_if<std::is_same<A, double>::value || std::is_same<B, double>::value ||
+                                          std::is_same<C, double>::value,
+                                      double>::type
+lxm_clamp(A x, B lo, C hi) {
+    return lxm_min(lxm_max((double)x, (double)lo), (double)hi);
+}
+static inline double lxm_sqrt(double x) { return std::sqrt(x); }
+static inline double lxm_abs(double x) { return std::fabs(x); }
+static inline double lxm_floor(double x) { return std::floor(x); }
+static inline int lxm_isnan(double x) { return x != x; }
+static inline int lxm_isfinite(double x) { return std::isfinite(x) ? 1 : 0; }
+static inline double lxm_sin(double x) { return std::sin(x); }
+static inline double lxm_cos(double x) { return std::cos(x); }
+static inline double lxm_atan(double x) { return std::atan(x); }
+static inline double lxm_acos(double x) { return std::acos(x); }
+static inline double lxm_log2(double x) { return std::log2(x); }
+static inline double lxm_exp2(double x) { return std::exp2(x); }
+#else
+typedef float lxr;
+#endif
 
 namespace {
 
 /* ------------------------------------------------------------ vec3 (glm) */
 struct V3 {
-    float x, y, z;
+    lxr x, y, z;
 };
 struct V2 {
-    float x, y;
+    lxr x, y;
 };
-static inline V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+static inline V3 v3(lxr x, lxr y, lxr z) { return V3{x, y, z}; }
 static inline V3 add(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
 static inline V3 sub(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
 static inline V3 mul(V3 a, V3 b) { return v3(a.x * b.x, a.y * b.y, a.z * b.z); }
-static inline V3 muls(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
-static inline V3 smul(float s, V3 a) { return v3(s * a.x, s * a.y, s * a.z); }
-static inline V3 divs(V3 a, float s) { return v3(a.x / s, a.y / s, a.z / s); } /* glm vec / scalar */
+static inline V3 muls(V3 a, lxr s) { return v3(a.x * s, a.y * s, a.z * s); }
+static inline V3 smul(lxr s, V3 a) { return v3(s * a.x, s * a.y, s * a.z); }
+static inline V3 divs(V3 a, lxr s) { return v3(a.x / s, a.y / s, a.z / s); } /* glm vec / scalar */
 static inline V3 divv(V3 a, V3 b) { return v3(a.x / b.x, a.y / b.y, a.z / b.z); }
 static inline V3 neg(V3 a) { return v3(-a.x, -a.y, -a.z); }
 /* glm compute_dot<tvec3>: tmp = x * y; tmp.x + tmp.y + tmp.z */
-static inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline lxr dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 static inline V3 cross(V3 x, V3 y) {
     return v3(x.y * y.z - y.y * x.z, x.z * y.x - y.z * x.x, x.x * y.y - y.x * x.y);
 }
 /* glm normalize: x * inversesqrt(dot(x, x)), inversesqrt = 1 / sqrt */
 static inline V3 normalize(V3 a) { return muls(a, 1.0f / lxm_sqrt(dot(a, a))); }
-static inline float length(V3 a) { return lxm_sqrt(dot(a, a)); }
+static inline lxr length(V3 a) { return lxm_sqrt(dot(a, a)); }
 static inline V3 vmax0(V3 a) { return v3(lxm_max(a.x, 0.0f), lxm_max(a.y, 0.0f), lxm_max(a.z, 0.0f)); }
 
 /* 3x4 row-major affine map; point = (m0 x + m1 y) + (m2 z + m3) (glm mat4 *
  * vec4 pairwise order, glm/detail/type_mat4x4.inl operator*). */
-static inline V3 xform_point(const float* m, V3 p) {
+static inline V3 xform_point(const lxr* m, V3 p) {
     return v3((m[0] * p.x + m[1] * p.y) + (m[2] * p.z + m[3]), (m[4] * p.x + m[5] * p.y) + (m[6] * p.z + m[7]),
               (m[8] * p.x + m[9] * p.y) + (m[10] * p.z + m[11]));
 }
-static inline V3 xform_vector(const float* m, V3 v) {
+static inline V3 xform_vector(const lxr* m, V3 v) {
     return v3((m[0] * v.x + m[1] * v.y) + m[2] * v.z, (m[4] * v.x + m[5] * v.y) + m[6] * v.z,
               (m[8] * v.x + m[9] * v.y) + m[10] * v.z);
 }
 /* normal object->world by the transpose of the world->object map */
-static inline V3 xform_normal(const float* inv, V3 n) {
+static inline V3 xform_normal(const lxr* inv, V3 n) {
     return v3((inv[0] * n.x + inv[4] * n.y) + inv[8] * n.z, (inv[1] * n.x + inv[5] * n.y) + inv[9] * n.z,
               (inv[2] * n.x + inv[6] * n.y) + inv[10] * n.z);
 }
 
-/* Affine inverse in double, rounded once to float. */
-static void affine_inverse(const float* m, float* out) {
+/* Affine inverse in double, rounded once to the arithmetic type. */
+static void affine_inverse(const lxr* m, lxr* out) {
     double a = m[0], b = m[1], c = m[2], d = m[4], e = m[5], f = m[6], g = m[8], h = m[9], i = m[10];
     double A = e * i - f * h, B = -(d * i - f * g), C = d * h - e * g;
     double det = a * A + b * B + c * C;
@@ -101,10 +144,10 @@ static void affine_inverse(const float* m, float* out) {
                    (a * e - b * d) * inv_det};
     double tx = m[3], ty = m[7], tz = m[11];
     for (int row = 0; row < 3; row++) {
-        out[row * 4 + 0] = (float)r[row * 3 + 0];
-        out[row * 4 + 1] = (float)r[row * 3 + 1];
-        out[row * 4 + 2] = (float)r[row * 3 + 2];
-        out[row * 4 + 3] = (float)(-(r[row * 3 + 0] * tx + r[row * 3 + 1] * ty + r[row * 3 + 2] * tz));
+        out[row * 4 + 0] = (lxr)r[row * 3 + 0];
+        out[row * 4 + 1] = (lxr)r[row * 3 + 1];
+        out[row * 4 + 2] = (lxr)r[row * 3 + 2];
+        out[row * 4 + 3] = (lxr)(-(r[row * 3 + 0] * tx + r[row * 3 + 1] * ty + r[row * 3 + 2] * tz));
     }
 }
 
@@ -144,34 +187,34 @@ static inline Rng make_rng(int iter, int index, int depth, uint32_t seed) {
 }
 
 /* ------------------------------------------------------------- constants */
-static const float PI_F = 3.1415926535897932384626422832795028841971f; /* utilities.h:13 */
-static const float TWO_PI_F = 6.2831853071795864769252867665590057683943f;
-static const float SQRT_OF_ONE_THIRD = 0.5773502691896257645091487805019574556476f;
-static const float INV_PI_CT = 0.3183098f;                             /* cook_torrance.h:14 */
-static const float GLM_PI_F = (float)3.14159265358979323846264338327950288; /* glm::pi<float>() */
-static const float CT_EPS = 1e-30f;                                     /* cook_torrance.cu:3 */
-static const float PDF_CLAMP = 1e-5f;                                   /* cook_torrance.cu:4 */
+static const lxr PI_F = 3.1415926535897932384626422832795028841971f; /* utilities.h:13 */
+static const lxr TWO_PI_F = 6.2831853071795864769252867665590057683943f;
+static const lxr SQRT_OF_ONE_THIRD = 0.5773502691896257645091487805019574556476f;
+static const lxr INV_PI_CT = 0.3183098f;                             /* cook_torrance.h:14 */
+static const lxr GLM_PI_F = (lxr)3.14159265358979323846264338327950288; /* glm::pi<float>() */
+static const lxr CT_EPS = 1e-30f;                                     /* cook_torrance.cu:3 */
+static const lxr PDF_CLAMP = 1e-5f;                                   /* cook_torrance.cu:4 */
 static const double SHADE_M_PI = 3.14159;                               /* shade_path.cu:23 */
 
 /* -------------------------------------------------------------- textures */
 struct Tex {
     int w = 0, h = 0;
-    std::vector<float> rgba; /* decoded texels */
+    std::vector<lxr> rgba; /* decoded texels */
 };
 
 /* Exact-weight bilinear filter, wrap addressing, normalised coordinates: the
  * documented semantics of tex2D<float4> with cudaFilterModeLinear /
  * cudaAddressModeWrap (texture.cu:52-56), without the hardware's 8-bit weight
  * quantisation. */
-static inline void tex_sample(const Tex& t, float u, float v, float* out) {
+static inline void tex_sample(const Tex& t, lxr u, lxr v, lxr* out) {
     if (!lxm_isfinite(u)) u = 0.0f;
     if (!lxm_isfinite(v)) v = 0.0f;
     u = u - lxm_floor(u);
     v = v - lxm_floor(v);
-    float x = u * (float)t.w - 0.5f;
-    float y = v * (float)t.h - 0.5f;
-    float fx = lxm_floor(x), fy = lxm_floor(y);
-    float ax = x - fx, ay = y - fy;
+    lxr x = u * (lxr)t.w - 0.5f;
+    lxr y = v * (lxr)t.h - 0.5f;
+    lxr fx = lxm_floor(x), fy = lxm_floor(y);
+    lxr ax = x - fx, ay = y - fy;
     int i0 = (int)fx, j0 = (int)fy;
     if (i0 < 0) i0 += t.w;
     if (i0 >= t.w) i0 -= t.w;
@@ -179,14 +222,14 @@ static inline void tex_sample(const Tex& t, float u, float v, float* out) {
     if (j0 >= t.h) j0 -= t.h;
     int i1 = i0 + 1 == t.w ? 0 : i0 + 1;
     int j1 = j0 + 1 == t.h ? 0 : j0 + 1;
-    const float* t00 = &t.rgba[((size_t)j0 * t.w + i0) * 4];
-    const float* t10 = &t.rgba[((size_t)j0 * t.w + i1) * 4];
-    const float* t01 = &t.rgba[((size_t)j1 * t.w + i0) * 4];
-    const float* t11 = &t.rgba[((size_t)j1 * t.w + i1) * 4];
+    const lxr* t00 = &t.rgba[((size_t)j0 * t.w + i0) * 4];
+    const lxr* t10 = &t.rgba[((size_t)j0 * t.w + i1) * 4];
+    const lxr* t01 = &t.rgba[((size_t)j1 * t.w + i0) * 4];
+    const lxr* t11 = &t.rgba[((size_t)j1 * t.w + i1) * 4];
     /* nested lerps: exact for constant textures and at texel centres */
     for (int c = 0; c < 4; c++) {
-        float a = t00[c] + ax * (t10[c] - t00[c]);
-        float b = t01[c] + ax * (t11[c] - t01[c]);
+        lxr a = t00[c] + ax * (t10[c] - t00[c]);
+        lxr b = t01[c] + ax * (t11[c] - t01[c]);
         out[c] = a + ay * (b - a);
     }
 }
@@ -198,10 +241,10 @@ static void decode_texture(const lx_texture_desc& d, Tex& t) {
     t.rgba.resize(n);
     if (d.format == LX_TEX_RGBA8) {
         const uint8_t* s = (const uint8_t*)d.data;
-        for (size_t i = 0; i < n; i++) t.rgba[i] = ((float)s[i]) / 255.0f;
+        for (size_t i = 0; i < n; i++) t.rgba[i] = ((lxr)s[i]) / 255.0f;
     } else if (d.format == LX_TEX_RGBA16) {
         const uint16_t* s = (const uint16_t*)d.data;
-        for (size_t i = 0; i < n; i++) t.rgba[i] = ((float)s[i]) / 65535.0f;
+        for (size_t i = 0; i < n; i++) t.rgba[i] = ((lxr)s[i]) / 65535.0f;
     } else {
         const float* s = (const float*)d.data;
         for (size_t i = 0; i < n; i++) t.rgba[i] = s[i];
@@ -245,8 +288,8 @@ struct BBox {
         mx.y = lxm_max(mx.y, p.y);
         mx.z = lxm_max(mx.z, p.z);
     }
-    float surface_area() const {
-        float dx = mx.x - mn.x, dy = mx.y - mn.y, dz = mx.z - mn.z;
+    lxr surface_area() const {
+        lxr dx = mx.x - mn.x, dy = mx.y - mn.y, dz = mx.z - mn.z;
         return 2.0f * (dx * dy + dy * dz + dz * dx);
     }
 };
@@ -297,7 +340,7 @@ static BBox fill_bvh(int idx, int start, int end, std::vector<BNode>& h, const M
     int cnt = end - start + 1;
     std::vector<V3> cen(cnt);
     for (int i = start; i <= end; i++) cen[i - start] = centroid(m, ti[i]);
-    float mnx = FLT_MAX, mxx = -FLT_MAX, mny = FLT_MAX, mxy = -FLT_MAX, mnz = FLT_MAX, mxz = -FLT_MAX;
+    lxr mnx = FLT_MAX, mxx = -FLT_MAX, mny = FLT_MAX, mxy = -FLT_MAX, mnz = FLT_MAX, mxz = -FLT_MAX;
     for (auto& c : cen) {
         mnx = lxm_min(mnx, c.x);
         mxx = lxm_max(mxx, c.x);
@@ -306,11 +349,11 @@ static BBox fill_bvh(int idx, int start, int end, std::vector<BNode>& h, const M
         mnz = lxm_min(mnz, c.z);
         mxz = lxm_max(mxz, c.z);
     }
-    float dx = mxx - mnx, dy = mxy - mny, dz = mxz - mnz;
-    float wx = (dx > 0) ? dx / BIN_COUNT : 1.0f;
-    float wy = (dy > 0) ? dy / BIN_COUNT : 1.0f;
-    float wz = (dz > 0) ? dz / BIN_COUNT : 1.0f;
-    auto bin_of = [&](float c, float mn, float w) {
+    lxr dx = mxx - mnx, dy = mxy - mny, dz = mxz - mnz;
+    lxr wx = (dx > 0) ? dx / BIN_COUNT : 1.0f;
+    lxr wy = (dy > 0) ? dy / BIN_COUNT : 1.0f;
+    lxr wz = (dz > 0) ? dz / BIN_COUNT : 1.0f;
+    auto bin_of = [&](lxr c, lxr mn, lxr w) {
         int b = (int)((c - mn) / w);
         return b < 0 ? 0 : (b > BIN_COUNT - 1 ? BIN_COUNT - 1 : b);
     };
@@ -352,11 +395,11 @@ static BBox fill_bvh(int idx, int start, int end, std::vector<BNode>& h, const M
             rb[i] = rb[i + 1];
             rb[i].expand_box_ref(b[i]);
         }
-        float min_cost = FLT_MAX;
+        lxr min_cost = FLT_MAX;
         best_bin = -1;
         for (int i = 0; i <= 14; i++) {
-            float cost = 1 + (float(lc[i]) / float(cnt)) * lb[i].surface_area() +
-                         (float(rc[i + 1]) / float(cnt)) * rb[i + 1].surface_area();
+            lxr cost = 1 + (lxr(lc[i]) / lxr(cnt)) * lb[i].surface_area() +
+                         (lxr(rc[i + 1]) / lxr(cnt)) * rb[i + 1].surface_area();
             if (cost < min_cost) {
                 min_cost = cost;
                 best_bin = i;
@@ -365,7 +408,7 @@ static BBox fill_bvh(int idx, int start, int end, std::vector<BNode>& h, const M
         return min_cost;
     };
     int bbx, bby, bbz;
-    float cxm = best_split(cx, bx, bbx), cym = best_split(cy, by, bby), czm = best_split(cz, bz, bbz);
+    lxr cxm = best_split(cx, bx, bbx), cym = best_split(cy, by, bby), czm = best_split(cz, bz, bbz);
     int axis, best_bin;
     if (cxm <= cym && cxm <= czm) {
         axis = 0;
@@ -386,14 +429,14 @@ static BBox fill_bvh(int idx, int start, int end, std::vector<BNode>& h, const M
         return b <= best_bin;
     });
     int mid = (int)std::distance(ti.begin(), pivot);
-    float lower = float(mid - start) / float(cnt);
-    float upper = float(end - mid + 1) / float(cnt);
+    lxr lower = lxr(mid - start) / lxr(cnt);
+    lxr upper = lxr(end - mid + 1) / lxr(cnt);
     if (lower < 0.15f || upper < 0.15f) {
         int new_mid = (start + end + 1) / 2;
         std::nth_element(ti.begin() + start, ti.begin() + new_mid, ti.begin() + end + 1, [&](int a, int b) {
             V3 ca = centroid(m, a), cb = centroid(m, b);
-            float va = axis == 0 ? ca.x : (axis == 1 ? ca.y : ca.z);
-            float vb = axis == 0 ? cb.x : (axis == 1 ? cb.y : cb.z);
+            lxr va = axis == 0 ? ca.x : (axis == 1 ? ca.y : ca.z);
+            lxr vb = axis == 0 ? cb.x : (axis == 1 ? cb.y : cb.z);
             return va < vb;
         });
         mid = new_mid;
@@ -417,24 +460,24 @@ struct Mat {
     V3 color;
     int albedo_tex;
     lx_tex_transform albedo_xf;
-    float roughness, metallic;
+    lxr roughness, metallic;
     int mr_tex;
     lx_tex_transform mr_xf;
     int normal_tex;
     lx_tex_transform normal_xf;
     V3 emission_color;
-    float emission_strength;
+    lxr emission_strength;
     int emissive_tex;
     lx_tex_transform emissive_xf;
-    float ior, transmission, alpha;
+    lxr ior, transmission, alpha;
     int alpha_mode;
-    float alpha_cutoff;
+    lxr alpha_cutoff;
     int type; /* LX_MATERIAL_*: legacy shading class */
 };
 
 struct Inst {
     uint32_t mesh, mat;
-    float m[12], inv[12];
+    lxr m[12], inv[12];
     BBox wbox; /* conservative world box, used only to skip instances */
 };
 
@@ -451,17 +494,17 @@ struct lxo_scene {
     int trace_depth = 8;
     /* precompute_emissive_mesh_area, scene_precompute.cpp:71-122 */
     std::vector<uint32_t> emissive;         /* instance ids */
-    std::vector<float> emissive_prefix;     /* normalised */
-    std::vector<std::vector<float>> tri_cdf; /* per emissive instance */
-    float total_area = 0.0f;
+    std::vector<lxr> emissive_prefix;     /* normalised */
+    std::vector<std::vector<lxr>> tri_cdf; /* per emissive instance */
+    lxr total_area = 0.0f;
     /* precompute_hdri_emission, scene_precompute.cpp:125-193 */
-    std::vector<float> marginal, conditional;
+    std::vector<lxr> marginal, conditional;
     double build_ms = 0;
 };
 
 namespace {
 
-static float emission_len(const Mat& m) { return length(m.emission_color); }
+static lxr emission_len(const Mat& m) { return length(m.emission_color); }
 
 /* scene_precompute.cpp:71-122; fp64 mode accumulates the prefix sums in
  * double and rounds the normalised values once. */
@@ -469,7 +512,7 @@ static void precompute_emitters(lxo_scene& s, int cdf_mode) {
     s.emissive.clear();
     s.emissive_prefix.clear();
     s.tri_cdf.clear();
-    float area_f = 0.0f;
+    lxr area_f = 0.0f;
     double area_d = 0.0;
     std::vector<double> prefix_d;
     for (uint32_t gi = 0; gi < s.insts.size(); gi++) {
@@ -478,15 +521,15 @@ static void precompute_emitters(lxo_scene& s, int cdf_mode) {
         if (!(emission_len(mat) * mat.emission_strength > 1e-5f)) continue;
         const Mesh& m = s.meshes[g.mesh];
         size_t nt = m.idx.size() / 3;
-        std::vector<float> cdf(nt);
-        float cum_f = 0.0f;
+        std::vector<lxr> cdf(nt);
+        lxr cum_f = 0.0f;
         double cum_d = 0.0;
         std::vector<double> cd(nt);
         for (size_t t = 0; t < nt; t++) {
             V3 A = xform_point(g.m, m.pos[m.idx[3 * t + 0]]);
             V3 B = xform_point(g.m, m.pos[m.idx[3 * t + 1]]);
             V3 C = xform_point(g.m, m.pos[m.idx[3 * t + 2]]);
-            float a = 0.5f * length(cross(sub(B, A), sub(C, A)));
+            lxr a = 0.5f * length(cross(sub(B, A), sub(C, A)));
             cum_f += a;
             cum_d += (double)a;
             cdf[t] = cum_f;
@@ -497,7 +540,7 @@ static void precompute_emitters(lxo_scene& s, int cdf_mode) {
             area_f += cum_f;
             s.emissive_prefix.push_back(area_f);
         } else {
-            for (size_t t = 0; t < nt; t++) cdf[t] = (float)(cd[t] / cum_d);
+            for (size_t t = 0; t < nt; t++) cdf[t] = (lxr)(cd[t] / cum_d);
             area_d += cum_d;
             prefix_d.push_back(area_d);
         }
@@ -508,8 +551,8 @@ static void precompute_emitters(lxo_scene& s, int cdf_mode) {
         for (auto& p : s.emissive_prefix) p /= area_f;
         s.total_area = area_f;
     } else {
-        for (double p : prefix_d) s.emissive_prefix.push_back((float)(p / area_d));
-        s.total_area = (float)area_d;
+        for (double p : prefix_d) s.emissive_prefix.push_back((lxr)(p / area_d));
+        s.total_area = (lxr)area_d;
     }
 }
 
@@ -519,19 +562,19 @@ static void precompute_env(lxo_scene& s, int cdf_mode) {
     int W = s.env.w, H = s.env.h;
     s.conditional.assign((size_t)W * H, 0.0f);
     s.marginal.assign(H, 0.0f);
-    float total_f = 0.0f;
+    lxr total_f = 0.0f;
     double total_d = 0.0;
     std::vector<double> marg_d(H);
     std::vector<double> row_d(W);
     for (int y = 0; y < H; y++) {
-        float row_f = 0.0f;
+        lxr row_f = 0.0f;
         double rowsum_d = 0.0;
-        float theta = PI_F * (y + 0.5f) / H;
-        float st = lxm_sin(theta);
+        lxr theta = PI_F * (y + 0.5f) / H;
+        lxr st = lxm_sin(theta);
         for (int x = 0; x < W; x++) {
-            const float* t = &s.env.rgba[((size_t)y * W + x) * 4];
-            float e = lxm_sqrt(((t[0] * t[0] + t[1] * t[1]) + (t[2] * t[2] + t[3] * t[3])));
-            float w = e * st;
+            const lxr* t = &s.env.rgba[((size_t)y * W + x) * 4];
+            lxr e = lxm_sqrt(((t[0] * t[0] + t[1] * t[1]) + (t[2] * t[2] + t[3] * t[3])));
+            lxr w = e * st;
             row_f += w;
             rowsum_d += (double)w;
             s.conditional[(size_t)y * W + x] = row_f;
@@ -541,16 +584,16 @@ static void precompute_env(lxo_scene& s, int cdf_mode) {
             if (row_f > 0.0f) {
                 for (int x = 0; x < W; x++) s.conditional[(size_t)y * W + x] /= row_f;
             } else {
-                for (int x = 0; x < W; x++) s.conditional[(size_t)y * W + x] = (float)(x + 1) / (float)W;
+                for (int x = 0; x < W; x++) s.conditional[(size_t)y * W + x] = (lxr)(x + 1) / (lxr)W;
             }
             s.conditional[(size_t)y * W + (W - 1)] = 1.0f;
             total_f += row_f;
             s.marginal[y] = total_f;
         } else {
             if (rowsum_d > 0.0) {
-                for (int x = 0; x < W; x++) s.conditional[(size_t)y * W + x] = (float)(row_d[x] / rowsum_d);
+                for (int x = 0; x < W; x++) s.conditional[(size_t)y * W + x] = (lxr)(row_d[x] / rowsum_d);
             } else {
-                for (int x = 0; x < W; x++) s.conditional[(size_t)y * W + x] = (float)(x + 1) / (float)W;
+                for (int x = 0; x < W; x++) s.conditional[(size_t)y * W + x] = (lxr)(x + 1) / (lxr)W;
             }
             s.conditional[(size_t)y * W + (W - 1)] = 1.0f;
             total_d += rowsum_d;
@@ -560,7 +603,7 @@ static void precompute_env(lxo_scene& s, int cdf_mode) {
     if (cdf_mode == LX_CDF_FP32_REFERENCE) {
         for (int y = 0; y < H; y++) s.marginal[y] /= total_f;
     } else {
-        for (int y = 0; y < H; y++) s.marginal[y] = (float)(marg_d[y] / total_d);
+        for (int y = 0; y < H; y++) s.marginal[y] = (lxr)(marg_d[y] / total_d);
     }
     s.marginal[H - 1] = 1.0f;
 }
@@ -568,16 +611,16 @@ static void precompute_env(lxo_scene& s, int cdf_mode) {
 /* ------------------------------------------------------ triangle tests */
 struct Ray {
     V3 o, d;
-    float tmax;
+    lxr tmax;
 };
 
 struct Hit {
-    float t;
+    lxr t;
     uint32_t inst, prim;
-    float u, v;
+    lxr u, v;
 };
 
-static inline bool better(float t, uint32_t inst, uint32_t prim, const Hit& h) {
+static inline bool better(lxr t, uint32_t inst, uint32_t prim, const Hit& h) {
     if (t < h.t) return true;
     if (t == h.t) {
         if (inst < h.inst) return true;
@@ -588,15 +631,15 @@ static inline bool better(float t, uint32_t inst, uint32_t prim, const Hit& h) {
 
 /* Moller-Trumbore, src/intersections/mesh_isect.cu:77-137 (epsilon
  * 1.1920929e-7, inflated barycentric bounds, t > epsilon), plus t < tmax. */
-static inline bool mt_test(const Mesh& m, uint32_t tri, V3 o, V3 d, float tmax, float& t, float& u, float& v) {
-    const float eps = (float)(1.1920929E-7F);
+static inline bool mt_test(const Mesh& m, uint32_t tri, V3 o, V3 d, lxr tmax, lxr& t, lxr& u, lxr& v) {
+    const lxr eps = (lxr)(1.1920929E-7F);
     V3 a = m.pos[m.idx[3 * tri + 0]];
     V3 e1 = sub(m.pos[m.idx[3 * tri + 1]], a);
     V3 e2 = sub(m.pos[m.idx[3 * tri + 2]], a);
     V3 c = cross(d, e2);
-    float det = dot(e1, c);
+    lxr det = dot(e1, c);
     if (det > -eps && det < eps) return false;
-    float inv_det = 1.0f / det;
+    lxr inv_det = 1.0f / det;
     a = sub(o, a);
     u = inv_det * dot(a, c);
     if (u < -eps || u > 1.0f + eps) return false;
@@ -619,31 +662,31 @@ static uint32_t hash_u32(uint32_t x) {
     return x;
 }
 
-static inline V2 v2(float x, float y) { return V2{x, y}; }
+static inline V2 v2(lxr x, lxr y) { return V2{x, y}; }
 
 /* __anyhit__ms_all, optix_triangle.cu:317-366: true if the candidate is
  * accepted.  dir is the world-space ray direction. */
-static bool any_hit_accept(const lxo_scene& s, uint32_t inst, uint32_t prim, float bu, float bv, V3 wdir,
+static bool any_hit_accept(const lxo_scene& s, uint32_t inst, uint32_t prim, lxr bu, lxr bv, V3 wdir,
                            uint32_t key, uint32_t iteration) {
     const Inst& g = s.insts[inst];
     const Mat& mat = s.mats[g.mat];
     if (mat.alpha_mode == LX_ALPHA_OPAQUE) return true;
-    float alpha = mat.alpha;
+    lxr alpha = mat.alpha;
     if (mat.albedo_tex >= 0) {
         const Mesh& m = s.meshes[g.mesh];
         V2 uv = v2(0.0f, 0.0f);
         if (m.has_uv) {
             V2 A = m.uv[m.idx[3 * prim + 0]], B = m.uv[m.idx[3 * prim + 1]], C = m.uv[m.idx[3 * prim + 2]];
-            float bA = 1.0f - bu - bv;
+            lxr bA = 1.0f - bu - bv;
             uv = v2((bA * A.x + bu * B.x) + bv * C.x, (bA * A.y + bu * B.y) + bv * C.y);
         }
         uv = v2(uv.x * mat.albedo_xf.scale[0], uv.y * mat.albedo_xf.scale[1]);
         if (mat.albedo_xf.rotation != 0.0f) {
-            float c = lxm_cos(mat.albedo_xf.rotation), sn = lxm_sin(mat.albedo_xf.rotation);
+            lxr c = lxm_cos(mat.albedo_xf.rotation), sn = lxm_sin(mat.albedo_xf.rotation);
             uv = v2(c * uv.x - sn * uv.y, sn * uv.x + c * uv.y);
         }
         uv = v2(uv.x + mat.albedo_xf.offset[0], uv.y + mat.albedo_xf.offset[1]);
-        float tex[4];
+        lxr tex[4];
         tex_sample(s.texs[mat.albedo_tex], uv.x, uv.y, tex);
         alpha *= tex[3];
     }
@@ -654,22 +697,22 @@ static bool any_hit_accept(const lxo_scene& s, uint32_t inst, uint32_t prim, flo
     h = hash_u32(h ^ lxm_f2u(wdir.z));
     h = hash_u32(h ^ inst);
     h = hash_u32(h ^ prim);
-    float rnd = (h >> 8) * (1.0f / 16777216.0f);
+    lxr rnd = (h >> 8) * (1.0f / 16777216.0f);
     return !(rnd >= alpha);
 }
 
 /* RayBoxInterection, src/mesh/bounding_box.cu:7-40 */
-static inline bool ray_box(const BBox& b, V3 o, V3 invd, const int* sign, float& thit) {
+static inline bool ray_box(const BBox& b, V3 o, V3 invd, const int* sign, lxr& thit) {
     V3 bounds[2] = {b.mn, b.mx};
-    float t_min = (bounds[sign[0]].x - o.x) * invd.x;
-    float t_max = (bounds[1 - sign[0]].x - o.x) * invd.x;
-    float t_ymin = (bounds[sign[1]].y - o.y) * invd.y;
-    float t_ymax = (bounds[1 - sign[1]].y - o.y) * invd.y;
+    lxr t_min = (bounds[sign[0]].x - o.x) * invd.x;
+    lxr t_max = (bounds[1 - sign[0]].x - o.x) * invd.x;
+    lxr t_ymin = (bounds[sign[1]].y - o.y) * invd.y;
+    lxr t_ymax = (bounds[1 - sign[1]].y - o.y) * invd.y;
     if (t_min > t_ymax || t_ymin > t_max) return false;
     t_min = lxm_max(t_min, t_ymin);
     t_max = lxm_min(t_max, t_ymax);
-    float t_zmin = (bounds[sign[2]].z - o.z) * invd.z;
-    float t_zmax = (bounds[1 - sign[2]].z - o.z) * invd.z;
+    lxr t_zmin = (bounds[sign[2]].z - o.z) * invd.z;
+    lxr t_zmax = (bounds[1 - sign[2]].z - o.z) * invd.z;
     if (t_min > t_zmax || t_zmin > t_max) return false;
     t_min = lxm_max(t_min, t_zmin);
     if (t_max < 0.0f) return false;
@@ -686,8 +729,8 @@ struct TraceCtx {
 
 /* Candidate handling shared by BVH and brute-force traversal. */
 static inline bool consider(const lxo_scene& s, uint32_t inst, const Mesh& m, uint32_t tri, V3 o, V3 d, V3 wdir,
-                            float tmax, Hit& best, const TraceCtx& tc) {
-    float t, u, v;
+                            lxr tmax, Hit& best, const TraceCtx& tc) {
+    lxr t, u, v;
     if (!mt_test(m, tri, o, d, tmax, t, u, v)) return false;
     if (!tc.any && !better(t, inst, tri, best)) return false;
     if (tc.alpha && !any_hit_accept(s, inst, tri, u, v, wdir, tc.key, tc.iteration)) return false;
@@ -702,7 +745,7 @@ static inline bool consider(const lxo_scene& s, uint32_t inst, const Mesh& m, ui
 /* Per-instance DFS, src/intersections/mesh_isect.cu:6-149 (stack of node
  * indices, nearer child first, prune when the entry distance exceeds the
  * best t), with the deviations listed at the top of the file. */
-static bool trace_instance_bvh(const lxo_scene& s, uint32_t gi, V3 wo, V3 wd, float tmax, Hit& best,
+static bool trace_instance_bvh(const lxo_scene& s, uint32_t gi, V3 wo, V3 wd, lxr tmax, Hit& best,
                                const TraceCtx& tc) {
     const Inst& g = s.insts[gi];
     const Mesh& m = s.meshes[g.mesh];
@@ -721,12 +764,12 @@ static bool trace_instance_bvh(const lxo_scene& s, uint32_t gi, V3 wo, V3 wd, fl
         const BNode& n = bvh[ni];
         if (!n.leaf) {
             int l = n.left, r = n.left + 1;
-            float tl = -FLT_MAX, tr = -FLT_MAX;
+            lxr tl = -FLT_MAX, tr = -FLT_MAX;
             bool hl = bvh[l].leaf || ray_box(bvh[l].box, o, invd, sign, tl);
             bool hr = bvh[r].leaf || ray_box(bvh[r].box, o, invd, sign, tr);
             /* prune only beyond a 2^-16 relative margin so that box rounding
              * and triangle-test rounding can never change the closest hit */
-            float cut = best.inst == 0xffffffffu ? FLT_MAX : best.t * 1.00001526f;
+            lxr cut = best.inst == 0xffffffffu ? FLT_MAX : best.t * 1.00001526f;
             if (hl && tl > cut) hl = false;
             if (hr && tr > cut) hr = false;
             if (hl && hr) {
@@ -747,7 +790,7 @@ static bool trace_instance_bvh(const lxo_scene& s, uint32_t gi, V3 wo, V3 wd, fl
     return found;
 }
 
-static bool trace_instance_brute(const lxo_scene& s, uint32_t gi, V3 wo, V3 wd, float tmax, Hit& best,
+static bool trace_instance_brute(const lxo_scene& s, uint32_t gi, V3 wo, V3 wd, lxr tmax, Hit& best,
                                  const TraceCtx& tc) {
     const Inst& g = s.insts[gi];
     const Mesh& m = s.meshes[g.mesh];
@@ -765,12 +808,12 @@ static bool trace_instance_brute(const lxo_scene& s, uint32_t gi, V3 wo, V3 wd, 
 }
 
 /* Conservative slab test used only to skip whole instances. */
-static inline bool world_box_hit(const BBox& b, V3 o, V3 d, float tmax) {
-    float t0 = 0.0f, t1 = tmax;
-    const float* bo = &o.x;
-    const float* bd = &d.x;
-    const float* mn = &b.mn.x;
-    const float* mx = &b.mx.x;
+static inline bool world_box_hit(const BBox& b, V3 o, V3 d, lxr tmax) {
+    lxr t0 = 0.0f, t1 = tmax;
+    const lxr* bo = &o.x;
+    const lxr* bd = &d.x;
+    const lxr* mn = &b.mn.x;
+    const lxr* mx = &b.mx.x;
     for (int a = 0; a < 3; a++) {
         if (bd[a] == 0.0f) {
             if (bo[a] < mn[a] || bo[a] > mx[a]) return false;
@@ -779,14 +822,14 @@ static inline bool world_box_hit(const BBox& b, V3 o, V3 d, float tmax) {
         double inv = 1.0 / (double)bd[a];
         double ta = ((double)mn[a] - bo[a]) * inv, tb = ((double)mx[a] - bo[a]) * inv;
         if (ta > tb) std::swap(ta, tb);
-        if (ta > t0) t0 = (float)ta;
-        if (tb < t1) t1 = (float)(tb * (1.0 + 1e-6) + 1e-30);
+        if (ta > t0) t0 = (lxr)ta;
+        if (tb < t1) t1 = (lxr)(tb * (1.0 + 1e-6) + 1e-30);
         if (t0 > t1 * 1.000001f + 1e-6f) return false;
     }
     return true;
 }
 
-static Hit trace_scene(const lxo_scene& s, V3 o, V3 d, float tmax, int mode, const TraceCtx& tc) {
+static Hit trace_scene(const lxo_scene& s, V3 o, V3 d, lxr tmax, int mode, const TraceCtx& tc) {
     Hit best;
     best.t = -1.0f;
     best.inst = 0xffffffffu;
@@ -806,7 +849,7 @@ static Hit trace_scene(const lxo_scene& s, V3 o, V3 d, float tmax, int mode, con
 
 /* ---------------------------------------------------- hit attributes */
 struct Rec {
-    float t;
+    lxr t;
     V3 N, T;
     int mat;
     V2 uv;
@@ -820,7 +863,7 @@ static Rec closest_hit_attrs(const lxo_scene& s, const Hit& h) {
     V3 A = m.pos[i0], B = m.pos[i1], C = m.pos[i2];
     V3 e1 = sub(B, A), e2 = sub(C, A);
     V3 normal = normalize(cross(e1, e2));
-    float bA = 1.0f - h.u - h.v;
+    lxr bA = 1.0f - h.u - h.v;
     if (m.has_n) {
         V3 nA = m.nrm[i0], nB = m.nrm[i1], nC = m.nrm[i2];
         normal = normalize(add(add(smul(bA, nA), smul(h.u, nB)), smul(h.v, nC)));
@@ -831,8 +874,8 @@ static Rec closest_hit_attrs(const lxo_scene& s, const Hit& h) {
         V2 a = m.uv[i0], b = m.uv[i1], c = m.uv[i2];
         uv = v2((bA * a.x + h.u * b.x) + h.v * c.x, (bA * a.y + h.u * b.y) + h.v * c.y);
         V2 d1 = v2(b.x - a.x, b.y - a.y), d2 = v2(c.x - a.x, c.y - a.y);
-        float det = (d1.x * d2.y - d2.x * d1.y);
-        float f = (lxm_abs(det) < 1e-10f) ? 0.0f : 1.0f / det;
+        lxr det = (d1.x * d2.y - d2.x * d1.y);
+        lxr f = (lxm_abs(det) < 1e-10f) ? 0.0f : 1.0f / det;
         if (f == 0.0f) {
             tangent = v3(1.0f, 0.0f, 0.0f);
         } else {
@@ -843,7 +886,7 @@ static Rec closest_hit_attrs(const lxo_scene& s, const Hit& h) {
     normal = normalize(xform_normal(g.inv, normal));
     tangent = normalize(xform_vector(g.m, tangent));
     V3 T = sub(tangent, smul(dot(tangent, normal), normal));
-    float len2 = dot(T, T);
+    lxr len2 = dot(T, T);
     if (len2 > 1e-10f) {
         tangent = muls(T, 1.0f / lxm_sqrt(len2));
     } else {
@@ -864,7 +907,7 @@ static Rec dl_hit_attrs(const lxo_scene& s, const Hit& h) {
     const Inst& g = s.insts[h.inst];
     const Mesh& m = s.meshes[g.mesh];
     uint32_t i0 = m.idx[3 * h.prim + 0], i1 = m.idx[3 * h.prim + 1], i2 = m.idx[3 * h.prim + 2];
-    float bA = 1.0f - h.u - h.v;
+    lxr bA = 1.0f - h.u - h.v;
     V3 normal;
     if (m.has_n) {
         normal = normalize(add(add(smul(bA, m.nrm[i0]), smul(h.u, m.nrm[i1])), smul(h.v, m.nrm[i2])));
@@ -892,7 +935,7 @@ static Rec dl_hit_attrs(const lxo_scene& s, const Hit& h) {
 static V2 apply_xf(V2 uv, const lx_tex_transform& x) {
     uv = v2(uv.x * x.scale[0], uv.y * x.scale[1]);
     if (x.rotation != 0.0f) {
-        float c = lxm_cos(x.rotation), s = lxm_sin(x.rotation);
+        lxr c = lxm_cos(x.rotation), s = lxm_sin(x.rotation);
         uv = v2(c * uv.x - s * uv.y, s * uv.x + c * uv.y);
     }
     return v2(uv.x + x.offset[0], uv.y + x.offset[1]);
@@ -902,7 +945,7 @@ static V2 apply_xf(V2 uv, const lx_tex_transform& x) {
 static V3 get_albedo(const lxo_scene& s, const Mat& m, V2 uv) {
     if (m.albedo_tex >= 0) {
         uv = apply_xf(uv, m.albedo_xf);
-        float t[4];
+        lxr t[4];
         tex_sample(s.texs[m.albedo_tex], uv.x, uv.y, t);
         return v3(lxm_pow(t[0], 2.2f), lxm_pow(t[1], 2.2f), lxm_pow(t[2], 2.2f));
     }
@@ -914,20 +957,20 @@ static V3 get_albedo(const lxo_scene& s, const Mat& m, V2 uv) {
 static V3 get_normal(const lxo_scene& s, const Mat& m, const Rec& r) {
     if (m.normal_tex < 0) return r.N;
     V2 uv = apply_xf(r.uv, m.normal_xf);
-    float t[4];
+    lxr t[4];
     tex_sample(s.texs[m.normal_tex], uv.x, uv.y, t);
     V3 ln = v3(t[0] * 2.0f - 1.0f, t[1] * 2.0f - 1.0f, t[2] * 2.0f - 1.0f);
     V3 N = r.N;
-    float l2 = dot(ln, ln);
+    lxr l2 = dot(ln, ln);
     if (!(l2 > 1e-12f)) return N;
     V3 T = sub(r.T, smul(dot(r.T, N), N));
-    float t2 = dot(T, T);
+    lxr t2 = dot(T, T);
     if (!(t2 > 1e-12f)) return N;
     T = muls(T, 1.0f / lxm_sqrt(t2));
     V3 B = cross(N, T);
     V3 mapped = v3(T.x * ln.x + B.x * ln.y + N.x * ln.z, T.y * ln.x + B.y * ln.y + N.y * ln.z,
                    T.z * ln.x + B.z * ln.y + N.z * ln.z);
-    float m2 = dot(mapped, mapped);
+    lxr m2 = dot(mapped, mapped);
     if (!(m2 > 1e-12f)) return N;
     mapped = muls(mapped, 1.0f / lxm_sqrt(m2));
     if (!(dot(mapped, N) > 1e-4f)) return N;
@@ -935,12 +978,12 @@ static V3 get_normal(const lxo_scene& s, const Mat& m, const Rec& r) {
 }
 
 /* get_metallic_roughness, material_queries.cu:92-106 */
-static void get_mr(const lxo_scene& s, const Mat& m, V2 uv, float& rough, float& metal) {
+static void get_mr(const lxo_scene& s, const Mat& m, V2 uv, lxr& rough, lxr& metal) {
     rough = m.roughness;
     metal = m.metallic;
     if (m.mr_tex >= 0) {
         uv = apply_xf(uv, m.mr_xf);
-        float t[4];
+        lxr t[4];
         tex_sample(s.texs[m.mr_tex], uv.x, uv.y, t);
         rough *= t[1];
         metal *= t[2];
@@ -952,7 +995,7 @@ static V3 get_emission(const lxo_scene& s, const Mat& m, V2 uv) {
     V3 e = muls(m.emission_color, m.emission_strength);
     if (m.emissive_tex >= 0) {
         uv = apply_xf(uv, m.emissive_xf);
-        float t[4];
+        lxr t[4];
         tex_sample(s.texs[m.emissive_tex], uv.x, uv.y, t);
         e = mul(e, v3(lxm_pow(t[0], 2.2f), lxm_pow(t[1], 2.2f), lxm_pow(t[2], 2.2f)));
     }
@@ -960,12 +1003,12 @@ static V3 get_emission(const lxo_scene& s, const Mat& m, V2 uv) {
 }
 
 /* get_alpha, material_queries.cu:119-137 */
-static float get_alpha(const lxo_scene& s, const Mat& m, V2 uv) {
+static lxr get_alpha(const lxo_scene& s, const Mat& m, V2 uv) {
     if (m.alpha_mode == LX_ALPHA_OPAQUE) return 1.0f;
-    float alpha = m.alpha;
+    lxr alpha = m.alpha;
     if (m.albedo_tex >= 0) {
         V2 tuv = apply_xf(uv, m.albedo_xf);
-        float t[4];
+        lxr t[4];
         tex_sample(s.texs[m.albedo_tex], tuv.x, tuv.y, t);
         alpha = alpha * t[3];
     }
@@ -976,21 +1019,21 @@ static float get_alpha(const lxo_scene& s, const Mat& m, V2 uv) {
 /* render_material_debug_mode, src/material_debug_render.cu:3-26, with the
  * normal-map texel get_normal writes to out_normal_map
  * (material_queries.cu:36-39 and 83). */
-static V3 debug_view(const lxo_scene& s, const Mat& m, int mode, V3 albedo, V3 normal, V2 uv, float rough,
-                     float metal) {
+static V3 debug_view(const lxo_scene& s, const Mat& m, int mode, V3 albedo, V3 normal, V2 uv, lxr rough,
+                     lxr metal) {
     if (mode == LX_DEBUG_ALBEDO) return albedo;
     if (mode == LX_DEBUG_WORLD_NORMAL) return add(muls(normal, 0.5f), v3(0.5f, 0.5f, 0.5f));
     if (mode == LX_DEBUG_NORMAL_MAP) {
         if (m.normal_tex < 0) return v3(0.5f, 0.5f, 1.0f);
         V2 tuv = apply_xf(uv, m.normal_xf);
-        float t[4];
+        lxr t[4];
         tex_sample(s.texs[m.normal_tex], tuv.x, tuv.y, t);
         return v3(t[0], t[1], t[2]);
     }
     if (mode == LX_DEBUG_ROUGHNESS) return v3(rough, rough, rough);
     if (mode == LX_DEBUG_METALLIC) return v3(metal, metal, metal);
     if (mode == LX_DEBUG_ALPHA) {
-        float a = get_alpha(s, m, uv);
+        lxr a = get_alpha(s, m, uv);
         return v3(a, a, a);
     }
     return v3(0.0f, 0.0f, 0.0f);
@@ -998,62 +1041,62 @@ static V3 debug_view(const lxo_scene& s, const Mat& m, int mode, V3 albedo, V3 n
 
 /* ------------------------------------------------------ Cook-Torrance */
 /* cook_torrance.cu:8-25 */
-static float D_TR(V3 h, V3 n, float alpha) {
-    float ndh = dot(n, h);
+static lxr D_TR(V3 h, V3 n, lxr alpha) {
+    lxr ndh = dot(n, h);
     if (ndh <= 0.0f) return 0.0f;
-    float a2 = alpha * alpha;
-    float ndh2 = ndh * ndh;
-    float dc = (ndh2 * a2) + (1.0f - ndh2);
-    float den = dc * dc * GLM_PI_F;
+    lxr a2 = alpha * alpha;
+    lxr ndh2 = ndh * ndh;
+    lxr dc = (ndh2 * a2) + (1.0f - ndh2);
+    lxr den = dc * dc * GLM_PI_F;
     return a2 / lxm_max(den, CT_EPS);
 }
 /* cook_torrance.cu:27-31 */
-static V3 F_schlick(float vdh, V3 f0) {
-    float m = 1.0f - vdh;
-    float m2 = m * m;
+static V3 F_schlick(lxr vdh, V3 f0) {
+    lxr m = 1.0f - vdh;
+    lxr m2 = m * m;
     return add(f0, muls(sub(v3(1, 1, 1), f0), m2 * m2 * m));
 }
 /* cook_torrance.cu:34-50 */
-static float F_diel(float cosI, float etaI, float etaT) {
+static lxr F_diel(lxr cosI, lxr etaI, lxr etaT) {
     cosI = lxm_clamp(cosI, 0.0f, 1.0f);
-    float sinI = lxm_sqrt(lxm_max(0.0f, 1.0f - cosI * cosI));
-    float sinT = (etaI / etaT) * sinI;
+    lxr sinI = lxm_sqrt(lxm_max(0.0f, 1.0f - cosI * cosI));
+    lxr sinT = (etaI / etaT) * sinI;
     if (sinT >= 1.0f) return 1.0f;
-    float cosT = lxm_sqrt(lxm_max(0.0f, 1.0f - sinT * sinT));
-    float rpar = ((etaT * cosI) - (etaI * cosT)) / ((etaT * cosI) + (etaI * cosT));
-    float rperp = ((etaI * cosI) - (etaT * cosT)) / ((etaI * cosI) + (etaT * cosT));
+    lxr cosT = lxm_sqrt(lxm_max(0.0f, 1.0f - sinT * sinT));
+    lxr rpar = ((etaT * cosI) - (etaI * cosT)) / ((etaT * cosI) + (etaI * cosT));
+    lxr rperp = ((etaI * cosI) - (etaT * cosT)) / ((etaI * cosI) + (etaT * cosT));
     return (rpar * rpar + rperp * rperp) / 2.0f;
 }
 /* cook_torrance.cu:56-66 */
-static float G1(V3 w, V3 n, float alpha) {
-    float a2 = alpha * alpha;
-    float ndw = lxm_max(dot(n, w), 0.0f);
-    float num = 2.0f * ndw;
-    float under = a2 + (1.0f - a2) * (ndw * ndw);
-    float den = ndw + lxm_sqrt(under);
+static lxr G1(V3 w, V3 n, lxr alpha) {
+    lxr a2 = alpha * alpha;
+    lxr ndw = lxm_max(dot(n, w), 0.0f);
+    lxr num = 2.0f * ndw;
+    lxr under = a2 + (1.0f - a2) * (ndw * ndw);
+    lxr den = ndw + lxm_sqrt(under);
     return num / lxm_max(den, CT_EPS);
 }
 /* cook_torrance.cu:72-77 (glm::mix(x, y, a) = x + a * (y - x)) */
-static V3 Fresnel(float c, V3 albedo, float metallic, float etaI, float etaT) {
-    float fd = F_diel(c, etaI, etaT);
+static V3 Fresnel(lxr c, V3 albedo, lxr metallic, lxr etaI, lxr etaT) {
+    lxr fd = F_diel(c, etaI, etaT);
     V3 diel = v3(fd, fd, fd);
     V3 cond = F_schlick(c, albedo);
     return add(diel, smul(metallic, sub(cond, diel)));
 }
 /* cook_torrance.cu:79-110 */
-static V3 BRDF(V3 v, V3 n, V3 l, V3 albedo, float roughness, float metallic, float ior, float transmission) {
+static V3 BRDF(V3 v, V3 n, V3 l, V3 albedo, lxr roughness, lxr metallic, lxr ior, lxr transmission) {
     ior = lxm_max(ior, 1.0001f);
     bool enter = dot(v, n) > 0.0f;
     V3 nf = enter ? n : neg(n);
-    float etaI = enter ? 1.0f : ior, etaT = enter ? ior : 1.0f;
+    lxr etaI = enter ? 1.0f : ior, etaT = enter ? ior : 1.0f;
     V3 h = normalize(add(v, l));
     roughness = lxm_clamp(roughness, 0.0001f, 1.0f);
-    float alpha = roughness * roughness;
-    float D = D_TR(h, nf, alpha);
+    lxr alpha = roughness * roughness;
+    lxr D = D_TR(h, nf, alpha);
     V3 F = Fresnel(lxm_max(dot(v, h), 0.0f), albedo, metallic, etaI, etaT);
-    float G = G1(v, nf, alpha) * G1(l, nf, alpha);
+    lxr G = G1(v, nf, alpha) * G1(l, nf, alpha);
     V3 num = muls(smul(D, F), G);
-    float den = 4 * lxm_max(dot(nf, v), 0.0f) * lxm_max(dot(nf, l), 0.0f);
+    lxr den = 4 * lxm_max(dot(nf, v), 0.0f) * lxm_max(dot(nf, l), 0.0f);
     V3 spec = divs(num, lxm_max(den, CT_EPS));
     V3 nonspec = sub(v3(1, 1, 1), F);
     V3 kD = muls(smul(1.0f - metallic, nonspec), 1.0f - transmission);
@@ -1061,34 +1104,34 @@ static V3 BRDF(V3 v, V3 n, V3 l, V3 albedo, float roughness, float metallic, flo
     return add(diffuse, spec);
 }
 /* cook_torrance.cu:112-142 */
-static V3 BTDF(V3 wo, V3 n, V3 wi, V3 albedo, float roughness, float metallic, float ior, float transmission) {
-    float cosI = dot(wo, n);
+static V3 BTDF(V3 wo, V3 n, V3 wi, V3 albedo, lxr roughness, lxr metallic, lxr ior, lxr transmission) {
+    lxr cosI = dot(wo, n);
     bool enter = cosI > 0.f;
-    float etaI = enter ? 1.0f : ior, etaT = enter ? ior : 1.0f;
+    lxr etaI = enter ? 1.0f : ior, etaT = enter ? ior : 1.0f;
     V3 ht = normalize(neg(add(smul(etaI, wo), smul(etaT, wi))));
-    float jd = (etaT * dot(wi, ht) + etaI * dot(wo, ht));
-    float jac = (etaI * etaI) / lxm_max(jd * jd, CT_EPS);
-    float ctn = lxm_abs(dot(wi, ht)) * lxm_abs(dot(wo, ht));
-    float ctd = lxm_abs(dot(wi, n)) * lxm_abs(dot(wo, n));
+    lxr jd = (etaT * dot(wi, ht) + etaI * dot(wo, ht));
+    lxr jac = (etaI * etaI) / lxm_max(jd * jd, CT_EPS);
+    lxr ctn = lxm_abs(dot(wi, ht)) * lxm_abs(dot(wo, ht));
+    lxr ctd = lxm_abs(dot(wi, n)) * lxm_abs(dot(wo, n));
     roughness = lxm_clamp(roughness, 0.0001f, 1.0f);
-    float alpha = roughness * roughness;
-    float D = D_TR(ht, n, alpha);
-    float F = F_diel(lxm_abs(dot(wo, ht)), etaI, etaT);
+    lxr alpha = roughness * roughness;
+    lxr D = D_TR(ht, n, alpha);
+    lxr F = F_diel(lxm_abs(dot(wo, ht)), etaI, etaT);
     bool gv = dot(wi, ht) * dot(wi, n) > 0.0f && dot(wo, ht) * dot(wo, n) > 0.0f;
-    float G = gv ? G1(dot(wi, n) > 0.0f ? wi : neg(wi), n, alpha) * G1(dot(wo, n) > 0.0f ? wo : neg(wo), n, alpha)
+    lxr G = gv ? G1(dot(wi, n) > 0.0f ? wi : neg(wi), n, alpha) * G1(dot(wo, n) > 0.0f ? wo : neg(wo), n, alpha)
                  : 0.0f;
-    float DFG = D * G * (1.0f - F);
+    lxr DFG = D * G * (1.0f - F);
     return muls(albedo, transmission * (1.0f - metallic) * (ctn / ctd) * jac * DFG);
 }
 
-static void lobe_probs(V3 wo, V3 n, V3 color, float metallic, float ior, float transmission, float& pRefr,
-                       float& pDiff, float& pSpec) {
+static void lobe_probs(V3 wo, V3 n, V3 color, lxr metallic, lxr ior, lxr transmission, lxr& pRefr,
+                       lxr& pDiff, lxr& pSpec) {
     bool enter = dot(wo, n) > 0.0f;
-    float etaI = enter ? 1.0f : ior, etaT = enter ? ior : 1.0f;
-    float cm = lxm_abs(dot(wo, n));
+    lxr etaI = enter ? 1.0f : ior, etaT = enter ? ior : 1.0f;
+    lxr cm = lxm_abs(dot(wo, n));
     V3 Fa = Fresnel(cm, color, metallic, etaI, etaT);
-    float sr = lxm_clamp((Fa.x + Fa.y + Fa.z) / 3.0f, 0.01f, 0.99f);
-    float Fm = F_diel(cm, etaI, etaT);
+    lxr sr = lxm_clamp((Fa.x + Fa.y + Fa.z) / 3.0f, 0.01f, 0.99f);
+    lxr Fm = F_diel(cm, etaI, etaT);
     pRefr = transmission * (1.0f - metallic) * (1.0f - Fm);
     pDiff = (1.0f - pRefr) * (1.0f - sr) * (1.0f - metallic);
     pSpec = 1.0f - pRefr - pDiff;
@@ -1096,9 +1139,9 @@ static void lobe_probs(V3 wo, V3 n, V3 color, float metallic, float ior, float t
 
 /* calculateRandomDirectionInHemisphere, src/interactions.cu:7-45 */
 static V3 cosine_hemisphere(V3 n, Rng& rng) {
-    float up = lxm_sqrt(rng.u01());
-    float over = lxm_sqrt(1 - up * up);
-    float around = rng.u01() * TWO_PI_F;
+    lxr up = lxm_sqrt(rng.u01());
+    lxr over = lxm_sqrt(1 - up * up);
+    lxr around = rng.u01() * TWO_PI_F;
     V3 dnn;
     if (lxm_abs(n.x) < SQRT_OF_ONE_THIRD) dnn = v3(1, 0, 0);
     else if (lxm_abs(n.y) < SQRT_OF_ONE_THIRD) dnn = v3(0, 1, 0);
@@ -1109,23 +1152,23 @@ static V3 cosine_hemisphere(V3 n, Rng& rng) {
 }
 
 /* sampleCookTorrance, cook_torrance.cu:156-224 */
-static V3 sample_ct(V3 wo, V3 n, float roughness, float metallic, Rng& rng, V3 color, float ior, float transmission) {
-    float r = rng.u01();
+static V3 sample_ct(V3 wo, V3 n, lxr roughness, lxr metallic, Rng& rng, V3 color, lxr ior, lxr transmission) {
+    lxr r = rng.u01();
     ior = lxm_max(ior, 1.0001f);
     roughness = lxm_clamp(roughness, 0.0001f, 1.0f);
-    float alpha = roughness * roughness;
+    lxr alpha = roughness * roughness;
     bool enter = dot(wo, n) > 0.0f;
     V3 nf = enter ? n : neg(n);
-    float etaI = enter ? 1.0f : ior, etaT = enter ? ior : 1.0f;
-    float pRefr, pDiff, pSpec;
+    lxr etaI = enter ? 1.0f : ior, etaT = enter ? ior : 1.0f;
+    lxr pRefr, pDiff, pSpec;
     lobe_probs(wo, n, color, metallic, ior, transmission, pRefr, pDiff, pSpec);
     if (r > pSpec && r <= pSpec + pDiff) return cosine_hemisphere(nf, rng);
-    float x1 = rng.u01();
-    float x2 = rng.u01();
-    float th = lxm_atan((alpha * lxm_sqrt(x1)) / lxm_sqrt(1.0f - x1));
-    float ph = 2.0f * PI_F * x2;
+    lxr x1 = rng.u01();
+    lxr x2 = rng.u01();
+    lxr th = lxm_atan((alpha * lxm_sqrt(x1)) / lxm_sqrt(1.0f - x1));
+    lxr ph = 2.0f * PI_F * x2;
     /* sphericalToCartesian(vec3(1, theta, phi)), cook_torrance.cu:144-154 */
-    float st = lxm_sin(th);
+    lxr st = lxm_sin(th);
     V3 hs = v3(1.0f * st * lxm_cos(ph), 1.0f * st * lxm_sin(ph), 1.0f * lxm_cos(th));
     V3 h = normalize(hs);
     V3 t = normalize(lxm_abs(nf.z) < 0.999f ? cross(v3(0, 0, 1), nf) : cross(v3(1, 0, 0), nf));
@@ -1139,9 +1182,9 @@ static V3 sample_ct(V3 wo, V3 n, float roughness, float metallic, Rng& rng, V3 c
         return normalize(wi);
     }
     /* glm::refract(I, N, eta) */
-    float eta = etaI / etaT;
-    float dv = dot(hw, I);
-    float k = 1.0f - eta * eta * (1.0f - dv * dv);
+    lxr eta = etaI / etaT;
+    lxr dv = dot(hw, I);
+    lxr k = 1.0f - eta * eta * (1.0f - dv * dv);
     if (!(k >= 0.0f)) return v3(0, 0, 0); /* glm yields NaN * 0 -> rejected by isnan */
     V3 wi = sub(smul(eta, I), smul(eta * dv + lxm_sqrt(k), hw));
     if (dot(wi, wi) < CT_EPS || lxm_isnan(wi.x) || lxm_isnan(wi.y) || lxm_isnan(wi.z) || dot(wi, nf) >= 0.0f)
@@ -1150,35 +1193,35 @@ static V3 sample_ct(V3 wo, V3 n, float roughness, float metallic, Rng& rng, V3 c
 }
 
 /* PDF_GGX, cook_torrance.cu:226-237 */
-static float pdf_ggx(V3 wo, V3 wi, V3 n, float roughness) {
+static lxr pdf_ggx(V3 wo, V3 wi, V3 n, lxr roughness) {
     V3 h = normalize(add(wo, wi));
     roughness = lxm_clamp(roughness, 0.0001f, 1.0f);
-    float alpha = roughness * roughness;
-    float ph = D_TR(h, n, alpha) * dot(n, h);
+    lxr alpha = roughness * roughness;
+    lxr ph = D_TR(h, n, alpha) * dot(n, h);
     return ph / lxm_max((4 * lxm_abs(dot(wo, h))), CT_EPS);
 }
 /* PDF_GGX_Transmission, cook_torrance.cu:239-258 */
-static float pdf_ggx_t(V3 wo, V3 wi, V3 n, float roughness, float ior) {
-    float cosI = dot(wo, n);
+static lxr pdf_ggx_t(V3 wo, V3 wi, V3 n, lxr roughness, lxr ior) {
+    lxr cosI = dot(wo, n);
     bool enter = cosI > 0.f;
-    float etaI = enter ? 1.0f : ior, etaT = enter ? ior : 1.0f;
+    lxr etaI = enter ? 1.0f : ior, etaT = enter ? ior : 1.0f;
     V3 ht = normalize(neg(add(smul(etaI, wo), smul(etaT, wi))));
     roughness = lxm_clamp(roughness, 0.0001f, 1.0f);
-    float alpha = roughness * roughness;
-    float den = etaT * dot(wi, ht) + etaI * dot(wo, ht);
-    float ph = D_TR(ht, n, alpha) * lxm_abs(dot(n, ht));
+    lxr alpha = roughness * roughness;
+    lxr den = etaT * dot(wi, ht) + etaI * dot(wo, ht);
+    lxr ph = D_TR(ht, n, alpha) * lxm_abs(dot(n, ht));
     return ph * (etaT * etaT * lxm_abs(dot(wi, ht))) / lxm_max(den * den, CT_EPS);
 }
 /* PDF, cook_torrance.cu:260-295 */
-static float PDF(V3 wo, V3 wi, V3 n, float roughness, float metallic, V3 color, float ior, float transmission) {
+static lxr PDF(V3 wo, V3 wi, V3 n, lxr roughness, lxr metallic, V3 color, lxr ior, lxr transmission) {
     ior = lxm_max(ior, 1.0001f);
     V3 nf = dot(wo, n) < 0.0f ? neg(n) : n;
-    float dwn = dot(wi, nf);
-    float pd = (dwn > 0.0f ? dwn : 0.0f) * INV_PI_CT;
+    lxr dwn = dot(wi, nf);
+    lxr pd = (dwn > 0.0f ? dwn : 0.0f) * INV_PI_CT;
     bool refract = (dot(wi, n) * dot(wo, n)) < 0.0f;
-    float pRefr, pDiff, pSpec;
+    lxr pRefr, pDiff, pSpec;
     lobe_probs(wo, n, color, metallic, ior, transmission, pRefr, pDiff, pSpec);
-    float pdf;
+    lxr pdf;
     if (refract) {
         pdf = pRefr * pdf_ggx_t(wo, wi, n, roughness, ior);
     } else {
@@ -1187,8 +1230,8 @@ static float PDF(V3 wo, V3 wi, V3 n, float roughness, float metallic, V3 color, 
     return lxm_max(pdf, PDF_CLAMP);
 }
 /* shadePathCookTorrance, cook_torrance.cu:298-331 */
-static V3 ct_eval_cos(V3 wo, V3 normal, V3 wi, V3 albedo, float roughness, float metallic, float ior,
-                      float transmission) {
+static V3 ct_eval_cos(V3 wo, V3 normal, V3 wi, V3 albedo, lxr roughness, lxr metallic, lxr ior,
+                      lxr transmission) {
     ior = lxm_max(ior, 1.0001f);
     bool refract = (dot(wi, normal) * dot(wo, normal)) < 0.0f;
     V3 f = refract ? BTDF(wo, normal, wi, albedo, roughness, metallic, ior, transmission)
@@ -1201,47 +1244,47 @@ static V3 ct_eval_cos(V3 wo, V3 normal, V3 wi, V3 albedo, float roughness, float
 static V3 glm_reflect(V3 I, V3 N) { return sub(I, muls(muls(N, dot(N, I)), 2.0f)); }
 
 /* Lambert::PDF, src/shaders/lambert.cu:9-12: max(1e-6, max(0, cos) / pi) */
-static float lambert_pdf(V3 wi, V3 n) {
-    float c = dot(wi, n);
+static lxr lambert_pdf(V3 wi, V3 n) {
+    lxr c = dot(wi, n);
     if (!(c > 0.0f)) c = 0.0f;
-    float p = c * INV_PI_CT;
+    lxr p = c * INV_PI_CT;
     return p > 1e-6f ? p : 1e-6f;
 }
 
 /* TransmissiveGlass::FresnelDielectricEval, src/shaders/glass.cu:27-55 */
-static float glass_fresnel(float cos_i, float ior) {
-    float eta_i = 1.0f, eta_t = ior;
+static lxr glass_fresnel(lxr cos_i, lxr ior) {
+    lxr eta_i = 1.0f, eta_t = ior;
     cos_i = lxm_clamp(cos_i, -1.0f, 1.0f);
     if (!(cos_i > 0.0f)) {
         eta_i = ior;
         eta_t = 1.0f;
         cos_i = lxm_abs(cos_i);
     }
-    float sin_i = lxm_sqrt(lxm_max(0.0f, 1.0f - cos_i * cos_i));
-    float sin_t = (eta_i / eta_t) * sin_i;
+    lxr sin_i = lxm_sqrt(lxm_max(0.0f, 1.0f - cos_i * cos_i));
+    lxr sin_t = (eta_i / eta_t) * sin_i;
     if (sin_t >= 1.0f) return 1.0f; /* total internal reflection */
-    float cos_t = lxm_sqrt(lxm_max(0.0f, 1.0f - sin_t * sin_t));
-    float par = ((eta_t * cos_i) - (eta_i * cos_t)) / ((eta_t * cos_i) + (eta_i * cos_t));
-    float per = ((eta_i * cos_i) - (eta_t * cos_t)) / ((eta_i * cos_i) + (eta_t * cos_t));
+    lxr cos_t = lxm_sqrt(lxm_max(0.0f, 1.0f - sin_t * sin_t));
+    lxr par = ((eta_t * cos_i) - (eta_i * cos_t)) / ((eta_t * cos_i) + (eta_i * cos_t));
+    lxr per = ((eta_i * cos_i) - (eta_t * cos_t)) / ((eta_i * cos_i) + (eta_t * cos_t));
     return (par * par + per * per) / 2.0f;
 }
 
 /* glm::refract(I, N, eta) as the vector overload computes it: NaN when k < 0
  * (sqrt of a negative times 0). */
-static V3 glm_refract(V3 I, V3 N, float eta) {
-    float dv = dot(N, I);
-    float k = 1.0f - eta * eta * (1.0f - dv * dv);
-    float mask = k >= 0.0f ? 1.0f : 0.0f;
-    float root = k >= 0.0f ? lxm_sqrt(k) : std::numeric_limits<float>::quiet_NaN();
+static V3 glm_refract(V3 I, V3 N, lxr eta) {
+    lxr dv = dot(N, I);
+    lxr k = 1.0f - eta * eta * (1.0f - dv * dv);
+    lxr mask = k >= 0.0f ? 1.0f : 0.0f;
+    lxr root = k >= 0.0f ? lxm_sqrt(k) : std::numeric_limits<lxr>::quiet_NaN();
     return muls(sub(smul(eta, I), smul(eta * dv + root, N)), mask);
 }
 
 /* sampleGlass (glass.cu:57-75) with sampleSpecularRefl / sampleSpecularTrans
  * (glass.cu:5-25) */
-static V3 sample_glass(V3 dir, V3 n, float ior, Rng& rng) {
+static V3 sample_glass(V3 dir, V3 n, lxr ior, Rng& rng) {
     V3 wo = neg(dir);
-    float r = rng.u01();
-    float F = lxm_clamp(glass_fresnel(dot(wo, n), ior), 0.0f, 1.0f);
+    lxr r = rng.u01();
+    lxr F = lxm_clamp(glass_fresnel(dot(wo, n), ior), 0.0f, 1.0f);
     bool enter = dot(wo, n) > 0.0f;
     V3 loc = enter ? n : neg(n);
     if (r < F) return glm_reflect(neg(wo), loc);
@@ -1251,7 +1294,7 @@ static V3 sample_glass(V3 dir, V3 n, float ior, Rng& rng) {
 }
 
 /* sample_materials, src/sample_materials.cu:24-37, non-microfacet classes */
-static V3 legacy_sample(int type, V3 dir, V3 n, float ior, Rng& rng) {
+static V3 legacy_sample(int type, V3 dir, V3 n, lxr ior, Rng& rng) {
     switch (type) {
     case LX_MATERIAL_DIFFUSE:
     case LX_MATERIAL_EMISSIVE: return cosine_hemisphere(n, rng); /* Lambert::sampleHemisphere */
@@ -1262,12 +1305,12 @@ static V3 legacy_sample(int type, V3 dir, V3 n, float ior, Rng& rng) {
 
 /* update_throughput_materials, src/update_throughput_materials.cu:36-62,
  * non-microfacet classes; returns the new last_pdf. */
-static float legacy_update(int type, V3 dir, V3 n, V3 wi, V3 color, float ior, float transmission, V3& thr,
-                           float last_pdf) {
+static lxr legacy_update(int type, V3 dir, V3 n, V3 wi, V3 color, lxr ior, lxr transmission, V3& thr,
+                           lxr last_pdf) {
     if (type == LX_MATERIAL_DIFFUSE) {
-        float c = dot(wi, n);
+        lxr c = dot(wi, n);
         V3 f = muls(muls(color, INV_PI_CT), c > 0.0f ? c : 0.0f);
-        float pdf = lambert_pdf(wi, n);
+        lxr pdf = lambert_pdf(wi, n);
         thr = mul(thr, divs(f, pdf));
         return pdf;
     }
@@ -1278,10 +1321,10 @@ static float legacy_update(int type, V3 dir, V3 n, V3 wi, V3 color, float ior, f
     if (type == LX_MATERIAL_GLASS) {
         V3 wo = neg(dir);
         bool enter = dot(wo, n) > 0.f;
-        float eta_i = enter ? 1.0f : ior;
-        float eta_t = enter ? ior : 1.0f;
+        lxr eta_i = enter ? 1.0f : ior;
+        lxr eta_t = enter ? ior : 1.0f;
         bool refracted = dot(wo, n) * dot(wi, n) < 0.0f;
-        float att = refracted ? (eta_i / eta_t) : 1.0f;
+        lxr att = refracted ? (eta_i / eta_t) : 1.0f;
         att = att * att;
         V3 one = v3(1.0f, 1.0f, 1.0f);
         V3 tint = refracted ? add(one, smul(transmission, sub(color, one))) : one; /* glm::mix(1, color, T) */
@@ -1293,7 +1336,7 @@ static float legacy_update(int type, V3 dir, V3 n, V3 wi, V3 color, float ior, f
 
 /* ------------------------------------------------------- CDF sampling */
 /* cudaUtils::select_from_cdf, src/common.cu:3-15 */
-static int select_from_cdf(const float* cdf, int size, float xi) {
+static int select_from_cdf(const lxr* cdf, int size, lxr xi) {
     int lo = 0, hi = size - 1;
     while (lo < hi) {
         int mid = lo + (hi - lo) / 2;
@@ -1304,52 +1347,52 @@ static int select_from_cdf(const float* cdf, int size, float xi) {
 }
 
 /* envmap_pdf, shade_path.cu:27-42 */
-static float envmap_pdf(const lxo_scene& s, V3 d) {
+static lxr envmap_pdf(const lxo_scene& s, V3 d) {
     int W = s.env.w, H = s.env.h;
-    float phi = lxm_atan2(d.z, d.x);
-    float theta = lxm_acos(lxm_clamp(d.y, -1.0f, 1.0f));
-    float u = (phi + PI_F) / (2.0f * PI_F);
-    float v = theta / PI_F;
+    lxr phi = lxm_atan2(d.z, d.x);
+    lxr theta = lxm_acos(lxm_clamp(d.y, -1.0f, 1.0f));
+    lxr u = (phi + PI_F) / (2.0f * PI_F);
+    lxr v = theta / PI_F;
     int col = (int)(u * W);
     col = col < 0 ? 0 : (col > W - 1 ? W - 1 : col);
     int row = (int)(v * H);
     row = row < 0 ? 0 : (row > H - 1 ? H - 1 : row);
-    float pm = (s.marginal[row] - (row == 0 ? 0.0f : s.marginal[row - 1])) * H;
-    float pc = (s.conditional[(size_t)row * W + col] - (col == 0 ? 0.0f : s.conditional[(size_t)row * W + col - 1])) * W;
-    float st = lxm_sin(theta);
+    lxr pm = (s.marginal[row] - (row == 0 ? 0.0f : s.marginal[row - 1])) * H;
+    lxr pc = (s.conditional[(size_t)row * W + col] - (col == 0 ? 0.0f : s.conditional[(size_t)row * W + col - 1])) * W;
+    lxr st = lxm_sin(theta);
     return (st > 1e-6f) ? (pm * pc) / (2.0f * PI_F * PI_F * st) : 0.0f;
 }
 
 /* Environment texel lookup of shade_path.cu:205-215 / 309-318: note the
  * double-precision M_PI = 3.14159 of shade_path.cu:23. */
-static V3 env_lookup(const lxo_scene& s, V3 d, float intensity) {
-    float phi = lxm_atan2(d.z, d.x);
-    float theta = lxm_acos(lxm_clamp(d.y, -1.0f, 1.0f));
-    float u = (float)(((double)phi + SHADE_M_PI) * (double)(1.0f / (2.0f * SHADE_M_PI)));
-    float v = (float)((double)theta * (double)(1.0f / SHADE_M_PI));
-    float t[4];
+static V3 env_lookup(const lxo_scene& s, V3 d, lxr intensity) {
+    lxr phi = lxm_atan2(d.z, d.x);
+    lxr theta = lxm_acos(lxm_clamp(d.y, -1.0f, 1.0f));
+    lxr u = (lxr)(((double)phi + SHADE_M_PI) * (double)(1.0f / (2.0f * SHADE_M_PI)));
+    lxr v = (lxr)((double)theta * (double)(1.0f / SHADE_M_PI));
+    lxr t[4];
     tex_sample(s.env, u, v, t);
     return muls(v3(t[0], t[1], t[2]), intensity);
 }
 
 /* sampleDirectLight, src/kernels/sample_direct_light.cu:32-66 (emitter) */
 static V3 sample_light_point(const lxo_scene& s, Rng& rng) {
-    float rnd = rng.u01();
+    lxr rnd = rng.u01();
     int sel = select_from_cdf(s.emissive_prefix.data(), (int)s.emissive.size(), rnd);
     const Inst& g = s.insts[s.emissive[sel]];
     const Mesh& m = s.meshes[g.mesh];
-    float lower = (sel == 0) ? 0.0f : s.emissive_prefix[sel - 1];
-    float upper = s.emissive_prefix[sel];
-    float den = upper - lower;
-    float to = (den > 1e-10f) ? (rnd - lower) / den : 0.0f;
+    lxr lower = (sel == 0) ? 0.0f : s.emissive_prefix[sel - 1];
+    lxr upper = s.emissive_prefix[sel];
+    lxr den = upper - lower;
+    lxr to = (den > 1e-10f) ? (rnd - lower) / den : 0.0f;
     to = lxm_clamp(to, 0.0f, 1.0f);
-    const std::vector<float>& cdf = s.tri_cdf[sel];
+    const std::vector<lxr>& cdf = s.tri_cdf[sel];
     int tri = select_from_cdf(cdf.data(), (int)cdf.size(), to);
     V3 v0 = m.pos[m.idx[3 * tri + 0]], v1 = m.pos[m.idx[3 * tri + 1]], v2 = m.pos[m.idx[3 * tri + 2]];
-    float r1 = lxm_sqrt(rng.u01());
-    float r2 = rng.u01();
-    float u = 1.0f - r1;
-    float v = r2 * r1;
+    lxr r1 = lxm_sqrt(rng.u01());
+    lxr r2 = rng.u01();
+    lxr u = 1.0f - r1;
+    lxr v = r2 * r1;
     V3 local = add(add(smul(u, v0), smul(v, v1)), smul(1.0f - u - v, v2));
     return xform_point(g.m, local);
 }
@@ -1357,19 +1400,19 @@ static V3 sample_light_point(const lxo_scene& s, Rng& rng) {
 /* sampleDirectLight, sample_direct_light.cu:68-100 (environment) */
 static V3 sample_env_dir(const lxo_scene& s, Rng& rng) {
     int W = s.env.w, H = s.env.h;
-    float rnd = rng.u01();
+    lxr rnd = rng.u01();
     int mg = select_from_cdf(s.marginal.data(), H, rnd);
-    float lower = (mg == 0) ? 0.0f : s.marginal[mg - 1];
-    float upper = s.marginal[mg];
-    float den = upper - lower;
-    float off = (den > 1e-10f) ? (rnd - lower) / den : 0.0f;
+    lxr lower = (mg == 0) ? 0.0f : s.marginal[mg - 1];
+    lxr upper = s.marginal[mg];
+    lxr den = upper - lower;
+    lxr off = (den > 1e-10f) ? (rnd - lower) / den : 0.0f;
     off = lxm_clamp(off, 0.0f, 1.0f);
     int cd = select_from_cdf(s.conditional.data() + (size_t)mg * W, W, off);
-    float u = (cd + 0.5f) / W;
-    float v = (mg + 0.5f) / H;
-    float phi = (float)((double)u * 2.0 * (double)PI_F - (double)PI_F);
-    float theta = v * PI_F;
-    float st = lxm_sin(theta);
+    lxr u = (cd + 0.5f) / W;
+    lxr v = (mg + 0.5f) / H;
+    lxr phi = (lxr)((double)u * 2.0 * (double)PI_F - (double)PI_F);
+    lxr theta = v * PI_F;
+    lxr st = lxm_sin(theta);
     return v3(st * lxm_cos(phi), lxm_cos(theta), st * lxm_sin(phi));
 }
 
@@ -1380,12 +1423,12 @@ struct Path {
     V3 sample_dir, dl_sample, env_sample;
     uint32_t pixel;
     bool kill, spec;
-    float last_pdf;
+    lxr last_pdf;
 };
 
 struct BounceRecords {
     Rec isect, dl;
-    float env_t;
+    lxr env_t;
 };
 
 struct RenderCfg {
@@ -1393,7 +1436,7 @@ struct RenderCfg {
     bool dmis, emis;
     int debug_mode; /* LX_DEBUG_* */
     bool legacy;    /* LX_SHADING_LEGACY */
-    float intensity;
+    lxr intensity;
     uint32_t seed;
     lx_camera_desc cam;
     int mode;
@@ -1403,10 +1446,10 @@ struct RenderCfg {
 static void camera_ray(const lx_camera_desc& c, int iter, int x, int y, int trace_depth, uint32_t seed, V3& dir) {
     int index = x + y * c.width;
     Rng rng = make_rng(iter, index, trace_depth, seed);
-    float x1 = rng.u01() - 0.5f;
-    float x2 = rng.u01() - 0.5f;
-    float pX = (float(x) + x1 + 0.5f) - (float)c.width * 0.5f;
-    float pY = (float(y) + x2 + 0.5f) - (float)c.height * 0.5f;
+    lxr x1 = rng.u01() - 0.5f;
+    lxr x2 = rng.u01() - 0.5f;
+    lxr pX = (lxr(x) + x1 + 0.5f) - (lxr)c.width * 0.5f;
+    lxr pY = (lxr(y) + x2 + 0.5f) - (lxr)c.height * 0.5f;
     V3 view = v3(c.view[0], c.view[1], c.view[2]);
     V3 right = v3(c.right[0], c.right[1], c.right[2]);
     V3 up = v3(c.up[0], c.up[1], c.up[2]);
@@ -1433,7 +1476,7 @@ static bool shade_one(const lxo_scene& s, const RenderCfg& cfg, Path& p, const B
         V3 color = get_albedo(s, mat, is.uv);
         if (type == LX_MATERIAL_EMISSIVE && mat.albedo_tex < 0) color = mat.emission_color; /* gltf.cpp:211-213 */
         V3 normal = get_normal(s, mat, is);
-        float rough, metal;
+        lxr rough, metal;
         get_mr(s, mat, is.uv, rough, metal);
         V3 emission = get_emission(s, mat, is.uv);
         bool is_specular = cfg.legacy ? (type == LX_MATERIAL_SPECULAR || type == LX_MATERIAL_GLASS) : rough < 0.05f;
@@ -1458,22 +1501,22 @@ static bool shade_one(const lxo_scene& s, const RenderCfg& cfg, Path& p, const B
         };
         if (cfg.emis) {
             if (emissive) {
-                float w = 1.0f;
+                lxr w = 1.0f;
                 if (depth > 1 && !p.spec) {
-                    float pb = p.last_pdf;
-                    float pe = envmap_pdf(s, dir);
+                    lxr pb = p.last_pdf;
+                    lxr pe = envmap_pdf(s, dir);
                     w = (pb * pb) / (pb * pb + pe * pe);
                 }
                 p.color = add(p.color, mul(p.thr, smul(w, emission)));
                 p.kill = true;
             } else if (br.env_t < 0.0f && !is_specular && !p.kill && has_exr) {
                 V3 ed = p.env_sample;
-                float pe = envmap_pdf(s, ed);
-                float pb = pdf_bsdf(ed);
-                float w = (pe * pe) / (pe * pe + pb * pb);
+                lxr pe = envmap_pdf(s, ed);
+                lxr pb = pdf_bsdf(ed);
+                lxr w = (pe * pe) / (pe * pe + pb * pb);
                 V3 brdf = brdf_of(ed);
                 V3 L = env_lookup(s, normalize(ed), cfg.intensity);
-                float cs = dot(nf, ed);
+                lxr cs = dot(nf, ed);
                 if (cs > 0.0f && pe > 1e-8f) {
                     V3 contrib = divs(muls(mul(L, brdf), cs), pe);
                     p.color = add(p.color, muls(mul(p.thr, contrib), w));
@@ -1481,14 +1524,14 @@ static bool shade_one(const lxo_scene& s, const RenderCfg& cfg, Path& p, const B
             }
         } else if (cfg.dmis) {
             if (emissive) {
-                float w = 1.0f;
+                lxr w = 1.0f;
                 if (depth > 1 && !p.spec) {
-                    float pb = p.last_pdf;
-                    float dist_sq = is.t * is.t;
-                    float cl = dot(is.N, neg(dir));
+                    lxr pb = p.last_pdf;
+                    lxr dist_sq = is.t * is.t;
+                    lxr cl = dot(is.N, neg(dir));
                     if (cl > 0.0001f) {
-                        float pa = 1.0f / s.total_area;
-                        float psa = pa * (dist_sq / cl);
+                        lxr pa = 1.0f / s.total_area;
+                        lxr psa = pa * (dist_sq / cl);
                         w = (pb * pb) / (pb * pb + psa * psa);
                     } else {
                         w = 0.0f;
@@ -1499,16 +1542,16 @@ static bool shade_one(const lxo_scene& s, const RenderCfg& cfg, Path& p, const B
             } else if (br.dl.t > 0.0f && !is_specular && !p.kill) {
                 V3 sp = add(smul(is.t, dir), p.o);
                 V3 lv = sub(p.dl_sample, sp);
-                float dsq = dot(lv, lv);
-                float dist = lxm_sqrt(dsq);
+                lxr dsq = dot(lv, lv);
+                lxr dist = lxm_sqrt(dsq);
                 V3 ld = divs(lv, dist);
-                float cs = dot(nf, ld);
-                float cl = dot(br.dl.N, neg(ld));
+                lxr cs = dot(nf, ld);
+                lxr cl = dot(br.dl.N, neg(ld));
                 if (cs > 0.0f && cl > 0.0f) {
-                    float pa = 1.0f / s.total_area;
-                    float psa = pa * (dsq / cl);
-                    float pb = pdf_bsdf(ld);
-                    float w = (psa * psa) / (psa * psa + pb * pb);
+                    lxr pa = 1.0f / s.total_area;
+                    lxr psa = pa * (dsq / cl);
+                    lxr pb = pdf_bsdf(ld);
+                    lxr w = (psa * psa) / (psa * psa + pb * pb);
                     V3 brdf = brdf_of(ld);
                     V3 Le = get_emission(s, s.mats[br.dl.mat], br.dl.uv);
                     V3 contrib = divs(muls(mul(Le, brdf), cs), psa);
@@ -1525,7 +1568,7 @@ static bool shade_one(const lxo_scene& s, const RenderCfg& cfg, Path& p, const B
                 bool transmitted = !cfg.legacy && dot(neg(dir), normal) * dot(p.sample_dir, normal) < 0.0f;
                 bool spec = is_specular || transmitted;
                 V3 ct = ct_eval_cos(neg(dir), normal, p.sample_dir, color, rough, metal, mat.ior, mat.transmission);
-                float pdf = PDF(neg(dir), p.sample_dir, normal, rough, metal, color, mat.ior, mat.transmission);
+                lxr pdf = PDF(neg(dir), p.sample_dir, normal, rough, metal, color, mat.ior, mat.transmission);
                 p.thr = mul(p.thr, divs(ct, pdf));
                 p.last_pdf = pdf;
                 p.spec = spec;
@@ -1535,8 +1578,8 @@ static bool shade_one(const lxo_scene& s, const RenderCfg& cfg, Path& p, const B
                 p.spec = is_specular;
             }
             if (depth >= 3) {
-                float mt = lxm_max(p.thr.x, lxm_max(p.thr.y, p.thr.z));
-                float sv = lxm_clamp(mt, 0.0f, 1.0f);
+                lxr mt = lxm_max(p.thr.x, lxm_max(p.thr.y, p.thr.z));
+                lxr sv = lxm_clamp(mt, 0.0f, 1.0f);
                 if (rng.u01() >= sv) p.kill = true;
                 else p.thr = divs(p.thr, sv);
             }
@@ -1545,10 +1588,10 @@ static bool shade_one(const lxo_scene& s, const RenderCfg& cfg, Path& p, const B
     } else if (!p.kill && has_exr) {
         V3 d = normalize(dir);
         V3 L = env_lookup(s, d, cfg.intensity);
-        float w = 1.0f;
+        lxr w = 1.0f;
         if (cfg.emis && depth > 1 && !p.spec) {
-            float pb = p.last_pdf;
-            float pe = envmap_pdf(s, d);
+            lxr pb = p.last_pdf;
+            lxr pe = envmap_pdf(s, d);
             w = (pb * pb) / (pb * pb + pe * pe);
         }
         p.color = add(p.color, muls(mul(p.thr, L), w));
@@ -1559,7 +1602,7 @@ static bool shade_one(const lxo_scene& s, const RenderCfg& cfg, Path& p, const B
         /* getPointOnRay, include/intersections.h:29-32 */
         V3 hp = add(p.o, smul(is.t - .0001f, normalize(dir)));
         V3 n = is.N;
-        float l2 = dot(p.sample_dir, p.sample_dir);
+        lxr l2 = dot(p.sample_dir, p.sample_dir);
         if (!(l2 > 1e-12f) || !lxm_isfinite(l2)) {
             p.kill = true;
             return false;
@@ -1574,8 +1617,8 @@ static bool shade_one(const lxo_scene& s, const RenderCfg& cfg, Path& p, const B
 /* One iteration of pathtrace(), src/pathtrace.cu:97-367, over a list of
  * pixels. */
 static void render_iteration(const lxo_scene& s, const RenderCfg& cfg, int iter, const uint32_t* pixels, uint32_t n,
-                             float* rgb_sum, Counters& cnt, std::vector<std::vector<uint32_t>>* survivors,
-                             std::vector<std::vector<float>>* records) {
+                             lxr* rgb_sum, Counters& cnt, std::vector<std::vector<uint32_t>>* survivors,
+                             std::vector<std::vector<lxr>>* records) {
     std::vector<Path> paths(n);
     std::vector<uint32_t> queue(n), next;
     next.reserve(n);
@@ -1622,7 +1665,7 @@ static void render_iteration(const lxo_scene& s, const RenderCfg& cfg, int iter,
                 V3 so = add(hp, muls(br.isect.N, 1e-4f));
                 if (cfg.dmis) {
                     V3 ld = sub(p.dl_sample, so);
-                    float dist = length(ld);
+                    lxr dist = length(ld);
                     ld = muls(ld, 1.0f / dist);
                     Hit hd = trace_scene(s, so, ld, dist * 1.0001f + 1e-4f, cfg.mode, tc);
                     cnt.dl++;
@@ -1653,16 +1696,16 @@ static void render_iteration(const lxo_scene& s, const RenderCfg& cfg, int iter,
             cnt.bounces++;
             if (records) {
                 const Rec& r = recs[q].isect;
-                std::vector<float>& rv = (*records)[depth - 1];
-                float t = r.t > 0.0f ? r.t : -1.0f;
+                std::vector<lxr>& rv = (*records)[depth - 1];
+                lxr t = r.t > 0.0f ? r.t : -1.0f;
                 rv.push_back(t);
                 rv.push_back(r.t > 0.0f ? r.N.x : 0.0f);
                 rv.push_back(r.t > 0.0f ? r.N.y : 0.0f);
                 rv.push_back(r.t > 0.0f ? r.N.z : 0.0f);
                 rv.push_back(r.t > 0.0f ? r.uv.x : 0.0f);
                 rv.push_back(r.t > 0.0f ? r.uv.y : 0.0f);
-                rv.push_back(r.t > 0.0f ? (float)r.mat : -1.0f);
-                rv.push_back((float)paths[q].pixel);
+                rv.push_back(r.t > 0.0f ? (lxr)r.mat : -1.0f);
+                rv.push_back((lxr)paths[q].pixel);
             }
             if (shade_one(s, cfg, paths[q], recs[q], iter, depth, compact)) next.push_back(q);
         }
@@ -1776,7 +1819,7 @@ lxo_scene* lxo_scene_create(const lx_scene_desc* d, int cdf_mode, int build_bvh)
         Inst& g = s->insts[i];
         g.mesh = d->instances[i].mesh;
         g.mat = d->instances[i].material;
-        std::memcpy(g.m, d->instances[i].transform, sizeof(g.m));
+        for (int k = 0; k < 12; k++) g.m[k] = d->instances[i].transform[k];
         affine_inverse(g.m, g.inv);
         const Mesh& m = s->meshes[g.mesh];
         BBox b;
@@ -1813,15 +1856,15 @@ int lxo_scene_bvh_depth(const lxo_scene* s, uint32_t mesh) { return s->meshes[me
 
 int lxo_read_env_cdfs(const lxo_scene* s, float* marginal, float* conditional) {
     if (!s->has_env) return -1;
-    if (marginal) std::memcpy(marginal, s->marginal.data(), s->marginal.size() * 4);
-    if (conditional) std::memcpy(conditional, s->conditional.data(), s->conditional.size() * 4);
+    if (marginal) std::copy(s->marginal.begin(), s->marginal.end(), marginal);
+    if (conditional) std::copy(s->conditional.begin(), s->conditional.end(), conditional);
     return 0;
 }
 
 int lxo_read_emitter_cdfs(const lxo_scene* s, uint32_t* inst, float* prefix, uint32_t* num, float* total) {
     if (num) *num = (uint32_t)s->emissive.size();
     if (inst) std::memcpy(inst, s->emissive.data(), s->emissive.size() * 4);
-    if (prefix) std::memcpy(prefix, s->emissive_prefix.data(), s->emissive_prefix.size() * 4);
+    if (prefix) std::copy(s->emissive_prefix.begin(), s->emissive_prefix.end(), prefix);
     if (total) *total = s->total_area;
     return 0;
 }
@@ -1829,7 +1872,7 @@ int lxo_read_emitter_cdfs(const lxo_scene* s, uint32_t* inst, float* prefix, uin
 int lxo_read_instance_tri_cdf(const lxo_scene* s, uint32_t k, float* tri_prefix, uint32_t* n) {
     if (k >= s->tri_cdf.size()) return -1;
     if (n) *n = (uint32_t)s->tri_cdf[k].size();
-    if (tri_prefix) std::memcpy(tri_prefix, s->tri_cdf[k].data(), s->tri_cdf[k].size() * 4);
+    if (tri_prefix) std::copy(s->tri_cdf[k].begin(), s->tri_cdf[k].end(), tri_prefix);
     return 0;
 }
 
@@ -1864,7 +1907,7 @@ int lxo_render(const lxo_scene* s, const lxo_render_opts* o, uint32_t first_iter
                const uint32_t* pixels, uint32_t n_pixels, float* rgb_sum, lxo_stats* stats) {
     RenderCfg cfg = make_cfg(*s, o);
     uint32_t n = pixels ? n_pixels : (uint32_t)(cfg.W * cfg.H);
-    std::memset(rgb_sum, 0, sizeof(float) * 3 * n);
+    std::vector<lxr> acc(3 * (size_t)n, (lxr)0);
     int threads = o->threads > 0 ? o->threads : 1;
     if ((uint32_t)threads > n) threads = n > 0 ? (int)n : 1;
     std::vector<uint32_t> all;
@@ -1880,11 +1923,12 @@ int lxo_render(const lxo_scene* s, const lxo_render_opts* o, uint32_t first_iter
         pool.emplace_back([&, t]() {
             uint32_t b = (uint32_t)((uint64_t)n * t / threads), e = (uint32_t)((uint64_t)n * (t + 1) / threads);
             for (uint32_t it = 0; it < n_iters; it++)
-                render_iteration(*s, cfg, (int)(first_iter + it), pixels + b, e - b, rgb_sum + 3 * (size_t)b, cnt[t],
-                                 nullptr, nullptr);
+                render_iteration(*s, cfg, (int)(first_iter + it), pixels + b, e - b, acc.data() + 3 * (size_t)b,
+                                 cnt[t], nullptr, nullptr);
         });
     }
     for (auto& th : pool) th.join();
+    std::copy(acc.begin(), acc.end(), rgb_sum);
     double t1 = now_ms();
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
@@ -1899,6 +1943,9 @@ int lxo_render(const lxo_scene* s, const lxo_render_opts* o, uint32_t first_iter
     return 0;
 }
 
+#ifndef LXO_FP64
+/* Survivor sets, hit records and unit functions: the fp32 oracle's checks
+ * only (the FP64 build exports scenes, tracing and rendering). */
 int lxo_debug_bounce(const lxo_scene* s, const lxo_render_opts* o, uint32_t iter, const uint32_t* pixels,
                      uint32_t n_pixels, uint32_t* survivors, uint32_t* survivor_counts, float* records,
                      uint32_t* record_counts, float* rgb_sum) {
@@ -2060,5 +2107,7 @@ void lxo_math_eval(int fn, const float* x, const float* y, float* out, uint32_t 
         out[i] = r;
     }
 }
+
+#endif /* LXO_FP64 */
 
 } /* extern "C" */
