@@ -1,9 +1,8 @@
 """Material debug views and the legacy (UBER_SHADER 0) shading model on the
 GPU, through the C ABI, against the oracle on the same seeded inputs.
 
-Bars: debug views bit-exact; legacy renders within 1e-4 relative per pixel
-(reported exact-match fraction, > 0.999 required) for the plain, -dmis and
--emis branches; the tail kernel and the per-bounce kernels give the same
+Bars: debug views bit-exact; legacy renders bit-exact per pixel for the
+plain, -dmis and -emis branches; the tail kernel and the per-bounce kernels give the same
 bits; a batch of debug / legacy jobs gives each job's single-render bits;
 the LDR read-back of a debug view is untonemapped.
 """
@@ -28,7 +27,7 @@ def _close(gpu, ref, what):
     ref = np.asarray(ref, np.float32).reshape(-1)
     exact = float(np.mean(gpu == ref))
     print(f"{what}: exact-match fraction {exact:.6f}, max |diff| {np.abs(gpu - ref).max():.3e}", flush=True)
-    np.testing.assert_allclose(gpu, ref, rtol=1e-4, atol=1e-6)
+    assert np.array_equal(gpu.view(np.uint32), ref.view(np.uint32)), what
     return exact
 
 
@@ -40,7 +39,7 @@ def test_legacy_render_matches_oracle(ctx, zoo, flags):
     img, spp = r.accum()
     want, _ = ref.render(1, 2, shading_model=F.LX_SHADING_LEGACY, threads=8, **flags)
     assert spp == 2
-    assert _close(img, want, f"legacy {flags}") > 0.999
+    assert _close(img, want, f"legacy {flags}") == 1.0
     # the uber shader renders the same scene differently
     u = lx.Render(sc, **flags)
     u.iterate(1, 2)

@@ -4,8 +4,9 @@ the same seeded inputs.
 Bars (from the path's specification):
   - closest-hit instance / primitive ids and t: bit-exact;
   - survivor sets after compaction: identical per bounce;
-  - per-pixel 1-spp radiance: within 1e-4 relative (the build aims for and
-    reports bit-exact agreement);
+  - per-pixel 1-spp radiance: within 1e-4 relative by the specification;
+    this build renders the oracle's bits, and the tests require bit
+    equality;
   - deterministic math: bit-exact over sweeps.
 """
 import math
@@ -18,8 +19,6 @@ from luxion_amd import scenes
 from luxion_amd import _ffi as F
 
 pytestmark = pytest.mark.gpu
-
-RTOL = 1e-4
 
 
 def _rays(n, lo, hi, seed, tmax=1e16):
@@ -36,7 +35,7 @@ def _close(gpu, ref, what):
     ref = np.asarray(ref, np.float32).reshape(-1)
     exact = np.mean(gpu == ref)
     print(f"{what}: exact-match fraction {exact:.6f}, max |diff| {np.abs(gpu - ref).max():.3e}", flush=True)
-    np.testing.assert_allclose(gpu, ref, rtol=RTOL, atol=1e-6)
+    assert np.array_equal(gpu.view(np.uint32), ref.view(np.uint32)), what
     return exact
 
 
@@ -129,7 +128,7 @@ def test_render_cornell_1spp_matches_oracle(ctx, oracle):
     want, _ = ref.render(1, 1, dmis=True, threads=8)
     assert spp == 1
     exact = _close(img, want, "cornell 1 spp -dmis")
-    assert exact > 0.999
+    assert exact == 1.0
 
 
 def test_survivor_sets_and_records_match_oracle(ctx, oracle):
@@ -168,7 +167,7 @@ def test_render_icosphere_emis_matches_oracle(ctx, oracle):
     img, _ = r.accum()
     want, _ = ref.render(1, 2, emis=True, threads=8)
     exact = _close(img, want, "icosphere -emis 2 spp")
-    assert exact > 0.999
+    assert exact == 1.0
     surv, rec = r.debug_bounce(3)
     surv_o, rec_o, _ = ref.debug_bounce(3, emis=True)
     for d in range(data.trace_depth):
@@ -189,7 +188,7 @@ def test_render_full_cfg1_subset_matches_oracle(ctx, oracle):
     want, _ = ref.render(1, 1, pixels=pix, emis=True, threads=8)
     got = img.reshape(-1, 3)[pix]
     exact = _close(got, want, "cfg1 full frame, 3000 pixels")
-    assert exact > 0.999
+    assert exact == 1.0
 
 
 def test_graph_replay_equals_eager(ctx, oracle):
@@ -253,4 +252,4 @@ def test_render_instanced_interior_dmis_emis_matches_oracle(ctx, oracle):
         img, _ = r.accum()
         want, _ = ref.render(2, 1, threads=8, **kw)
         exact = _close(img, want, f"instanced interior {kw}")
-        assert exact > 0.999
+        assert exact == 1.0
