@@ -5,10 +5,11 @@ sun of radiance 1e4, depth 12, environment importance sampling + MIS).
 
 A step is one 1-spp frame (one round): camera rays, then the bounces (by
 default all in the persistent tail kernel: radiance + environment shadow
-rays, uber-shader, MIS, Russian roulette, accumulation).  Consecutive rounds
-overlap on two streams (tuning overlap_rounds), so one frame's last long
-paths share the GPU with the next frame's first bounces; the accumulated
-image is the same bits.  Inputs are resident in HBM before the timed
+rays, uber-shader, MIS, Russian roulette, accumulation).  A round renders
+twenty consecutive frames as twenty copies of the job (tuning round_frames),
+so the last long paths of a frame share the GPU with the other frames' paths
+and the launch's final drain is paid once per twenty frames; the
+accumulated image is the same bits as one frame after another.  Inputs are resident in HBM before the timed
 region.  Per-kernel times for the roofline come from a separate pass with
 HIP events around every kernel.  Output: one JSON line.
 """
@@ -29,12 +30,12 @@ RAY_BYTES, HIT_BYTES, OCC_BYTES = 32, 16, 4
 # rays, one queue word (texels not counted)
 SHADE_BYTES = 2 * 48 + 3 * 16 + 12 + 36 + 24 + 64 + 32 + 2 * 32 + 4
 # launch shapes for this workload, applied unless --autotune 1: the fastest
-# measured for the 1080p frame with overlapped rounds (tail kernel for every
-# bounce, 256-lane blocks, 16 waves per CU, ray-cell queue sort, 16x16 camera
-# squares in bands of 8 square rows); fixed so that runs, and the profiles of
-# the same command, time the same kernels
+# measured for the 1080p frame (tail kernel for every bounce, 256-lane
+# blocks, 16 waves per CU, no queue sort, 16x16 camera squares in bands of 8
+# square rows, twenty frames per round, one round at a time); fixed so that
+# runs, and the profiles of the same command, time the same kernels
 WORKLOAD_TUNING = dict(wavefront_bounces=0, block_size=256, waves_per_cu=16, queue_chunk=32, tail_min_waves=4,
-                       bvh_width=4, sort_mode=1, pixel_tile=16, tile_band=8, overlap_rounds=1)
+                       bvh_width=4, sort_mode=0, pixel_tile=16, tile_band=8, overlap_rounds=0, round_frames=20)
 
 
 def measured_traffic(kernel):
@@ -147,9 +148,9 @@ def main():
     r.sync()
     log(f"warmup done ({args.warmup} steps)")
 
-    # timed region: the frames as a caller renders them (consecutive rounds
-    # overlapped on two streams when the tuning says so), synchronised on
-    # both sides
+    # timed region: the frames as a caller renders them (rounds of
+    # round_frames frames, overlapped when overlap_rounds says so),
+    # synchronised on both sides
     first = args.warmup + 1
     r.reset()
     t0 = time.perf_counter()
@@ -159,10 +160,10 @@ def main():
     st_main = r.stats()
     rays = st_main["radiance_rays"] + st_main["dl_shadow_rays"] + st_main["env_shadow_rays"]
     log(f"timed region: {args.steps} steps in {t_timed * 1e3:.1f} ms, {rays} rays "
-        f"(overlap_rounds {tuning['overlap_rounds']})")
+        f"(overlap_rounds {tuning['overlap_rounds']}, round_frames {tuning['round_frames']})")
 
     # one round after another, replayed from a captured hipGraph
-    ctx.set_tuning(overlap_rounds=0)
+    ctx.set_tuning(overlap_rounds=0, round_frames=1)
     r.reset()
     r.iterate(first, 1)
     r.sync()
@@ -174,9 +175,10 @@ def main():
     st_graph = r.stats()
     log(f"sequential rounds (graph replay): {args.steps} steps in {t_graph * 1e3:.1f} ms")
 
-    # kernel timing pass: the same frames with HIP events before and after
-    # each kernel on the render's stream (per-kernel device time for the
-    # roofline; rounds run one after another)
+    # kernel timing pass: the timed region's rounds again, with HIP events
+    # before and after each kernel on the stream it runs on (per-kernel
+    # device time for the roofline)
+    ctx.set_tuning(overlap_rounds=tuning["overlap_rounds"], round_frames=tuning["round_frames"])
     r.reset()
     r.set_timing(events=True)
     t0 = time.perf_counter()
@@ -184,7 +186,6 @@ def main():
     r.sync()
     t_events = time.perf_counter() - t0
     st = r.stats()
-    ctx.set_tuning(overlap_rounds=tuning["overlap_rounds"])
     assert st["radiance_rays"] == st_main["radiance_rays"], "kernel timing pass traced a different ray batch"
     log(f"kernel timing pass: {args.steps} steps in {t_events * 1e3:.1f} ms")
 
@@ -371,10 +372,11 @@ def main():
                           "env_shadow": st["env_shadow_rays"] // args.steps,
                           "dl_shadow": st["dl_shadow_rays"] // args.steps},
         "sequential_rounds": {"value": round(value_graph, 2), "ms_per_step": round(t_graph * 1e3 / args.steps, 3),
-                              "note": "rounds one after another (overlap_rounds 0), graph replay"},
+                              "note": "one frame per round, rounds one after another (round_frames 1, "
+                                      "overlap_rounds 0), graph replay"},
         "kernel_timing_pass": {"value": round(value_events, 2), "ms_per_step": round(t_events * 1e3 / args.steps, 3),
-                               "note": "rounds one after another, eager launches with HIP events around "
-                                       "every kernel (source of kernel_ms_per_step and roofline)"},
+                               "note": "the timed region's rounds with HIP events around every kernel "
+                                       "(source of kernel_ms_per_step and roofline)"},
         "kernel_ms_per_step": {"trace": round(st["trace_ms"] / args.steps, 3),
                                "shade": round(st["shade_ms"] / args.steps, 3),
                                "tail": round(st["tail_ms"] / args.steps, 3),
@@ -382,15 +384,16 @@ def main():
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
                      "bytes_per_launch": int(bytes_per_launch), "avg_launch_ms": round(launch_ms, 4),
+                     "frames_per_launch": round(args.steps / max(1, klaunch), 2),
                      "issue": measured_issue(kname),
                      "limiter": "VALU issue with lanes idle behind each wave's longest ray (traversal "
                                 "divergence), not HBM bandwidth: HBM traffic is under half the algorithmic bytes",
                      "per_ray": {k: round(v, 2) for k, v in per_ray.items()},
-                     "overlapped_frame": {
+                     "timed_region_frame": {
                          "achieved": round(kbytes / args.steps / (t_timed / args.steps) / 1e9, 1),
                          "frac": round(kbytes / args.steps / (t_timed / args.steps) / 1e9 / HBM_PEAK_GBPS, 4),
                          "note": "the same algorithmic bytes per frame over the timed region's frame time "
-                                 "(consecutive rounds overlapped)"}},
+                                 "(its rounds shaped by round_frames / overlap_rounds)"}},
         "tuning": tuning,
         "batched": batched,
         "served": served,

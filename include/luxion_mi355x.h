@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define LX_ABI_VERSION 3
+#define LX_ABI_VERSION 4
 
 typedef enum lx_status {
     LX_OK = 0,
@@ -422,6 +422,17 @@ typedef struct lx_tuning {
                                   server four rounds whenever this is nonzero.
                                   0: one round after another on the render's
                                   stream (captured in a hipGraph with use_graph) */
+    int32_t round_frames;      /* 1..32 (default 1): lx_render_iterate of two or more
+                                  samples (without work counting; event timing
+                                  brackets each round) renders up to this many
+                                  consecutive samples in one round (one job copy
+                                  per sample, each into a colour buffer of its own,
+                                  added to the accumulation in sample order; same
+                                  bits), so a round's last long paths are shared
+                                  by that many frames; with overlap_rounds, rounds
+                                  of this width overlap.  Path buffers grow by
+                                  this factor; a render whose HBM budget cannot
+                                  hold them renders as with 1 */
 } lx_tuning;
 
 /* Coherence stages (the reference's RAY_SORTING and MATERIAL_SORTING,

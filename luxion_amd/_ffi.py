@@ -130,7 +130,8 @@ class lx_tuning(C.Structure):
     _fields_ = [("block_size", C.c_int32), ("bvh_width", C.c_int32), ("queue_chunk", C.c_int32),
                 ("waves_per_cu", C.c_int32), ("use_graph", C.c_int32), ("max_paths_per_round", C.c_uint32),
                 ("wavefront_bounces", C.c_int32), ("tail_min_waves", C.c_int32), ("sort_mode", C.c_int32),
-                ("pixel_tile", C.c_int32), ("tile_band", C.c_int32), ("overlap_rounds", C.c_int32)]
+                ("pixel_tile", C.c_int32), ("tile_band", C.c_int32), ("overlap_rounds", C.c_int32),
+                ("round_frames", C.c_int32)]
 
 
 class lx_server_opts(C.Structure):

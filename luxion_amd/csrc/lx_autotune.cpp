@@ -47,7 +47,7 @@ std::vector<Knob> knobs() {
 
 const char* kFields[] = {"block_size", "bvh_width", "queue_chunk", "waves_per_cu", "use_graph",
                          "max_paths_per_round", "wavefront_bounces", "tail_min_waves", "sort_mode",
-                         "pixel_tile", "tile_band", "overlap_rounds"};
+                         "pixel_tile", "tile_band", "overlap_rounds", "round_frames"};
 
 int32_t* field_ptr(lx_tuning& t, const std::string& k) {
     if (k == "block_size") return &t.block_size;
@@ -62,6 +62,7 @@ int32_t* field_ptr(lx_tuning& t, const std::string& k) {
     if (k == "pixel_tile") return &t.pixel_tile;
     if (k == "tile_band") return &t.tile_band;
     if (k == "overlap_rounds") return &t.overlap_rounds;
+    if (k == "round_frames") return &t.round_frames;
     return nullptr;
 }
 
@@ -124,6 +125,7 @@ void check_tuning(const lx_tuning& t) {
                "pixel_tile must be 1, 8, 16 or 32");
     LX_REQUIRE(t.tile_band >= 0 && t.tile_band <= 64, LX_ERR_INVALID, "tile_band must be 0..64");
     LX_REQUIRE(t.overlap_rounds >= 0 && t.overlap_rounds <= 3, LX_ERR_INVALID, "overlap_rounds must be 0..3");
+    LX_REQUIRE(t.round_frames >= 1 && t.round_frames <= 32, LX_ERR_INVALID, "round_frames must be 1..32");
 }
 
 } // namespace

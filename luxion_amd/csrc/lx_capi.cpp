@@ -74,6 +74,7 @@ lx_status lx_context_create(int device, uint64_t budget, lx_context** out) {
         c->tuning.pixel_tile = 8;
         c->tuning.tile_band = 0;
         c->tuning.overlap_rounds = 1;
+        c->tuning.round_frames = 1;
     } catch (...) {
         if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
@@ -461,6 +462,154 @@ static void iterate_overlapped(lx_render* r, uint32_t first_iter, uint32_t n, in
     }
 }
 
+/* Buffers for wide rounds of R samples, D sets, made on first use (and
+ * remade when R changes).  Returns false when the HBM budget cannot hold
+ * them (the render then iterates without wide rounds). */
+static bool wide_setup(lx_render* r, uint32_t R, int D) {
+    lx_context* ctx = r->ctx;
+    uint64_t cap64 = (uint64_t)R * r->npix;
+    if (cap64 >= (1ull << 31)) return false;
+    const uint32_t cap = (uint32_t)cap64;
+    try {
+        if (!r->wide_counters.ptr) {
+            r->wide_counters.alloc(ctx, lx_render::kMaxRounds * sizeof(LxCounters));
+            LX_HIP(lx_launch_fill_u32(r->wide_counters.as<uint32_t>(),
+                                      lx_render::kMaxRounds * sizeof(LxCounters) / 4, 0u, ctx->stream));
+        }
+    } catch (const LxError& e) {
+        if (e.status != LX_ERR_BUDGET) throw;
+        return false;
+    }
+    if (r->wide_frames != R && r->wide_sets > 0) {
+        /* the previous calls' rounds end before their buffers are freed:
+         * every round's merge waits for the round on the context stream */
+        LX_HIP(hipStreamSynchronize(ctx->stream));
+        for (int p = 0; p < lx_render::kMaxRounds; p++) r->wide[p] = lx_render::WideSet();
+        r->wide_sets = 0;
+    }
+    r->wide_frames = R;
+    if (r->wide_sets < D) {
+        uint32_t tg, sg, cg;
+        render_grids(ctx, cap, tg, sg, cg);
+        try {
+            for (int p = r->wide_sets; p < D; p++) {
+                auto& W = r->wide[p];
+                W.wb.alloc(ctx, cap, r->opts.dmis != 0, r->opts.emis != 0, scene_has_normal_maps(r->scene));
+                W.jobs.alloc(ctx, (size_t)R * sizeof(LxJob));
+                W.job_spp.alloc(ctx, (size_t)R * 4);
+                W.sched.alloc(ctx, sizeof(LxSched));
+                W.spill.alloc(ctx, (size_t)lx_spill_words(tg) * 4);
+                W.color.alloc(ctx, (size_t)cap * 12);
+                W.hj.assign(R, LxJob{});
+                W.hs.assign(R, 0);
+            }
+        } catch (const LxError& e) {
+            if (e.status != LX_ERR_BUDGET) throw;
+            for (int p = r->wide_sets; p < D; p++) r->wide[p] = lx_render::WideSet();
+            return false;
+        }
+        r->wide_sets = D;
+        r->wide_grid[0] = tg;
+        r->wide_grid[1] = sg;
+        r->wide_grid[2] = cg;
+    }
+    for (int p = 0; p < D; p++) {
+        if (!r->pstream[p]) LX_HIP(hipStreamCreateWithFlags(&r->pstream[p], hipStreamNonBlocking));
+        if (!r->ev_done[p]) LX_HIP(hipEventCreateWithFlags(&r->ev_done[p], hipEventDisableTiming));
+        if (!r->ev_free[p]) LX_HIP(hipEventCreateWithFlags(&r->ev_free[p], hipEventDisableTiming));
+    }
+    if (!r->ev_up) LX_HIP(hipEventCreateWithFlags(&r->ev_up, hipEventDisableTiming));
+    return true;
+}
+
+/* n >= 2 samples in wide rounds of R samples: chunk c (samples first_iter +
+ * c R .. + R - 1, fewer in the last) runs on stream c % D with set c % D,
+ * D = min(rounds, chunks).  Copy k of set p starts at sample p R + k and
+ * steps by D R, into region k of the set's colour pool; a chunk's regions
+ * are added to the accumulation in sample order after the previous chunk's,
+ * and chunk c + D waits for that add before it reuses the pool.  With
+ * timing on, events on each chunk's stream bracket its kernels, read back
+ * after a single synchronisation. */
+static void iterate_wide(lx_render* r, uint32_t first_iter, uint32_t n, int rounds) {
+    lx_context* ctx = r->ctx;
+    hipStream_t ms = ctx->stream;
+    const uint32_t R = r->wide_frames;
+    const uint32_t chunks = (n + R - 1) / R;
+    const int D = (int)std::min<uint32_t>((uint32_t)rounds, chunks);
+    const uint32_t step = (uint32_t)D * R;
+    for (int p = 0; p < D; p++) {
+        auto& W = r->wide[p];
+        for (uint32_t k = 0; k < R; k++) {
+            LxJob& J = W.hj[k];
+            const uint32_t v = (uint32_t)p * R + k;
+            J = r->job;
+            J.iter = (int32_t)first_iter + (int32_t)v - (int32_t)step;
+            J.spp_done = 0;
+            J.active = 0;
+            J.accum_off = k * r->npix;
+            J.flags = (J.flags & ~LX_JF_STEP_MASK) | (step << LX_JF_STEP_SHIFT);
+            W.hs[k] = v < n ? (int32_t)((n - v + step - 1) / step) : 0;
+        }
+        LX_HIP(hipMemcpyAsync(W.jobs.ptr, W.hj.data(), (size_t)R * sizeof(LxJob), hipMemcpyHostToDevice, ms));
+        LX_HIP(hipMemcpyAsync(W.job_spp.ptr, W.hs.data(), (size_t)R * 4, hipMemcpyHostToDevice, ms));
+    }
+    LX_HIP(hipEventRecord(r->ev_up, ms));
+    for (int p = 0; p < D; p++) LX_HIP(hipStreamWaitEvent(r->pstream[p], r->ev_up, 0));
+    const size_t nf = (size_t)r->npix * 3;
+    LxCounters* cnt = r->wide_counters.as<LxCounters>();
+    const int per = 2 * r->depth + 1;
+    std::vector<hipEvent_t> ev;
+    if (r->timing) {
+        ev.resize((size_t)(per + 2) * chunks);
+        for (auto& e : ev) LX_HIP(hipEventCreate(&e));
+    }
+    for (uint32_t c = 0; c < chunks; c++) {
+        const int p = (int)(c % (uint32_t)D);
+        auto& W = r->wide[p];
+        hipStream_t s = r->pstream[p];
+        const uint32_t rc = std::min(R, n - c * R); /* the copies this chunk schedules: k < rc */
+        if (c >= (uint32_t)D) LX_HIP(hipStreamWaitEvent(s, r->ev_free[p], 0));
+        LX_HIP(lx_launch_fill_u32(W.color.as<uint32_t>(), nf * rc, 0x80000000u, s));
+        hipEvent_t* e = r->timing ? &ev[(size_t)c * (per + 2)] : nullptr;
+        if (e) LX_HIP(hipEventRecord(e[0], s));
+        lx_enqueue_round(ctx, W.jobs.as<LxJob>(), W.job_spp.as<int32_t>(), R, cnt + p, W.sched.as<LxSched>(),
+                         R * r->npix, W.wb, W.color.as<float>(), W.spill.as<uint32_t>(), r->depth, r->wide_grid[0],
+                         r->wide_grid[1], r->wide_grid[2], r->scene->dev.bvh_width,
+                         r->scene->d_scene.as<LxDevScene>(), e ? e + 1 : nullptr, false, s);
+        if (e) LX_HIP(hipEventRecord(e[per + 1], s));
+        LX_HIP(hipEventRecord(r->ev_done[p], s));
+        LX_HIP(hipStreamWaitEvent(ms, r->ev_done[p], 0));
+        LX_HIP(lx_launch_add_regions(r->accum.as<float>(), W.color.as<float>(), nf, rc, ms));
+        LX_HIP(hipEventRecord(r->ev_free[p], ms));
+    }
+    if (r->timing) {
+        LX_HIP(hipStreamSynchronize(ms));
+        const int wf = lx_wavefront_bounces(ctx, r->depth);
+        for (uint32_t c = 0; c < chunks; c++) {
+            hipEvent_t* e = &ev[(size_t)c * (per + 2)];
+            float t;
+            for (int b = 0; b < wf; b++) {
+                LX_HIP(hipEventElapsedTime(&t, e[1 + 2 * b], e[2 + 2 * b]));
+                r->trace_ms += t;
+                LX_HIP(hipEventElapsedTime(&t, e[2 + 2 * b], e[3 + 2 * b]));
+                r->shade_ms += t;
+            }
+            if (wf < r->depth) {
+                LX_HIP(hipEventElapsedTime(&t, e[1 + 2 * wf], e[2 + 2 * wf]));
+                r->tail_ms += t;
+            }
+            LX_HIP(hipEventElapsedTime(&t, e[0], e[1]));
+            r->other_ms += t;
+            LX_HIP(hipEventElapsedTime(&t, e[per], e[per + 1]));
+            r->other_ms += t;
+        }
+        r->trace_launches += (uint32_t)wf * chunks;
+        r->shade_launches += (uint32_t)wf * chunks;
+        if (wf < r->depth) r->tail_launches += chunks;
+        for (auto& x : ev) (void)hipEventDestroy(x);
+    }
+}
+
 /* Every round set's counters. */
 static void read_counters(lx_render* r, LxCounters& c) {
     LX_HIP(hipMemcpy(&c, r->counters.ptr, sizeof(c), hipMemcpyDeviceToHost));
@@ -468,6 +617,11 @@ static void read_counters(lx_render* r, LxCounters& c) {
         LxCounters o;
         LX_HIP(hipMemcpy(&o, r->extra[p - 1].counters.ptr, sizeof(o), hipMemcpyDeviceToHost));
         lx_counters_add(c, o);
+    }
+    if (r->wide_counters.ptr) {
+        LxCounters w[lx_render::kMaxRounds];
+        LX_HIP(hipMemcpy(w, r->wide_counters.ptr, sizeof(w), hipMemcpyDeviceToHost));
+        for (auto& o : w) lx_counters_add(c, o);
     }
 }
 
@@ -480,6 +634,12 @@ lx_status lx_render_iterate(lx_render* r, uint32_t first_iter, uint32_t n) {
     std::lock_guard<std::mutex> lk(ctx->mu);
     ctx->set_device();
     const int rounds = 1 + ctx->tuning.overlap_rounds;
+    const uint32_t R = (uint32_t)std::max(1, ctx->tuning.round_frames);
+    if (n >= 2 && R >= 2 && !r->counting && wide_setup(r, R, rounds)) {
+        iterate_wide(r, first_iter, n, rounds);
+        r->spp += n;
+        return LX_OK;
+    }
     if (n >= 2 && rounds >= 2 && !r->timing && !r->counting && overlap_setup(r, rounds)) {
         iterate_overlapped(r, first_iter, n, rounds);
         r->spp += n;
@@ -571,6 +731,12 @@ lx_status lx_render_sync(lx_render* r) {
                          hipMemcpyDeviceToHost));
         ov |= o;
     }
+    for (int p = 0; r->wide_counters.ptr && p < lx_render::kMaxRounds; p++) {
+        uint32_t o = 0;
+        LX_HIP(hipMemcpy(&o, (uint8_t*)r->wide_counters.ptr + p * sizeof(LxCounters) + offsetof(LxCounters, stack_overflow),
+                         4, hipMemcpyDeviceToHost));
+        ov |= o;
+    }
     LX_REQUIRE(ov == 0, LX_ERR_STATE, "traversal stack overflow (BVH deeper than the supported stack)");
     LX_CATCH
 }
@@ -610,6 +776,9 @@ lx_status lx_render_reset(lx_render* r) {
     LX_HIP(lx_launch_fill_u32(r->counters.as<uint32_t>(), sizeof(LxCounters) / 4, 0u, r->ctx->stream));
     for (int p = 1; p < r->sets_ready; p++)
         LX_HIP(lx_launch_fill_u32(r->extra[p - 1].counters.as<uint32_t>(), sizeof(LxCounters) / 4, 0u,
+                                  r->ctx->stream));
+    if (r->wide_counters.ptr)
+        LX_HIP(lx_launch_fill_u32(r->wide_counters.as<uint32_t>(), lx_render::kMaxRounds * sizeof(LxCounters) / 4, 0u,
                                   r->ctx->stream));
     LX_HIP(hipStreamSynchronize(r->ctx->stream));
     r->spp = 0;

@@ -241,6 +241,22 @@ struct lx_render {
     DevBuf color[kMaxRounds];
     hipStream_t pstream[kMaxRounds] = {};
     hipEvent_t ev_up = nullptr, ev_done[kMaxRounds] = {}, ev_free[kMaxRounds] = {};
+
+    /* wide rounds (tuning.round_frames = R > 1): a round renders R
+     * consecutive samples as R copies of the job (copy k into region k of
+     * the set's colour pool), with the overlapped rounds' streams and
+     * events; sets sized for wide_frames samples, counters kept apart so
+     * that resizing the sets keeps the statistics */
+    struct WideSet {
+        WaveBufs wb;
+        DevBuf jobs, job_spp, sched, spill, color;
+        std::vector<LxJob> hj;   /* sources of async copies */
+        std::vector<int32_t> hs;
+    } wide[kMaxRounds];
+    int wide_sets = 0;
+    uint32_t wide_frames = 0;
+    uint32_t wide_grid[3] = {};
+    DevBuf wide_counters; /* kMaxRounds LxCounters */
 };
 
 /* Records one round (begin, camera, bounces, end) on the stream. */

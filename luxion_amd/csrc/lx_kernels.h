@@ -64,6 +64,8 @@ hipError_t lx_launch_math_eval(int fn, const float* x, const float* y, float* ou
 hipError_t lx_launch_copy16(const void* src, void* dst, size_t n16, uint32_t grid, hipStream_t s);
 hipError_t lx_launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 hipError_t lx_launch_add_f32(float* dst, const float* src, size_t n, hipStream_t s);
+/* dst[i] += src[0 * n + i], then += src[1 * n + i], ... in region order */
+hipError_t lx_launch_add_regions(float* dst, const float* src, size_t n, uint32_t regions, hipStream_t s);
 /* ranges: nranges pairs (first pixel, pixel count) of an RGB float pool, in device memory
  * (nranges <= 65535) */
 hipError_t lx_launch_fill_ranges(uint32_t* p, const uint32_t* ranges, uint32_t nranges, uint32_t v, hipStream_t s);

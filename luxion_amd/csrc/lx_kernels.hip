@@ -738,6 +738,16 @@ __global__ void k_add_f32(float* __restrict__ dst, const float* __restrict__ src
         dst[i] = dst[i] + src[i];
 }
 
+/* The regions of a wide round's colour pool added in sample order: the same
+ * float sums as one add per region. */
+__global__ void k_add_regions(float* __restrict__ dst, const float* __restrict__ src, size_t n, uint32_t regions) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        float a = dst[i];
+        for (uint32_t k = 0; k < regions; k++) a = a + src[(size_t)k * n + i];
+        dst[i] = a;
+    }
+}
+
 /* ------------------------------------------------------------ launchers */
 hipError_t lx_launch_begin_round(LxJob* jobs, const int32_t* job_spp, uint32_t njobs, LxCounters* C, LxSched* sched,
                                  uint32_t cap, hipStream_t s) {
@@ -871,5 +881,12 @@ hipError_t lx_launch_add_f32(float* dst, const float* src, size_t n, hipStream_t
     if (blocks > 65535) blocks = 65535;
     if (blocks == 0) blocks = 1;
     hipLaunchKernelGGL(k_add_f32, dim3((uint32_t)blocks), dim3(256), 0, s, dst, src, n);
+    return hipGetLastError();
+}
+hipError_t lx_launch_add_regions(float* dst, const float* src, size_t n, uint32_t regions, hipStream_t s) {
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 65535) blocks = 65535;
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(k_add_regions, dim3((uint32_t)blocks), dim3(256), 0, s, dst, src, n, regions);
     return hipGetLastError();
 }

@@ -38,7 +38,7 @@ def test_ffi_signature_table_covers_header():
 
 def test_abi_version_and_clean_error_state():
     lib = F.load()
-    assert lib.lx_abi_version() == 3
+    assert lib.lx_abi_version() == 4
     assert lib.lx_last_error() == b""
 
 
