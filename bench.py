@@ -36,6 +36,10 @@ SHADE_BYTES = 2 * 48 + 3 * 16 + 12 + 36 + 24 + 64 + 32 + 2 * 32 + 4
 # runs, and the profiles of the same command, time the same kernels
 WORKLOAD_TUNING = dict(wavefront_bounces=0, block_size=256, waves_per_cu=16, queue_chunk=32, tail_min_waves=4,
                        bvh_width=4, sort_mode=0, pixel_tile=16, tile_band=8, overlap_rounds=0, round_frames=20)
+# the batched and served legs (and the per-job renders they are compared
+# with) run one sample per round, overlapped, with the ray-cell sort: the
+# library's round shape for plain per-job launches
+FORMS_TUNING = dict(WORKLOAD_TUNING, sort_mode=1, overlap_rounds=1, round_frames=1)
 
 
 def measured_traffic(kernel):
@@ -186,6 +190,7 @@ def main():
     r.sync()
     t_events = time.perf_counter() - t0
     st = r.stats()
+    r.close()  # frees the wide rounds' path buffers before the other legs
     assert st["radiance_rays"] == st_main["radiance_rays"], "kernel timing pass traced a different ray batch"
     log(f"kernel timing pass: {args.steps} steps in {t_events * 1e3:.1f} ms")
 
@@ -312,12 +317,25 @@ def main():
         log(f"accuracy: {accuracy}")
 
     batched = served = None
+    if not args.autotune:
+        ctx.set_tuning(**FORMS_TUNING)
     if args.batched:
         from luxion_amd import workloads
         spec = workloads.mixed_jobs([(sc, dict(emis=True), (0.0, 0.0, 0.0), 3.0)], args.batched, seed=3)
         log(f"batched: {len(spec)} jobs, {workloads.path_samples(spec) / 1e6:.1f} M path-samples")
         batched, _ = workloads.run_batched(ctx, spec)
         log(f"batched: {batched}")
+        if not args.autotune:
+            # the same per-job renders with the timed region's wide rounds
+            # (each job's samples twenty to a launch)
+            ctx.set_tuning(round_frames=WORKLOAD_TUNING["round_frames"])
+            wide, _ = workloads.run_batched(ctx, spec)
+            ctx.set_tuning(**FORMS_TUNING)
+            batched["vs_per_job_wide_rounds"] = {
+                "round_frames": WORKLOAD_TUNING["round_frames"], "per_job_s": wide["per_job_s"],
+                "speedup": round(wide["per_job_s"] / batched["batched_s"], 3),
+                "images_bitexact": wide["images_bitexact"]}
+            log(f"batched vs per-job wide rounds: {batched['vs_per_job_wide_rounds']}")
     if args.served != "off":
         from luxion_amd import workloads
         specs = [(sc, dict(emis=True), (0.0, 0.0, 0.0), 3.0)]
@@ -395,6 +413,8 @@ def main():
                          "note": "the same algorithmic bytes per frame over the timed region's frame time "
                                  "(its rounds shaped by round_frames / overlap_rounds)"}},
         "tuning": tuning,
+        "forms_tuning": None if args.autotune else {k: FORMS_TUNING[k] for k in ("sort_mode", "overlap_rounds",
+                                                                            "round_frames")},
         "batched": batched,
         "served": served,
         "cpu_baseline": cpu,
