@@ -3,7 +3,7 @@ MI355X, on the single-mesh + HDRI workload (1,310,720-triangle displaced
 icosphere in 64 primitives + ground plane, 2048x1024 HDRI with a 0.5-degree
 sun of radiance 1e4, depth 12, environment importance sampling + MIS).
 
-A step is one 1-spp frame (one round): camera rays, then the bounces (by
+A step is one 1-spp frame: camera rays, then the bounces (by
 default all in the persistent tail kernel: radiance + environment shadow
 rays, uber-shader, MIS, Russian roulette, accumulation).  A round renders
 twenty consecutive frames as twenty copies of the job (tuning round_frames),
