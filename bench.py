@@ -326,16 +326,17 @@ def main():
         batched, _ = workloads.run_batched(ctx, spec)
         log(f"batched: {batched}")
         if not args.autotune:
-            # the same per-job renders with the timed region's wide rounds
-            # (each job's samples twenty to a launch)
+            # both forms with wide rounds: per-job renders take their samples
+            # twenty to a launch, the batch up to 2048 / jobs samples of every
+            # job per round
             ctx.set_tuning(round_frames=WORKLOAD_TUNING["round_frames"])
             wide, _ = workloads.run_batched(ctx, spec)
             ctx.set_tuning(**FORMS_TUNING)
-            batched["vs_per_job_wide_rounds"] = {
-                "round_frames": WORKLOAD_TUNING["round_frames"], "per_job_s": wide["per_job_s"],
-                "speedup": round(wide["per_job_s"] / batched["batched_s"], 3),
-                "images_bitexact": wide["images_bitexact"]}
-            log(f"batched vs per-job wide rounds: {batched['vs_per_job_wide_rounds']}")
+            batched["wide_rounds"] = {
+                "round_frames": WORKLOAD_TUNING["round_frames"], "batched_s": wide["batched_s"],
+                "batched_spp_per_s": wide["batched_spp_per_s"], "per_job_s": wide["per_job_s"],
+                "speedup_vs_per_job": wide["speedup_vs_per_job"], "images_bitexact": wide["images_bitexact"]}
+            log(f"batched with wide rounds: {batched['wide_rounds']}")
     if args.served != "off":
         from luxion_amd import workloads
         specs = [(sc, dict(emis=True), (0.0, 0.0, 0.0), 3.0)]

@@ -432,7 +432,10 @@ typedef struct lx_tuning {
                                   by that many frames; with overlap_rounds, rounds
                                   of this width overlap.  Path buffers grow by
                                   this factor; a render whose HBM budget cannot
-                                  hold them renders as with 1 */
+                                  hold them renders as with 1.  lx_batch_run
+                                  takes up to this many samples of every job per
+                                  round (at most 2048 job copies) when its
+                                  budget holds them (same bits) */
 } lx_tuning;
 
 /* Coherence stages (the reference's RAY_SORTING and MATERIAL_SORTING,

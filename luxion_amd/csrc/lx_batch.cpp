@@ -145,6 +145,35 @@ extern "C" lx_status lx_batch_run(lx_context* ctx, const lx_job* jobs, uint32_t 
         DevBuf d_jobs, d_spp, d_cnt, d_sched, d_spill, d_accum;
         d_accum.alloc(ctx, accum_pix * 12);
         uint32_t cap = lx_batch_path_cap(ctx, npix, any_dmis, (uint64_t)lx_spill_words(ctx->num_cus * 16u) * 4 + (1u << 20));
+        /* wide rounds (tuning.round_frames = R > 1): each round takes up to R
+         * samples of every job as R copies of it (copy k: samples k, k + R,
+         * ... into region k of a colour pool added in sample order), when the
+         * budget holds R copies of every job's paths at once */
+        uint32_t R = (uint32_t)std::max(1, ctx->tuning.round_frames);
+        int32_t max_spp = 1;
+        for (int32_t v : hspp) max_spp = std::max(max_spp, v);
+        R = std::min<uint32_t>(R, (uint32_t)max_spp);
+        R = std::min<uint32_t>(R, LX_SCHED_MAX / std::max<uint32_t>(1u, n)); /* copies the schedule holds */
+        bool wide = R > 1 && (uint64_t)n * R <= LX_SCHED_MAX && accum_pix * R < (1ull << 31) &&
+                    (uint64_t)cap >= accum_pix;
+        DevBuf d_colw;
+        if (wide) {
+            std::vector<uint32_t> npw;
+            for (uint32_t k = 0; k < R; k++) npw.insert(npw.end(), npix.begin(), npix.end());
+            try {
+                uint32_t capw = lx_batch_path_cap(ctx, npw, any_dmis,
+                                                  (uint64_t)lx_spill_words(ctx->num_cus * 16u) * 4 + (1u << 20) +
+                                                      accum_pix * R * 12);
+                wide = (uint64_t)capw >= accum_pix * R;
+                if (wide) {
+                    d_colw.alloc(ctx, accum_pix * R * 12);
+                    cap = capw;
+                }
+            } catch (const LxError& e) {
+                if (e.status != LX_ERR_BUDGET) throw;
+                wide = false;
+            }
+        }
         WaveBufs wb;
         wb.alloc(ctx, cap, any_dmis, true, any_tan);
         uint32_t waves = (uint32_t)std::max(1, ctx->tuning.waves_per_cu);
@@ -162,10 +191,35 @@ extern "C" lx_status lx_batch_run(lx_context* ctx, const lx_job* jobs, uint32_t 
         LX_HIP(lx_launch_fill_u32(d_cnt.as<uint32_t>(), sizeof(LxCounters) / 4, 0u, s));
         std::vector<int> fin;
         int rounds = lx_simulate_rounds(npix, hspp, cap, &fin);
+        DevBuf d_jobsw, d_sppw;
+        std::vector<LxJob> hjw;
+        std::vector<int32_t> hsw;
+        if (wide) {
+            /* copy k of job j (index k n + j): first sample k, step R, region k */
+            hjw.resize((size_t)n * R);
+            hsw.resize((size_t)n * R);
+            rounds = 0;
+            for (uint32_t k = 0; k < R; k++)
+                for (uint32_t i = 0; i < n; i++) {
+                    LxJob J = hj[i];
+                    J.iter = hj[i].iter + 1 + (int32_t)k - (int32_t)R;
+                    J.flags = (J.flags & ~LX_JF_STEP_MASK) | (R << LX_JF_STEP_SHIFT);
+                    J.accum_off = (uint32_t)(k * accum_pix) + hj[i].accum_off;
+                    hjw[(size_t)k * n + i] = J;
+                    hsw[(size_t)k * n + i] = (int32_t)k < hspp[i] ? (hspp[i] - (int32_t)k + (int32_t)R - 1) / (int32_t)R : 0;
+                }
+            for (uint32_t i = 0; i < n; i++) {
+                fin[i] = (hspp[i] + (int32_t)R - 1) / (int32_t)R - 1;
+                rounds = std::max(rounds, fin[i] + 1);
+            }
+            d_jobsw.upload(ctx, hjw.data(), hjw.size() * sizeof(LxJob));
+            d_sppw.upload(ctx, hsw.data(), hsw.size() * 4);
+        }
         /* overlapped rounds (tuning.overlap_rounds) need every job in every
          * round until it is done, so that round k is sample k of each job in
          * both forms and the per-pixel sums keep their order */
-        bool overlap = ctx->tuning.overlap_rounds != 0 && rounds >= 2 && n <= LX_SCHED_MAX && (uint64_t)cap >= accum_pix;
+        bool overlap = !wide && ctx->tuning.overlap_rounds != 0 && rounds >= 2 && n <= LX_SCHED_MAX &&
+                       (uint64_t)cap >= accum_pix;
         WaveBufs wb1;
         DevBuf d_jobs1, d_spp1, d_cnt1, d_sched1, d_spill1, d_col[2];
         std::vector<LxJob> hjp[2];
@@ -204,7 +258,7 @@ extern "C" lx_status lx_batch_run(lx_context* ctx, const lx_job* jobs, uint32_t 
         /* one captured round, replayed; an event after every round dates job completion */
         hipGraph_t g = nullptr;
         hipGraphExec_t ge = nullptr;
-        bool use_graph = ctx->tuning.use_graph != 0 && !overlap;
+        bool use_graph = ctx->tuning.use_graph != 0 && !overlap && !wide;
         if (use_graph) {
             LX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             try {
@@ -257,7 +311,13 @@ extern "C" lx_status lx_batch_run(lx_context* ctx, const lx_job* jobs, uint32_t 
         LX_HIP(hipEventRecord(ev[0], s));
         double t_start = now_ms();
         for (int k = 0; k < rounds; k++) {
-            if (overlap) {
+            if (wide) {
+                LX_HIP(lx_launch_fill_u32(d_colw.as<uint32_t>(), accum_pix * R * 3, 0x80000000u, s));
+                lx_enqueue_round(ctx, d_jobsw.as<LxJob>(), d_sppw.as<int32_t>(), n * R, d_cnt.as<LxCounters>(),
+                                 d_sched.as<LxSched>(), cap, wb, d_colw.as<float>(), d_spill.as<uint32_t>(), max_depth,
+                                 tg, sg, cg, width, uscene, nullptr);
+                LX_HIP(lx_launch_add_regions(d_accum.as<float>(), d_colw.as<float>(), accum_pix * 3, R, s));
+            } else if (overlap) {
                 int p = k & 1;
                 if (k >= 2) LX_HIP(hipStreamWaitEvent(ps[p], e_free[p], 0));
                 LX_HIP(lx_launch_fill_u32(d_col[p].as<uint32_t>(), accum_pix * 3, 0x80000000u, ps[p]));
@@ -292,7 +352,15 @@ extern "C" lx_status lx_batch_run(lx_context* ctx, const lx_job* jobs, uint32_t 
         if (g) (void)hipGraphDestroy(g);
         LX_REQUIRE(c.stack_overflow == 0, LX_ERR_STATE, "traversal stack overflow");
         std::vector<LxJob> done(n);
-        LX_HIP(hipMemcpy(done.data(), d_jobs.ptr, n * sizeof(LxJob), hipMemcpyDeviceToHost));
+        if (wide) {
+            LX_HIP(hipMemcpy(hjw.data(), d_jobsw.ptr, hjw.size() * sizeof(LxJob), hipMemcpyDeviceToHost));
+            for (uint32_t i = 0; i < n; i++) {
+                done[i] = hjw[i];
+                for (uint32_t k = 1; k < R; k++) done[i].spp_done += hjw[(size_t)k * n + i].spp_done;
+            }
+        } else {
+            LX_HIP(hipMemcpy(done.data(), d_jobs.ptr, n * sizeof(LxJob), hipMemcpyDeviceToHost));
+        }
         if (overlap) {
             std::vector<LxJob> done1(n);
             LX_HIP(hipMemcpy(done1.data(), d_jobs1.ptr, n * sizeof(LxJob), hipMemcpyDeviceToHost));

@@ -143,3 +143,28 @@ def test_wide_falls_back_when_the_budget_is_tight():
             c.close()
         return
     pytest.fail("no budget fitted the render")
+
+
+def test_batch_wide_rounds_equal_one_sample_per_round(tuned):
+    """The batched form with round_frames R takes up to R samples of every job
+    per round (job copies, colour regions added in sample order): the same
+    images, sample counts and status as one sample per round, for two scenes,
+    spp 1..9 and later first iterations; with fewer path slots than R copies
+    need it renders one sample per round."""
+    a = lx.Scene(tuned, scenes.cornell(64, 64))
+    zd = scenes.material_zoo(80, 48, level=2, env_size=(64, 32))
+    b = lx.Scene(tuned, zd)
+    jobs = lambda: [lx.make_job(a, scenes.cornell(64, 64).camera, 5, dmis=True, seed=5),
+                    lx.make_job(b, zd.camera, 2, emis=True, seed=9),
+                    lx.make_job(a, scenes.cornell(48, 80).camera, 1, dmis=True, seed=11),
+                    lx.make_job(b, zd.camera, 9, emis=True, first_iter=7),
+                    lx.make_job(a, scenes.cornell(33, 17).camera, 3, dmis=True)]
+    tuned.set_tuning(round_frames=1, overlap_rounds=0)
+    want = lx.batch_run(tuned, jobs())
+    for frames, paths in ((2, 0), (3, 0), (4, 0), (8, 0), (4, 3 * 64 * 64)):
+        tuned.set_tuning(round_frames=frames, overlap_rounds=1, max_paths_per_round=paths)
+        got = lx.batch_run(tuned, jobs())
+        for x, y in zip(want, got):
+            assert x["status"] == y["status"] == 0, frames
+            assert x["spp"] == y["spp"], frames
+            assert np.array_equal(x["image"], y["image"]), (frames, paths)
