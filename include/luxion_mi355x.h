@@ -344,6 +344,21 @@ lx_status lx_trace_closest(lx_scene* scene, const lx_ray* rays, uint32_t n, int3
 lx_status lx_trace_occluded(lx_scene* scene, const lx_ray* rays, uint32_t n, int32_t any_hit_alpha,
                             uint32_t iteration, uint8_t* occluded);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): lx_trace_closest on DEVICE arrays (d_rays: n lx_ray,
+ * d_hits: n lx_hit, both allocations of this process on the scene's device),
+ * with no hipMalloc / hipFree and no allocating library call.  The traversal
+ * stack's spill columns and its overflow flag live in a fixed scratch array
+ * of 32 MiB + 256 B per device (held until the process ends), so n is at
+ * most 131072 rays per call (larger n: LX_ERR_INVALID).  Runs on the
+ * context's stream and returns once the hits are written.  Every call on a
+ * device shares that device's array: calls that overlap on one device (from
+ * different contexts or threads) race; calls through one context are
+ * serialised. */
+lx_status lx_trace_closest_noalloc(lx_scene* scene, const lx_ray* d_rays, uint32_t n, int32_t any_hit_alpha,
+                                   uint32_t iteration, lx_hit* d_hits);
+
 /* Run one bounce-resolved iteration and dump, per bounce d = 1..depth, the
  * sorted pixel indices that survive compaction (survivors_out, with counts in
  * survivor_counts[d-1]) plus the radiance hit record of every shaded path

@@ -60,6 +60,7 @@ lx_status lx_context_create(int device, uint64_t budget, lx_context** out) {
         hipDeviceProp_t prop;
         LX_HIP(hipGetDeviceProperties(&prop, device));
         c->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        LX_HIP(lx_noalloc_scratch_address(device, &c->noalloc_scratch));
         size_t fr = 0, tot = 0;
         LX_HIP(hipMemGetInfo(&fr, &tot));
         c->budget = budget ? budget : (uint64_t)fr;
@@ -241,6 +242,59 @@ lx_status lx_trace_closest(lx_scene* s, const lx_ray* rays, uint32_t n, int32_t 
 lx_status lx_trace_occluded(lx_scene* s, const lx_ray* rays, uint32_t n, int32_t alpha, uint32_t iteration,
                             uint8_t* occ) {
     return trace_hook(s, rays, n, 1, alpha, iteration, nullptr, occ);
+}
+
+/* The allocation-free variant of lx_trace_closest: device-resident rays and
+ * hits, the spill columns and overflow flag in this device's scratch array
+ * (one lane per ray, so n is capped by the array).  Calls that overlap on one
+ * device race on the array; a context serialises its own calls. */
+static void require_device_range(const lx_context* ctx, const void* p, size_t bytes, const char* what) {
+    hipPointerAttribute_t at{};
+    hipError_t e = hipPointerGetAttributes(&at, p);
+    if (e != hipSuccess || at.type != hipMemoryTypeDevice || at.device != ctx->device) {
+        (void)hipGetLastError();
+        throw LxError{LX_ERR_INVALID, std::string(what) + " is not a device allocation of this process on the "
+                                                          "scene's device"};
+    }
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    e = hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw LxError{LX_ERR_INVALID, std::string(what) + " is not a device allocation of this process on the "
+                                                          "scene's device"};
+    }
+    uintptr_t b = (uintptr_t)base, q = (uintptr_t)p;
+    LX_REQUIRE(q >= b && q - b <= size && size - (q - b) >= bytes, LX_ERR_INVALID,
+               std::string(what) + " extends past the end of its device allocation");
+}
+
+lx_status lx_trace_closest_noalloc(lx_scene* s, const lx_ray* d_rays, uint32_t n, int32_t alpha, uint32_t iteration,
+                                   lx_hit* d_hits) {
+    LX_TRY
+    LX_REQUIRE(s && s->accel_built, LX_ERR_STATE, "call lx_scene_build_accel first");
+    LX_REQUIRE(d_rays && d_hits, LX_ERR_INVALID, "null argument");
+    LX_REQUIRE(n <= LX_NOALLOC_MAX_RAYS, LX_ERR_INVALID,
+               "lx_trace_closest_noalloc takes at most " + std::to_string(LX_NOALLOC_MAX_RAYS) +
+                   " rays per call (got " + std::to_string(n) + "); split the batch");
+    if (n == 0) return LX_OK;
+    lx_context* ctx = s->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->set_device();
+    LX_REQUIRE(ctx->noalloc_scratch, LX_ERR_STATE, "scratch array address unavailable on this device");
+    require_device_range(ctx, d_rays, (size_t)n * sizeof(lx_ray), "d_rays");
+    require_device_range(ctx, d_hits, (size_t)n * sizeof(lx_hit), "d_hits");
+    uint32_t grid = (n + LX_TRACE_BLOCK - 1) / LX_TRACE_BLOCK; /* <= LX_NOALLOC_MAX_RAYS / LX_TRACE_BLOCK */
+    uint32_t* spill = ctx->noalloc_scratch;
+    uint32_t* overflow = ctx->noalloc_scratch + LX_NOALLOC_SPILL_WORDS;
+    LX_HIP(lx_launch_fill_u32(overflow, 64, 0u, ctx->stream));
+    LX_HIP(lx_launch_trace_rays(s->d_scene.as<LxDevScene>(), d_rays, n, 0, alpha, iteration, d_hits, nullptr, spill,
+                                overflow, grid, s->dev.bvh_width, ctx->stream));
+    uint32_t ov = 0;
+    LX_HIP(hipMemcpyAsync(&ov, overflow, 4, hipMemcpyDeviceToHost, ctx->stream));
+    LX_HIP(hipStreamSynchronize(ctx->stream));
+    LX_REQUIRE(ov == 0, LX_ERR_STATE, "traversal stack overflow (BVH deeper than the supported stack)");
+    LX_CATCH
 }
 
 lx_status lx_debug_math_eval(lx_context* ctx, int32_t fn, const float* x, const float* y, float* out, uint32_t n) {

@@ -71,6 +71,9 @@ struct lx_context {
     lx_tuning tuning{};
     std::mutex mu; /* serialises device work from the served form's thread */
     std::atomic<int> refs{1}; /* user handle + live scenes */
+    /* this device's copy of the allocation-free trace's scratch array
+     * (lx_noalloc_scratch_address, looked up at context creation) */
+    uint32_t* noalloc_scratch = nullptr;
 
     void* dalloc(size_t bytes);           /* charged against the budget */
     void dfree(void* p, size_t bytes);

@@ -18,6 +18,12 @@
 #define LX_STACK 24 /* per-lane LDS stack entries */
 #endif
 #define LX_SPILL 64 /* per-lane global spill entries beyond LX_STACK */
+/* lx_trace_closest_noalloc: at most this many rays per call (one lane per
+ * ray, a multiple of LX_TRACE_BLOCK); its scratch array holds their spill
+ * columns and one 256-byte line for the stack-overflow flag (32 MiB + 256 B) */
+#define LX_NOALLOC_MAX_RAYS 131072u
+#define LX_NOALLOC_SPILL_WORDS ((size_t)LX_SPILL * LX_NOALLOC_MAX_RAYS)
+#define LX_NOALLOC_SCRATCH_WORDS (LX_NOALLOC_SPILL_WORDS + 64)
 
 struct LxSched {
     uint32_t n_active;
@@ -59,6 +65,10 @@ hipError_t lx_launch_finish(const LxJob* jobs, const LxPathBuf& P, LxCounters* C
 hipError_t lx_launch_trace_rays(const LxDevScene* S, const lx_ray* rays, uint32_t n, int any, int alpha,
                                 uint32_t iteration, lx_hit* hits, uint8_t* occ, uint32_t* spill, uint32_t* overflow,
                                 uint32_t grid, int width, hipStream_t s);
+/* Device address of the allocation-free trace's scratch array on the current
+ * device (ordinal `device`, which must be current); looked up once per device
+ * with hipGetSymbolAddress and cached. */
+hipError_t lx_noalloc_scratch_address(int device, uint32_t** out);
 hipError_t lx_launch_tonemap(const float* accum, uint32_t npix, float spp, int mode, uint8_t* out, hipStream_t s);
 hipError_t lx_launch_math_eval(int fn, const float* x, const float* y, float* out, uint32_t n, hipStream_t s);
 hipError_t lx_launch_copy16(const void* src, void* dst, size_t n16, uint32_t grid, hipStream_t s);

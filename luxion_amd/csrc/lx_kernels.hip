@@ -27,6 +27,9 @@
 #include "lx_path_dev.h"
 #include "lx_kernels.h"
 
+#include <mutex>
+#include <vector>
+
 #define LX_WAVE 64
 
 LXD uint32_t lx_lane() { return __lane_id(); }
@@ -889,4 +892,34 @@ hipError_t lx_launch_add_regions(float* dst, const float* src, size_t n, uint32_
     if (blocks == 0) blocks = 1;
     hipLaunchKernelGGL(k_add_regions, dim3((uint32_t)blocks), dim3(256), 0, s, dst, src, n, regions);
     return hipGetLastError();
+}
+
+/* Scratch of lx_trace_closest_noalloc, the trace entry point that allocates
+ * nothing: the traversal-stack spill columns of LX_NOALLOC_MAX_RAYS lanes,
+ * then one 256-byte line whose first word is the stack-overflow flag.  Each
+ * device holds its own copy, at its own address, until the process ends.
+ * In host code the name `scratch` is only a host-side stand-in: host code
+ * uses the address hipGetSymbolAddress returns for the current device. */
+static __device__ __align__(256) uint32_t scratch[LX_NOALLOC_SCRATCH_WORDS];
+
+hipError_t lx_noalloc_scratch_address(int device, uint32_t** out) {
+    static std::mutex mu;
+    static std::vector<uint32_t*> by_device;
+    *out = nullptr;
+    if (device < 0) return hipErrorInvalidDevice;
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(mu);
+    if ((size_t)device >= by_device.size()) by_device.resize((size_t)device + 1, nullptr);
+    if (!by_device[device]) {
+        uint32_t* p = nullptr;
+        e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        by_device[device] = p;
+    }
+    *out = by_device[device];
+    return hipSuccess;
 }
