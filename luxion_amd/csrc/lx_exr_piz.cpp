@@ -1,17 +1,21 @@
 /*
  * lx_exr_piz.cpp -- OpenEXR PIZ block decoder.
  *
- * PIZ is defined by the OpenEXR file format, so this follows its published
- * steps exactly (any other arithmetic would decode different values):
- *   1. a bitmap of the 16-bit values present and the reverse look-up table
- *      that maps dense indices back to them;
- *   2. Huffman decoding: 6-bit code lengths with zero-run codes (59..63),
- *      canonical codes assigned from the longest length down, and a
- *      run-length symbol (the largest symbol) that repeats the previous value;
- *   3. the 2D Haar-like wavelet inverse per channel plane, in its 14-bit form
- *      when the largest index fits in 14 bits and its 16-bit modular form
- *      otherwise;
- *   4. the look-up table applied, then planes re-interleaved per scanline.
+ * A PIZ block holds, in order: the byte range and bytes of a 65536-bit
+ * "values present" bitmap; the byte length of a Huffman-coded stream; the
+ * stream.  Decoding, as the OpenEXR file format defines it:
+ *   1. the stream decodes to 16-bit dense indices (symbols; the largest
+ *      symbol of the table is the run-length escape: 8 bits follow with a
+ *      repeat count of the previous index);
+ *   2. each channel's plane (lines x width x words-per-sample) is run through
+ *      the inverse of the format's integer Haar wavelet, coarsest level first
+ *      (14-bit lifting when the largest dense index fits in 14 bits, 16-bit
+ *      modular lifting otherwise);
+ *   3. dense indices map back to the values whose bitmap bits are set (the
+ *      value 0 is always present); planes are re-interleaved per scanline.
+ * The arithmetic of each step is fixed by the format; the code below is
+ * organised around explicit (x, y) plane coordinates and per-length code
+ * ranges.
  */
 #include "lx_image.h"
 
@@ -25,198 +29,199 @@ namespace {
 
 uint32_t rd32(const uint8_t* p) { return (uint32_t)p[0] | (uint32_t)p[1] << 8 | (uint32_t)p[2] << 16 | (uint32_t)p[3] << 24; }
 
-struct Bits {
-    const uint8_t* p;
-    const uint8_t* end;
-    uint64_t buf = 0;
-    int n = 0;
-    uint64_t take(int k) {
-        while (n < k) {
-            if (p >= end) bad("truncated code table");
-            buf = (buf << 8) | *p++;
-            n += 8;
+constexpr uint32_t kSymbols = 65537; /* 65536 dense indices + the run-length escape */
+constexpr int kMaxLen = 58;          /* longest code the format allows */
+
+/* MSB-first reader over the packed code-length table. */
+class BitReader {
+public:
+    BitReader(const uint8_t* p, const uint8_t* end) : p_(p), end_(end) {}
+    uint32_t get(int k) {
+        while (have_ < k) {
+            if (p_ >= end_) bad("truncated code table");
+            acc_ = (acc_ << 8) | *p_++;
+            have_ += 8;
         }
-        n -= k;
-        return (buf >> n) & ((1ull << k) - 1ull);
+        have_ -= k;
+        return (uint32_t)((acc_ >> have_) & ((1u << k) - 1u));
     }
+    const uint8_t* next_byte() const { return p_; }
+
+private:
+    const uint8_t* p_;
+    const uint8_t* end_;
+    uint64_t acc_ = 0;
+    int have_ = 0;
 };
 
-const int kMaxSym = 65537; /* 65536 values + the run-length symbol */
-
-/* Code lengths for symbols im..iM, then canonical codes. */
-void read_table(const uint8_t*& p, const uint8_t* end, uint32_t im, uint32_t iM, std::vector<uint64_t>& len,
-                std::vector<uint64_t>& code) {
-    len.assign(kMaxSym, 0);
-    Bits b{p, end};
-    for (uint32_t i = im; i <= iM; i++) {
-        uint64_t l = b.take(6);
-        if (l == 63) {
-            uint32_t run = (uint32_t)b.take(8) + 6;
-            if (i + run > iM + 1) bad("code table run overflows");
-            for (uint32_t k = 0; k < run; k++) len[i + k] = 0;
-            i += run - 1;
-        } else if (l >= 59) {
-            uint32_t run = (uint32_t)(l - 59 + 2);
-            if (i + run > iM + 1) bad("code table run overflows");
-            for (uint32_t k = 0; k < run; k++) len[i + k] = 0;
-            i += run - 1;
+/* Code lengths of symbols lo..hi: a 6-bit length per symbol, except that
+ * 59..62 stand for runs of 2..5 zero lengths and 63 is followed by an 8-bit
+ * count of a run of 6..261 zero lengths. */
+std::vector<uint8_t> unpack_lengths(const uint8_t*& p, const uint8_t* end, uint32_t lo, uint32_t hi) {
+    std::vector<uint8_t> len(kSymbols, 0);
+    BitReader br(p, end);
+    uint32_t s = lo;
+    while (s <= hi) {
+        uint32_t v = br.get(6);
+        uint32_t zeros = 0;
+        if (v == 63) zeros = br.get(8) + 6;
+        else if (v >= 59) zeros = v - 57;
+        if (zeros) {
+            if (zeros > hi + 1 - s) bad("code table run overflows");
+            s += zeros; /* lengths already zero */
         } else {
-            len[i] = l;
+            len[s++] = (uint8_t)v;
         }
     }
-    p = b.p;
-    /* canonical codes: start codes from the longest length down */
-    uint64_t count[59] = {0};
-    for (int i = 0; i < kMaxSym; i++) count[len[i]]++;
-    uint64_t c = 0;
-    for (int l = 58; l > 0; l--) {
-        uint64_t nc = (c + count[l]) >> 1;
-        count[l] = c;
-        c = nc;
-    }
-    code.assign(kMaxSym, 0);
-    for (int i = 0; i < kMaxSym; i++)
-        if (len[i] > 0) code[i] = count[len[i]]++;
+    p = br.next_byte();
+    return len;
 }
 
-void huf_decode(const uint8_t* src, size_t n, uint16_t* out, size_t nout) {
+/* Canonical code values.  For each length L the format gives the codes of
+ * that length one contiguous range, in symbol order, starting at the summed
+ * weight of every longer code, sum over M > L of count[M] * 2^(58 - M),
+ * counted in units of 2^(58 - L) and truncated.  Weights summing past 2^58
+ * (Kraft sum above 1) mean an oversubscribed table whose codes would not fit
+ * their lengths: rejected. */
+std::vector<uint64_t> canonical_codes(const std::vector<uint8_t>& len, uint32_t lo, uint32_t hi) {
+    uint64_t per_len[kMaxLen + 1] = {0};
+    for (uint32_t s = lo; s <= hi; s++) per_len[len[s]]++;
+    uint64_t next[kMaxLen + 1] = {0};
+    unsigned __int128 longer = 0; /* weight of the codes longer than the current length */
+    for (int L = kMaxLen; L >= 1; --L) {
+        next[L] = (uint64_t)(longer >> (kMaxLen - L));
+        longer += (unsigned __int128)per_len[L] << (kMaxLen - L);
+    }
+    if (longer > ((unsigned __int128)1 << kMaxLen)) bad("oversubscribed Huffman code lengths");
+    std::vector<uint64_t> code(kSymbols, 0);
+    for (uint32_t s = lo; s <= hi; s++) {
+        int L = len[s];
+        if (!L) continue;
+        code[s] = next[L]++;
+        if (code[s] >> L) bad("Huffman code does not fit its length");
+    }
+    return code;
+}
+
+/* Decode the Huffman block src[0..n) into exactly nout 16-bit indices. */
+void huffman_decode(const uint8_t* src, size_t n, uint16_t* out, size_t nout) {
     if (n < 20) bad("short Huffman block");
-    uint32_t im = rd32(src), iM = rd32(src + 4), nbits = rd32(src + 12);
-    if (im >= (uint32_t)kMaxSym || iM >= (uint32_t)kMaxSym || im > iM) bad("bad symbol range");
+    const uint32_t lo = rd32(src), hi = rd32(src + 4), nbits = rd32(src + 12);
+    if (lo > hi || hi >= kSymbols) bad("bad symbol range");
     const uint8_t* p = src + 20;
     const uint8_t* end = src + n;
-    std::vector<uint64_t> len, code;
-    read_table(p, end, im, iM, len, code);
-    if ((size_t)(end - p) * 8 < nbits) bad("truncated bit stream");
-    /* fast table for codes of up to FB bits; longer codes by length */
-    const int FB = 14;
-    std::vector<int32_t> fast((size_t)1 << FB, -1);
-    std::vector<std::unordered_map<uint64_t, uint32_t>> slow(59);
-    for (uint32_t s = im; s <= iM; s++) {
-        int l = (int)len[s];
-        if (!l) continue;
-        if (l <= FB) {
-            uint64_t first = code[s] << (FB - l), cnt = 1ull << (FB - l);
-            for (uint64_t k = 0; k < cnt; k++) fast[first + k] = (int32_t)(s << 6 | (uint32_t)l);
+    std::vector<uint8_t> len = unpack_lengths(p, end, lo, hi);
+    std::vector<uint64_t> code = canonical_codes(len, lo, hi);
+    if ((uint64_t)(end - p) * 8 < nbits) bad("truncated bit stream");
+    /* one table look-up for codes of up to kFast bits; longer codes are found
+     * per length */
+    const int kFast = 14;
+    std::vector<int32_t> fast((size_t)1 << kFast, -1);
+    std::vector<std::unordered_map<uint64_t, uint32_t>> longer(kMaxLen + 1);
+    for (uint32_t s = lo; s <= hi; s++) {
+        const int L = len[s];
+        if (!L) continue;
+        if (L <= kFast) {
+            const uint64_t span = 1ull << (kFast - L), base = code[s] << (kFast - L);
+            std::fill(fast.begin() + (ptrdiff_t)base, fast.begin() + (ptrdiff_t)(base + span),
+                      (int32_t)(s << 6 | (uint32_t)L));
         } else {
-            slow[l][code[s]] = s;
+            longer[L].emplace(code[s], s);
         }
     }
-    uint32_t rlc = iM;
-    size_t pos = 0; /* bit position */
-    size_t o = 0;
-    auto peek = [&](int k) -> uint64_t {
-        /* k bits at pos, MSB first, zero past the end of the stream */
+    const uint32_t escape = hi;
+    size_t bit = 0;
+    /* k bits at `bit`, MSB first, zeros past the end of the stream */
+    auto window = [&](int k) -> uint64_t {
         uint64_t v = 0;
         for (int j = 0; j < k; j++) {
-            size_t bp = pos + (size_t)j;
-            uint64_t bit = bp < nbits ? (p[bp >> 3] >> (7 - (bp & 7))) & 1u : 0u;
-            v = (v << 1) | bit;
+            const size_t b = bit + (size_t)j;
+            v = (v << 1) | (b < nbits ? (uint64_t)((p[b >> 3] >> (7 - (b & 7))) & 1u) : 0u);
         }
         return v;
     };
-    while (pos < nbits) {
-        uint32_t sym;
-        int l;
-        int32_t f = fast[peek(FB)];
+    size_t o = 0;
+    while (bit < nbits) {
+        uint32_t sym = 0;
+        int L = 0;
+        const int32_t f = fast[window(kFast)];
         if (f >= 0) {
             sym = (uint32_t)f >> 6;
-            l = f & 63;
+            L = f & 63;
         } else {
-            sym = 0;
-            l = 0;
-            for (int L = FB + 1; L <= 58; L++) {
-                auto it = slow[L].find(peek(L));
-                if (it != slow[L].end()) {
+            for (int k = kFast + 1; k <= kMaxLen && !L; k++) {
+                auto it = longer[k].find(window(k));
+                if (it != longer[k].end()) {
                     sym = it->second;
-                    l = L;
-                    break;
+                    L = k;
                 }
             }
-            if (!l) bad("invalid Huffman code");
+            if (!L) bad("invalid Huffman code");
         }
-        if (pos + (size_t)l > nbits) bad("code runs past the stream");
-        pos += (size_t)l;
-        if (sym == rlc) {
-            if (pos + 8 > nbits) bad("truncated run length");
-            uint32_t run = (uint32_t)peek(8);
-            pos += 8;
-            if (o == 0) bad("run length without a previous value");
-            if (o + run > nout) bad("run overflows the output");
-            uint16_t v = out[o - 1];
-            for (uint32_t k = 0; k < run; k++) out[o++] = v;
-        } else {
+        if (bit + (size_t)L > nbits) bad("code runs past the stream");
+        bit += (size_t)L;
+        if (sym != escape) {
             if (o >= nout) bad("too many symbols");
             out[o++] = (uint16_t)sym;
+            continue;
         }
+        if (bit + 8 > nbits) bad("truncated run length");
+        const uint32_t repeat = (uint32_t)window(8);
+        bit += 8;
+        if (o == 0) bad("run length without a previous value");
+        if (repeat > nout - o) bad("run overflows the output");
+        std::fill(out + o, out + o + repeat, out[o - 1]);
+        o += repeat;
     }
     if (o != nout) bad("too few symbols");
 }
 
-inline void wdec14(uint16_t l, uint16_t h, uint16_t& a, uint16_t& b) {
-    int16_t ls = (int16_t)l, hs = (int16_t)h;
-    int hi = hs;
-    int ai = ls + (hi & 1) + (hi >> 1);
-    a = (uint16_t)(int16_t)ai;
-    b = (uint16_t)(int16_t)(ai - hi);
-}
+/* Inverse lifting of one (average, difference) pair, in place: on return
+ * (a, b) hold the two original samples. */
+struct Unlift14 {
+    void operator()(uint16_t& a, uint16_t& b) const {
+        const int avg = (int16_t)a, diff = (int16_t)b;
+        const int first = avg + (diff & 1) + (diff >> 1);
+        a = (uint16_t)first;
+        b = (uint16_t)(first - diff);
+    }
+};
+struct Unlift16 {
+    void operator()(uint16_t& a, uint16_t& b) const {
+        const uint32_t avg = a, diff = b;
+        const uint32_t second = (avg - (diff >> 1)) & 0xffffu;
+        const uint32_t first = (diff + second + 0x8000u) & 0xffffu; /* - 2^15, mod 2^16 */
+        a = (uint16_t)first;
+        b = (uint16_t)second;
+    }
+};
 
-inline void wdec16(uint16_t l, uint16_t h, uint16_t& a, uint16_t& b) {
-    int m = l, d = h;
-    int bb = (m - (d >> 1)) & 0xffff;
-    int aa = (d + bb - 0x8000) & 0xffff;
-    b = (uint16_t)bb;
-    a = (uint16_t)aa;
-}
-
-/* in-place inverse wavelet of an nx x ny plane with element stride ox and
- * row stride oy */
-void wav2_decode(uint16_t* in, int nx, int ox, int ny, int oy, uint16_t mx) {
-    bool w14 = mx < (1 << 14);
-    int n = std::min(nx, ny);
-    int p = 1;
-    while (p <= n) p <<= 1;
-    p >>= 1;
-    int p2 = p;
-    p >>= 1;
-    auto dec = [&](uint16_t l, uint16_t h, uint16_t& a, uint16_t& b) {
-        if (w14) wdec14(l, h, a, b);
-        else wdec16(l, h, a, b);
-    };
-    while (p >= 1) {
-        uint16_t* py = in;
-        uint16_t* ey = in + (ptrdiff_t)oy * (ny - p2);
-        int oy1 = oy * p, oy2 = oy * p2, ox1 = ox * p, ox2 = ox * p2;
-        uint16_t i00, i01, i10, i11;
-        for (; py <= ey; py += oy2) {
-            uint16_t* px = py;
-            uint16_t* ex = py + (ptrdiff_t)ox * (nx - p2);
-            for (; px <= ex; px += ox2) {
-                uint16_t* p01 = px + ox1;
-                uint16_t* p10 = px + oy1;
-                uint16_t* p11 = p10 + ox1;
-                dec(*px, *p10, i00, i10);
-                dec(*p01, *p11, i01, i11);
-                dec(i00, i01, *px, *p01);
-                dec(i10, i11, *p10, *p11);
+/* Inverse wavelet of one nx x ny plane whose sample (x, y) sits at
+ * base[x * sx + y * sy].  Level h (h = largest power of two <= min(nx, ny),
+ * halved, down to 1) pairs samples h apart inside 2h x 2h cells: columns
+ * first, then rows.  A column of cells cut by the right edge (nx & h) has
+ * only its vertical pairs, a row cut by the bottom edge (ny & h) only its
+ * horizontal pairs, and the corner cell neither. */
+template <class Unlift>
+void inverse_wavelet(uint16_t* base, int nx, int ny, ptrdiff_t sx, ptrdiff_t sy, Unlift unlift) {
+    auto at = [&](int x, int y) -> uint16_t& { return base[(ptrdiff_t)x * sx + (ptrdiff_t)y * sy]; };
+    int top = 1;
+    while (top * 2 <= std::min(nx, ny)) top *= 2;
+    for (int h = top / 2; h >= 1; h /= 2) {
+        const int cell = 2 * h;
+        const int cells_x = nx / cell * cell, cells_y = ny / cell * cell;
+        const bool cut_x = (nx & h) != 0, cut_y = (ny & h) != 0;
+        for (int y = 0; y < cells_y; y += cell) {
+            for (int x = 0; x < cells_x; x += cell) {
+                unlift(at(x, y), at(x, y + h));
+                unlift(at(x + h, y), at(x + h, y + h));
+                unlift(at(x, y), at(x + h, y));
+                unlift(at(x, y + h), at(x + h, y + h));
             }
-            if (nx & p) {
-                uint16_t* p10 = px + oy1;
-                dec(*px, *p10, i00, *p10);
-                *px = i00;
-            }
+            if (cut_x) unlift(at(cells_x, y), at(cells_x, y + h));
         }
-        if (ny & p) {
-            uint16_t* px = py;
-            uint16_t* ex = py + (ptrdiff_t)ox * (nx - p2);
-            for (; px <= ex; px += ox2) {
-                uint16_t* p01 = px + ox1;
-                dec(*px, *p01, i00, *p01);
-                *px = i00;
-            }
-        }
-        p2 = p;
-        p >>= 1;
+        if (cut_y)
+            for (int x = 0; x < cells_x; x += cell) unlift(at(x, cells_y), at(x + h, cells_y));
     }
 }
 
@@ -225,49 +230,60 @@ void wav2_decode(uint16_t* in, int nx, int ox, int ny, int oy, uint16_t mx) {
 void lx_exr_piz_decode(const uint8_t* src, size_t n, uint8_t* out, size_t expect, const std::vector<int>& ch_bytes,
                        int width, int lines) {
     if (n < 4) bad("short block");
-    uint16_t minnz = (uint16_t)(src[0] | src[1] << 8), maxnz = (uint16_t)(src[2] | src[3] << 8);
+    const uint16_t first_byte = (uint16_t)(src[0] | src[1] << 8), last_byte = (uint16_t)(src[2] | src[3] << 8);
     size_t i = 4;
-    std::vector<uint8_t> bitmap(8192, 0);
-    if (maxnz >= 8192) bad("bad bitmap range");
-    if (minnz <= maxnz) {
-        size_t cnt = (size_t)maxnz - minnz + 1;
-        if (i + cnt > n) bad("truncated bitmap");
-        std::memcpy(&bitmap[minnz], src + i, cnt);
+    std::vector<uint8_t> present(8192, 0);
+    if (first_byte <= last_byte) {
+        if (last_byte >= 8192) bad("bad bitmap range");
+        const size_t cnt = (size_t)last_byte - first_byte + 1;
+        if (cnt > n - i) bad("truncated bitmap");
+        std::memcpy(&present[first_byte], src + i, cnt);
         i += cnt;
     }
-    std::vector<uint16_t> lut(65536, 0);
-    int k = 0;
-    for (int v = 0; v < 65536; v++)
-        if (v == 0 || (bitmap[v >> 3] & (1 << (v & 7)))) lut[k++] = (uint16_t)v;
-    uint16_t maxv = (uint16_t)(k - 1);
-    if (i + 4 > n) bad("missing Huffman length");
-    uint32_t hl = rd32(src + i);
+    /* dense index -> value: the values whose bit is set, in increasing order */
+    std::vector<uint16_t> value_of;
+    value_of.reserve(65536);
+    for (uint32_t v = 0; v < 65536; v++)
+        if (v == 0 || ((present[v >> 3] >> (v & 7)) & 1)) value_of.push_back((uint16_t)v);
+    const uint32_t largest_index = (uint32_t)value_of.size() - 1;
+    if (n - i < 4) bad("missing Huffman length");
+    const uint32_t huf_len = rd32(src + i);
     i += 4;
-    if (i + hl > n) bad("truncated Huffman data");
+    if (huf_len > n - i) bad("truncated Huffman data");
     if (expect % 2) bad("odd block size");
-    size_t nv = expect / 2;
-    std::vector<uint16_t> tmp(nv);
-    huf_decode(src + i, hl, tmp.data(), nv);
-    /* planes: channel c occupies lines * width * size values, size = bytes/2 */
-    size_t off = 0;
-    std::vector<size_t> start(ch_bytes.size());
+    const size_t nwords = expect / 2;
+    /* channel planes, one after another: lines x width x words-per-sample */
+    std::vector<size_t> plane_at(ch_bytes.size());
+    size_t total = 0;
     for (size_t c = 0; c < ch_bytes.size(); c++) {
-        int size = ch_bytes[c] / 2;
-        start[c] = off;
-        for (int j = 0; j < size; j++) wav2_decode(tmp.data() + off + j, width, size, lines, width * size, maxv);
-        off += (size_t)lines * width * size;
+        plane_at[c] = total;
+        total += (size_t)lines * (size_t)width * (size_t)(ch_bytes[c] / 2);
     }
-    if (off != nv) bad("channel sizes do not match the block");
-    for (auto& v : tmp) v = lut[v];
-    /* re-interleave: line by line, channel by channel */
+    if (total != nwords) bad("channel sizes do not match the block");
+    std::vector<uint16_t> words(nwords);
+    huffman_decode(src + i, huf_len, words.data(), nwords);
+    for (size_t c = 0; c < ch_bytes.size(); c++) {
+        const int per_sample = ch_bytes[c] / 2;
+        for (int w = 0; w < per_sample; w++) {
+            uint16_t* base = words.data() + plane_at[c] + w;
+            const ptrdiff_t sx = per_sample, sy = (ptrdiff_t)width * per_sample;
+            if (largest_index < (1u << 14)) inverse_wavelet(base, width, lines, sx, sy, Unlift14{});
+            else inverse_wavelet(base, width, lines, sx, sy, Unlift16{});
+        }
+    }
+    for (uint16_t& w : words) {
+        if (w > largest_index) bad("index beyond the value table");
+        w = value_of[w];
+    }
+    /* scanline order: line by line, channel by channel, little-endian words */
     uint8_t* o = out;
     for (int y = 0; y < lines; y++)
         for (size_t c = 0; c < ch_bytes.size(); c++) {
-            size_t cnt = (size_t)width * (ch_bytes[c] / 2);
-            const uint16_t* s = tmp.data() + start[c] + (size_t)y * cnt;
+            const size_t cnt = (size_t)width * (size_t)(ch_bytes[c] / 2);
+            const uint16_t* row = words.data() + plane_at[c] + (size_t)y * cnt;
             for (size_t q = 0; q < cnt; q++) {
-                *o++ = (uint8_t)(s[q] & 0xff);
-                *o++ = (uint8_t)(s[q] >> 8);
+                *o++ = (uint8_t)(row[q] & 0xff);
+                *o++ = (uint8_t)(row[q] >> 8);
             }
         }
 }

@@ -703,10 +703,10 @@ LxImageF lx_load_exr(const std::string& path) {
     std::vector<uint8_t> dec;
     for (int k = 0; k < nblocks; k++) {
         uint64_t off = le64(&b[i + 8 * (size_t)k]);
-        if (off + 8 > n) bad("EXR: bad block offset");
+        if (off > n || n - off < 8) bad("EXR: bad block offset");
         int y = (int)le32(&b[off]);
         uint32_t len = le32(&b[off + 4]);
-        if (off + 8 + len > n) bad("EXR: truncated block");
+        if (len > n - off - 8) bad("EXR: truncated block");
         int ly = y - y0;
         if (ly < 0 || ly >= H) bad("EXR: block outside the data window");
         int lines = std::min(lpb, H - ly);

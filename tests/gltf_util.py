@@ -74,9 +74,188 @@ def _attr(name, typ, val):
     return name.encode() + b"\0" + typ.encode() + b"\0" + struct.pack("<I", len(val)) + val
 
 
-def write_exr(path, planes, compression, half=True):
+def _wavelet_encode(P, wide):
+    """Forward integer Haar wavelet of the OpenEXR PIZ format over the 2-D
+    uint16 plane P (in place), finest level first: in each 2h x 2h cell,
+    rows first, then columns; cells cut by the right / bottom edge get only
+    their vertical / horizontal pairs.  wide: 16-bit modular lifting (else
+    14-bit)."""
+    def lift(a, b):
+        if not wide:
+            a16, b16 = a.astype(np.int16).astype(np.int32), b.astype(np.int16).astype(np.int32)
+            return ((a16 + b16) >> 1).astype(np.uint16), (a16 - b16).astype(np.uint16)
+        ao = (a.astype(np.int32) + 0x8000) & 0xffff
+        m = (ao + b.astype(np.int32)) >> 1
+        d = ao - b.astype(np.int32)
+        m = np.where(d < 0, (m + 0x8000) & 0xffff, m)
+        return m.astype(np.uint16), (d & 0xffff).astype(np.uint16)
+
+    ny, nx = P.shape
+    h = 1
+    while 2 * h <= min(nx, ny):
+        s = 2 * h
+        cx, cy = nx // s * s, ny // s * s
+        if cx and cy:
+            p00, p01 = P[0:cy:s, 0:cx:s].copy(), P[0:cy:s, h:cx:s].copy()
+            p10, p11 = P[h:cy:s, 0:cx:s].copy(), P[h:cy:s, h:cx:s].copy()
+            i00, i01 = lift(p00, p01)
+            i10, i11 = lift(p10, p11)
+            P[0:cy:s, 0:cx:s], P[h:cy:s, 0:cx:s] = lift(i00, i10)
+            P[0:cy:s, h:cx:s], P[h:cy:s, h:cx:s] = lift(i01, i11)
+        if nx & h and cy:
+            P[0:cy:s, cx], P[h:cy:s, cx] = lift(P[0:cy:s, cx].copy(), P[h:cy:s, cx].copy())
+        if ny & h and cx:
+            P[cy, 0:cx:s], P[cy, h:cx:s] = lift(P[cy, 0:cx:s].copy(), P[cy, h:cx:s].copy())
+        h = s
+
+
+def _huffman_lengths(freq):
+    """Code length per symbol (0 = unused) from a Huffman tree over freq."""
+    import heapq
+    syms = [s for s in range(len(freq)) if freq[s]]
+    lengths = np.zeros(len(freq), np.int64)
+    if len(syms) == 1:
+        lengths[syms[0]] = 1
+        return lengths
+    heap = [(int(freq[s]), k, [s]) for k, s in enumerate(syms)]
+    heapq.heapify(heap)
+    tick = len(heap)
+    while len(heap) > 1:
+        fa, _, a = heapq.heappop(heap)
+        fb, _, b = heapq.heappop(heap)
+        for s in a + b:
+            lengths[s] += 1
+        heapq.heappush(heap, (fa + fb, tick, a + b))
+        tick += 1
+    assert lengths.max() <= 58
+    return lengths
+
+
+def _canonical(lengths, lo, hi):
+    """Code value per symbol under the format's canonical numbering: walking
+    from length 58 down, each length's codes start where the (halved) run of
+    the longer codes ended; symbols of one length count up in symbol order."""
+    count = np.bincount(lengths[lo:hi + 1], minlength=59)
+    start, c = [0] * 59, 0
+    for L in range(58, 0, -1):
+        start[L] = c
+        c = (c + int(count[L])) >> 1
+    codes = {}
+    for s in range(lo, hi + 1):
+        L = int(lengths[s])
+        if L:
+            codes[s] = (start[L], L)
+            start[L] += 1
+    return codes
+
+
+class _BitWriter:
+    def __init__(self):
+        self.bits = []
+
+    def put(self, value, n):
+        self.bits.extend((value >> (n - 1 - k)) & 1 for k in range(n))
+
+    def bytes(self):
+        b = self.bits + [0] * (-len(self.bits) % 8)
+        return np.packbits(np.array(b, np.uint8)).tobytes() if b else b""
+
+
+def huffman_block(words, lengths_override=None):
+    """OpenEXR Huffman block for the uint16 sequence `words`: header (lowest
+    and highest symbol, table bytes, bit count, 4 spare bytes), packed code
+    lengths (6 bits each; 59..62 = 2..5 zero lengths; 63 + 8 bits = 6..261
+    zero lengths), then the codes, MSB first; runs of a repeated value use
+    the escape symbol (highest symbol + 1) and an 8-bit repeat count.
+    lengths_override: code lengths to write instead (malformed tables)."""
+    words = np.asarray(words, np.uint16)
+    freq = np.bincount(words, minlength=65537).astype(np.int64)
+    lo, hi = int(words.min()), int(words.max()) + 1  # hi: the escape symbol
+    freq[hi] = 1
+    lengths = _huffman_lengths(freq) if lengths_override is None else np.asarray(lengths_override, np.int64)
+    table = _BitWriter()
+    s = lo
+    while s <= hi:
+        if lengths[s]:
+            table.put(int(lengths[s]), 6)
+            s += 1
+            continue
+        run = 1
+        while s + run <= hi and lengths[s + run] == 0 and run < 261:
+            run += 1
+        if run >= 6:
+            table.put(63, 6)
+            table.put(run - 6, 8)
+        elif run >= 2:
+            table.put(59 + run - 2, 6)
+        else:
+            table.put(0, 6)
+        s += run
+    tb = table.bytes()
+    codes = _canonical(lengths, lo, hi)
+    out = _BitWriter()
+    i, n = 0, len(words)
+    esc = codes[hi]
+    while i < n:
+        v = int(words[i])
+        j = i + 1
+        while j < n and words[j] == v and j - i <= 255:
+            j += 1
+        rep = j - i - 1
+        c = codes[v]
+        out.put(*c)
+        if rep >= 2:
+            out.put(*esc)
+            out.put(rep, 8)
+        else:
+            for _ in range(rep):
+                out.put(*c)
+        i = j
+    nbits = len(out.bits)
+    return struct.pack("<IIIII", lo, hi, len(tb), nbits, 0) + tb + out.bytes()
+
+
+def piz_block(raw, chan_words, width, lines, lengths_override=None):
+    """PIZ-compressed block of the interleaved scanline bytes `raw`
+    (lines x channels x width samples, chan_words[c] uint16 words per sample
+    of channel c): channel planes, the bitmap of present values and its dense
+    index table, the wavelet per plane, then the Huffman block."""
+    w16 = np.frombuffer(raw, np.uint16)
+    planes, pos = [[] for _ in chan_words], 0
+    for _ in range(lines):
+        for c, k in enumerate(chan_words):
+            planes[c].append(w16[pos:pos + width * k])
+            pos += width * k
+    planes = [np.concatenate(p).reshape(lines, width * k) for p, k in zip(planes, chan_words)]
+    present = np.zeros(65536, bool)
+    for p in planes:
+        present[p.ravel()] = True
+    present[0] = False
+    bitmap = np.packbits(present.reshape(-1, 8)[:, ::-1], axis=1).ravel()  # bit v & 7 of byte v >> 3
+    dense = np.cumsum(np.concatenate([[True], present[1:]])).astype(np.int64) - 1
+    nz = np.nonzero(bitmap)[0]
+    first, last = (int(nz[0]), int(nz[-1])) if len(nz) else (8191, 0)
+    largest = int(np.count_nonzero(present))  # dense indices run 0 .. count of present non-zero values
+    words = []
+    for p, k in zip(planes, chan_words):
+        q = dense[p].astype(np.uint16)
+        for j in range(k):
+            sub = q[:, j::k]
+            _wavelet_encode(sub, largest >= (1 << 14))
+            q[:, j::k] = sub
+        words.append(q.ravel())
+    words = np.concatenate(words)
+    huf = huffman_block(words, lengths_override)
+    head = struct.pack("<HH", first, last) + (bitmap[first:last + 1].tobytes() if first <= last else b"")
+    return head + struct.pack("<I", len(huf)) + huf
+
+
+def write_exr(path, planes, compression, half=True, piz_lengths=None, block_offset=None):
     """Scanline EXR; planes: dict name -> (H, W) float32.  compression 0
-    (NONE), 1 (RLE), 2 (ZIPS) or 3 (ZIP)."""
+    (NONE), 1 (RLE), 2 (ZIPS), 3 (ZIP) or 4 (PIZ).  piz_lengths: code
+    lengths written into every PIZ block instead of the real ones;
+    block_offset: a value written over the first entry of the offset table
+    (malformed files)."""
     names = sorted(planes)
     H, W = planes[names[0]].shape
     chl = b""
@@ -93,7 +272,7 @@ def write_exr(path, planes, compression, half=True):
     hdr += _attr("screenWindowCenter", "v2f", struct.pack("<ff", 0, 0))
     hdr += _attr("screenWindowWidth", "float", struct.pack("<f", 1.0))
     hdr += b"\0"
-    lpb = 16 if compression == 3 else 1
+    lpb = {3: 16, 4: 32}.get(compression, 1)
     nb = (H + lpb - 1) // lpb
     blocks = []
     for b in range(nb):
@@ -104,7 +283,11 @@ def write_exr(path, planes, compression, half=True):
                 row = planes[n][y]
                 raw += (row.astype(np.float16).tobytes() if half else row.astype(np.float32).tobytes())
         data = raw
-        if compression in (1, 2, 3):
+        if compression == 4:
+            comp = piz_block(raw, [1 if half else 2] * len(names), W, min(H, y0 + lpb) - y0, piz_lengths)
+            if len(comp) < len(raw) or piz_lengths is not None:
+                data = comp
+        elif compression in (1, 2, 3):
             t = np.frombuffer(raw, np.uint8)
             inter = np.concatenate([t[0::2], t[1::2]])
             d = inter.astype(np.int32)
@@ -136,8 +319,8 @@ def write_exr(path, planes, compression, half=True):
         blocks.append(struct.pack("<iI", y0, len(data)) + data)
     off = len(hdr) + 8 * nb
     table = b""
-    for blk in blocks:
-        table += struct.pack("<Q", off)
+    for k, blk in enumerate(blocks):
+        table += struct.pack("<Q", block_offset if (k == 0 and block_offset is not None) else off)
         off += len(blk)
     with open(path, "wb") as f:
         f.write(hdr + table + b"".join(blocks))

@@ -12,8 +12,13 @@ Normals (normalised per vertex): 1 ulp-level.  Transforms: 1e-6 relative
 the float32 arithmetic).  JPEG texels: different decoders (PIL's libjpeg vs
 this build's; the reference uses stb_image, absent here) round differently:
 mean |diff| <= 1 level, max <= 24 (parity unpinned beyond that).  PIZ EXR:
-no independent decoder in this image; checked by invariants (parity
-unpinned).
+no independent decoder exists in this image, so PIZ files written by a PIZ
+encoder restated from the format (tests/gltf_util.py: value bitmap and
+dense indices, forward wavelet in both lifting widths, canonical Huffman
+code with zero-run length packing and run-length escapes) must decode back
+exactly, malformed code tables and block offsets must come back as
+LX_ERR_FORMAT, and the reference's own PIZ HDRIs must decode with the
+invariants of an HDR capture.
 """
 import glob
 import os
@@ -207,6 +212,67 @@ def test_white_furnace_exr_matches_oracle():
     got = lx.parse_glb(glb, exr, env_head=want.size)
     assert (got["env"][0], got["env"][1]) == (want.shape[1], want.shape[0])
     assert np.array_equal(got["env"][2].reshape(want.shape), want)
+
+
+def _env_of(tmp_path, exr):
+    p = str(tmp_path / "s.glb")
+    gltf_util.write_glb(p)
+    return lx.parse_glb(p, exr, env_head=1 << 22)
+
+
+@pytest.mark.parametrize("case", ["rgb_half_odd", "rgba_half_blocks", "y_half", "rgb_float_wide", "constant"])
+def test_piz_round_trip_exact(tmp_path, case):
+    """PIZ files from the format's encoder decode to the written samples
+    bit for bit: odd sizes (edge columns and rows at every wavelet level),
+    several 32-line blocks with a short last one, one channel, float
+    channels (two words per sample) with more than 2^14 distinct values
+    (the 16-bit modular lifting), and a constant image (one symbol)."""
+    rng = np.random.default_rng(len(case))
+    spec = {"rgb_half_odd": ("RGB", 41, 67, True), "rgba_half_blocks": ("RGBA", 70, 37, True),
+            "y_half": ("Y", 33, 29, True), "rgb_float_wide": ("RGB", 40, 256, False),
+            "constant": ("RGB", 35, 20, True)}[case]
+    chans, H, W, half = spec
+    planes = {}
+    for c in chans:
+        v = np.exp(rng.normal(0, 2, (H, W))).astype(np.float32)
+        v[3:6, 4:20] = 0.5  # repeated values: run-length escapes
+        planes[c] = np.zeros((H, W), np.float32) if case == "constant" else v
+    if not half:
+        words = np.concatenate([planes[c][:32].view(np.uint16).ravel() for c in chans])
+        assert len(np.unique(words)) >= (1 << 14)  # the first block takes the 16-bit lifting
+    exr = str(tmp_path / "p.exr")
+    gltf_util.write_exr(exr, planes, 4, half=half)
+    w, h, rgba = _env_of(tmp_path, exr)["env"]
+    assert (w, h) == (W, H)
+    rgba = rgba[:H * W * 4].reshape(H, W, 4)
+    want = {c: (planes[c].astype(np.float16).astype(np.float32) if half else planes[c]) for c in chans}
+    if chans == "Y":
+        for k in range(4):
+            assert np.array_equal(rgba[..., k], want["Y"])
+    else:
+        for k, c in enumerate("RGB"):
+            assert np.array_equal(rgba[..., k].view(np.uint32), want[c].view(np.uint32)), c
+        assert np.array_equal(rgba[..., 3], want["A"] if "A" in want else np.ones((H, W), np.float32))
+
+
+def test_piz_malformed_tables_and_offsets_are_format_errors(tmp_path):
+    """An oversubscribed code table (five symbols of length 1), a table
+    whose zero run passes the last symbol, and a block offset near 2^64 are
+    reported as LX_ERR_FORMAT, not decoded out of bounds."""
+    planes = {c: np.full((8, 8), 0.25 * (k + 1), np.float32) for k, c in enumerate("RGB")}
+    over = np.zeros(65537, np.int64)
+    exr = str(tmp_path / "o.exr")
+    # the block's symbols: dense indices 0..3 and the escape 4; all given length 1
+    over[0:5] = 1
+    gltf_util.write_exr(exr, planes, 4, piz_lengths=over)
+    with pytest.raises(lx.LuxionError, match="oversubscribed") as e:
+        _env_of(tmp_path, exr)
+    assert e.value.status == 5
+    exr2 = str(tmp_path / "w.exr")
+    gltf_util.write_exr(exr2, planes, 1, block_offset=(1 << 64) - 4)
+    with pytest.raises(lx.LuxionError, match="bad block offset") as e:
+        _env_of(tmp_path, exr2)
+    assert e.value.status == 5
 
 
 @pytest.mark.skipif(not os.path.isdir(os.path.join(REF_SCENES, "exr")), reason="reference checkout not present")
