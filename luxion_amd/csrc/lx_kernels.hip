@@ -371,6 +371,24 @@ __global__ __launch_bounds__(LX_SHADE_BLOCK) void k_shade(const LxJob* __restric
 }
 
 /* ------------------------------------------------------------ tail */
+/* One lane's column of tail-kernel path state in LDS (stride BLOCK floats):
+ * throughput, colour, last pdf and the pending environment colour term. */
+#define LX_TAIL_LDS_STATE 10
+template <int BLOCK>
+struct LxTailColumn {
+    LX_LDS float* p;
+    LXD float get(int k) const { return p[k * BLOCK]; }
+    LXD void set(int k, float v) const { p[k * BLOCK] = v; }
+    LXD f3 thr() const { return mk3(get(0), get(1), get(2)); }
+    LXD f3 col() const { return mk3(get(3), get(4), get(5)); }
+    LXD float pdf() const { return get(6); }
+    LXD f3 pend_col() const { return mk3(get(7), get(8), get(9)); }
+    LXD void put_thr(f3 v) const { set(0, v.x); set(1, v.y); set(2, v.z); }
+    LXD void put_col(f3 v) const { set(3, v.x); set(4, v.y); set(5, v.z); }
+    LXD void put_pdf(float v) const { set(6, v); }
+    LXD void put_pend_col(f3 v) const { set(7, v.x); set(8, v.y); set(9, v.z); }
+};
+
 /* BLOCK threads per workgroup; OCC > 0 asks the register allocator for at
  * least OCC waves per SIMD (spilling to scratch if it must). */
 /* UNI 1: every job of the launch renders the scene at `uscene`, which then
@@ -382,6 +400,10 @@ void k_finish(const LxJob* __restrict__ jobs, LxPathBuf P, LxCounters* __restric
               uint32_t* __restrict__ spill, float* __restrict__ accum, uint32_t chunk,
               const LxDevScene* __restrict__ uscene, const uint32_t* __restrict__ order) {
     __shared__ uint32_t s_stack[LX_STACK * BLOCK];
+    /* per-lane path state that only shading reads: it waits in an LDS
+     * column while the lane traverses, instead of holding registers */
+    __shared__ float s_path[LX_TAIL_LDS_STATE * BLOCK];
+    LxTailColumn<BLOCK> ps{(LX_LDS float*)(s_path + threadIdx.x)};
     uint32_t n = C->count[bounce0];
     uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t gsz = gridDim.x * blockDim.x;
@@ -397,13 +419,12 @@ void k_finish(const LxJob* __restrict__ jobs, LxPathBuf P, LxCounters* __restric
     bool have = false, exhausted = false;
     uint32_t job = 0, pix = 0;
     int b = bounce0;
-    f3 o = mk3(0, 0, 0), d = mk3(0, 0, 0), thr = mk3(0, 0, 0), col = mk3(0, 0, 0);
-    float pdf = 0.0f;
+    f3 o = mk3(0, 0, 0), d = mk3(0, 0, 0);
     bool spec = false;
     /* the environment shadow ray of the last shaded bounce, traced together
-     * with the next radiance ray; its colour term waits in pend_col */
+     * with the next radiance ray; its colour term waits in the LDS column */
     bool pend = false, pend_add = false, flush = false;
-    f3 pend_o = mk3(0, 0, 0), pend_d = mk3(0, 0, 0), pend_col = mk3(0, 0, 0);
+    f3 pend_o = mk3(0, 0, 0), pend_d = mk3(0, 0, 0);
     while (true) {
         /* refill idle lanes from the live-path queue, one atomic per wave,
          * once at least `chunk` lanes are idle (or all of them are) */
@@ -423,9 +444,9 @@ void k_finish(const LxJob* __restrict__ jobs, LxPathBuf P, LxCounters* __restric
                     pix = pw & 0x7fffffffu;
                     o = mk3(lx_gptr(P.ox)[my], lx_gptr(P.oy)[my], lx_gptr(P.oz)[my]);
                     d = mk3(lx_gptr(P.dx)[my], lx_gptr(P.dy)[my], lx_gptr(P.dz)[my]);
-                    thr = mk3(lx_gptr(P.tx)[my], lx_gptr(P.ty)[my], lx_gptr(P.tz)[my]);
-                    col = mk3(lx_gptr(P.cx)[my], lx_gptr(P.cy)[my], lx_gptr(P.cz)[my]);
-                    pdf = lx_gptr(P.pdf)[my];
+                    ps.put_thr(mk3(lx_gptr(P.tx)[my], lx_gptr(P.ty)[my], lx_gptr(P.tz)[my]));
+                    ps.put_col(mk3(lx_gptr(P.cx)[my], lx_gptr(P.cy)[my], lx_gptr(P.cz)[my]));
+                    ps.put_pdf(lx_gptr(P.pdf)[my]);
                     b = bounce0;
                     have = true;
                 }
@@ -440,8 +461,9 @@ void k_finish(const LxJob* __restrict__ jobs, LxPathBuf P, LxCounters* __restric
             bool occluded = false;
             lx_trace_path_d<CNT, W>(J, S, o, d, pix, b, st, &C->stack_overflow, pend, pend_o, pend_d, have, occluded,
                                     sink, rc, &wc, &wa);
+            f3 col = ps.col();
             if (pend) {
-                if (!occluded && pend_add) col = col + pend_col;
+                if (!occluded && pend_add) col = col + ps.pend_col();
                 pend = false;
             }
             if (flush) {
@@ -449,6 +471,8 @@ void k_finish(const LxJob* __restrict__ jobs, LxPathBuf P, LxCounters* __restric
                 flush = false;
             } else {
                 int depth = b + 1;
+                f3 thr = ps.thr(), pend_col = mk3(0, 0, 0);
+                float pdf = ps.pdf();
                 bool cont = lx_shade_path<true>(J, S, r, depth, depth < J.depth, o, d, thr, col, pdf, spec, pix,
                                                 &pend_add, &pend_col);
                 nsh++;
@@ -460,10 +484,19 @@ void k_finish(const LxJob* __restrict__ jobs, LxPathBuf P, LxCounters* __restric
                 }
                 if (cont) {
                     b++;
+                    ps.put_thr(thr);
+                    ps.put_pdf(pdf);
+                    ps.put_col(col);
+                    ps.put_pend_col(pend_col);
                 } else {
                     have = false;
-                    if (pend) flush = true;
-                    else lx_accumulate(accum, J, pix, col);
+                    if (pend) {
+                        flush = true;
+                        ps.put_col(col);
+                        ps.put_pend_col(pend_col);
+                    } else {
+                        lx_accumulate(accum, J, pix, col);
+                    }
                 }
             }
         }

@@ -124,17 +124,19 @@ template <bool CNT, int W>
 LXD void lx_trace_path_d(const LxJob& J, const LxDevScene& S, f3 o, f3 d, uint32_t pix, int bounce, LxStack& st,
                          uint32_t* overflow, bool pend, f3 pend_o, f3 pend_d, bool has_rad, bool& pend_occluded,
                          LxRecSink& out, LxRayCount& rc, LxWork* wc, LxWork* wa) {
-    f3 lp = mk3(0, 0, 0), ed = mk3(0, 0, 0);
-    if (has_rad && (J.dmis || J.emis)) {
-        /* sampleDirectLight: RNG keyed on the pre-increment depth */
-        LxRng rng = lx_make_rng(J.iter, (int)pix, bounce, J.seed);
-        if (J.dmis) lp = lx_sample_light_point(S, rng);
-        if (J.emis) ed = lx_sample_env_dir(S, rng);
-    }
     LxHitD h = lx_trace_pair<CNT, W>(S, pend, pend_o, pend_d, 1e16f, has_rad, o, d, 1e16f, pix, (uint32_t)J.iter,
                                       LX_INST_ALPHA, st, overflow, pend_occluded, wc, wa);
     if (pend) rc.env++;
     if (!has_rad) return;
+    /* sampleDirectLight: RNG keyed on the pre-increment depth.  The samples
+     * do not depend on the trace, so they are drawn after it (fewer registers
+     * live across the traversal loop; the same values). */
+    f3 lp = mk3(0, 0, 0), ed = mk3(0, 0, 0);
+    if (J.dmis || J.emis) {
+        LxRng rng = lx_make_rng(J.iter, (int)pix, bounce, J.seed);
+        if (J.dmis) lp = lx_sample_light_point(S, rng);
+        if (J.emis) ed = lx_sample_env_dir(S, rng);
+    }
     rc.rad++;
     if (J.dmis) out.light(lp);
     if (J.emis) out.env_dir(ed);
