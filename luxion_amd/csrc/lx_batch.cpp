@@ -255,16 +255,32 @@ extern "C" lx_status lx_batch_run(lx_context* ctx, const lx_job* jobs, uint32_t 
             LX_HIP(hipMemcpyAsync(d_spp1.ptr, hsp[1].data(), n * 4, hipMemcpyHostToDevice, s));
             LX_HIP(lx_launch_fill_u32(d_cnt1.as<uint32_t>(), sizeof(LxCounters) / 4, 0u, s));
         }
-        /* one captured round, replayed; an event after every round dates job completion */
-        hipGraph_t g = nullptr;
-        hipGraphExec_t ge = nullptr;
-        bool use_graph = ctx->tuning.use_graph != 0 && !overlap && !wide;
-        if (use_graph) {
-            LX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            try {
+        /* a round on the context stream: with wide rounds, the job copies'
+         * colour regions filled with -0.0, the round, then the regions added
+         * into the accumulation in sample order */
+        auto plain_round = [&]() {
+            if (wide) {
+                LX_HIP(lx_launch_fill_u32(d_colw.as<uint32_t>(), accum_pix * R * 3, 0x80000000u, s));
+                lx_enqueue_round(ctx, d_jobsw.as<LxJob>(), d_sppw.as<int32_t>(), n * R, d_cnt.as<LxCounters>(),
+                                 d_sched.as<LxSched>(), cap, wb, d_colw.as<float>(), d_spill.as<uint32_t>(), max_depth,
+                                 tg, sg, cg, width, uscene, nullptr);
+                LX_HIP(lx_launch_add_regions(d_accum.as<float>(), d_colw.as<float>(), accum_pix * 3, R, s));
+            } else {
                 lx_enqueue_round(ctx, d_jobs.as<LxJob>(), d_spp.as<int32_t>(), n, d_cnt.as<LxCounters>(),
                                  d_sched.as<LxSched>(), cap, wb, d_accum.as<float>(), d_spill.as<uint32_t>(), max_depth,
                                  tg, sg, cg, width, uscene, nullptr);
+            }
+        };
+        /* one captured round, replayed (every round is the same launches over
+         * device-side job state; no memset nodes); an event after every round
+         * dates job completion */
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        bool use_graph = ctx->tuning.use_graph != 0 && !overlap;
+        if (use_graph) {
+            LX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            try {
+                plain_round();
             } catch (...) {
                 hipGraph_t tmp;
                 (void)hipStreamEndCapture(s, &tmp);
@@ -311,13 +327,7 @@ extern "C" lx_status lx_batch_run(lx_context* ctx, const lx_job* jobs, uint32_t 
         LX_HIP(hipEventRecord(ev[0], s));
         double t_start = now_ms();
         for (int k = 0; k < rounds; k++) {
-            if (wide) {
-                LX_HIP(lx_launch_fill_u32(d_colw.as<uint32_t>(), accum_pix * R * 3, 0x80000000u, s));
-                lx_enqueue_round(ctx, d_jobsw.as<LxJob>(), d_sppw.as<int32_t>(), n * R, d_cnt.as<LxCounters>(),
-                                 d_sched.as<LxSched>(), cap, wb, d_colw.as<float>(), d_spill.as<uint32_t>(), max_depth,
-                                 tg, sg, cg, width, uscene, nullptr);
-                LX_HIP(lx_launch_add_regions(d_accum.as<float>(), d_colw.as<float>(), accum_pix * 3, R, s));
-            } else if (overlap) {
+            if (overlap) {
                 int p = k & 1;
                 if (k >= 2) LX_HIP(hipStreamWaitEvent(ps[p], e_free[p], 0));
                 LX_HIP(lx_launch_fill_u32(d_col[p].as<uint32_t>(), accum_pix * 3, 0x80000000u, ps[p]));
@@ -330,9 +340,7 @@ extern "C" lx_status lx_batch_run(lx_context* ctx, const lx_job* jobs, uint32_t 
             } else if (use_graph) {
                 LX_HIP(hipGraphLaunch(ge, s));
             } else {
-                lx_enqueue_round(ctx, d_jobs.as<LxJob>(), d_spp.as<int32_t>(), n, d_cnt.as<LxCounters>(),
-                                 d_sched.as<LxSched>(), cap, wb, d_accum.as<float>(), d_spill.as<uint32_t>(), max_depth,
-                                 tg, sg, cg, width, uscene, nullptr);
+                plain_round();
             }
             LX_HIP(hipEventRecord(ev[k + 1], s));
         }

@@ -9,6 +9,10 @@ after another through their own lx_render (graph-replayed rounds), with the
 images of both compared bit for bit (so the two reach the same RMSE against
 any reference).
 
+run_per_job: the same jobs rendered one after another, each through its
+own lx_render (the plain per-job launches the batched form is measured
+against).
+
 run_served: Poisson arrivals at rate lam into an lx_server; latency is
 submit -> result ready, per job.
 """
@@ -56,8 +60,10 @@ def run_batched(ctx, spec, per_job=True):
     dev_ms = max(r["finish_ms"] for r in res) - min(r["start_ms"] for r in res)
     assert all(r["status"] == 0 for r in res)
     ps = path_samples(spec)
+    rays = sum(r["rays"] for r in res)
     out = {"jobs": len(spec), "path_samples": ps, "batched_s": round(t_batch, 4),
-           "batched_device_s": round(dev_ms / 1e3, 4), "batched_spp_per_s": round(ps / t_batch, 1)}
+           "batched_device_s": round(dev_ms / 1e3, 4), "batched_spp_per_s": round(ps / t_batch, 1),
+           "batched_mrays_per_s": round(rays / t_batch / 1e6, 1), "rays_per_path_sample": round(rays / ps, 3)}
     if not per_job:
         return out, res
     # per-job launches: each job's own render, buffers and graph prepared
@@ -82,6 +88,28 @@ def run_batched(ctx, spec, per_job=True):
     out.update({"per_job_s": round(t_jobs, 4), "per_job_spp_per_s": round(ps / t_jobs, 1),
                 "speedup_vs_per_job": round(t_jobs / t_batch, 3), "images_bitexact": same})
     return out, res
+
+
+def run_per_job(spec):
+    """Plain per-job launches: every job through its own lx_render, under
+    the context's current tuning, buffers and graph prepared beforehand (a
+    warm-up sample, then reset), timed from each job's first launch to its
+    last sample being ready.  Returns the summed time and the images."""
+    t_jobs = 0.0
+    images = []
+    for sc, cam, kw in spec:
+        r = Render(sc, camera=cam, seed=kw["seed"], dmis=kw.get("dmis", False), emis=kw.get("emis", False))
+        r.iterate(1, 1)
+        r.sync()
+        r.reset()
+        t0 = time.perf_counter()
+        r.iterate(kw["first_iter"], kw["spp"])
+        r.sync()
+        t_jobs += time.perf_counter() - t0
+        images.append(r.accum()[0])
+        r.close()
+    ps = path_samples(spec)
+    return {"per_job_s": round(t_jobs, 4), "per_job_spp_per_s": round(ps / t_jobs, 1), "images": images}
 
 
 def run_served(ctx, spec, lam, seed=11, max_jobs=64, max_depth=12, srv=None):
