@@ -170,7 +170,7 @@ typedef struct lx_scene_desc {
 
 /* Replaces Scene::Scene(std::string, const char*) for procedural input
  * (src/scene/scene.cpp:22-49) plus the emitter / HDRI CDF precompute
- * (src/scene/scene_precompute.cpp:71-193). */
+ * (src/scene/scene_precompute.cpp:22-144). */
 lx_status lx_scene_create(lx_context* ctx, const lx_scene_desc* desc, lx_scene** out);
 
 /* Replaces Scene::Scene for .glb input (src/scene/scene.cpp:22-49 ->

@@ -4,7 +4,7 @@
  *
  * Replaces Scene::Scene + Scene::precompute_emissive_mesh_area +
  * Scene::precompute_hdri_emission (src/scene/scene.cpp:22-49,
- * src/scene/scene_precompute.cpp:71-193), Mesh::make_mesh_device
+ * src/scene/scene_precompute.cpp:22-144), Mesh::make_mesh_device
  * (src/mesh/mesh_device.cu:11-45) and the GAS / IAS builds
  * (src/myoptix/optix_accel.cpp:9-110, src/myoptix/optix_ias.cpp:9-63).
  *
@@ -94,7 +94,7 @@ static float texel_f(const HostTex& t, size_t k, int c) {
     return v;
 }
 
-/* precompute_emissive_mesh_area, scene_precompute.cpp:71-122 */
+/* precompute_emissive_mesh_area, scene_precompute.cpp:22-74 */
 static void precompute_emitters(lx_scene* s) {
     s->emissive.clear();
     s->emissive_prefix.clear();
@@ -149,7 +149,7 @@ static void precompute_emitters(lx_scene* s) {
     }
 }
 
-/* precompute_hdri_emission, scene_precompute.cpp:125-193 */
+/* precompute_hdri_emission, scene_precompute.cpp:76-144 */
 static void precompute_env(lx_scene* s) {
     int W = (int)s->env.w, H = (int)s->env.h;
     const float PIF = 3.1415926535897932384626422832795028841971f;

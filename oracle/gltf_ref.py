@@ -11,7 +11,9 @@ exactly.
 PNG and JPEG texels come from PIL (PNG is lossless, so exact; JPEG decoders
 differ in rounding, so JPEG is compared with a tolerance).  EXR: ZIP / ZIPS /
 uncompressed scanline files are decoded here with zlib; PIZ has no decoder in
-this image ("parity unpinned" for PIZ texels: checked by invariants).
+this image: PIZ files written by the tests' format-restated PIZ encoder
+(tests/gltf_util.py) must decode back exactly, and the reference's own PIZ
+HDRIs are checked by invariants.
 """
 import io
 import json

@@ -107,6 +107,13 @@ def lib():
         L.lxo_sample_env.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_int, P]
         L.lxo_sample_light.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_uint32, P]
         L.lxo_math_eval.argtypes = [C.c_int, P, P, P, C.c_uint32]
+        L.lxo_ct_eval_batch.argtypes = [C.c_int, P, P, C.c_uint32, P, C.c_float, C.c_float, P, C.c_float, C.c_float, P]
+        L.lxo_ct_sample_batch.argtypes = [P, P, C.c_float, C.c_float, P, C.c_float, C.c_float, C.c_int, C.c_int,
+                                          C.c_uint32, C.c_int, C.c_uint32, P]
+        L.lxo_envmap_pdf_batch.argtypes = [P, P, C.c_uint32, P]
+        L.lxo_sample_env_batch.argtypes = [P, C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_uint32, P]
+        L.lxo_sample_light_batch.argtypes = [P, C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_uint32, P]
+        L.lxo_get_normal.argtypes = [P, C.c_uint32, P, P, P, P]
         _lib = L
     return _lib
 
@@ -125,6 +132,27 @@ def math_eval(fn, x, y=None):
     out = np.empty_like(x)
     lib().lxo_math_eval(MATH_FNS[fn], x.ctypes.data, None if yy is None else yy.ctypes.data, out.ctypes.data,
                         len(x))
+    return out
+
+
+def ct_eval(what, wo, wi, n, roughness, metallic, color, ior, transmission):
+    """CookTorrance::PDF (what 0), BTDF (1) or BRDF (2) of the oracle at the
+    directions wi (k, 3) for one outgoing direction wo."""
+    wi = np.ascontiguousarray(wi, np.float32).reshape(-1, 3)
+    out = np.zeros(len(wi) if what == 0 else (len(wi), 3), np.float32)
+    wo_, n_, c_ = f3(wo), f3(n), f3(color)  # held until the call returns
+    lib().lxo_ct_eval_batch(what, wo_.ctypes.data, wi.ctypes.data, len(wi), n_.ctypes.data, roughness,
+                            metallic, c_.ctypes.data, ior, transmission, out.ctypes.data)
+    return out
+
+
+def ct_sample(wo, n, roughness, metallic, color, ior, transmission, count, iter=1, depth=1, seed=0, index0=0):
+    """sampleCookTorrance directions for RNG(iter, index0 + i, depth), i < count
+    (zero vectors where the sampler rejects)."""
+    out = np.zeros((count, 3), np.float32)
+    wo_, n_, c_ = f3(wo), f3(n), f3(color)  # held until the call returns
+    lib().lxo_ct_sample_batch(wo_.ctypes.data, n_.ctypes.data, roughness, metallic, c_.ctypes.data, ior,
+                              transmission, iter, depth, seed, index0, count, out.ctypes.data)
     return out
 
 
@@ -202,6 +230,28 @@ class OracleScene:
         self.L.lxo_render(self.h, C.byref(o), first_iter, n_iters, None if pix is None else pix.ctypes.data, n,
                          out.ctypes.data, C.byref(st))
         return out, {k: getattr(st, k) for k, _ in lxo_stats._fields_}
+
+    def envmap_pdf(self, dirs):
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        out = np.zeros(len(d), np.float32)
+        self.L.lxo_envmap_pdf_batch(self.h, d.ctypes.data, len(d), out.ctypes.data)
+        return out
+
+    def sample_env(self, count, iter=1, depth=1, seed=0, index0=0):
+        out = np.zeros((count, 3), np.float32)
+        self.L.lxo_sample_env_batch(self.h, iter, depth, seed, index0, count, out.ctypes.data)
+        return out
+
+    def sample_light(self, count, iter=1, depth=1, seed=0, index0=0):
+        out = np.zeros((count, 3), np.float32)
+        self.L.lxo_sample_light_batch(self.h, iter, depth, seed, index0, count, out.ctypes.data)
+        return out
+
+    def get_normal(self, mat, N, T, uv):
+        out = np.zeros(3, np.float32)
+        n_, t_, uv_ = f3(N), f3(T), np.ascontiguousarray(uv, np.float32)  # held until the call returns
+        self.L.lxo_get_normal(self.h, mat, n_.ctypes.data, t_.ctypes.data, uv_.ctypes.data, out.ctypes.data)
+        return out
 
     def debug_bounce(self, iteration, pixels=None, **kw):
         o = self.opts(**kw)

@@ -100,6 +100,23 @@ float lxo_envmap_pdf(const lxo_scene* s, const float* d);
 void lxo_sample_env(const lxo_scene* s, int iter, int index, int depth, uint32_t seed, int dmis, float* dir3);
 void lxo_sample_light(const lxo_scene* s, int iter, int index, int depth, uint32_t seed, float* p3);
 
+/* Batched forms for Monte-Carlo and quadrature checks.  lxo_ct_eval_batch:
+ * what 0 = PDF (1 float per direction), 1 = BTDF, 2 = BRDF (3 floats) at
+ * n_dirs directions wi.  The samplers draw RNG(iter, index0 + i, depth). */
+void lxo_ct_eval_batch(int what, const float* wo, const float* wi, uint32_t n_dirs, const float* n, float roughness,
+                       float metallic, const float* color, float ior, float transmission, float* out);
+void lxo_ct_sample_batch(const float* wo, const float* n, float roughness, float metallic, const float* color,
+                         float ior, float transmission, int iter, int depth, uint32_t seed, int index0, uint32_t count,
+                         float* dirs);
+void lxo_envmap_pdf_batch(const lxo_scene* s, const float* dirs, uint32_t count, float* out);
+void lxo_sample_env_batch(const lxo_scene* s, int iter, int depth, uint32_t seed, int index0, uint32_t count,
+                          float* dirs);
+void lxo_sample_light_batch(const lxo_scene* s, int iter, int depth, uint32_t seed, int index0, uint32_t count,
+                            float* pts);
+/* get_normal (material_queries.cu:33-89) of material `mat` for a hit with
+ * normal N, tangent T and uv. */
+void lxo_get_normal(const lxo_scene* s, uint32_t mat, const float* N, const float* T, const float* uv, float* out3);
+
 /* Deterministic math, exported for the accuracy sweep. */
 void lxo_math_eval(int fn, const float* x, const float* y, float* out, uint32_t n);
 

@@ -61,7 +61,8 @@ def test_lambert_pdf_normalised_and_floored(oracle):
     n = _f3([0.3, 0.8, -0.52] / np.linalg.norm([0.3, 0.8, -0.52]))
     d = rng.normal(size=(40000, 3))
     d /= np.linalg.norm(d, axis=1, keepdims=True)
-    p = np.array([L.lxo_lambert_pdf(_f3(w).ctypes.data, n.ctypes.data) for w in d])
+    dd = _f3(d)  # one array held for the calls (a temporary's buffer may be freed before the call)
+    p = np.array([L.lxo_lambert_pdf(dd[i].ctypes.data, n.ctypes.data) for i in range(len(dd))])
     assert abs(np.mean(p) * 4 * math.pi - 1.0) < 0.02  # uniform-sphere estimate of the integral
     below = (d @ n) < 0
     assert np.all(p[below] == np.float32(1e-6))
