@@ -1787,16 +1787,28 @@ lxo_scene* lxo_scene_create(const lx_scene_desc* d, int cdf_mode, int build_bvh)
     }
     double t0 = now_ms();
     if (build_bvh) {
-        for (auto& m : s->meshes) {
-            int nt = (int)(m.idx.size() / 3);
-            if (nt == 0) continue;
-            m.bvh.resize(2 * nt - 1);
-            std::vector<int> ti(nt);
-            std::iota(ti.begin(), ti.end(), 0);
-            int alloc = 1, md = 0;
-            fill_bvh(0, 0, nt - 1, m.bvh, m, ti, alloc, 0, md);
-            m.bvh_depth = md;
-        }
+        /* one tree per mesh (BVH::make_bvh per Geom, bvh.cu:18-38); the meshes
+         * are independent, so they are built on up to 16 threads */
+        std::atomic<size_t> next{0};
+        auto work = [&]() {
+            for (size_t k = next++; k < s->meshes.size(); k = next++) {
+                Mesh& m = s->meshes[k];
+                int nt = (int)(m.idx.size() / 3);
+                if (nt == 0) continue;
+                m.bvh.resize(2 * nt - 1);
+                std::vector<int> ti(nt);
+                std::iota(ti.begin(), ti.end(), 0);
+                int alloc = 1, md = 0;
+                fill_bvh(0, 0, nt - 1, m.bvh, m, ti, alloc, 0, md);
+                m.bvh_depth = md;
+            }
+        };
+        unsigned nthr = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        nthr = (unsigned)std::min<size_t>(nthr, std::max<size_t>(1, s->meshes.size()));
+        std::vector<std::thread> builders;
+        for (unsigned i = 1; i < nthr; i++) builders.emplace_back(work);
+        work();
+        for (auto& t : builders) t.join();
     }
     s->build_ms = now_ms() - t0;
     s->mats.resize(d->num_materials);

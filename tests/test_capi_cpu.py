@@ -3,6 +3,7 @@ declares (no GPU calls), and the ctypes layouts match the header."""
 import ctypes as C
 import os
 import re
+import subprocess
 
 import pytest
 
@@ -42,14 +43,50 @@ def test_abi_version_and_clean_error_state():
     assert lib.lx_last_error() == b""
 
 
-def test_struct_sizes_match_c_layout():
-    # sizes from the header's field lists (x86-64 SysV alignment)
-    assert C.sizeof(F.lx_tex_transform) == 20
-    assert C.sizeof(F.lx_material_desc) == 4 * (3 + 1 + 5 + 2 + 1 + 5 + 1 + 5 + 3 + 1 + 1 + 5 + 5 + 1)
-    assert C.sizeof(F.lx_camera_desc) == 4 * (2 + 12 + 2)
-    assert C.sizeof(F.lx_ray) == 32
-    assert C.sizeof(F.lx_hit) == 20
-    assert C.sizeof(F.lx_instance_desc) == 56
+def _header_structs():
+    txt = open(os.path.join(ROOT, "include", "luxion_mi355x.h")).read()
+    return sorted(set(re.findall(r"typedef struct (lx_\w+) \{", txt)))
+
+
+def test_struct_layouts_match_the_compiled_header(tmp_path):
+    """A C++ program compiled against include/luxion_mi355x.h prints sizeof
+    and offsetof of every field of every struct the header defines; the
+    ctypes mirror in luxion_amd/_ffi.py must agree field for field."""
+    structs = _header_structs()
+    assert len(structs) >= 15
+    missing = [s for s in structs if not hasattr(F, s)]
+    assert not missing, missing
+    lines = ['#include "luxion_mi355x.h"', "#include <cstddef>", "#include <cstdio>", "int main() {"]
+    for s in structs:
+        lines.append(f'    std::printf("{s} size %zu\\n", sizeof({s}));')
+        for fname, _ in getattr(F, s)._fields_:
+            lines.append(f'    std::printf("{s} {fname} %zu\\n", offsetof({s}, {fname}));')
+    lines += ["    return 0;", "}"]
+    src = tmp_path / "layout.cpp"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["g++", "-std=c++17", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    checked = 0
+    for line in filter(None, out):
+        s, key, val = line.split()
+        cls = getattr(F, s)
+        want = C.sizeof(cls) if key == "size" else getattr(cls, key).offset
+        assert int(val) == want, (s, key, int(val), want)
+        checked += 1
+    assert checked == sum(1 + len(getattr(F, s)._fields_) for s in structs)
+
+
+def test_integration_example_is_the_compiled_consumer():
+    """The C++ example in INTEGRATION.md is, line for line, the block that
+    tests/capi_consumer.cpp compiles (and the GPU suite runs)."""
+    doc = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    block = doc.split("```cpp", 1)[1].split("```", 1)[0].strip().split("\n")
+    assert block[0] == '#include "luxion_mi355x.h"'
+    snippet = "\n".join(block[1:]).strip()
+    src = open(os.path.join(ROOT, "tests", "capi_consumer.cpp")).read()
+    body = src.split("// INTEGRATION-BEGIN", 1)[1].split("// INTEGRATION-END", 1)[0].strip()
+    assert snippet == body
 
 
 def test_null_arguments_are_rejected_without_gpu():
