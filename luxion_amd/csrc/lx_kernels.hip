@@ -456,7 +456,7 @@ void k_finish(const LxJob* __restrict__ jobs, LxPathBuf P, LxCounters* __restric
         if (have || flush) {
             const LxJob& J = UNI == 2 ? jobs[0] : jobs[job];
             const LxDevScene& S = UNI ? *uscene : *J.scene;
-            LxRec r;
+            LxRec r{};
             LxRecSink sink{r};
             bool occluded = false;
             lx_trace_path_d<CNT, W>(J, S, o, d, pix, b, st, &C->stack_overflow, pend, pend_o, pend_d, have, occluded,

@@ -140,6 +140,13 @@ LXD void lx_trace_path_d(const LxJob& J, const LxDevScene& S, f3 o, f3 d, uint32
     rc.rad++;
     if (J.dmis) out.light(lp);
     if (J.emis) out.env_dir(ed);
+    /* the miss values first, overwritten below on hits: stores of the same
+     * constant to different fields in different branches would otherwise
+     * be merged into one store through a selected address, which keeps the
+     * record in scratch memory */
+    out.miss();
+    if (J.dmis) out.dl_miss();
+    out.env(-0.5f);
     if (h.inst != 0xffffffffu) {
         bool want_t = lx_gptr(S.mats)[lx_gptr(S.inst)[h.inst].material].normal_tex >= 0;
         f3 N, Tg;
@@ -162,16 +169,10 @@ LXD void lx_trace_path_d(const LxJob& J, const LxDevScene& S, f3 o, f3 d, uint32
                 int dmat;
                 lx_dl_attrs(S, hd, dN, duv, dmat);
                 out.dl(hd.t, dN, duv, dmat);
-            } else {
-                out.dl_miss();
             }
         }
         out.env_origin(so);
         out.env(1.0f);
-    } else {
-        out.miss();
-        if (J.dmis) out.dl_miss();
-        out.env(-0.5f);
     }
 }
 
