@@ -632,5 +632,7 @@ bool lx_build_blas_gpu(lx_context* ctx, const float* pos, const uint32_t* idx, u
     for (size_t k = 0; k < top.size(); k++) out.nodes[k] = top[k];
     out.root = 0;
     out.depth = depth;
+    lx_wide_dfs_order(out.nodes, out.root);
+    out.sah = lx_wide_sah(out.nodes, out.root);
     return true;
 }

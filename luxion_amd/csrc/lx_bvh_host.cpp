@@ -253,6 +253,73 @@ static uint32_t collapse(const std::vector<BNode>& bn, int32_t bi, int W, std::v
     return me;
 }
 
+double lx_wide_sah(const std::vector<LxWideNode>& nodes, uint32_t root) {
+    if (nodes.empty() || (root & LX_LEAF_BIT)) return 0.0;
+    auto node_box = [&](uint32_t i, float* lo, float* hi) {
+        const LxWideNode& w = nodes[i];
+        for (int a = 0; a < 3; a++) {
+            lo[a] = FLT_MAX;
+            hi[a] = -FLT_MAX;
+        }
+        for (uint32_t k = 0; k < 8; k++) {
+            if (w.child[k] == LX_CHILD_EMPTY) continue;
+            for (int a = 0; a < 3; a++) {
+                lo[a] = std::min(lo[a], w.lo[k][a]);
+                hi[a] = std::max(hi[a], w.hi[k][a]);
+            }
+        }
+    };
+    float rlo[3], rhi[3];
+    node_box(root, rlo, rhi);
+    double ra = (double)area(rlo, rhi);
+    if (!(ra > 0.0)) return 0.0;
+    double sah = 0.0;
+    std::vector<uint32_t> todo{root};
+    while (!todo.empty()) {
+        uint32_t i = todo.back();
+        todo.pop_back();
+        float lo[3], hi[3];
+        node_box(i, lo, hi);
+        sah += (double)area(lo, hi) / ra;
+        const LxWideNode& w = nodes[i];
+        for (uint32_t k = 0; k < 8; k++) {
+            uint32_t c = w.child[k];
+            if (c == LX_CHILD_EMPTY) continue;
+            if (c & LX_LEAF_BIT) sah += (double)area(w.lo[k], w.hi[k]) / ra * (double)(((c >> 28) & 7u) + 1u);
+            else todo.push_back(c);
+        }
+    }
+    return sah;
+}
+
+void lx_wide_dfs_order(std::vector<LxWideNode>& nodes, uint32_t& root) {
+    if (nodes.empty() || (root & LX_LEAF_BIT)) return;
+    std::vector<uint32_t> order;
+    order.reserve(nodes.size());
+    std::vector<uint32_t> todo{root};
+    while (!todo.empty()) {
+        uint32_t i = todo.back();
+        todo.pop_back();
+        order.push_back(i);
+        const LxWideNode& w = nodes[i];
+        for (int k = 7; k >= 0; k--) { /* slot 0 is visited first */
+            uint32_t c = w.child[k];
+            if (c != LX_CHILD_EMPTY && !(c & LX_LEAF_BIT)) todo.push_back(c);
+        }
+    }
+    std::vector<uint32_t> newid(nodes.size(), LX_CHILD_EMPTY);
+    for (uint32_t k = 0; k < (uint32_t)order.size(); k++) newid[order[k]] = k;
+    std::vector<LxWideNode> out(order.size());
+    for (uint32_t k = 0; k < (uint32_t)order.size(); k++) {
+        LxWideNode w = nodes[order[k]];
+        for (int c = 0; c < 8; c++)
+            if (w.child[c] != LX_CHILD_EMPTY && !(w.child[c] & LX_LEAF_BIT)) w.child[c] = newid[w.child[c]];
+        out[k] = w;
+    }
+    nodes.swap(out);
+    root = 0;
+}
+
 static uint32_t make_leaf_ref(uint32_t first, uint32_t count) {
     return LX_LEAF_BIT | ((count - 1u) << 28) | first;
 }

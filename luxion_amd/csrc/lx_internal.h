@@ -144,6 +144,14 @@ void lx_build_tlas_host(const std::vector<float>& boxes /* 6 per instance */, in
 bool lx_build_blas_gpu(lx_context* ctx, const float* pos, const uint32_t* idx, uint32_t ntri, int leaf_size, int bins,
                        int width, BvhBuildResult& out);
 void lx_pad_box(float lo[3], float hi[3]);
+/* SAH cost of a wide tree, in the units of the host builder's cost: every
+ * node's box area (the union of its child boxes) plus every leaf's box area
+ * times its triangle count, over the root box area */
+double lx_wide_sah(const std::vector<LxWideNode>& nodes, uint32_t root);
+/* Renumber a wide tree's nodes in depth-first pre-order (children in slot
+ * order), so that each subtree's nodes are contiguous in memory as the host
+ * builder's recursion lays them out; root becomes 0. */
+void lx_wide_dfs_order(std::vector<LxWideNode>& nodes, uint32_t& root);
 
 /* -------------------------------------------------------------- scene */
 struct HostMesh {
