@@ -297,6 +297,9 @@ def main():
                      "max_depth": depth, "flags": flag_name,
                      "launch_shape": "one frame (1 spp) per tail-kernel launch; %d round(s) in flight"
                                      % (1 + tuning["overlap_rounds"])}
+    out["accel_build"] = {"ms": round(info["build_ms"], 1), "bvh_nodes": info["bvh_nodes"],
+                          "sah_cost": round(info["sah_cost"], 4),
+                          "builder": "auto: BLASes of 2^14 triangles or more on the GPU, smaller ones on the host"}
     cam_rays = W * H
     if st_main is not None:
         rays = rays_of(st_main)

@@ -188,9 +188,9 @@ enum { LX_CDF_FP64 = 0, LX_CDF_FP32_REFERENCE = 1 };
 typedef struct lx_build_opts {
     int32_t bvh_width;      /* 4 or 8; 0 = the context tuning's bvh_width */
     int32_t leaf_size;      /* max triangles per leaf, 1..8 (default 4) */
-    int32_t builder;        /* 0 = GPU (Morton LBVH, binned SAH over clusters at the top, SAH
-                               leaf test), 1 = host binned SAH, 2 = auto (default): GPU for
-                               BLASes of 2^22 triangles or more, host otherwise */
+    int32_t builder;        /* 0 = GPU (top-down binned SAH on the device, the host builder's
+                               split and leaf rules), 1 = host binned SAH, 2 = auto (default):
+                               GPU for BLASes of 2^14 triangles or more, host otherwise */
     int32_t sah_bins;       /* binned-SAH bins (default 16) */
     int32_t cdf_precision;  /* LX_CDF_FP64 (default) or LX_CDF_FP32_REFERENCE */
 } lx_build_opts;

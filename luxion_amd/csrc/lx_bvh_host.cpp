@@ -11,10 +11,10 @@
 #include "lx_internal.h"
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cfloat>
-#include <functional>
 #include <thread>
 
 namespace {
@@ -200,41 +200,42 @@ void lx_pad_box(float lo[3], float hi[3]) {
     }
 }
 
-/* Collapse the binary tree into W-wide nodes (W = 4 or 8): each wide node
- * adopts the children of its largest-area internal children until it has W. */
-static uint32_t collapse(const std::vector<BNode>& bn, int32_t bi, int W, std::vector<LxWideNode>& out,
-                         const std::function<uint32_t(uint32_t, uint32_t)>& leaf_ref, int depth, int& max_depth,
-                         double& sah, double root_area) {
-    if (depth > max_depth) max_depth = depth;
-    const BNode& b = bn[bi];
-    if (b.left < 0) {
-        sah += (double)area(b.lo, b.hi) / root_area * (double)b.count;
-        return leaf_ref(b.first, b.count);
-    }
-    sah += (double)area(b.lo, b.hi) / root_area;
-    int32_t kids[8] = {b.left, b.right, -1, -1, -1, -1, -1, -1};
+/* The wide node over binary internal node b: starting from b's two
+ * children, the largest-area internal child is replaced by its own two
+ * children until there are W (or no internal child is left).  Returns the
+ * number of children. */
+static int wide_kids(const std::vector<BNode>& bn, const BNode& b, int W, int32_t kids[8]) {
+    float karea[8];
+    kids[0] = b.left;
+    kids[1] = b.right;
+    karea[0] = area(bn[b.left].lo, bn[b.left].hi);
+    karea[1] = area(bn[b.right].lo, bn[b.right].hi);
     int nk = 2;
     while (nk < W) {
         int best = -1;
         float ba = -1.0f;
         for (int k = 0; k < nk; k++) {
-            const BNode& c = bn[kids[k]];
-            if (c.left < 0) continue;
-            float a = area(c.lo, c.hi);
-            if (a > ba) {
-                ba = a;
+            if (bn[kids[k]].left < 0) continue;
+            if (karea[k] > ba) {
+                ba = karea[k];
                 best = k;
             }
         }
         if (best < 0) break;
-        int32_t c = kids[best];
-        kids[best] = bn[c].left;
-        kids[nk++] = bn[c].right;
+        const BNode& c = bn[kids[best]];
+        kids[best] = c.left;
+        karea[best] = area(bn[c.left].lo, bn[c.left].hi);
+        kids[nk] = c.right;
+        karea[nk++] = area(bn[c.right].lo, bn[c.right].hi);
     }
-    uint32_t me = (uint32_t)out.size();
-    out.emplace_back();
-    LxWideNode n;
-    std::memset(&n, 0, sizeof(n));
+    return nk;
+}
+
+/* Starts wide node `me` of `out` for binary internal node b: child count,
+ * padded child boxes, empty slots; the child refs of used slots are left to
+ * the caller. */
+static int begin_wide(const std::vector<BNode>& bn, const BNode& b, int W, LxWideNode& n, int32_t kids[8]) {
+    int nk = wide_kids(bn, b, W, kids);
     n.n = (uint32_t)nk;
     for (int k = 0; k < 8; k++) {
         if (k < nk) {
@@ -244,80 +245,157 @@ static uint32_t collapse(const std::vector<BNode>& bn, int32_t bi, int W, std::v
                 n.hi[k][a] = c.hi[a];
             }
             lx_pad_box(n.lo[k], n.hi[k]);
-            n.child[k] = collapse(bn, kids[k], W, out, leaf_ref, depth + 1, max_depth, sah, root_area);
         } else {
+            for (int a = 0; a < 3; a++) n.lo[k][a] = n.hi[k][a] = 0.0f;
             n.child[k] = LX_CHILD_EMPTY;
         }
     }
-    out[me] = n;
+    return nk;
+}
+
+struct CollapseSums {
+    int max_depth = 0;
+    double sah = 0.0;
+};
+
+/* Appends the wide subtree of binary node bi to `out` in depth-first
+ * pre-order (a node, then its children's subtrees in slot order); returns
+ * its child ref (a node index of `out`, or a leaf ref). */
+template <class LeafRef>
+static uint32_t collapse_rec(const std::vector<BNode>& bn, int32_t bi, int W, std::vector<LxWideNode>& out,
+                             const LeafRef& leaf_ref, int depth, CollapseSums& acc, double inv_root_area) {
+    if (depth > acc.max_depth) acc.max_depth = depth;
+    const BNode& b = bn[bi];
+    if (b.left < 0) {
+        acc.sah += (double)area(b.lo, b.hi) * inv_root_area * (double)b.count;
+        return leaf_ref(b.first, b.count);
+    }
+    acc.sah += (double)area(b.lo, b.hi) * inv_root_area;
+    uint32_t me = (uint32_t)out.size();
+    out.emplace_back();
+    int32_t kids[8];
+    int nk = begin_wide(bn, b, W, out[me], kids);
+    for (int k = 0; k < nk; k++) {
+        uint32_t ref = collapse_rec(bn, kids[k], W, out, leaf_ref, depth + 1, acc, inv_root_area);
+        out[me].child[k] = ref; /* `out` may have grown: index again */
+    }
     return me;
 }
 
-double lx_wide_sah(const std::vector<LxWideNode>& nodes, uint32_t root) {
-    if (nodes.empty() || (root & LX_LEAF_BIT)) return 0.0;
-    auto node_box = [&](uint32_t i, float* lo, float* hi) {
-        const LxWideNode& w = nodes[i];
-        for (int a = 0; a < 3; a++) {
-            lo[a] = FLT_MAX;
-            hi[a] = -FLT_MAX;
-        }
-        for (uint32_t k = 0; k < 8; k++) {
-            if (w.child[k] == LX_CHILD_EMPTY) continue;
-            for (int a = 0; a < 3; a++) {
-                lo[a] = std::min(lo[a], w.lo[k][a]);
-                hi[a] = std::max(hi[a], w.hi[k][a]);
-            }
-        }
-    };
-    float rlo[3], rhi[3];
-    node_box(root, rlo, rhi);
-    double ra = (double)area(rlo, rhi);
-    if (!(ra > 0.0)) return 0.0;
-    double sah = 0.0;
-    std::vector<uint32_t> todo{root};
-    while (!todo.empty()) {
-        uint32_t i = todo.back();
-        todo.pop_back();
-        float lo[3], hi[3];
-        node_box(i, lo, hi);
-        sah += (double)area(lo, hi) / ra;
-        const LxWideNode& w = nodes[i];
-        for (uint32_t k = 0; k < 8; k++) {
-            uint32_t c = w.child[k];
-            if (c == LX_CHILD_EMPTY) continue;
-            if (c & LX_LEAF_BIT) sah += (double)area(w.lo[k], w.hi[k]) / ra * (double)(((c >> 28) & 7u) + 1u);
-            else todo.push_back(c);
-        }
+/* The top of a large tree, collapsed serially down to wide depth `cut`; the
+ * subtrees below the cut become tasks.  `seq` lists the top nodes and tasks
+ * in the depth-first pre-order the serial collapse would write them in
+ * (k >= 0: top node k; k < 0: task -k - 1). */
+struct CollapseTop {
+    std::vector<LxWideNode> nodes;
+    std::vector<std::array<int32_t, 8>> task_of; /* per slot: the task it refers to, or -1 */
+    std::vector<std::pair<int32_t, int>> tasks;  /* binary node, wide depth */
+    std::vector<int64_t> seq;
+    CollapseSums acc;
+};
+
+template <class LeafRef>
+static uint32_t collapse_top(const std::vector<BNode>& bn, int32_t bi, int W, const LeafRef& leaf_ref, int depth,
+                             int cut, CollapseTop& T, double inv_root_area, int32_t& task) {
+    task = -1;
+    const BNode& b = bn[bi];
+    if (b.left >= 0 && depth >= cut) {
+        task = (int32_t)T.tasks.size();
+        T.tasks.push_back({bi, depth});
+        T.seq.push_back(-(int64_t)task - 1);
+        return 0u;
     }
-    return sah;
+    if (depth > T.acc.max_depth) T.acc.max_depth = depth;
+    if (b.left < 0) {
+        T.acc.sah += (double)area(b.lo, b.hi) * inv_root_area * (double)b.count;
+        return leaf_ref(b.first, b.count);
+    }
+    T.acc.sah += (double)area(b.lo, b.hi) * inv_root_area;
+    uint32_t me = (uint32_t)T.nodes.size();
+    T.nodes.emplace_back();
+    T.task_of.emplace_back();
+    T.task_of[me].fill(-1);
+    T.seq.push_back((int64_t)me);
+    int32_t kids[8];
+    int nk = begin_wide(bn, b, W, T.nodes[me], kids);
+    for (int k = 0; k < nk; k++) {
+        int32_t t;
+        uint32_t ref = collapse_top(bn, kids[k], W, leaf_ref, depth + 1, cut, T, inv_root_area, t);
+        T.nodes[me].child[k] = ref;
+        T.task_of[me][k] = t;
+    }
+    return me;
 }
 
-void lx_wide_dfs_order(std::vector<LxWideNode>& nodes, uint32_t& root) {
-    if (nodes.empty() || (root & LX_LEAF_BIT)) return;
-    std::vector<uint32_t> order;
-    order.reserve(nodes.size());
-    std::vector<uint32_t> todo{root};
-    while (!todo.empty()) {
-        uint32_t i = todo.back();
-        todo.pop_back();
-        order.push_back(i);
-        const LxWideNode& w = nodes[i];
-        for (int k = 7; k >= 0; k--) { /* slot 0 is visited first */
-            uint32_t c = w.child[k];
-            if (c != LX_CHILD_EMPTY && !(c & LX_LEAF_BIT)) todo.push_back(c);
+/* Collapse the binary tree into W-wide nodes (W = 4 or 8), nodes in
+ * depth-first pre-order.  Large trees are collapsed on host threads: the
+ * top levels serially, the subtrees below them concurrently, each into a
+ * node array of its own, then the arrays are placed where the serial
+ * pre-order puts them and their child refs offset -- the result is the
+ * serial collapse's node for node. */
+template <class LeafRef>
+static uint32_t collapse(const std::vector<BNode>& bn, int32_t root, int W, std::vector<LxWideNode>& out,
+                         const LeafRef& leaf_ref, int& max_depth, double& sah, double root_area) {
+    const double inv_ra = 1.0 / root_area;
+    out.clear();
+    if (bn.size() < (1u << 17)) {
+        out.reserve(bn.size() / 2 + 1); /* at most one wide node per internal binary node */
+        CollapseSums acc;
+        uint32_t r = collapse_rec(bn, root, W, out, leaf_ref, 0, acc, inv_ra);
+        max_depth = acc.max_depth;
+        sah = acc.sah;
+        return r;
+    }
+    CollapseTop T;
+    const int cut = W == 8 ? 3 : 4; /* up to 512 or 256 tasks */
+    int32_t root_task;
+    uint32_t r = collapse_top(bn, root, W, leaf_ref, 0, cut, T, inv_ra, root_task);
+    const size_t nt = T.tasks.size();
+    std::vector<std::vector<LxWideNode>> sub(nt);
+    std::vector<CollapseSums> sums(nt);
+    lx_parallel_for((uint32_t)nt, [&](uint32_t t) {
+        sub[t].reserve(64);
+        collapse_rec(bn, T.tasks[t].first, W, sub[t], leaf_ref, T.tasks[t].second, sums[t], inv_ra);
+    });
+    /* final positions, in the serial pre-order */
+    std::vector<uint32_t> top_at(T.nodes.size()), task_at(nt);
+    uint64_t pos = 0;
+    for (int64_t e : T.seq) {
+        if (e >= 0) top_at[(size_t)e] = (uint32_t)pos++;
+        else {
+            task_at[(size_t)(-e - 1)] = (uint32_t)pos;
+            pos += sub[(size_t)(-e - 1)].size();
         }
     }
-    std::vector<uint32_t> newid(nodes.size(), LX_CHILD_EMPTY);
-    for (uint32_t k = 0; k < (uint32_t)order.size(); k++) newid[order[k]] = k;
-    std::vector<LxWideNode> out(order.size());
-    for (uint32_t k = 0; k < (uint32_t)order.size(); k++) {
-        LxWideNode w = nodes[order[k]];
-        for (int c = 0; c < 8; c++)
-            if (w.child[c] != LX_CHILD_EMPTY && !(w.child[c] & LX_LEAF_BIT)) w.child[c] = newid[w.child[c]];
-        out[k] = w;
+    LX_REQUIRE(pos < LX_LEAF_BIT, LX_ERR_INVALID, "BVH too large for the node encoding");
+    out.resize((size_t)pos);
+    for (size_t k = 0; k < T.nodes.size(); k++) {
+        LxWideNode n = T.nodes[k];
+        for (int c = 0; c < 8; c++) {
+            if (n.child[c] == LX_CHILD_EMPTY) continue;
+            if (T.task_of[k][c] >= 0) n.child[c] = task_at[(size_t)T.task_of[k][c]]; /* a task's root is its node 0 */
+            else if (!(n.child[c] & LX_LEAF_BIT)) n.child[c] = top_at[n.child[c]];
+        }
+        out[top_at[k]] = n;
     }
-    nodes.swap(out);
-    root = 0;
+    lx_parallel_for((uint32_t)nt, [&](uint32_t t) {
+        const uint32_t base = task_at[t];
+        const std::vector<LxWideNode>& v = sub[t];
+        for (size_t k = 0; k < v.size(); k++) {
+            LxWideNode n = v[k];
+            for (int c = 0; c < 8; c++)
+                if (n.child[c] != LX_CHILD_EMPTY && !(n.child[c] & LX_LEAF_BIT)) n.child[c] += base;
+            out[base + k] = n;
+        }
+    });
+    max_depth = T.acc.max_depth;
+    sah = T.acc.sah;
+    for (size_t t = 0; t < nt; t++) {
+        max_depth = std::max(max_depth, sums[t].max_depth);
+        sah += sums[t].sah;
+    }
+    if (root_task >= 0) return task_at[(size_t)root_task]; /* cut == 0 only */
+    return (r & LX_LEAF_BIT) ? r : top_at[r];
 }
 
 static uint32_t make_leaf_ref(uint32_t first, uint32_t count) {
@@ -364,7 +442,7 @@ void lx_build_blas_host(const float* pos, const uint32_t* idx, uint32_t ntri, in
     int md = 0;
     double sah = 0.0;
     double ra = std::max(1e-30, (double)area(B.nodes[root].lo, B.nodes[root].hi));
-    out.root = collapse(B.nodes, root, width == 8 ? 8 : 4, out.nodes, make_leaf_ref, 0, md, sah, ra);
+    out.root = collapse(B.nodes, root, width == 8 ? 8 : 4, out.nodes, make_leaf_ref, md, sah, ra);
     out.depth = md;
     out.sah = sah;
 }
@@ -395,7 +473,7 @@ void lx_build_tlas_host(const std::vector<float>& boxes, int width, BvhBuildResu
         (void)count;
         return make_leaf_ref(prims[first].id, 1);
     };
-    out.root = collapse(B.nodes, root, width == 8 ? 8 : 4, out.nodes, ref, 0, md, sah, ra);
+    out.root = collapse(B.nodes, root, width == 8 ? 8 : 4, out.nodes, ref, md, sah, ra);
     out.depth = md;
     out.sah = sah;
 }

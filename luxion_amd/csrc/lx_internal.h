@@ -6,11 +6,14 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <array>
 #include <atomic>
 #include <cstdint>
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "lx_kernels.h"
@@ -120,6 +123,29 @@ struct DevBuf {
     }
 };
 
+/* ------------------------------------------------------ host threads */
+/* Runs fn(k) for every k in [0, n) on up to 16 host threads, the calling
+ * thread among them; indices are handed out one at a time from a shared
+ * counter, so fn must be safe to run concurrently for distinct k. */
+template <class Fn>
+void lx_parallel_for(uint32_t n, const Fn& fn) {
+    unsigned hw = std::thread::hardware_concurrency();
+    uint32_t nt = std::min<uint32_t>(n, std::min<unsigned>(16u, hw ? hw : 1u));
+    if (nt <= 1) {
+        for (uint32_t k = 0; k < n; k++) fn(k);
+        return;
+    }
+    std::atomic<uint32_t> next{0};
+    auto worker = [&]() {
+        for (uint32_t k = next.fetch_add(1); k < n; k = next.fetch_add(1)) fn(k);
+    };
+    std::vector<std::thread> pool;
+    pool.reserve(nt - 1);
+    for (uint32_t t = 1; t < nt; t++) pool.emplace_back(worker);
+    worker();
+    for (std::thread& t : pool) t.join();
+}
+
 /* ---------------------------------------------------------- BVH build */
 struct BvhBuildResult {
     std::vector<LxWideNode> nodes; /* local indices */
@@ -127,6 +153,16 @@ struct BvhBuildResult {
     uint32_t root = 0;            /* child ref (node index or leaf ref), local */
     int depth = 0;
     double sah = 0.0;
+    /* A tree the GPU builder leaves on the device (nodes and tris above stay
+     * empty): dev_count nodes already packed at the traversal's width with
+     * absolute child refs (the node and triangle bases it was built for),
+     * and its leaf-ordered triangles, tags applied. */
+    DevBuf d_nodes, d_tris;
+    uint32_t dev_count = 0;
+    /* the boxes two levels below the root (at most 64; leaf children of the
+     * root count as one), for the instance world boxes */
+    std::vector<std::array<float, 6>> top_boxes;
+    bool on_device() const { return d_nodes.ptr != nullptr || d_tris.ptr != nullptr; }
 };
 
 /* Host binned-SAH build + collapse to BVH4 (builder 1). */
@@ -135,23 +171,21 @@ void lx_build_blas_host(const float* pos, const uint32_t* idx, uint32_t ntri, in
 /* TLAS over instance world boxes (one instance per leaf). */
 void lx_build_tlas_host(const std::vector<float>& boxes /* 6 per instance */, int width, BvhBuildResult& out);
 /* GPU builder (builder 0; builder 2 uses it for BLASes of at least
- * LX_GPU_BUILD_MIN_TRIS triangles, where the host build would take seconds):
- * linear BVH from Morton codes, binned SAH over clusters at the top, SAH leaf
- * test at the bottom; returns false if the GPU path is unavailable for this
- * input.  Its trees are built faster and traversed slower than the host
- * builder's. */
-#define LX_GPU_BUILD_MIN_TRIS (1u << 22)
-bool lx_build_blas_gpu(lx_context* ctx, const float* pos, const uint32_t* idx, uint32_t ntri, int leaf_size, int bins,
-                       int width, BvhBuildResult& out);
+ * LX_GPU_BUILD_MIN_TRIS triangles): top-down binned SAH on the device over
+ * the Morton-ordered triangles, breadth-first levels while a node is larger
+ * than one thread's share and one thread per subtree below that, with the
+ * host builder's split and leaf rules; then the host builder's collapse, run
+ * level by level on the device.  The tree stays on the device (out.d_nodes,
+ * out.d_tris), its nodes numbered from node_base and its leaves from
+ * tri_base.  `nv` is the mesh's vertex count; with ntag > 0 each triangle
+ * is tagged as the scene's triangle records of a group are (its index within
+ * its mesh, and its instance).  Returns false if the GPU path is
+ * unavailable for this input (the caller then builds on the host). */
+#define LX_GPU_BUILD_MIN_TRIS (1u << 14)
+bool lx_build_blas_gpu(lx_context* ctx, const float* pos, size_t nv, const uint32_t* idx, uint32_t ntri,
+                       int leaf_size, int bins, int width, uint32_t node_base, uint32_t tri_base,
+                       const uint32_t* tag_off, const uint32_t* tag_inst, size_t ntag, BvhBuildResult& out);
 void lx_pad_box(float lo[3], float hi[3]);
-/* SAH cost of a wide tree, in the units of the host builder's cost: every
- * node's box area (the union of its child boxes) plus every leaf's box area
- * times its triangle count, over the root box area */
-double lx_wide_sah(const std::vector<LxWideNode>& nodes, uint32_t root);
-/* Renumber a wide tree's nodes in depth-first pre-order (children in slot
- * order), so that each subtree's nodes are contiguous in memory as the host
- * builder's recursion lays them out; root becomes 0. */
-void lx_wide_dfs_order(std::vector<LxWideNode>& nodes, uint32_t& root);
 
 /* -------------------------------------------------------------- scene */
 struct HostMesh {

@@ -275,13 +275,13 @@ void lx_scene_init(lx_scene* s, const lx_scene_desc* d) {
     for (auto& g : s->insts) s->info.num_instanced_tris += s->meshes[g.mesh].idx.size() / 3;
 }
 
-/* Node k's child boxes in per-axis SoA float4 groups (four children per
- * group), children refs, and meta.x = 1 for TLAS nodes (index >= tlas_first).
- * Empty child slots get zero boxes and LX_CHILD_EMPTY. */
+/* Nodes in[0, n) packed: child boxes in per-axis SoA float4 groups (four
+ * children per group), children refs, meta.x = 1 for TLAS nodes, meta.y =
+ * the child count.  Empty child slots get zero boxes and LX_CHILD_EMPTY. */
 template <class N, int W>
-static std::vector<N> pack_nodes(const std::vector<LxWideNode>& in, uint32_t tlas_first) {
-    std::vector<N> out(in.size());
-    for (size_t i = 0; i < in.size(); i++) {
+static std::vector<N> pack_nodes(const LxWideNode* in, size_t count, bool tlas) {
+    std::vector<N> out(count);
+    for (size_t i = 0; i < count; i++) {
         const LxWideNode& w = in[i];
         N& n = out[i];
         std::memset(&n, 0, sizeof(N));
@@ -303,10 +303,57 @@ static std::vector<N> pack_nodes(const std::vector<LxWideNode>& in, uint32_t tla
             ch[k] = used ? w.child[k] : LX_CHILD_EMPTY;
         }
         int* meta = (int*)&n.meta;
-        meta[0] = i >= tlas_first ? 1 : 0;
+        meta[0] = tlas ? 1 : 0;
         meta[1] = (int)w.n;
     }
     return out;
+}
+
+/* A run of BLAS nodes and triangles in the scene's arrays: built on the host
+ * (held in the host arrays from host_node / host_tri) or left on the device
+ * by the GPU builder (dev >= 0: its result). */
+struct BlasSegment {
+    uint32_t node_first, node_count, tri_first, tri_count;
+    size_t host_node, host_tri;
+    int dev;
+};
+
+/* Writes the BLAS segments, then the TLAS nodes (packed as type N), into
+ * dst_nodes / dst_tris. */
+template <class N, int W>
+static void upload_bvh(lx_context* ctx, const std::vector<BlasSegment>& segs, const std::vector<LxWideNode>& nodes,
+                       const std::vector<float4>& tris, std::vector<BvhBuildResult>& dev, const BvhBuildResult& tl,
+                       uint32_t tlas_first, uint32_t tri_total, DevBuf& dst_nodes, DevBuf& dst_tris,
+                       std::vector<std::vector<N>>& hold) {
+    hipStream_t st = ctx->stream;
+    dst_nodes.alloc(ctx, ((size_t)tlas_first + tl.nodes.size()) * sizeof(N));
+    dst_tris.alloc(ctx, (size_t)tri_total * 48);
+    char* dn = (char*)dst_nodes.ptr;
+    char* dt = (char*)dst_tris.ptr;
+    for (const BlasSegment& g : segs) {
+        char* n_at = dn + (size_t)g.node_first * sizeof(N);
+        char* t_at = dt + (size_t)g.tri_first * 48;
+        if (g.dev >= 0) {
+            const BvhBuildResult& r = dev[(size_t)g.dev];
+            if (g.node_count)
+                LX_HIP(hipMemcpyAsync(n_at, r.d_nodes.ptr, (size_t)g.node_count * sizeof(N), hipMemcpyDeviceToDevice,
+                                      st));
+            LX_HIP(hipMemcpyAsync(t_at, r.d_tris.ptr, (size_t)g.tri_count * 48, hipMemcpyDeviceToDevice, st));
+        } else {
+            if (g.node_count) {
+                hold.push_back(pack_nodes<N, W>(nodes.data() + g.host_node, g.node_count, false));
+                LX_HIP(hipMemcpyAsync(n_at, hold.back().data(), (size_t)g.node_count * sizeof(N),
+                                      hipMemcpyHostToDevice, st));
+            }
+            LX_HIP(hipMemcpyAsync(t_at, tris.data() + 3 * g.host_tri, (size_t)g.tri_count * 48, hipMemcpyHostToDevice,
+                                  st));
+        }
+    }
+    if (!tl.nodes.empty()) {
+        hold.push_back(pack_nodes<N, W>(tl.nodes.data(), tl.nodes.size(), true));
+        LX_HIP(hipMemcpyAsync(dn + (size_t)tlas_first * sizeof(N), hold.back().data(), tl.nodes.size() * sizeof(N),
+                              hipMemcpyHostToDevice, st));
+    }
 }
 
 /* Precompute and upload everything; builds the BVHs. */
@@ -378,66 +425,89 @@ void lx_scene_accel(lx_scene* s, const lx_build_opts* o) {
         }
     }
 
-    std::vector<LxWideNode> nodes;
-    std::vector<float4> tris;
+    std::vector<LxWideNode> nodes; /* host-built BLAS nodes, absolute child refs */
+    std::vector<float4> tris;      /* host-built BLAS triangles */
+    std::vector<BlasSegment> segs;
+    std::vector<BvhBuildResult> dev_blas;
+    uint32_t node_total = 0, tri_total = 0;
     int max_depth = 0;
     double sah = 0.0;
-    /* Builds a BLAS into the shared node / triangle arrays; returns its root
-     * and, in `boxes`, object-space boxes bounding it: the boxes two levels
-     * below the root (at most 64), or the box of all its vertices.  A rotated
-     * instance's world box is the union of these boxes' transformed corners,
-     * much tighter than the transformed box of the whole mesh. */
+    /* Builds a BLAS as the next segment of the scene's node / triangle
+     * arrays; returns its root and, in `boxes`, object-space boxes bounding
+     * it: the boxes two levels below the root (at most 64), or the box of all
+     * its vertices.  A rotated instance's world box is the union of these
+     * boxes' transformed corners, much tighter than the transformed box of
+     * the whole mesh. */
     auto add_blas = [&](const float* pos, size_t nv, const uint32_t* idx, uint32_t nt, const uint32_t* tag_off,
                         const uint32_t* tag_inst, size_t ntag, std::vector<std::array<float, 6>>& boxes) {
+        const uint32_t noff = node_total, toff = tri_total;
+        LX_REQUIRE((size_t)toff + nt < (1u << 28) - 1, LX_ERR_INVALID, "scene too large for the leaf encoding");
         BvhBuildResult r;
         bool ok = false;
         bool gpu = opt.builder == 0 || (opt.builder == 2 && nt >= LX_GPU_BUILD_MIN_TRIS);
-        if (gpu) ok = lx_build_blas_gpu(ctx, pos, idx, nt, opt.leaf_size, opt.sah_bins, opt.bvh_width, r);
-        if (!ok) lx_build_blas_host(pos, idx, nt, opt.leaf_size, opt.sah_bins, opt.bvh_width, r);
-        if (ntag) {
-            /* group-wide triangle index -> (instance, triangle of its mesh) */
-            for (size_t k = 0; k < r.tris.size() / 3; k++) {
-                uint32_t t;
-                std::memcpy(&t, &r.tris[3 * k].w, 4);
-                size_t m = (size_t)(std::upper_bound(tag_off, tag_off + ntag, t) - tag_off) - 1;
-                uint32_t local = t - tag_off[m];
-                std::memcpy(&r.tris[3 * k].w, &local, 4);
-                std::memcpy(&r.tris[3 * k + 1].w, &tag_inst[m], 4);
-            }
-        }
-        uint32_t noff = (uint32_t)nodes.size(), toff = (uint32_t)(tris.size() / 3);
-        LX_REQUIRE((size_t)toff + nt < (1u << 28) - 1, LX_ERR_INVALID, "scene too large for the leaf encoding");
-        for (LxWideNode n : r.nodes) {
-            for (int k = 0; k < 8; k++) {
-                uint32_t v = n.child[k];
-                if (v == LX_CHILD_EMPTY) continue;
-                if (v & LX_LEAF_BIT) v = (v & ~LX_LEAF_FIRST_MASK) | ((v & LX_LEAF_FIRST_MASK) + toff);
-                else v += noff;
-                n.child[k] = v;
-            }
-            nodes.push_back(n);
-        }
-        if (!(r.root & LX_LEAF_BIT) && r.root != LX_CHILD_EMPTY) {
-            const LxWideNode& rn = r.nodes[r.root];
-            for (int k = 0; k < 8; k++) {
-                uint32_t c = rn.child[k];
-                if (c == LX_CHILD_EMPTY) continue;
-                if (c & LX_LEAF_BIT) {
-                    boxes.push_back({rn.lo[k][0], rn.lo[k][1], rn.lo[k][2], rn.hi[k][0], rn.hi[k][1], rn.hi[k][2]});
-                    continue;
+        if (gpu)
+            ok = lx_build_blas_gpu(ctx, pos, nv, idx, nt, opt.leaf_size, opt.sah_bins, opt.bvh_width, noff, toff,
+                                   tag_off, tag_inst, ntag, r);
+        BlasSegment g{noff, 0, toff, nt, nodes.size(), tris.size() / 3, -1};
+        if (ok) {
+            g.node_count = r.dev_count;
+            g.dev = (int)dev_blas.size();
+            boxes = r.top_boxes;
+        } else {
+            r = BvhBuildResult();
+            lx_build_blas_host(pos, idx, nt, opt.leaf_size, opt.sah_bins, opt.bvh_width, r);
+            if (ntag) {
+                /* group-wide triangle index -> (instance, triangle of its mesh) */
+                for (size_t k = 0; k < r.tris.size() / 3; k++) {
+                    uint32_t t;
+                    std::memcpy(&t, &r.tris[3 * k].w, 4);
+                    size_t m = (size_t)(std::upper_bound(tag_off, tag_off + ntag, t) - tag_off) - 1;
+                    uint32_t local = t - tag_off[m];
+                    std::memcpy(&r.tris[3 * k].w, &local, 4);
+                    std::memcpy(&r.tris[3 * k + 1].w, &tag_inst[m], 4);
                 }
-                const LxWideNode& cn = r.nodes[c];
-                for (int q = 0; q < 8; q++)
-                    if (cn.child[q] != LX_CHILD_EMPTY)
-                        boxes.push_back({cn.lo[q][0], cn.lo[q][1], cn.lo[q][2], cn.hi[q][0], cn.hi[q][1], cn.hi[q][2]});
             }
+            g.node_count = (uint32_t)r.nodes.size();
+            LX_REQUIRE((size_t)noff + r.nodes.size() < LX_LEAF_BIT, LX_ERR_INVALID,
+                       "scene too large for the node encoding");
+            for (LxWideNode n : r.nodes) {
+                for (int k = 0; k < 8; k++) {
+                    uint32_t v = n.child[k];
+                    if (v == LX_CHILD_EMPTY) continue;
+                    if (v & LX_LEAF_BIT) v = (v & ~LX_LEAF_FIRST_MASK) | ((v & LX_LEAF_FIRST_MASK) + toff);
+                    else v += noff;
+                    n.child[k] = v;
+                }
+                nodes.push_back(n);
+            }
+            if (!(r.root & LX_LEAF_BIT) && r.root != LX_CHILD_EMPTY) {
+                const LxWideNode& rn = r.nodes[r.root];
+                for (int k = 0; k < 8; k++) {
+                    uint32_t c = rn.child[k];
+                    if (c == LX_CHILD_EMPTY) continue;
+                    if (c & LX_LEAF_BIT) {
+                        boxes.push_back(
+                            {rn.lo[k][0], rn.lo[k][1], rn.lo[k][2], rn.hi[k][0], rn.hi[k][1], rn.hi[k][2]});
+                        continue;
+                    }
+                    const LxWideNode& cn = r.nodes[c];
+                    for (int q = 0; q < 8; q++)
+                        if (cn.child[q] != LX_CHILD_EMPTY)
+                            boxes.push_back(
+                                {cn.lo[q][0], cn.lo[q][1], cn.lo[q][2], cn.hi[q][0], cn.hi[q][1], cn.hi[q][2]});
+                }
+            }
+            tris.insert(tris.end(), r.tris.begin(), r.tris.end());
         }
         uint32_t rr = r.root;
         if (rr & LX_LEAF_BIT) rr = (rr & ~LX_LEAF_FIRST_MASK) | ((rr & LX_LEAF_FIRST_MASK) + toff);
         else rr += noff;
-        tris.insert(tris.end(), r.tris.begin(), r.tris.end());
         max_depth = std::max(max_depth, r.depth);
         sah += r.sah;
+        segs.push_back(g);
+        if (ok) dev_blas.push_back(std::move(r));
+        node_total += g.node_count;
+        tri_total += nt;
         if (boxes.empty()) {
             std::array<float, 6> b = {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX};
             for (size_t v = 0; v < nv; v++)
@@ -550,26 +620,27 @@ void lx_scene_accel(lx_scene* s, const lx_build_opts* o) {
     /* TLAS over the groups */
     BvhBuildResult tl;
     lx_build_tlas_host(wboxes, opt.bvh_width, tl);
-    uint32_t noff = (uint32_t)nodes.size();
-    for (LxWideNode n : tl.nodes) {
+    const uint32_t tlas_first = node_total;
+    LX_REQUIRE((size_t)tlas_first + tl.nodes.size() < LX_LEAF_BIT, LX_ERR_INVALID,
+               "scene too large for the node encoding");
+    for (LxWideNode& n : tl.nodes)
         for (int k = 0; k < 8; k++) {
             uint32_t v = n.child[k];
             if (v == LX_CHILD_EMPTY || (v & LX_LEAF_BIT)) continue;
-            n.child[k] = v + noff;
+            n.child[k] = v + tlas_first;
         }
-        nodes.push_back(n);
-    }
-    uint32_t tlas_root = (tl.root & LX_LEAF_BIT) ? tl.root : tl.root + noff;
+    uint32_t tlas_root = (tl.root & LX_LEAF_BIT) ? tl.root : tl.root + tlas_first;
 
-    /* uploads: nodes packed at the traversal's width */
-    if (opt.bvh_width == 8) {
-        std::vector<LxNode8> packed = pack_nodes<LxNode8, 8>(nodes, noff);
-        s->d_nodes.upload(ctx, packed.data(), packed.size() * sizeof(LxNode8));
-    } else {
-        std::vector<LxNode4> packed = pack_nodes<LxNode4, 4>(nodes, noff);
-        s->d_nodes.upload(ctx, packed.data(), packed.size() * sizeof(LxNode4));
-    }
-    s->d_tris.upload(ctx, tris.data(), tris.size() * sizeof(float4));
+    /* uploads: nodes packed at the traversal's width; BLASes the GPU builder
+     * left on the device are copied there */
+    std::vector<std::vector<LxNode8>> hold8;
+    std::vector<std::vector<LxNode4>> hold4;
+    if (opt.bvh_width == 8)
+        upload_bvh<LxNode8, 8>(ctx, segs, nodes, tris, dev_blas, tl, tlas_first, tri_total, s->d_nodes, s->d_tris,
+                               hold8);
+    else
+        upload_bvh<LxNode4, 4>(ctx, segs, nodes, tris, dev_blas, tl, tlas_first, tri_total, s->d_nodes, s->d_tris,
+                               hold4);
     s->d_inst.upload(ctx, dinst.data(), dinst.size() * sizeof(LxInstance));
     s->d_groups.upload(ctx, dgroups.data(), dgroups.size() * sizeof(LxGroup));
     s->d_idx.upload(ctx, all_idx.data(), all_idx.size() * 4);
@@ -623,7 +694,7 @@ void lx_scene_accel(lx_scene* s, const lx_build_opts* o) {
     s->info.total_emissive_area = s->total_area;
     s->info.env_width = s->has_env ? (int)s->env.w : 0;
     s->info.env_height = s->has_env ? (int)s->env.h : 0;
-    s->info.bvh_nodes = (uint32_t)nodes.size();
+    s->info.bvh_nodes = tlas_first + (uint32_t)tl.nodes.size();
     s->info.bvh_depth = max_depth + tl.depth + 1;
     s->info.build_ms = now_ms() - t0;
     s->info.sah_cost = sah;
