@@ -619,7 +619,7 @@ __device__ __forceinline__ float sah_node_area(const SahNode& d) {
 __device__ __forceinline__ void wide_pad_box(float lo[3], float hi[3]) {
     for (int a = 0; a < 3; a++) {
         float m = fmaxf(fabsf(lo[a]), fabsf(hi[a]));
-        float pad = (m + (hi[a] - lo[a])) * 1.9073486e-6f + 1e-30f;
+        float pad = (m + (hi[a] - lo[a])) * 1.9073486e-6f + 8.6736174e-19f;
         lo[a] = lo[a] - pad;
         hi[a] = hi[a] + pad;
     }
@@ -688,7 +688,9 @@ __global__ void k_wide_emit(const uint32_t* __restrict__ level, uint32_t n, cons
     for (int k = 0; k < W; k++) {
         int32_t kid = kids[8 * (size_t)i + k];
         if (kid < 0) {
-            lox[k] = loy[k] = loz[k] = hix[k] = hiy[k] = hiz[k] = 0.0f;
+            /* an empty slot's box is inverted (lo = +inf, hi = -inf): every slab test misses it */
+            lox[k] = loy[k] = loz[k] = __builtin_inff();
+            hix[k] = hiy[k] = hiz[k] = -__builtin_inff();
             ch[k] = LX_CHILD_EMPTY;
             continue;
         }

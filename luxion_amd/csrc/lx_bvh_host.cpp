@@ -191,10 +191,12 @@ struct Builder {
 
 void lx_pad_box(float lo[3], float hi[3]) {
     /* conservative padding: covers the slab test's rounding and the
-     * triangle test's epsilon-inflated acceptance region */
+     * triangle test's epsilon-inflated acceptance region; the 2^-60 floor
+     * keeps a flat box's slab wider than 2^-100 t for the traversal's
+     * clamped reciprocal of a zero direction component */
     for (int a = 0; a < 3; a++) {
         float m = std::max(std::fabs(lo[a]), std::fabs(hi[a]));
-        float pad = (m + (hi[a] - lo[a])) * 1.9073486e-6f + 1e-30f; /* 2^-19 */
+        float pad = (m + (hi[a] - lo[a])) * 1.9073486e-6f + 8.6736174e-19f; /* 2^-19, 2^-60 */
         lo[a] = lo[a] - pad;
         hi[a] = hi[a] + pad;
     }

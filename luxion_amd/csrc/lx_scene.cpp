@@ -18,6 +18,7 @@
 #include <map>
 #include <chrono>
 #include <cmath>
+#include <limits>
 #include <cfloat>
 
 #include "../../include/luxion_detmath.h"
@@ -277,7 +278,8 @@ void lx_scene_init(lx_scene* s, const lx_scene_desc* d) {
 
 /* Nodes in[0, n) packed: child boxes in per-axis SoA float4 groups (four
  * children per group), children refs, meta.x = 1 for TLAS nodes, meta.y =
- * the child count.  Empty child slots get zero boxes and LX_CHILD_EMPTY. */
+ * the child count.  Empty child slots get inverted boxes (lo = +inf,
+ * hi = -inf, missed by every slab test) and LX_CHILD_EMPTY. */
 template <class N, int W>
 static std::vector<N> pack_nodes(const LxWideNode* in, size_t count, bool tlas) {
     std::vector<N> out(count);
@@ -294,12 +296,13 @@ static std::vector<N> pack_nodes(const LxWideNode* in, size_t count, bool tlas) 
         uint32_t* ch = (uint32_t*)&n.child;
         for (int k = 0; k < W; k++) {
             bool used = k < 8 && w.child[k] != LX_CHILD_EMPTY;
-            lox[k] = used ? w.lo[k][0] : 0.0f;
-            loy[k] = used ? w.lo[k][1] : 0.0f;
-            loz[k] = used ? w.lo[k][2] : 0.0f;
-            hix[k] = used ? w.hi[k][0] : 0.0f;
-            hiy[k] = used ? w.hi[k][1] : 0.0f;
-            hiz[k] = used ? w.hi[k][2] : 0.0f;
+            const float inf = std::numeric_limits<float>::infinity();
+            lox[k] = used ? w.lo[k][0] : inf;
+            loy[k] = used ? w.lo[k][1] : inf;
+            loz[k] = used ? w.lo[k][2] : inf;
+            hix[k] = used ? w.hi[k][0] : -inf;
+            hiy[k] = used ? w.hi[k][1] : -inf;
+            hiz[k] = used ? w.hi[k][2] : -inf;
             ch[k] = used ? w.child[k] : LX_CHILD_EMPTY;
         }
         int* meta = (int*)&n.meta;
@@ -623,6 +626,9 @@ void lx_scene_accel(lx_scene* s, const lx_build_opts* o) {
     const uint32_t tlas_first = node_total;
     LX_REQUIRE((size_t)tlas_first + tl.nodes.size() < LX_LEAF_BIT, LX_ERR_INVALID,
                "scene too large for the node encoding");
+    /* the traversal addresses a node's planes by 32-bit byte offsets from the node array */
+    LX_REQUIRE(((size_t)tlas_first + tl.nodes.size()) * (opt.bvh_width == 8 ? 256u : 128u) <= (size_t)UINT32_MAX,
+               LX_ERR_INVALID, "scene too large: BVH node array above 4 GiB (build at BVH width 4 or split the scene)");
     for (LxWideNode& n : tl.nodes)
         for (int k = 0; k < 8; k++) {
             uint32_t v = n.child[k];

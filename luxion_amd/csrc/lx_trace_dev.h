@@ -11,12 +11,14 @@
  *    16-byte loads;
  *  - a per-lane short stack in LDS (column per lane: conflict-free for a
  *    wave64), spilling to a per-lane global region past its depth;
- *  - robust slab tests: t = (lo - o) * (1/d), far distance inflated by
- *    2^-21 relative, and culling against the best hit only beyond a 2^-16
- *    relative margin, so culling never changes the result of the triangle
- *    test; the closest hit is the lexicographic minimum of
- *    (t, instance, primitive) over all accepted candidates and does not
- *    depend on traversal order or tree shape.
+ *  - robust slab tests: the near and far planes of each axis are picked by
+ *    the sign of the ray direction when they are loaded (per-lane byte
+ *    offsets into the node), t = fma(plane, 1/d, -o/d) with o/d rounded
+ *    outward, the far distance inflated by 2^-21 relative, and culling
+ *    against the best hit only beyond a 2^-16 relative margin, so culling
+ *    never changes the result of the triangle test; the closest hit is the
+ *    lexicographic minimum of (t, instance, primitive) over all accepted
+ *    candidates and does not depend on traversal order or tree shape.
  * The triangle test is the reference's software Moller-Trumbore
  * (src/intersections/mesh_isect.cu:77-137) computed in object space.
  */
@@ -143,42 +145,88 @@ struct LxNodeT<8> {
     typedef LxNode8 type;
 };
 
-/* Slab test of the W children of a BVH node; writes entry distances and
- * returns a W-bit hit mask. */
+/* Slab-test constants of a ray (o, d): ix = 1/d per axis, clamped to
+ * +-2^100 (+-2^60 on an axis where |o| >= 2^26, so that o * ix stays
+ * finite), and o * ix rounded outward by 2^-22 of its magnitude: pn up, pf
+ * down.  Then fma(plane, ix, -pn) <= (plane - o) * ix <= fma(plane, ix, -pf)
+ * up to the fma's own rounding (2^-24 relative, covered by the far
+ * inflation of 2^-21).  A clamped axis (|d| < 2^-100, d = 0 included) tests
+ * as a direction of 2^-100: its slab differs from the exact one only when
+ * the origin lies within 2^-100 t of a face, i.e. inside the box's padding
+ * (at least 2^-60), where no triangle of the box is accepted. */
+struct LxSlab {
+    f3 ix, pn, pf;
+};
+
+LXD float lx_slab_clamp(float ix, float o) {
+    const float lim = lxm_abs(o) < 67108864.0f ? 0x1p100f : 0x1p60f;
+    return __builtin_fminf(__builtin_fmaxf(ix, -lim), lim);
+}
+
+/* iv = 1/d as computed by the caller (+-inf for a zero component) */
+LXD LxSlab lx_slab(f3 o, f3 iv) {
+    LxSlab s;
+    s.ix = mk3(lx_slab_clamp(iv.x, o.x), lx_slab_clamp(iv.y, o.y), lx_slab_clamp(iv.z, o.z));
+    const float px = o.x * s.ix.x, py = o.y * s.ix.y, pz = o.z * s.ix.z;
+    s.pn = mk3(__builtin_fmaf(lxm_abs(px), 0x1p-22f, px), __builtin_fmaf(lxm_abs(py), 0x1p-22f, py),
+               __builtin_fmaf(lxm_abs(pz), 0x1p-22f, pz));
+    s.pf = mk3(__builtin_fmaf(-lxm_abs(px), 0x1p-22f, px), __builtin_fmaf(-lxm_abs(py), 0x1p-22f, py),
+               __builtin_fmaf(-lxm_abs(pz), 0x1p-22f, pz));
+    return s;
+}
+
+/* Slab test of the W children of node `node`; writes entry distances and
+ * child refs and returns a W-bit hit mask.  A node holds, per axis, the
+ * children's lo planes then their hi planes (W/4 float4 each); the near
+ * planes of an axis are its lo planes when the direction is positive, so
+ * each lane loads its near and far planes at byte offsets picked by the
+ * signs of its ray.  Empty slots carry inverted boxes (lo = +inf,
+ * hi = -inf), which fail the test whatever the direction. */
 template <int W>
-LXD uint32_t lx_boxw(const typename LxNodeT<W>::type* __restrict__ nd, f3 o, f3 ix, float tcut, float* tn,
-                     uint32_t* cs) {
-    float lxs[W], hxs[W], lys[W], hys[W], lzs[W], hzs[W];
+LXD uint32_t lx_boxw(const void* nodes, uint32_t node, const LxSlab& sl, float tcut, float* tn, uint32_t* cs) {
+    constexpr uint32_t G = W / 4;   /* float4 groups per plane set */
+    constexpr uint32_t HB = 16 * G; /* bytes from an axis's lo planes to its hi planes */
+    const LX_GLOBAL char* nb = (const LX_GLOBAL char*)lx_gptr((const char*)nodes);
+    const uint32_t base = node * (uint32_t)sizeof(typename LxNodeT<W>::type);
+    const uint32_t nx = base | ((lxm_f2u(sl.ix.x) >> 31) * HB);
+    const uint32_t ny = base | ((lxm_f2u(sl.ix.y) >> 31) * HB);
+    const uint32_t nz = base | ((lxm_f2u(sl.ix.z) >> 31) * HB);
+    float nxs[W], fxs[W], nys[W], fys[W], nzs[W], fzs[W];
 #pragma unroll
-    for (int g = 0; g < W / 4; g++) {
-        const LX_GLOBAL lx_v4f* L = (const LX_GLOBAL lx_v4f*)lx_gptr((const float4*)&nd->lox);
-        lx_v4f a = L[0 * (W / 4) + g], b = L[1 * (W / 4) + g], c = L[2 * (W / 4) + g];
-        lx_v4f d = L[3 * (W / 4) + g], e = L[4 * (W / 4) + g], f = L[5 * (W / 4) + g];
-        lx_v4i ch = ((const LX_GLOBAL lx_v4i*)lx_gptr((const int4*)&nd->child))[g];
-        lxs[4 * g + 0] = a.x, lxs[4 * g + 1] = a.y, lxs[4 * g + 2] = a.z, lxs[4 * g + 3] = a.w;
-        hxs[4 * g + 0] = b.x, hxs[4 * g + 1] = b.y, hxs[4 * g + 2] = b.z, hxs[4 * g + 3] = b.w;
-        lys[4 * g + 0] = c.x, lys[4 * g + 1] = c.y, lys[4 * g + 2] = c.z, lys[4 * g + 3] = c.w;
-        hys[4 * g + 0] = d.x, hys[4 * g + 1] = d.y, hys[4 * g + 2] = d.z, hys[4 * g + 3] = d.w;
-        lzs[4 * g + 0] = e.x, lzs[4 * g + 1] = e.y, lzs[4 * g + 2] = e.z, lzs[4 * g + 3] = e.w;
-        hzs[4 * g + 0] = f.x, hzs[4 * g + 1] = f.y, hzs[4 * g + 2] = f.z, hzs[4 * g + 3] = f.w;
+    for (int g = 0; g < (int)G; g++) {
+        lx_v4f a = *(const LX_GLOBAL lx_v4f*)(nb + nx + 16 * g);
+        lx_v4f b = *(const LX_GLOBAL lx_v4f*)(nb + (nx ^ HB) + 16 * g);
+        lx_v4f c = *(const LX_GLOBAL lx_v4f*)(nb + ny + 2 * HB + 16 * g);
+        lx_v4f d = *(const LX_GLOBAL lx_v4f*)(nb + (ny ^ HB) + 2 * HB + 16 * g);
+        lx_v4f e = *(const LX_GLOBAL lx_v4f*)(nb + nz + 4 * HB + 16 * g);
+        lx_v4f f = *(const LX_GLOBAL lx_v4f*)(nb + (nz ^ HB) + 4 * HB + 16 * g);
+        lx_v4i ch = *(const LX_GLOBAL lx_v4i*)(nb + base + 6 * HB + 16 * g);
+        nxs[4 * g + 0] = a.x, nxs[4 * g + 1] = a.y, nxs[4 * g + 2] = a.z, nxs[4 * g + 3] = a.w;
+        fxs[4 * g + 0] = b.x, fxs[4 * g + 1] = b.y, fxs[4 * g + 2] = b.z, fxs[4 * g + 3] = b.w;
+        nys[4 * g + 0] = c.x, nys[4 * g + 1] = c.y, nys[4 * g + 2] = c.z, nys[4 * g + 3] = c.w;
+        fys[4 * g + 0] = d.x, fys[4 * g + 1] = d.y, fys[4 * g + 2] = d.z, fys[4 * g + 3] = d.w;
+        nzs[4 * g + 0] = e.x, nzs[4 * g + 1] = e.y, nzs[4 * g + 2] = e.z, nzs[4 * g + 3] = e.w;
+        fzs[4 * g + 0] = f.x, fzs[4 * g + 1] = f.y, fzs[4 * g + 2] = f.z, fzs[4 * g + 3] = f.w;
         cs[4 * g + 0] = (uint32_t)ch.x, cs[4 * g + 1] = (uint32_t)ch.y;
         cs[4 * g + 2] = (uint32_t)ch.z, cs[4 * g + 3] = (uint32_t)ch.w;
     }
     uint32_t mask = 0;
 #pragma unroll
     for (int k = 0; k < W; k++) {
-        float ax = (lxs[k] - o.x) * ix.x, bx = (hxs[k] - o.x) * ix.x;
-        float ay = (lys[k] - o.y) * ix.y, by = (hys[k] - o.y) * ix.y;
-        float az = (lzs[k] - o.z) * ix.z, bz = (hzs[k] - o.z) * ix.z;
-        float n0 = __builtin_fminf(ax, bx), f0 = __builtin_fmaxf(ax, bx);
-        float n1 = __builtin_fminf(ay, by), f1 = __builtin_fmaxf(ay, by);
-        float n2 = __builtin_fminf(az, bz), f2v = __builtin_fmaxf(az, bz);
-        float tnear = __builtin_fmaxf(__builtin_fmaxf(n0, n1), __builtin_fmaxf(n2, 0.0f));
-        float tfar = __builtin_fminf(__builtin_fminf(f0, f1), __builtin_fminf(f2v, tcut));
-        bool h = (tnear <= tfar * LX_FAR_INFLATE) && cs[k] != LX_CHILD_EMPTY;
+        float ax = __builtin_fmaf(nxs[k], sl.ix.x, -sl.pn.x), bx = __builtin_fmaf(fxs[k], sl.ix.x, -sl.pf.x);
+        float ay = __builtin_fmaf(nys[k], sl.ix.y, -sl.pn.y), by = __builtin_fmaf(fys[k], sl.ix.y, -sl.pf.y);
+        float az = __builtin_fmaf(nzs[k], sl.ix.z, -sl.pn.z), bz = __builtin_fmaf(fzs[k], sl.ix.z, -sl.pf.z);
+        float tnear = __builtin_fmaxf(__builtin_fmaxf(ax, ay), __builtin_fmaxf(az, 0.0f));
+        float tfar = __builtin_fminf(__builtin_fminf(bx, by), __builtin_fminf(bz, tcut));
+        bool h = tnear <= tfar * LX_FAR_INFLATE;
         tn[k] = tnear;
         mask |= h ? (1u << k) : 0u;
     }
+    /* the child refs are used only when a child is hit; consuming them here
+     * keeps their load in flight with the planes' instead of issuing it
+     * after the test */
+#pragma unroll
+    for (int k = 0; k < W; k++) __asm__ volatile("" ::"v"(cs[k]));
     return mask;
 }
 
@@ -268,13 +316,13 @@ struct LxWork {
  * restores the world-space ray and is consumed in the same step, so an
  * iteration of the traversal loops never spends itself on a sentinel.
  * Returns false when the stack is empty (the ray is done). */
-LXD bool lx_pop_next(LxStack& st, uint32_t& node, bool& in_blas, f3& o, f3& d, f3& ix, f3 wo, f3 wd, f3 iwd) {
+LXD bool lx_pop_next(LxStack& st, uint32_t& node, bool& in_blas, f3& o, f3& d, LxSlab& sl, f3 wo, f3 wd, f3 iwd) {
     if (st.sp == 0) return false;
     node = st.pop();
     if (node == LX_SENTINEL) {
         o = wo;
         d = wd;
-        ix = iwd;
+        sl = lx_slab(wo, iwd);
         in_blas = false;
         if (st.sp == 0) return false;
         node = st.pop();
@@ -289,7 +337,7 @@ LXD bool lx_pop_next(LxStack& st, uint32_t& node, bool& in_blas, f3& o, f3& d, f
  * stack is full (the group is skipped and flagged). */
 template <bool CNT>
 LXD bool lx_enter_instance(const LxDevScene& S, uint32_t& node, uint32_t& cur_inst, uint32_t& inst_flags,
-                           bool& in_blas, f3& o, f3& d, f3& ix, f3 wo, f3 wd, uint32_t alpha_mask, LxStack& st,
+                           bool& in_blas, f3& o, f3& d, LxSlab& sl, f3 wo, f3 wd, uint32_t alpha_mask, LxStack& st,
                            uint32_t* overflow, LxWork* work) {
     if (st.sp >= LX_STACK + LX_SPILL) {
         atomicOr(overflow, 1u);
@@ -298,7 +346,7 @@ LXD bool lx_enter_instance(const LxDevScene& S, uint32_t& node, uint32_t& cur_in
     const LX_GLOBAL LxGroup& G = lx_gptr(S.groups)[node & LX_LEAF_FIRST_MASK];
     o = xf_point((const float*)G.inv, wo);
     d = xf_vector((const float*)G.inv, wd);
-    ix = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    sl = lx_slab(o, mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z));
     inst_flags = G.flags & alpha_mask;
     cur_inst = G.inst;
     in_blas = true;
@@ -349,10 +397,9 @@ LXD LxHitD lx_trace(const LxDevScene& S, f3 wo, f3 wd, float tmax, uint32_t key,
     best.inst = 0xffffffffu;
     best.prim = 0xffffffffu;
     best.u = best.v = 0.0f;
-    typedef typename LxNodeT<W>::type Node;
-    const Node* __restrict__ nodes = (const Node*)S.nodes;
     f3 iwd = mk3(1.0f / wd.x, 1.0f / wd.y, 1.0f / wd.z);
-    f3 o = wo, d = wd, ix = iwd;
+    f3 o = wo, d = wd;
+    LxSlab sl = lx_slab(wo, iwd);
     bool in_blas = false;
     uint32_t cur_inst = 0;
     uint32_t inst_flags = 0;
@@ -361,9 +408,9 @@ LXD LxHitD lx_trace(const LxDevScene& S, f3 wo, f3 wd, float tmax, uint32_t key,
     while (true) {
         bool more = true;
         if ((node & LX_LEAF_BIT) && !in_blas) {
-            if (!lx_enter_instance<CNT>(S, node, cur_inst, inst_flags, in_blas, o, d, ix, wo, wd, alpha_mask, st,
+            if (!lx_enter_instance<CNT>(S, node, cur_inst, inst_flags, in_blas, o, d, sl, wo, wd, alpha_mask, st,
                                         overflow, work))
-                more = lx_pop_next(st, node, in_blas, o, d, ix, wo, wd, iwd);
+                more = lx_pop_next(st, node, in_blas, o, d, sl, wo, wd, iwd);
         }
         if (!more) break;
         if (node & LX_LEAF_BIT) {
@@ -371,15 +418,15 @@ LXD LxHitD lx_trace(const LxDevScene& S, f3 wo, f3 wd, float tmax, uint32_t key,
             if (!in_blas) continue;
             if (lx_leaf<ANY, CNT>(S, node, cur_inst, inst_flags, o, d, wd, tmax, key, iteration, best, work))
                 return best;
-            if (!lx_pop_next(st, node, in_blas, o, d, ix, wo, wd, iwd)) break;
+            if (!lx_pop_next(st, node, in_blas, o, d, sl, wo, wd, iwd)) break;
         } else {
             float tcut = (best.inst == 0xffffffffu ? tmax : best.t) * LX_CULL_MARGIN;
             float tn[W];
             uint32_t cs[W];
-            uint32_t m = lx_boxw<W>(nodes + node, o, ix, tcut, tn, cs);
+            uint32_t m = lx_boxw<W>(S.nodes, node, sl, tcut, tn, cs);
             if (CNT) work->nodes++;
             if (m == 0u) {
-                if (!lx_pop_next(st, node, in_blas, o, d, ix, wo, wd, iwd)) break;
+                if (!lx_pop_next(st, node, in_blas, o, d, sl, wo, wd, iwd)) break;
             } else {
                 node = lx_order_children<W>(tn, cs, m, st, overflow);
             }
@@ -400,12 +447,11 @@ LXD LxHitD lx_trace_pair(const LxDevScene& S, bool has0, f3 o0, f3 d0, float tma
                          float tmax1, uint32_t key, uint32_t iteration, uint32_t alpha_mask, LxStack& st,
                          uint32_t* overflow, bool& occluded0, LxWork* wclosest, LxWork* wany) {
     LxHitD best;
-    typedef typename LxNodeT<W>::type Node;
-    const Node* __restrict__ nodes = (const Node*)S.nodes;
     bool any = has0;
     f3 wo = any ? o0 : o1, wd = any ? d0 : d1;
     float tmax = any ? tmax0 : tmax1;
-    f3 iwd, o, d, ix;
+    f3 iwd, o, d;
+    LxSlab sl;
     bool in_blas;
     uint32_t cur_inst = 0, inst_flags = 0, node;
     occluded0 = false;
@@ -418,7 +464,7 @@ LXD LxHitD lx_trace_pair(const LxDevScene& S, bool has0, f3 o0, f3 d0, float tma
         iwd = mk3(1.0f / wd.x, 1.0f / wd.y, 1.0f / wd.z);
         o = wo;
         d = wd;
-        ix = iwd;
+        sl = lx_slab(wo, iwd);
         in_blas = false;
         st.sp = 0;
         node = S.tlas_root;
@@ -428,9 +474,9 @@ LXD LxHitD lx_trace_pair(const LxDevScene& S, bool has0, f3 o0, f3 d0, float tma
         LxWork* work = any ? wany : wclosest;
         bool more = true, found_any = false;
         if ((node & LX_LEAF_BIT) && !in_blas) {
-            if (!lx_enter_instance<CNT>(S, node, cur_inst, inst_flags, in_blas, o, d, ix, wo, wd, alpha_mask, st,
+            if (!lx_enter_instance<CNT>(S, node, cur_inst, inst_flags, in_blas, o, d, sl, wo, wd, alpha_mask, st,
                                         overflow, work))
-                more = lx_pop_next(st, node, in_blas, o, d, ix, wo, wd, iwd);
+                more = lx_pop_next(st, node, in_blas, o, d, sl, wo, wd, iwd);
         }
         if (more) {
             if (node & LX_LEAF_BIT) {
@@ -440,14 +486,14 @@ LXD LxHitD lx_trace_pair(const LxDevScene& S, bool has0, f3 o0, f3 d0, float tma
                                                      best, work)
                                 : lx_leaf<false, CNT>(S, node, cur_inst, inst_flags, o, d, wd, tmax, key, iteration,
                                                       best, work);
-                if (in_blas && !found_any) more = lx_pop_next(st, node, in_blas, o, d, ix, wo, wd, iwd);
+                if (in_blas && !found_any) more = lx_pop_next(st, node, in_blas, o, d, sl, wo, wd, iwd);
             } else {
                 float tcut = (best.inst == 0xffffffffu ? tmax : best.t) * LX_CULL_MARGIN;
                 float tn[W];
                 uint32_t cs[W];
-                uint32_t m = lx_boxw<W>(nodes + node, o, ix, tcut, tn, cs);
+                uint32_t m = lx_boxw<W>(S.nodes, node, sl, tcut, tn, cs);
                 if (CNT) work->nodes++;
-                if (m == 0u) more = lx_pop_next(st, node, in_blas, o, d, ix, wo, wd, iwd);
+                if (m == 0u) more = lx_pop_next(st, node, in_blas, o, d, sl, wo, wd, iwd);
                 else node = lx_order_children<W>(tn, cs, m, st, overflow);
             }
         }
