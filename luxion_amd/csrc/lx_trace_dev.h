@@ -163,7 +163,15 @@ LXD float lx_slab_clamp(float ix, float o) {
     return __builtin_fminf(__builtin_fmaxf(ix, -lim), lim);
 }
 
-/* iv = 1/d as computed by the caller (+-inf for a zero component) */
+/* Reciprocal direction for the slab test: the hardware reciprocal (1 ulp;
+ * +-inf for a zero component).  Its error scales an axis's entry and exit
+ * distances alike and differs between axes by at most 2^-22 relative, which
+ * the far inflation of 2^-21 covers with the fma's rounding. */
+LXD f3 lx_rcp3(f3 d) {
+    return mk3(__builtin_amdgcn_rcpf(d.x), __builtin_amdgcn_rcpf(d.y), __builtin_amdgcn_rcpf(d.z));
+}
+
+/* iv = lx_rcp3(d) */
 LXD LxSlab lx_slab(f3 o, f3 iv) {
     LxSlab s;
     s.ix = mk3(lx_slab_clamp(iv.x, o.x), lx_slab_clamp(iv.y, o.y), lx_slab_clamp(iv.z, o.z));
@@ -346,7 +354,7 @@ LXD bool lx_enter_instance(const LxDevScene& S, uint32_t& node, uint32_t& cur_in
     const LX_GLOBAL LxGroup& G = lx_gptr(S.groups)[node & LX_LEAF_FIRST_MASK];
     o = xf_point((const float*)G.inv, wo);
     d = xf_vector((const float*)G.inv, wd);
-    sl = lx_slab(o, mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z));
+    sl = lx_slab(o, lx_rcp3(d));
     inst_flags = G.flags & alpha_mask;
     cur_inst = G.inst;
     in_blas = true;
@@ -356,32 +364,45 @@ LXD bool lx_enter_instance(const LxDevScene& S, uint32_t& node, uint32_t& cur_in
     return true;
 }
 
-/* Triangle leaf of the current group; returns true if an any-hit ray found
- * its hit. */
-template <bool ANY, bool CNT>
-LXD bool lx_leaf(const LxDevScene& S, uint32_t node, uint32_t cur_inst, uint32_t inst_flags, f3 o, f3 d, f3 wd,
-                 float tmax, uint32_t key, uint32_t iteration, LxHitD& best, LxWork* work) {
-    uint32_t first = node & LX_LEAF_FIRST_MASK;
-    uint32_t cnt = ((node >> 28) & 7u) + 1u;
+/* One triangle of a leaf: the Moller-Trumbore test, for closest-hit rays
+ * the lexicographic order against the best hit so far, and the alpha
+ * any-hit program; returns true if the triangle is accepted (and recorded
+ * in best). */
+LXD bool lx_leaf_tri(const LxDevScene& S, bool any, const float4& A, const float4& E1, const float4& E2,
+                     uint32_t cur_inst, uint32_t inst_flags, f3 o, f3 d, f3 wd, float tmax, uint32_t key,
+                     uint32_t iteration, LxHitD& best) {
+    float t, u, v;
+    if (!lx_mt(o, d, A, E1, E2, tmax, t, u, v)) return false;
+    uint32_t prim = lxm_f2u(A.w);
+    uint32_t inst = cur_inst == LX_GROUP_TAGGED ? lxm_f2u(E1.w) : cur_inst;
+    if (!any && !lx_better(t, inst, prim, best)) return false;
+    if (inst_flags & LX_INST_ALPHA) {
+        const LX_GLOBAL LxInstance& I = lx_gptr(S.inst)[inst];
+        if ((I.flags & LX_INST_ALPHA) && !lx_any_hit_accept(S, I, inst, prim, u, v, wd, key, iteration)) return false;
+    }
+    best.t = t;
+    best.inst = inst;
+    best.prim = prim;
+    best.u = u;
+    best.v = v;
+    return true;
+}
+
+/* Triangle leaf of the current group, for an any-hit (any) or closest-hit
+ * ray (one copy of the code serves both, so a wave whose lanes trace both
+ * kinds runs it once); returns true if an any-hit ray found its hit. */
+template <bool CNT>
+LXD bool lx_leaf(const LxDevScene& S, bool any, uint32_t node, uint32_t cur_inst, uint32_t inst_flags, f3 o, f3 d,
+                 f3 wd, float tmax, uint32_t key, uint32_t iteration, LxHitD& best, LxWork* work) {
+    const uint32_t first = node & LX_LEAF_FIRST_MASK;
+    const uint32_t cnt = ((node >> 28) & 7u) + 1u;
+    const LX_GLOBAL lx_v4f* tb = (const LX_GLOBAL lx_v4f*)lx_gptr(S.tris) + 3u * (size_t)first;
     for (uint32_t k = 0; k < cnt; k++) {
-        const LX_GLOBAL lx_v4f* tp = (const LX_GLOBAL lx_v4f*)lx_gptr(S.tris) + 3u * (size_t)(first + k);
-        float4 A = lx_ld4(tp), E1 = lx_ld4(tp + 1), E2 = lx_ld4(tp + 2);
-        float t, u, v;
+        const LX_GLOBAL lx_v4f* p0 = tb + 3u * k;
+        float4 A0 = lx_ld4(p0), B0 = lx_ld4(p0 + 1), C0 = lx_ld4(p0 + 2);
         if (CNT) work->tris++;
-        if (!lx_mt(o, d, A, E1, E2, tmax, t, u, v)) continue;
-        uint32_t prim = lxm_f2u(A.w);
-        uint32_t inst = cur_inst == LX_GROUP_TAGGED ? lxm_f2u(E1.w) : cur_inst;
-        if (!ANY && !lx_better(t, inst, prim, best)) continue;
-        if (inst_flags & LX_INST_ALPHA) {
-            const LX_GLOBAL LxInstance& I = lx_gptr(S.inst)[inst];
-            if ((I.flags & LX_INST_ALPHA) && !lx_any_hit_accept(S, I, inst, prim, u, v, wd, key, iteration)) continue;
-        }
-        best.t = t;
-        best.inst = inst;
-        best.prim = prim;
-        best.u = u;
-        best.v = v;
-        if (ANY) return true;
+        if (lx_leaf_tri(S, any, A0, B0, C0, cur_inst, inst_flags, o, d, wd, tmax, key, iteration, best) && any)
+            return true;
     }
     return false;
 }
@@ -397,7 +418,7 @@ LXD LxHitD lx_trace(const LxDevScene& S, f3 wo, f3 wd, float tmax, uint32_t key,
     best.inst = 0xffffffffu;
     best.prim = 0xffffffffu;
     best.u = best.v = 0.0f;
-    f3 iwd = mk3(1.0f / wd.x, 1.0f / wd.y, 1.0f / wd.z);
+    f3 iwd = lx_rcp3(wd);
     f3 o = wo, d = wd;
     LxSlab sl = lx_slab(wo, iwd);
     bool in_blas = false;
@@ -406,31 +427,31 @@ LXD LxHitD lx_trace(const LxDevScene& S, f3 wo, f3 wd, float tmax, uint32_t key,
     st.sp = 0;
     uint32_t node = S.tlas_root;
     while (true) {
-        bool more = true;
-        if ((node & LX_LEAF_BIT) && !in_blas) {
-            if (!lx_enter_instance<CNT>(S, node, cur_inst, inst_flags, in_blas, o, d, sl, wo, wd, alpha_mask, st,
-                                        overflow, work))
-                more = lx_pop_next(st, node, in_blas, o, d, sl, wo, wd, iwd);
-        }
-        if (!more) break;
-        if (node & LX_LEAF_BIT) {
-            /* a TLAS leaf here (popped after a skipped instance) waits for the next iteration */
-            if (!in_blas) continue;
-            if (lx_leaf<ANY, CNT>(S, node, cur_inst, inst_flags, o, d, wd, tmax, key, iteration, best, work))
-                return best;
-            if (!lx_pop_next(st, node, in_blas, o, d, sl, wo, wd, iwd)) break;
-        } else {
-            float tcut = (best.inst == 0xffffffffu ? tmax : best.t) * LX_CULL_MARGIN;
-            float tn[W];
-            uint32_t cs[W];
-            uint32_t m = lx_boxw<W>(S.nodes, node, sl, tcut, tn, cs);
-            if (CNT) work->nodes++;
-            if (m == 0u) {
-                if (!lx_pop_next(st, node, in_blas, o, d, sl, wo, wd, iwd)) break;
+        /* one step: (an instance entry, then) one triangle leaf or one node
+         * test; every step that ends its subtree pops in one place */
+        bool pop = false;
+        if ((node & LX_LEAF_BIT) && !in_blas)
+            pop = !lx_enter_instance<CNT>(S, node, cur_inst, inst_flags, in_blas, o, d, sl, wo, wd, alpha_mask, st,
+                                          overflow, work);
+        if (!pop) {
+            if (node & LX_LEAF_BIT) {
+                /* a TLAS leaf here (popped after a skipped instance) waits for the next step */
+                if (in_blas) {
+                    if (lx_leaf<CNT>(S, ANY, node, cur_inst, inst_flags, o, d, wd, tmax, key, iteration, best, work))
+                        return best;
+                    pop = true;
+                }
             } else {
-                node = lx_order_children<W>(tn, cs, m, st, overflow);
+                float tcut = (best.inst == 0xffffffffu ? tmax : best.t) * LX_CULL_MARGIN;
+                float tn[W];
+                uint32_t cs[W];
+                uint32_t m = lx_boxw<W>(S.nodes, node, sl, tcut, tn, cs);
+                if (CNT) work->nodes++;
+                if (m == 0u) pop = true;
+                else node = lx_order_children<W>(tn, cs, m, st, overflow);
             }
         }
+        if (pop && !lx_pop_next(st, node, in_blas, o, d, sl, wo, wd, iwd)) break;
     }
     return best;
 }
@@ -461,7 +482,7 @@ LXD LxHitD lx_trace_pair(const LxDevScene& S, bool has0, f3 o0, f3 d0, float tma
         best.inst = 0xffffffffu;
         best.prim = 0xffffffffu;
         best.u = best.v = 0.0f;
-        iwd = mk3(1.0f / wd.x, 1.0f / wd.y, 1.0f / wd.z);
+        iwd = lx_rcp3(wd);
         o = wo;
         d = wd;
         sl = lx_slab(wo, iwd);
@@ -472,31 +493,30 @@ LXD LxHitD lx_trace_pair(const LxDevScene& S, bool has0, f3 o0, f3 d0, float tma
     begin();
     while (true) {
         LxWork* work = any ? wany : wclosest;
-        bool more = true, found_any = false;
-        if ((node & LX_LEAF_BIT) && !in_blas) {
-            if (!lx_enter_instance<CNT>(S, node, cur_inst, inst_flags, in_blas, o, d, sl, wo, wd, alpha_mask, st,
-                                        overflow, work))
-                more = lx_pop_next(st, node, in_blas, o, d, sl, wo, wd, iwd);
-        }
-        if (more) {
+        bool pop = false, found_any = false;
+        if ((node & LX_LEAF_BIT) && !in_blas)
+            pop = !lx_enter_instance<CNT>(S, node, cur_inst, inst_flags, in_blas, o, d, sl, wo, wd, alpha_mask, st,
+                                          overflow, work);
+        if (!pop) {
             if (node & LX_LEAF_BIT) {
-                /* a TLAS leaf here (popped after a skipped instance) waits for the next iteration */
-                if (in_blas)
-                found_any = any ? lx_leaf<true, CNT>(S, node, cur_inst, inst_flags, o, d, wd, tmax, key, iteration,
-                                                     best, work)
-                                : lx_leaf<false, CNT>(S, node, cur_inst, inst_flags, o, d, wd, tmax, key, iteration,
-                                                      best, work);
-                if (in_blas && !found_any) more = lx_pop_next(st, node, in_blas, o, d, sl, wo, wd, iwd);
+                /* a TLAS leaf here (popped after a skipped instance) waits for the next step */
+                if (in_blas) {
+                    found_any = lx_leaf<CNT>(S, any, node, cur_inst, inst_flags, o, d, wd, tmax, key, iteration, best,
+                                             work);
+                    pop = !found_any;
+                }
             } else {
                 float tcut = (best.inst == 0xffffffffu ? tmax : best.t) * LX_CULL_MARGIN;
                 float tn[W];
                 uint32_t cs[W];
                 uint32_t m = lx_boxw<W>(S.nodes, node, sl, tcut, tn, cs);
                 if (CNT) work->nodes++;
-                if (m == 0u) more = lx_pop_next(st, node, in_blas, o, d, sl, wo, wd, iwd);
+                if (m == 0u) pop = true;
                 else node = lx_order_children<W>(tn, cs, m, st, overflow);
             }
         }
+        bool more = true;
+        if (pop) more = lx_pop_next(st, node, in_blas, o, d, sl, wo, wd, iwd);
         if (found_any || !more) {
             if (!any) break;
             /* first ray finished: record it and start the second */
