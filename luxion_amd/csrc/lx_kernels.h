@@ -61,7 +61,12 @@ hipError_t lx_launch_finish(const LxJob* jobs, const LxPathBuf& P, LxCounters* C
                             float* accum, uint32_t threads, int block, int occ, int width, int chunk,
                             const LxDevScene* uscene /* the scene of every job, or NULL */,
                             bool single_job /* njobs == 1 */, hipStream_t s, bool count = false,
-                            const uint32_t* order = nullptr);
+                            const uint32_t* order = nullptr,
+                            /* non-NULL (bounce0 0, no order): the kernel makes the
+                             * camera paths of the scheduled jobs itself, as the
+                             * camera kernel would with this tile and band, instead
+                             * of reading them from P */
+                            const LxSched* cam_sched = nullptr, uint32_t tile = 1, uint32_t band = 0);
 hipError_t lx_launch_trace_rays(const LxDevScene* S, const lx_ray* rays, uint32_t n, int any, int alpha,
                                 uint32_t iteration, lx_hit* hits, uint8_t* occ, uint32_t* spill, uint32_t* overflow,
                                 uint32_t grid, int width, hipStream_t s);

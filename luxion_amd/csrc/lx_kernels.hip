@@ -124,7 +124,36 @@ LXD uint32_t lx_tile_pixel(uint32_t q, uint32_t W, uint32_t H, uint32_t tile, ui
     return (Ht + r / W) * W + r % W;
 }
 
-/* generateRayFromCamera, src/kernels/generate_ray_from_camera.cu:8-39 */
+/* generateRayFromCamera, src/kernels/generate_ray_from_camera.cu:8-39: the
+ * pixel of job J's q-th camera path (returned) and its ray direction. */
+LXD uint32_t lx_camera_ray(const LxJob& J, uint32_t q, uint32_t tile, uint32_t band, f3& dir) {
+    int W = J.cam.width;
+    uint32_t index = lx_tile_pixel(q, (uint32_t)W, (uint32_t)J.cam.height, tile, band);
+    int x = (int)(index % (uint32_t)W), y = (int)(index / (uint32_t)W);
+    LxRng rng = lx_make_rng(J.iter, (int)index, J.depth, J.seed);
+    float x1 = rng.u01() - 0.5f;
+    float x2 = rng.u01() - 0.5f;
+    float pX = (float(x) + x1 + 0.5f) - (float)J.cam.width * 0.5f;
+    float pY = (float(y) + x2 + 0.5f) - (float)J.cam.height * 0.5f;
+    f3 view = mk3(J.cam.view[0], J.cam.view[1], J.cam.view[2]);
+    f3 right = mk3(J.cam.right[0], J.cam.right[1], J.cam.right[2]);
+    f3 up = mk3(J.cam.up[0], J.cam.up[1], J.cam.up[2]);
+    dir = normalize((view - ((right * J.cam.pixel_length[0]) * pX)) - ((up * J.cam.pixel_length[1]) * pY));
+    return index;
+}
+
+/* The scheduled job whose path range holds slot i: binary search over the
+ * active jobs' first slots (off[0] = 0, increasing). */
+LXD uint32_t lx_sched_job(const uint32_t* off, const uint32_t* idx, uint32_t nact, uint32_t i) {
+    uint32_t lo = 0, hi = nact - 1;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    return idx[lo];
+}
+
 __global__ __launch_bounds__(256) void k_camera(const LxJob* __restrict__ jobs, const LxSched* __restrict__ sched,
                                                 const LxCounters* __restrict__ C, LxPathBuf P, uint32_t tile,
                                                 uint32_t band) {
@@ -138,27 +167,10 @@ __global__ __launch_bounds__(256) void k_camera(const LxJob* __restrict__ jobs, 
     __syncthreads();
     uint32_t total = C->count[0];
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
-        /* binary search over the active jobs' offsets */
-        uint32_t lo = 0, hi = nact - 1;
-        while (lo < hi) {
-            uint32_t mid = (lo + hi + 1) >> 1;
-            if (s_off[mid] <= i) lo = mid;
-            else hi = mid - 1;
-        }
-        uint32_t j = s_idx[lo];
+        uint32_t j = lx_sched_job(s_off, s_idx, nact, i);
         const LxJob& J = jobs[j];
-        int W = J.cam.width;
-        uint32_t index = lx_tile_pixel(i - J.path_off, (uint32_t)W, (uint32_t)J.cam.height, tile, band);
-        int x = (int)(index % (uint32_t)W), y = (int)(index / (uint32_t)W);
-        LxRng rng = lx_make_rng(J.iter, (int)index, J.depth, J.seed);
-        float x1 = rng.u01() - 0.5f;
-        float x2 = rng.u01() - 0.5f;
-        float pX = (float(x) + x1 + 0.5f) - (float)J.cam.width * 0.5f;
-        float pY = (float(y) + x2 + 0.5f) - (float)J.cam.height * 0.5f;
-        f3 view = mk3(J.cam.view[0], J.cam.view[1], J.cam.view[2]);
-        f3 right = mk3(J.cam.right[0], J.cam.right[1], J.cam.right[2]);
-        f3 up = mk3(J.cam.up[0], J.cam.up[1], J.cam.up[2]);
-        f3 dir = normalize((view - ((right * J.cam.pixel_length[0]) * pX)) - ((up * J.cam.pixel_length[1]) * pY));
+        f3 dir;
+        uint32_t index = lx_camera_ray(J, i - J.path_off, tile, band, dir);
         P.ox[i] = J.cam.position[0];
         P.oy[i] = J.cam.position[1];
         P.oz[i] = J.cam.position[2];
@@ -398,7 +410,8 @@ template <bool CNT, int BLOCK, int OCC, int W, int UNI>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1)))
 void k_finish(const LxJob* __restrict__ jobs, LxPathBuf P, LxCounters* __restrict__ C, int bounce0,
               uint32_t* __restrict__ spill, float* __restrict__ accum, uint32_t chunk,
-              const LxDevScene* __restrict__ uscene, const uint32_t* __restrict__ order) {
+              const LxDevScene* __restrict__ uscene, const uint32_t* __restrict__ order,
+              const LxSched* __restrict__ cam_sched, uint32_t cam_tile, uint32_t cam_band) {
     __shared__ uint32_t s_stack[LX_STACK * BLOCK];
     /* per-lane path state that only shading reads: it waits in an LDS
      * column while the lane traverses, instead of holding registers */
@@ -437,16 +450,30 @@ void k_finish(const LxJob* __restrict__ jobs, LxPathBuf P, LxCounters* __restric
             if (!have && !flush) {
                 uint32_t my = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
                 if (my < n) {
-                    if (order) my = lx_gptr(order)[my]; /* slot of the my-th queued path */
-                    job = lx_gptr(P.job)[my];
-                    uint32_t pw = lx_gptr(P.pix)[my];
-                    spec = (pw >> 31) != 0u;
-                    pix = pw & 0x7fffffffu;
-                    o = mk3(lx_gptr(P.ox)[my], lx_gptr(P.oy)[my], lx_gptr(P.oz)[my]);
-                    d = mk3(lx_gptr(P.dx)[my], lx_gptr(P.dy)[my], lx_gptr(P.dz)[my]);
-                    ps.put_thr(mk3(lx_gptr(P.tx)[my], lx_gptr(P.ty)[my], lx_gptr(P.tz)[my]));
-                    ps.put_col(mk3(lx_gptr(P.cx)[my], lx_gptr(P.cy)[my], lx_gptr(P.cz)[my]));
-                    ps.put_pdf(lx_gptr(P.pdf)[my]);
+                    if (cam_sched) {
+                        /* camera paths made here (the camera kernel's ray for
+                         * slot my) instead of loaded from the path buffer */
+                        job = UNI == 2 ? 0u
+                                       : lx_sched_job(cam_sched->off, cam_sched->idx, cam_sched->n_active, my);
+                        const LxJob& Jc = UNI == 2 ? jobs[0] : jobs[job];
+                        pix = lx_camera_ray(Jc, my - Jc.path_off, cam_tile, cam_band, d);
+                        o = mk3(Jc.cam.position[0], Jc.cam.position[1], Jc.cam.position[2]);
+                        spec = false;
+                        ps.put_thr(mk3(1.0f, 1.0f, 1.0f));
+                        ps.put_col(mk3(0.0f, 0.0f, 0.0f));
+                        ps.put_pdf(-1.0f);
+                    } else {
+                        if (order) my = lx_gptr(order)[my]; /* slot of the my-th queued path */
+                        job = lx_gptr(P.job)[my];
+                        uint32_t pw = lx_gptr(P.pix)[my];
+                        spec = (pw >> 31) != 0u;
+                        pix = pw & 0x7fffffffu;
+                        o = mk3(lx_gptr(P.ox)[my], lx_gptr(P.oy)[my], lx_gptr(P.oz)[my]);
+                        d = mk3(lx_gptr(P.dx)[my], lx_gptr(P.dy)[my], lx_gptr(P.dz)[my]);
+                        ps.put_thr(mk3(lx_gptr(P.tx)[my], lx_gptr(P.ty)[my], lx_gptr(P.tz)[my]));
+                        ps.put_col(mk3(lx_gptr(P.cx)[my], lx_gptr(P.cy)[my], lx_gptr(P.cz)[my]));
+                        ps.put_pdf(lx_gptr(P.pdf)[my]);
+                    }
                     b = bounce0;
                     have = true;
                 }
@@ -825,50 +852,60 @@ hipError_t lx_launch_sort(const LxJob* jobs, const LxPathBuf& P, const LxRecBuf&
     hipLaunchKernelGGL(k_sort_scatter, dim3(grid), dim3(256), 0, s, C, bounce, keys, hist, order);
     return hipGetLastError();
 }
+struct LxFinishArgs {
+    const LxJob* jobs;
+    const LxPathBuf* P;
+    LxCounters* C;
+    int bounce0;
+    uint32_t* spill;
+    float* accum;
+    uint32_t chunk;
+    const LxDevScene* us;
+    const uint32_t* order;
+    const LxSched* cam;
+    uint32_t tile, band;
+};
 template <int BLOCK, int OCC, int W, int UNI>
-static void lx_finish_at(const LxJob* jobs, const LxPathBuf& P, LxCounters* C, int bounce0, uint32_t* spill,
-                         float* accum, uint32_t grid, uint32_t chunk, const LxDevScene* us, const uint32_t* order,
-                         hipStream_t s) {
-    hipLaunchKernelGGL((k_finish<false, BLOCK, OCC, W, UNI>), dim3(grid), dim3(BLOCK), 0, s, jobs, P, C, bounce0,
-                       spill, accum, chunk, us, order);
+static void lx_finish_at(const LxFinishArgs& a, uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL((k_finish<false, BLOCK, OCC, W, UNI>), dim3(grid), dim3(BLOCK), 0, s, a.jobs, *a.P, a.C,
+                       a.bounce0, a.spill, a.accum, a.chunk, a.us, a.order, a.cam, a.tile, a.band);
 }
 template <int W, int UNI>
-static void lx_finish_w(const LxJob* jobs, const LxPathBuf& P, LxCounters* C, int bounce0, uint32_t* spill,
-                        float* accum, uint32_t threads, int block, int occ, uint32_t chunk, const LxDevScene* us,
-                        const uint32_t* order, hipStream_t s) {
+static void lx_finish_w(const LxFinishArgs& a, uint32_t threads, int block, int occ, hipStream_t s) {
     if (occ >= 4) {
-        if (block == 256 && threads >= 256)
-            lx_finish_at<256, 4, W, UNI>(jobs, P, C, bounce0, spill, accum, threads / 256, chunk, us, order, s);
-        else lx_finish_at<128, 4, W, UNI>(jobs, P, C, bounce0, spill, accum, threads / 128, chunk, us, order, s);
+        if (block == 256 && threads >= 256) lx_finish_at<256, 4, W, UNI>(a, threads / 256, s);
+        else lx_finish_at<128, 4, W, UNI>(a, threads / 128, s);
     } else {
-        lx_finish_at<128, 0, W, UNI>(jobs, P, C, bounce0, spill, accum, threads / 128, chunk, us, order, s);
+        lx_finish_at<128, 0, W, UNI>(a, threads / 128, s);
     }
 }
 hipError_t lx_launch_finish(const LxJob* jobs, const LxPathBuf& P, LxCounters* C, int bounce0, uint32_t* spill,
                             float* accum, uint32_t threads, int block, int occ, int width, int chunk,
                             const LxDevScene* uscene, bool single_job, hipStream_t s, bool count,
-                            const uint32_t* order) {
+                            const uint32_t* order, const LxSched* cam_sched, uint32_t tile, uint32_t band) {
     /* threads = grid * block lanes, the same for every shape, so one spill
      * allocation serves them all */
     uint32_t c = chunk < 1 ? 1u : (chunk > 64 ? 64u : (uint32_t)chunk);
+    LxFinishArgs a{jobs, &P, C, bounce0, spill, accum, c, uscene, order, cam_sched, tile, band};
     dim3 g(threads / 128), b(128);
     if (count) {
         /* the counting replay: same paths and rays, per-lane scene and job */
         if (width == 8)
             hipLaunchKernelGGL((k_finish<true, 128, 0, 8, 0>), g, b, 0, s, jobs, P, C, bounce0, spill, accum, c, uscene,
-                               order);
+                               order, cam_sched, tile, band);
         else
             hipLaunchKernelGGL((k_finish<true, 128, 0, 4, 0>), g, b, 0, s, jobs, P, C, bounce0, spill, accum, c, uscene,
-                               order);
+                               order, cam_sched, tile, band);
     } else if (uscene && single_job) {
-        if (width == 8) lx_finish_w<8, 2>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, uscene, order, s);
-        else lx_finish_w<4, 2>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, uscene, order, s);
+        if (width == 8) lx_finish_w<8, 2>(a, threads, block, occ, s);
+        else lx_finish_w<4, 2>(a, threads, block, occ, s);
     } else if (uscene) {
-        if (width == 8) lx_finish_w<8, 1>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, uscene, order, s);
-        else lx_finish_w<4, 1>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, uscene, order, s);
+        if (width == 8) lx_finish_w<8, 1>(a, threads, block, occ, s);
+        else lx_finish_w<4, 1>(a, threads, block, occ, s);
     } else {
-        if (width == 8) lx_finish_w<8, 0>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, nullptr, order, s);
-        else lx_finish_w<4, 0>(jobs, P, C, bounce0, spill, accum, threads, block, occ, c, nullptr, order, s);
+        a.us = nullptr;
+        if (width == 8) lx_finish_w<8, 0>(a, threads, block, occ, s);
+        else lx_finish_w<4, 0>(a, threads, block, occ, s);
     }
     return hipGetLastError();
 }

@@ -14,6 +14,10 @@
 #include <algorithm>
 #include <chrono>
 
+#ifndef LX_TAIL_CAMERA
+#define LX_TAIL_CAMERA 1
+#endif
+
 uint32_t lx_spill_words(uint32_t trace_grid) { return (uint32_t)LX_SPILL * trace_grid * LX_TRACE_BLOCK; }
 
 static void carve(uint8_t*& p, float** f, size_t n) {
@@ -96,12 +100,16 @@ void lx_enqueue_round(lx_context* ctx, LxJob* d_jobs, const int32_t* d_spp, uint
                       const LxDevScene* uscene, hipEvent_t* ev, bool count, hipStream_t stream) {
     hipStream_t s = stream ? stream : ctx->stream;
     int wf = lx_wavefront_bounces(ctx, max_depth);
+    const int sm = ctx->tuning.sort_mode;
+    const uint32_t tile = (uint32_t)ctx->tuning.pixel_tile, band = (uint32_t)ctx->tuning.tile_band;
+    /* a tail kernel that starts at bounce 0 with the queue in slot order makes
+     * the camera paths itself (the same rays, with no path-state round trip
+     * through memory); otherwise the camera kernel writes them first */
+    const bool tail_camera = LX_TAIL_CAMERA && wf == 0 && wf < max_depth && !(sm & LX_SORT_RAYS);
     LX_HIP(lx_launch_begin_round(d_jobs, d_spp, njobs, d_cnt, d_sched, path_cap, s));
-    LX_HIP(lx_launch_camera(d_jobs, d_sched, d_cnt, wb.A, cam_grid, s, (uint32_t)ctx->tuning.pixel_tile,
-                            (uint32_t)ctx->tuning.tile_band));
+    if (!tail_camera) LX_HIP(lx_launch_camera(d_jobs, d_sched, d_cnt, wb.A, cam_grid, s, tile, band));
     LxPathBuf* cur = &wb.A;
     LxPathBuf* nxt = &wb.B;
-    const int sm = ctx->tuning.sort_mode;
     /* the sort stages write wb.order and the next launch reads it; a
      * per-bounce stage sits inside the timed bracket of the launch it serves */
     const uint32_t sort_grid = std::min(shade_grid, 2u * (uint32_t)std::max(1, ctx->num_cus));
@@ -125,7 +133,7 @@ void lx_enqueue_round(lx_context* ctx, LxJob* d_jobs, const int32_t* d_spp, uint
         if (ev) LX_HIP(hipEventRecord(ev[2 * wf], s));
         LX_HIP(lx_launch_finish(d_jobs, *cur, d_cnt, wf, d_spill, d_accum, trace_grid * LX_TRACE_BLOCK,
                                 ctx->tuning.block_size, ctx->tuning.tail_min_waves, width, ctx->tuning.queue_chunk,
-                                uscene, njobs == 1, s, count, of));
+                                uscene, njobs == 1, s, count, of, tail_camera ? d_sched : nullptr, tile, band));
         if (ev && 2 * wf + 1 < 2 * max_depth) LX_HIP(hipEventRecord(ev[2 * wf + 1], s));
     }
     if (ev) LX_HIP(hipEventRecord(ev[2 * max_depth], s));
