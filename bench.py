@@ -40,18 +40,20 @@ RAY_BYTES, HIT_BYTES, OCC_BYTES = 32, 16, 4
 SHADE_BYTES = 2 * 48 + 3 * 16 + 12 + 36 + 24 + 64 + 32 + 2 * 32 + 4
 # launch shapes applied unless --autotune 1 (fixed, so that runs and the
 # profiles of the same command time the same kernels): tail kernel for every
-# bounce, 256-lane blocks, 16 waves per CU, BVH4, no queue sort; the
-# headline renders one frame per round with one more round in flight
+# bounce (making its camera paths itself), 256-lane blocks, 16 waves per CU,
+# BVH4, no queue sort, refills once 48 (HDRI) / 16 (interior) lanes of a wave
+# are idle; the headline renders one frame per round with one more round in
+# flight
 WORKLOAD_TUNING = {
-    "hdri": dict(wavefront_bounces=0, block_size=256, waves_per_cu=16, queue_chunk=32, tail_min_waves=4, bvh_width=4,
+    "hdri": dict(wavefront_bounces=0, block_size=256, waves_per_cu=16, queue_chunk=48, tail_min_waves=4, bvh_width=4,
                  sort_mode=0, pixel_tile=16, tile_band=8, overlap_rounds=1, round_frames=1),
-    "interior": dict(wavefront_bounces=0, block_size=256, waves_per_cu=16, queue_chunk=32, tail_min_waves=4,
+    "interior": dict(wavefront_bounces=0, block_size=256, waves_per_cu=16, queue_chunk=16, tail_min_waves=4,
                      bvh_width=4, sort_mode=0, pixel_tile=8, tile_band=0, overlap_rounds=1, round_frames=1),
 }
 WIDE_FRAMES = 20
 # the batched and served legs' plain per-job launches: one sample per round,
-# overlapped, with the ray-cell sort
-FORMS_TUNING = dict(WORKLOAD_TUNING["hdri"], sort_mode=1, overlap_rounds=1, round_frames=1)
+# overlapped, no queue sort
+FORMS_TUNING = dict(WORKLOAD_TUNING["hdri"], sort_mode=0, overlap_rounds=1, round_frames=1)
 # device sources of the kernels the profiles measure: a profile's figures are
 # this build's only if its recorded digest matches
 DEVICE_SOURCES = ["luxion_amd/csrc/lx_kernels.hip", "luxion_amd/csrc/lx_kernels.h", "luxion_amd/csrc/lx_path_dev.h",
@@ -360,6 +362,7 @@ def main():
                 "traffic": int(prof["traffic_bytes_per_launch"]) if prof and "traffic_bytes_per_launch" in prof
                 else None,
                 "bytes_per_launch": int(bytes_per_launch), "avg_launch_ms": round(launch_ms, 4),
+                "rays_per_frame": int(rays_of(st) // steps),
                 "frames_per_launch": round(steps / max(1, klaunch), 2),
                 "frac_basis": "algorithmic bytes (counting replay: ray, hit, node, triangle, instance bytes + 380 B "
                               "per shaded path-bounce) / event-timed launch / 8 TB/s",
