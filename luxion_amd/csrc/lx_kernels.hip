@@ -78,15 +78,16 @@ __global__ void k_end_round(LxJob* __restrict__ jobs, uint32_t njobs) {
     if (j < njobs && jobs[j].active) jobs[j].spp_done += 1;
 }
 
-/* The pixel of a job's q-th camera path.  tile <= 1: row-major.  Otherwise
- * the image's whole tile x tile squares come first, with pixels in Z (Morton)
- * order inside a square -- an 8x8 square is one wave64, so a wave's camera
- * rays and their first hits are compact -- then the right strip and the
- * bottom strip the squares leave, row-major.  Squares follow each other
- * row by row (band 0), or, with band > 0, in horizontal bands of `band`
- * square rows walked column by column, so that the waves running at one
- * time cover a compact region of the image.  A bijection of [0, W*H) onto
- * the pixels for any W, H. */
+/* The pixel (x, y) of a job's q-th camera path.  tile <= 1: row-major.
+ * Otherwise (tile a power of two) the image's whole tile x tile squares come
+ * first, with pixels in Z (Morton) order inside a square -- an 8x8 square is
+ * one wave64, so a wave's camera rays and their first hits are compact --
+ * then the right strip and the bottom strip the squares leave, row-major.
+ * Squares follow each other row by row (band 0), or, with band > 0, in
+ * horizontal bands of `band` square rows walked column by column, so that
+ * the waves running at one time cover a compact region of the image.  A
+ * bijection of [0, W*H) onto the pixels for any W, H; shifts stand for the
+ * divisions by the tile size, so at most two integer divisions remain. */
 LXD uint32_t lx_even_bits(uint32_t v) {
     v &= 0x55555555u;
     v = (v | (v >> 1)) & 0x33333333u;
@@ -95,41 +96,55 @@ LXD uint32_t lx_even_bits(uint32_t v) {
     v = (v | (v >> 8)) & 0x0000ffffu;
     return v;
 }
-LXD uint32_t lx_tile_pixel(uint32_t q, uint32_t W, uint32_t H, uint32_t tile, uint32_t band) {
-    if (tile <= 1u) return q;
-    uint32_t Wt = W / tile * tile, Ht = H / tile * tile;
-    uint32_t nt = Wt * Ht;
+LXD void lx_tile_xy(uint32_t q, uint32_t W, uint32_t H, uint32_t tile, uint32_t band, uint32_t& x, uint32_t& y) {
+    if (tile <= 1u) {
+        y = q / W;
+        x = q - y * W;
+        return;
+    }
+    const uint32_t lt = (uint32_t)__builtin_ctz(tile);
+    const uint32_t tiles_x = W >> lt, tiles_y = H >> lt;
+    const uint32_t Wt = tiles_x << lt, Ht = tiles_y << lt;
+    const uint32_t nt = Wt * Ht;
     if (q < nt) {
-        uint32_t per = tile * tile;
-        uint32_t t = q / per, w = q % per;
-        uint32_t tiles_x = Wt / tile, tiles_y = Ht / tile;
+        const uint32_t t = q >> (2u * lt), w = q & ((1u << (2u * lt)) - 1u);
         uint32_t tx, ty;
         if (band == 0u) {
-            tx = t % tiles_x;
             ty = t / tiles_x;
+            tx = t - ty * tiles_x;
         } else {
-            uint32_t b = t / (tiles_x * band); /* every band but the last is full */
-            uint32_t rows = min(band, tiles_y - b * band);
-            uint32_t u = t - b * tiles_x * band;
+            const uint32_t per_band = tiles_x * band; /* every band but the last is full */
+            const uint32_t b = t / per_band;
+            const uint32_t rows = min(band, tiles_y - b * band);
+            const uint32_t u = t - b * per_band;
             tx = u / rows;
-            ty = b * band + u % rows;
+            ty = b * band + (u - tx * rows);
         }
-        uint32_t x = tx * tile + lx_even_bits(w), y = ty * tile + lx_even_bits(w >> 1);
-        return y * W + x;
+        x = (tx << lt) + lx_even_bits(w);
+        y = (ty << lt) + lx_even_bits(w >> 1);
+        return;
     }
     uint32_t r = q - nt;
-    uint32_t rw = W - Wt, nr = rw * Ht; /* right strip: columns Wt.., rows ..Ht */
-    if (r < nr) return (r / rw) * W + Wt + r % rw;
+    const uint32_t rw = W - Wt, nr = rw * Ht; /* right strip: columns Wt.., rows ..Ht */
+    if (r < nr) {
+        y = r / rw;
+        x = Wt + (r - y * rw);
+        return;
+    }
     r -= nr; /* bottom strip: rows Ht.., every column */
-    return (Ht + r / W) * W + r % W;
+    const uint32_t yy = r / W;
+    y = Ht + yy;
+    x = r - yy * W;
 }
 
 /* generateRayFromCamera, src/kernels/generate_ray_from_camera.cu:8-39: the
  * pixel of job J's q-th camera path (returned) and its ray direction. */
 LXD uint32_t lx_camera_ray(const LxJob& J, uint32_t q, uint32_t tile, uint32_t band, f3& dir) {
-    int W = J.cam.width;
-    uint32_t index = lx_tile_pixel(q, (uint32_t)W, (uint32_t)J.cam.height, tile, band);
-    int x = (int)(index % (uint32_t)W), y = (int)(index / (uint32_t)W);
+    const uint32_t W = (uint32_t)J.cam.width;
+    uint32_t ux, uy;
+    lx_tile_xy(q, W, (uint32_t)J.cam.height, tile, band, ux, uy);
+    const uint32_t index = uy * W + ux;
+    int x = (int)ux, y = (int)uy;
     LxRng rng = lx_make_rng(J.iter, (int)index, J.depth, J.seed);
     float x1 = rng.u01() - 0.5f;
     float x2 = rng.u01() - 0.5f;
