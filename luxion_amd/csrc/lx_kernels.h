@@ -74,6 +74,14 @@ hipError_t lx_launch_trace_rays(const LxDevScene* S, const lx_ray* rays, uint32_
  * device (ordinal `device`, which must be current); looked up once per device
  * with hipGetSymbolAddress and cached. */
 hipError_t lx_noalloc_scratch_address(int device, uint32_t** out);
+/* World-space bounds of transformed vertex sets: item k maps the vertices
+ * [first, first + count) of pos (xyz float triples) by its object -> world
+ * 3x4 matrix, in double, and writes min xyz, max xyz (double) to out[6k..]. */
+struct LxBoundsItem {
+    float m[12];
+    uint32_t first, count;
+};
+hipError_t lx_launch_inst_bounds(const float* pos, const LxBoundsItem* items, uint32_t n, double* out, hipStream_t s);
 hipError_t lx_launch_tonemap(const float* accum, uint32_t npix, float spp, int mode, uint8_t* out, hipStream_t s);
 hipError_t lx_launch_math_eval(int fn, const float* x, const float* y, float* out, uint32_t n, hipStream_t s);
 hipError_t lx_launch_copy16(const void* src, void* dst, size_t n16, uint32_t grid, hipStream_t s);

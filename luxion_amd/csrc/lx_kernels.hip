@@ -935,6 +935,43 @@ hipError_t lx_launch_trace_rays(const LxDevScene* S, const lx_ray* rays, uint32_
                            hits, occ, spill, overflow);
     return hipGetLastError();
 }
+/* one workgroup per item: strided min / max over the item's vertices, then
+ * a reduction in LDS */
+__global__ __launch_bounds__(256) void k_inst_bounds(const float* __restrict__ pos,
+                                                     const LxBoundsItem* __restrict__ items, double* __restrict__ out) {
+    __shared__ double s_b[6][256];
+    const LxBoundsItem& it = items[blockIdx.x];
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (uint32_t v = threadIdx.x; v < it.count; v += blockDim.x) {
+        const float* p = pos + 3u * (size_t)(it.first + v);
+        const double x = p[0], y = p[1], z = p[2];
+        for (int r = 0; r < 3; r++) {
+            const double w = (double)it.m[4 * r] * x + (double)it.m[4 * r + 1] * y + (double)it.m[4 * r + 2] * z +
+                             (double)it.m[4 * r + 3];
+            lo[r] = fmin(lo[r], w);
+            hi[r] = fmax(hi[r], w);
+        }
+    }
+    for (int r = 0; r < 3; r++) {
+        s_b[r][threadIdx.x] = lo[r];
+        s_b[3 + r][threadIdx.x] = hi[r];
+    }
+    __syncthreads();
+    for (uint32_t h = blockDim.x / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h)
+            for (int r = 0; r < 3; r++) {
+                s_b[r][threadIdx.x] = fmin(s_b[r][threadIdx.x], s_b[r][threadIdx.x + h]);
+                s_b[3 + r][threadIdx.x] = fmax(s_b[3 + r][threadIdx.x], s_b[3 + r][threadIdx.x + h]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) out[6u * blockIdx.x + threadIdx.x] = s_b[threadIdx.x][0];
+}
+hipError_t lx_launch_inst_bounds(const float* pos, const LxBoundsItem* items, uint32_t n, double* out, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_inst_bounds, dim3(n), dim3(256), 0, s, pos, items, out);
+    return hipGetLastError();
+}
 hipError_t lx_launch_tonemap(const float* accum, uint32_t npix, float spp, int mode, uint8_t* out, hipStream_t s) {
     hipLaunchKernelGGL(k_tonemap, dim3((npix + 255) / 256), dim3(256), 0, s, accum, npix, spp, mode, out);
     return hipGetLastError();

@@ -597,6 +597,53 @@ void lx_scene_accel(lx_scene* s, const lx_build_opts* o) {
         D.pad = 0u;
     }
 
+    /* a group of one rotated or scaled instance: the union of its mesh's top
+     * BVH boxes, mapped to the world, is loose (on the instanced interior
+     * its surface area is about 1.8 times that of the mapped vertices' box),
+     * so the mesh's vertices themselves are mapped (in double) and bounded on
+     * the device, then rounded outward and padded as above */
+    s->d_pos.upload(ctx, all_pos.data(), all_pos.size() * 4);
+    {
+        std::vector<LxBoundsItem> items;
+        std::vector<size_t> item_group;
+        for (size_t gi = 0; gi < groups.size(); gi++) {
+            if (groups[gi].size() != 1) continue;
+            const lx_instance_desc& g0 = s->insts[groups[gi][0]];
+            const size_t nv = s->meshes[g0.mesh].pos.size() / 3;
+            if (nv == 0) continue;
+            LxBoundsItem it;
+            std::memcpy(it.m, g0.transform, 48);
+            it.first = vtx_off[g0.mesh];
+            it.count = (uint32_t)nv;
+            items.push_back(it);
+            item_group.push_back(gi);
+        }
+        if (!items.empty()) {
+            DevBuf d_items, d_out;
+            d_items.upload(ctx, items.data(), items.size() * sizeof(LxBoundsItem));
+            d_out.alloc(ctx, items.size() * 6 * sizeof(double));
+            LX_HIP(lx_launch_inst_bounds(s->d_pos.as<float>(), d_items.as<LxBoundsItem>(), (uint32_t)items.size(),
+                                         d_out.as<double>(), ctx->stream));
+            std::vector<double> hb(items.size() * 6);
+            LX_HIP(hipMemcpyAsync(hb.data(), d_out.ptr, hb.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                  ctx->stream));
+            LX_HIP(hipStreamSynchronize(ctx->stream));
+            for (size_t k = 0; k < items.size(); k++) {
+                float flo[3], fhi[3];
+                for (int r = 0; r < 3; r++) {
+                    flo[r] = std::nextafter((float)hb[6 * k + r], -FLT_MAX);
+                    fhi[r] = std::nextafter((float)hb[6 * k + 3 + r], FLT_MAX);
+                }
+                lx_pad_box(flo, fhi);
+                const size_t gi = item_group[k];
+                for (int r = 0; r < 3; r++) {
+                    wboxes[6 * gi + r] = flo[r];
+                    wboxes[6 * gi + 3 + r] = fhi[r];
+                }
+            }
+        }
+    }
+
     /* per-instance records for hit attributes and emitter sampling */
     std::vector<LxInstance> dinst(s->insts.size());
     for (size_t i = 0; i < s->insts.size(); i++) {
@@ -650,7 +697,6 @@ void lx_scene_accel(lx_scene* s, const lx_build_opts* o) {
     s->d_inst.upload(ctx, dinst.data(), dinst.size() * sizeof(LxInstance));
     s->d_groups.upload(ctx, dgroups.data(), dgroups.size() * sizeof(LxGroup));
     s->d_idx.upload(ctx, all_idx.data(), all_idx.size() * 4);
-    s->d_pos.upload(ctx, all_pos.data(), all_pos.size() * 4);
     if (any_n) s->d_nrm.upload(ctx, all_nrm.data(), all_nrm.size() * 4);
     if (any_uv) s->d_uv.upload(ctx, all_uv.data(), all_uv.size() * 4);
     s->d_mats.upload(ctx, s->mats.data(), s->mats.size() * sizeof(lx_material_desc));
