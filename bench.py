@@ -40,20 +40,20 @@ RAY_BYTES, HIT_BYTES, OCC_BYTES = 32, 16, 4
 SHADE_BYTES = 2 * 48 + 3 * 16 + 12 + 36 + 24 + 64 + 32 + 2 * 32 + 4
 # launch shapes applied unless --autotune 1 (fixed, so that runs and the
 # profiles of the same command time the same kernels): tail kernel for every
-# bounce (making its camera paths itself), 256-lane blocks, 16 waves per CU,
-# BVH4, no queue sort, refills once 48 (HDRI) / 16 (interior) lanes of a wave
-# are idle; the headline renders one frame per round with one more round in
-# flight
+# bounce (making its camera paths itself), 128-lane (HDRI) / 256-lane
+# (interior) blocks, 16 waves per CU, BVH4, no queue sort, refills once 48
+# (HDRI) / 16 (interior) lanes of a wave are idle; the headline renders one
+# frame per round with two more rounds in flight
 WORKLOAD_TUNING = {
-    "hdri": dict(wavefront_bounces=0, block_size=256, waves_per_cu=16, queue_chunk=48, tail_min_waves=4, bvh_width=4,
-                 sort_mode=0, pixel_tile=16, tile_band=8, overlap_rounds=1, round_frames=1),
+    "hdri": dict(wavefront_bounces=0, block_size=128, waves_per_cu=16, queue_chunk=48, tail_min_waves=4, bvh_width=4,
+                 sort_mode=0, pixel_tile=16, tile_band=8, overlap_rounds=2, round_frames=1),
     "interior": dict(wavefront_bounces=0, block_size=256, waves_per_cu=16, queue_chunk=16, tail_min_waves=4,
-                     bvh_width=4, sort_mode=0, pixel_tile=8, tile_band=0, overlap_rounds=1, round_frames=1),
+                     bvh_width=4, sort_mode=0, pixel_tile=8, tile_band=0, overlap_rounds=2, round_frames=1),
 }
 WIDE_FRAMES = 20
 # the batched and served legs' plain per-job launches: one sample per round,
 # overlapped, no queue sort
-FORMS_TUNING = dict(WORKLOAD_TUNING["hdri"], sort_mode=0, overlap_rounds=1, round_frames=1)
+FORMS_TUNING = dict(WORKLOAD_TUNING["hdri"], block_size=256, sort_mode=0, overlap_rounds=1, round_frames=1)
 # device sources of the kernels the profiles measure: a profile's figures are
 # this build's only if its recorded digest matches
 DEVICE_SOURCES = ["luxion_amd/csrc/lx_kernels.hip", "luxion_amd/csrc/lx_kernels.h", "luxion_amd/csrc/lx_path_dev.h",
