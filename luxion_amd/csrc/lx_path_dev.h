@@ -219,8 +219,15 @@ LXD bool lx_shade_path(const LxJob& J, const LxDevScene& S, const LxRec& R, int 
             col = lx_debug_color(S, mat, dbg, color, normal, uv, rough, metal);
             kill = true;
         } else {
-            if (mt == LX_MATERIAL_MICROFACET)
-                sample_dir = lx_sample_ct(-d, normal, rough, metal, rng, color, mat.ior, mat.transmission);
+            /* the lobe probabilities and the BRDF's view-side terms of this
+             * shading point, shared by the sampler and every PDF / BRDF below */
+            LxLobes lobes{};
+            LxBrdfView bview{};
+            if (mt == LX_MATERIAL_MICROFACET) {
+                lobes = lx_lobes_of(-d, normal, color, metal, mat.ior, mat.transmission);
+                bview = lx_brdf_view(-d, normal, rough, mat.ior);
+                sample_dir = lx_sample_ct_q(-d, normal, rough, rng, mat.ior, lobes);
+            }
             else
                 sample_dir = lx_sample_legacy(mt, d, normal, mat.ior, rng);
             bool emissive = legacy ? mt == LX_MATERIAL_EMISSIVE : length(emission) > 0.00001f;
@@ -238,11 +245,11 @@ LXD bool lx_shade_path(const LxJob& J, const LxDevScene& S, const LxRec& R, int 
                     f3 ed = R.ed;
                     float pe = lx_envmap_pdf(S, ed);
                     float pb = mt == LX_MATERIAL_MICROFACET
-                                   ? lx_pdf(-d, ed, normal, rough, metal, color, mat.ior, mat.transmission)
+                                   ? lx_pdf_q(-d, ed, normal, rough, mat.ior, lobes)
                                    : lx_legacy_pdf(mt, ed, normal);
                     float w = (pe * pe) / (pe * pe + pb * pb);
                     f3 brdf = mt == LX_MATERIAL_MICROFACET
-                                  ? lx_brdf(-d, normal, ed, color, rough, metal, mat.ior, mat.transmission)
+                                  ? lx_brdf_v(bview, ed, color, metal, mat.transmission)
                                   : lx_legacy_brdf(mt, color);
                     f3 L = lx_env_lookup(S, normalize(ed), J.intensity);
                     float cs = dot(nf, ed);
@@ -285,11 +292,11 @@ LXD bool lx_shade_path(const LxJob& J, const LxDevScene& S, const LxRec& R, int 
                         float pa = 1.0f / S.total_area;
                         float psa = pa * (dsq / cl);
                         float pb = mt == LX_MATERIAL_MICROFACET
-                                       ? lx_pdf(-d, ld, normal, rough, metal, color, mat.ior, mat.transmission)
+                                       ? lx_pdf_q(-d, ld, normal, rough, mat.ior, lobes)
                                        : lx_legacy_pdf(mt, ld, normal);
                         float w = (psa * psa) / (psa * psa + pb * pb);
                         f3 brdf = mt == LX_MATERIAL_MICROFACET
-                                      ? lx_brdf(-d, normal, ld, color, rough, metal, mat.ior, mat.transmission)
+                                      ? lx_brdf_v(bview, ld, color, metal, mat.transmission)
                                       : lx_legacy_brdf(mt, color);
                         f3 Le = lx_get_emission(S, S.mats[R.dlmat], R.dluv);
                         f3 contrib = ((Le * brdf) * cs) / psa;
@@ -306,8 +313,9 @@ LXD bool lx_shade_path(const LxJob& J, const LxDevScene& S, const LxRec& R, int 
                     /* only the uber build counts a transmission as specular */
                     bool transmitted = !legacy && dot(-d, normal) * dot(sample_dir, normal) < 0.0f;
                     bool nspec = is_specular || transmitted;
-                    f3 ct = lx_ct_eval_cos(-d, normal, sample_dir, color, rough, metal, mat.ior, mat.transmission);
-                    float pdf = lx_pdf(-d, sample_dir, normal, rough, metal, color, mat.ior, mat.transmission);
+                    f3 ct = lx_ct_eval_cos_v(bview, -d, normal, sample_dir, color, rough, metal, mat.ior,
+                                             mat.transmission);
+                    float pdf = lx_pdf_q(-d, sample_dir, normal, rough, mat.ior, lobes);
                     thr = thr * (ct / pdf);
                     last_pdf = pdf;
                     spec = nspec;

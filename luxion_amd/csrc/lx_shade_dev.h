@@ -285,25 +285,43 @@ LXD f3 lx_fresnel(float c, f3 albedo, float metallic, float ei, float et) {
     f3 cd = lx_schlick(c, albedo);
     return d + metallic * (cd - d);
 }
-/* BRDF, cook_torrance.cu:79-110 */
-LXD f3 lx_brdf(f3 v, f3 n, f3 l, f3 albedo, float rough, float metal, float ior, float tr) {
+/* The view-side terms of the BRDF for one shading point (v, n, rough, ior):
+ * the same for every light direction the bounce evaluates, so a bounce
+ * computes them once. */
+struct LxBrdfView {
+    f3 v, nf;
+    float ei, et, alpha, g1v, den4v;
+};
+LXD LxBrdfView lx_brdf_view(f3 v, f3 n, float rough, float ior) {
+    LxBrdfView b;
     ior = lxm_max(ior, 1.0001f);
     bool enter = dot(v, n) > 0.0f;
-    f3 nf = enter ? n : -n;
-    float ei = enter ? 1.0f : ior, et = enter ? ior : 1.0f;
-    f3 h = normalize(v + l);
+    b.v = v;
+    b.nf = enter ? n : -n;
+    b.ei = enter ? 1.0f : ior;
+    b.et = enter ? ior : 1.0f;
     rough = lxm_clamp(rough, 0.0001f, 1.0f);
-    float alpha = rough * rough;
-    float D = lx_D(h, nf, alpha);
-    f3 F = lx_fresnel(lxm_max(dot(v, h), 0.0f), albedo, metal, ei, et);
-    float G = lx_G1(v, nf, alpha) * lx_G1(l, nf, alpha);
+    b.alpha = rough * rough;
+    b.g1v = lx_G1(v, b.nf, b.alpha);
+    b.den4v = 4.0f * lxm_max(dot(b.nf, v), 0.0f);
+    return b;
+}
+/* BRDF, cook_torrance.cu:79-110, toward l */
+LXD f3 lx_brdf_v(const LxBrdfView& b, f3 l, f3 albedo, float metal, float tr) {
+    f3 h = normalize(b.v + l);
+    float D = lx_D(h, b.nf, b.alpha);
+    f3 F = lx_fresnel(lxm_max(dot(b.v, h), 0.0f), albedo, metal, b.ei, b.et);
+    float G = b.g1v * lx_G1(l, b.nf, b.alpha);
     f3 num = (D * F) * G;
-    float den = 4.0f * lxm_max(dot(nf, v), 0.0f) * lxm_max(dot(nf, l), 0.0f);
+    float den = b.den4v * lxm_max(dot(b.nf, l), 0.0f);
     f3 spec = num / lxm_max(den, LX_CT_EPS);
     f3 ns = mk3(1.0f, 1.0f, 1.0f) - F;
     f3 kD = ((1.0f - metal) * ns) * (1.0f - tr);
     f3 diff = (kD * albedo) * LX_INV_PI_CT;
     return diff + spec;
+}
+LXD f3 lx_brdf(f3 v, f3 n, f3 l, f3 albedo, float rough, float metal, float ior, float tr) {
+    return lx_brdf_v(lx_brdf_view(v, n, rough, ior), l, albedo, metal, tr);
 }
 /* BTDF, cook_torrance.cu:112-142 */
 LXD f3 lx_btdf(f3 wo, f3 n, f3 wi, f3 albedo, float rough, float metal, float ior, float tr) {
@@ -325,6 +343,9 @@ LXD f3 lx_btdf(f3 wo, f3 n, f3 wi, f3 albedo, float rough, float metal, float io
     float DFG = D * G * (1.0f - F);
     return albedo * (tr * (1.0f - metal) * (ctn / ctd) * jac * DFG);
 }
+struct LxLobes {
+    float pR, pD, pS;
+};
 /* lobe probabilities shared by sampleCookTorrance (cook_torrance.cu:170-183)
  * and PDF (cook_torrance.cu:272-285) */
 LXD void lx_lobes(f3 wo, f3 n, f3 color, float metal, float ior, float tr, float& pR, float& pD, float& pS) {
@@ -337,6 +358,12 @@ LXD void lx_lobes(f3 wo, f3 n, f3 color, float metal, float ior, float tr, float
     pR = tr * (1.0f - metal) * (1.0f - Fm);
     pD = (1.0f - pR) * (1.0f - sr) * (1.0f - metal);
     pS = 1.0f - pR - pD;
+}
+/* the lobes of a shading point, with PDF's and the sampler's ior clamp */
+LXD LxLobes lx_lobes_of(f3 wo, f3 n, f3 color, float metal, float ior, float tr) {
+    LxLobes q;
+    lx_lobes(wo, n, color, metal, lxm_max(ior, 1.0001f), tr, q.pR, q.pD, q.pS);
+    return q;
 }
 /* calculateRandomDirectionInHemisphere, src/interactions.cu:7-45 */
 LXD f3 lx_cos_hemisphere(f3 n, LxRng& rng) {
@@ -351,8 +378,8 @@ LXD f3 lx_cos_hemisphere(f3 n, LxRng& rng) {
     f3 p2 = normalize(cross(n, p1));
     return ((up * n) + ((lxm_cos(around) * over) * p1)) + ((lxm_sin(around) * over) * p2);
 }
-/* sampleCookTorrance, cook_torrance.cu:156-224 */
-LXD f3 lx_sample_ct(f3 wo, f3 n, float rough, float metal, LxRng& rng, f3 color, float ior, float tr) {
+/* sampleCookTorrance, cook_torrance.cu:156-224, with the lobe probabilities given */
+LXD f3 lx_sample_ct_q(f3 wo, f3 n, float rough, LxRng& rng, float ior, const LxLobes& q) {
     float r = rng.u01();
     ior = lxm_max(ior, 1.0001f);
     rough = lxm_clamp(rough, 0.0001f, 1.0f);
@@ -360,8 +387,7 @@ LXD f3 lx_sample_ct(f3 wo, f3 n, float rough, float metal, LxRng& rng, f3 color,
     bool enter = dot(wo, n) > 0.0f;
     f3 nf = enter ? n : -n;
     float ei = enter ? 1.0f : ior, et = enter ? ior : 1.0f;
-    float pR, pD, pS;
-    lx_lobes(wo, n, color, metal, ior, tr, pR, pD, pS);
+    const float pD = q.pD, pS = q.pS;
     if (r > pS && r <= pS + pD) return lx_cos_hemisphere(nf, rng);
     float x1 = rng.u01();
     float x2 = rng.u01();
@@ -387,6 +413,9 @@ LXD f3 lx_sample_ct(f3 wo, f3 n, float rough, float metal, LxRng& rng, f3 color,
         return mk3(0, 0, 0);
     return normalize(wi);
 }
+LXD f3 lx_sample_ct(f3 wo, f3 n, float rough, float metal, LxRng& rng, f3 color, float ior, float tr) {
+    return lx_sample_ct_q(wo, n, rough, rng, ior, lx_lobes_of(wo, n, color, metal, ior, tr));
+}
 /* PDF_GGX, cook_torrance.cu:226-237 */
 LXD float lx_pdf_ggx(f3 wo, f3 wi, f3 n, float rough) {
     f3 h = normalize(wo + wi);
@@ -407,26 +436,32 @@ LXD float lx_pdf_ggx_t(f3 wo, f3 wi, f3 n, float rough, float ior) {
     float ph = lx_D(ht, n, alpha) * lxm_abs(dot(n, ht));
     return ph * (et * et * lxm_abs(dot(wi, ht))) / lxm_max(den * den, LX_CT_EPS);
 }
-/* PDF, cook_torrance.cu:260-295 */
-LXD float lx_pdf(f3 wo, f3 wi, f3 n, float rough, float metal, f3 color, float ior, float tr) {
+/* PDF, cook_torrance.cu:260-295, with the lobe probabilities given */
+LXD float lx_pdf_q(f3 wo, f3 wi, f3 n, float rough, float ior, const LxLobes& q) {
     ior = lxm_max(ior, 1.0001f);
     f3 nf = dot(wo, n) < 0.0f ? -n : n;
     float dwn = dot(wi, nf);
     float pd = (dwn > 0.0f ? dwn : 0.0f) * LX_INV_PI_CT;
     bool refract = (dot(wi, n) * dot(wo, n)) < 0.0f;
-    float pR, pD, pS;
-    lx_lobes(wo, n, color, metal, ior, tr, pR, pD, pS);
     float pdf;
-    if (refract) pdf = pR * lx_pdf_ggx_t(wo, wi, n, rough, ior);
-    else pdf = pD * pd + pS * lx_pdf_ggx(wo, wi, nf, rough);
+    if (refract) pdf = q.pR * lx_pdf_ggx_t(wo, wi, n, rough, ior);
+    else pdf = q.pD * pd + q.pS * lx_pdf_ggx(wo, wi, nf, rough);
     return lxm_max(pdf, LX_PDF_CLAMP);
 }
-/* shadePathCookTorrance, cook_torrance.cu:298-331 */
-LXD f3 lx_ct_eval_cos(f3 wo, f3 n, f3 wi, f3 albedo, float rough, float metal, float ior, float tr) {
+LXD float lx_pdf(f3 wo, f3 wi, f3 n, float rough, float metal, f3 color, float ior, float tr) {
+    return lx_pdf_q(wo, wi, n, rough, ior, lx_lobes_of(wo, n, color, metal, ior, tr));
+}
+/* shadePathCookTorrance, cook_torrance.cu:298-331 (b: the BRDF's view-side
+ * terms of the same wo, n, rough, ior) */
+LXD f3 lx_ct_eval_cos_v(const LxBrdfView& b, f3 wo, f3 n, f3 wi, f3 albedo, float rough, float metal, float ior,
+                        float tr) {
     ior = lxm_max(ior, 1.0001f);
     bool refract = (dot(wi, n) * dot(wo, n)) < 0.0f;
-    f3 f = refract ? lx_btdf(wo, n, wi, albedo, rough, metal, ior, tr) : lx_brdf(wo, n, wi, albedo, rough, metal, ior, tr);
+    f3 f = refract ? lx_btdf(wo, n, wi, albedo, rough, metal, ior, tr) : lx_brdf_v(b, wi, albedo, metal, tr);
     return f * lxm_abs(dot(wi, n));
+}
+LXD f3 lx_ct_eval_cos(f3 wo, f3 n, f3 wi, f3 albedo, float rough, float metal, float ior, float tr) {
+    return lx_ct_eval_cos_v(lx_brdf_view(wo, n, rough, ior), wo, n, wi, albedo, rough, metal, ior, tr);
 }
 
 /* ---------------------------------------------------------- CDF sampling */
