@@ -399,8 +399,9 @@ __global__ __launch_bounds__(LX_SHADE_BLOCK) void k_shade(const LxJob* __restric
 
 /* ------------------------------------------------------------ tail */
 /* One lane's column of tail-kernel path state in LDS (stride BLOCK floats):
- * throughput, colour, last pdf and the pending environment colour term. */
-#define LX_TAIL_LDS_STATE 10
+ * throughput, colour, last pdf, the pending environment colour term and the
+ * pending environment shadow ray (origin, direction). */
+#define LX_TAIL_LDS_STATE 16
 template <int BLOCK>
 struct LxTailColumn {
     LX_LDS float* p;
@@ -414,6 +415,12 @@ struct LxTailColumn {
     LXD void put_col(f3 v) const { set(3, v.x); set(4, v.y); set(5, v.z); }
     LXD void put_pdf(float v) const { set(6, v); }
     LXD void put_pend_col(f3 v) const { set(7, v.x); set(8, v.y); set(9, v.z); }
+    LXD f3 pend_o() const { return mk3(get(10), get(11), get(12)); }
+    LXD f3 pend_d() const { return mk3(get(13), get(14), get(15)); }
+    LXD void put_pend_ray(f3 o, f3 d) const {
+        set(10, o.x); set(11, o.y); set(12, o.z);
+        set(13, d.x); set(14, d.y); set(15, d.z);
+    }
 };
 
 /* BLOCK threads per workgroup; OCC > 0 asks the register allocator for at
@@ -450,9 +457,9 @@ void k_finish(const LxJob* __restrict__ jobs, LxPathBuf P, LxCounters* __restric
     f3 o = mk3(0, 0, 0), d = mk3(0, 0, 0);
     bool spec = false;
     /* the environment shadow ray of the last shaded bounce, traced together
-     * with the next radiance ray; its colour term waits in the LDS column */
+     * with the next radiance ray; the ray and its colour term wait in the
+     * LDS column */
     bool pend = false, pend_add = false, flush = false;
-    f3 pend_o = mk3(0, 0, 0), pend_d = mk3(0, 0, 0);
     while (true) {
         /* refill idle lanes from the live-path queue, one atomic per wave,
          * once at least `chunk` lanes are idle (or all of them are) */
@@ -501,8 +508,8 @@ void k_finish(const LxJob* __restrict__ jobs, LxPathBuf P, LxCounters* __restric
             LxRec r{};
             LxRecSink sink{r};
             bool occluded = false;
-            lx_trace_path_d<CNT, W>(J, S, o, d, pix, b, st, &C->stack_overflow, pend, pend_o, pend_d, have, occluded,
-                                    sink, rc, &wc, &wa);
+            lx_trace_path_d<CNT, W>(J, S, o, d, pix, b, st, &C->stack_overflow, pend, ps.pend_o(), ps.pend_d(), have,
+                                    occluded, sink, rc, &wc, &wa);
             f3 col = ps.col();
             if (pend) {
                 if (!occluded && pend_add) col = col + ps.pend_col();
@@ -521,8 +528,7 @@ void k_finish(const LxJob* __restrict__ jobs, LxPathBuf P, LxCounters* __restric
                 if (J.emis && r.t > 0.0f) {
                     /* this bounce's environment shadow ray, traced next time */
                     pend = true;
-                    pend_o = r.so;
-                    pend_d = r.ed;
+                    ps.put_pend_ray(r.so, r.ed);
                 }
                 if (cont) {
                     b++;
