@@ -14,9 +14,6 @@
 #include <algorithm>
 #include <chrono>
 
-#ifndef LX_TAIL_CAMERA
-#define LX_TAIL_CAMERA 1
-#endif
 
 uint32_t lx_spill_words(uint32_t trace_grid) { return (uint32_t)LX_SPILL * trace_grid * LX_TRACE_BLOCK; }
 
@@ -105,7 +102,7 @@ void lx_enqueue_round(lx_context* ctx, LxJob* d_jobs, const int32_t* d_spp, uint
     /* a tail kernel that starts at bounce 0 with the queue in slot order makes
      * the camera paths itself (the same rays, with no path-state round trip
      * through memory); otherwise the camera kernel writes them first */
-    const bool tail_camera = LX_TAIL_CAMERA && wf == 0 && wf < max_depth && !(sm & LX_SORT_RAYS);
+    const bool tail_camera = wf == 0 && wf < max_depth && !(sm & LX_SORT_RAYS);
     LX_HIP(lx_launch_begin_round(d_jobs, d_spp, njobs, d_cnt, d_sched, path_cap, s));
     if (!tail_camera) LX_HIP(lx_launch_camera(d_jobs, d_sched, d_cnt, wb.A, cam_grid, s, tile, band));
     LxPathBuf* cur = &wb.A;
