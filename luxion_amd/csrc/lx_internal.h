@@ -313,7 +313,9 @@ void lx_enqueue_round(lx_context* ctx, LxJob* d_jobs, const int32_t* d_spp, uint
                       hipEvent_t* ev /* 2*max_depth+1 slots or NULL: [2b], [2b+1] bracket the trace and
                                         shade of bounce b < wavefront_bounces, [2w], [2w+1] the tail kernel,
                                         [2*max_depth] the round's end; other slots are not recorded */,
-                      bool count = false, hipStream_t stream = nullptr /* NULL: the context's stream */);
+                      bool count = false, hipStream_t stream = nullptr /* NULL: the context's stream */,
+                      bool dl_query = true /* some job may have -dmis: the tail kernel whose direct-light
+                                              rays are occlusion queries */);
 uint32_t lx_spill_words(uint32_t trace_grid);
 /* a += b for the cumulative fields (ray, bounce and work counts, overflow flag) */
 void lx_counters_add(LxCounters& a, const LxCounters& b);

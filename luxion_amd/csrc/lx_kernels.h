@@ -66,7 +66,10 @@ hipError_t lx_launch_finish(const LxJob* jobs, const LxPathBuf& P, LxCounters* C
                              * camera paths of the scheduled jobs itself, as the
                              * camera kernel would with this tile and band, instead
                              * of reading them from P */
-                            const LxSched* cam_sched = nullptr, uint32_t tile = 1, uint32_t band = 0);
+                            const LxSched* cam_sched = nullptr, uint32_t tile = 1, uint32_t band = 0,
+                            /* some job has -dmis: the kernel whose direct-light rays
+                             * are occlusion queries (same colours) */
+                            bool dl_query = false);
 hipError_t lx_launch_trace_rays(const LxDevScene* S, const lx_ray* rays, uint32_t n, int any, int alpha,
                                 uint32_t iteration, lx_hit* hits, uint8_t* occ, uint32_t* spill, uint32_t* overflow,
                                 uint32_t grid, int width, hipStream_t s);

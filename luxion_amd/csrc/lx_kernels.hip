@@ -428,7 +428,9 @@ struct LxTailColumn {
 /* UNI 1: every job of the launch renders the scene at `uscene`, which then
  * lives in scalar registers instead of one copy per lane; UNI 2: in
  * addition the launch has a single job (jobs[0]), also kept in scalars. */
-template <bool CNT, int BLOCK, int OCC, int W, int UNI>
+/* DLQ: direct-light shadow rays answered by lx_trace_dl (the kernel for jobs
+ * with -dmis; without it the kernel carries no occlusion-query code). */
+template <bool CNT, int BLOCK, int OCC, int W, int UNI, bool DLQ>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1)))
 void k_finish(const LxJob* __restrict__ jobs, LxPathBuf P, LxCounters* __restrict__ C, int bounce0,
               uint32_t* __restrict__ spill, float* __restrict__ accum, uint32_t chunk,
@@ -508,8 +510,8 @@ void k_finish(const LxJob* __restrict__ jobs, LxPathBuf P, LxCounters* __restric
             LxRec r{};
             LxRecSink sink{r};
             bool occluded = false;
-            lx_trace_path_d<CNT, W>(J, S, o, d, pix, b, st, &C->stack_overflow, pend, ps.pend_o(), ps.pend_d(), have,
-                                    occluded, sink, rc, &wc, &wa);
+            lx_trace_path_d<CNT, W, DLQ>(J, S, o, d, pix, b, st, &C->stack_overflow, pend, ps.pend_o(), ps.pend_d(),
+                                         have, occluded, sink, rc, &wc, &wa);
             f3 col = ps.col();
             if (pend) {
                 if (!occluded && pend_add) col = col + ps.pend_col();
@@ -885,11 +887,16 @@ struct LxFinishArgs {
     const uint32_t* order;
     const LxSched* cam;
     uint32_t tile, band;
+    bool dlq;
 };
 template <int BLOCK, int OCC, int W, int UNI>
 static void lx_finish_at(const LxFinishArgs& a, uint32_t grid, hipStream_t s) {
-    hipLaunchKernelGGL((k_finish<false, BLOCK, OCC, W, UNI>), dim3(grid), dim3(BLOCK), 0, s, a.jobs, *a.P, a.C,
-                       a.bounce0, a.spill, a.accum, a.chunk, a.us, a.order, a.cam, a.tile, a.band);
+    if (OCC > 0 && a.dlq)
+        hipLaunchKernelGGL((k_finish<false, BLOCK, OCC, W, UNI, true>), dim3(grid), dim3(BLOCK), 0, s, a.jobs, *a.P,
+                           a.C, a.bounce0, a.spill, a.accum, a.chunk, a.us, a.order, a.cam, a.tile, a.band);
+    else
+        hipLaunchKernelGGL((k_finish<false, BLOCK, OCC, W, UNI, false>), dim3(grid), dim3(BLOCK), 0, s, a.jobs, *a.P,
+                           a.C, a.bounce0, a.spill, a.accum, a.chunk, a.us, a.order, a.cam, a.tile, a.band);
 }
 template <int W, int UNI>
 static void lx_finish_w(const LxFinishArgs& a, uint32_t threads, int block, int occ, hipStream_t s) {
@@ -903,20 +910,27 @@ static void lx_finish_w(const LxFinishArgs& a, uint32_t threads, int block, int 
 hipError_t lx_launch_finish(const LxJob* jobs, const LxPathBuf& P, LxCounters* C, int bounce0, uint32_t* spill,
                             float* accum, uint32_t threads, int block, int occ, int width, int chunk,
                             const LxDevScene* uscene, bool single_job, hipStream_t s, bool count,
-                            const uint32_t* order, const LxSched* cam_sched, uint32_t tile, uint32_t band) {
+                            const uint32_t* order, const LxSched* cam_sched, uint32_t tile, uint32_t band,
+                            bool dl_query) {
     /* threads = grid * block lanes, the same for every shape, so one spill
      * allocation serves them all */
     uint32_t c = chunk < 1 ? 1u : (chunk > 64 ? 64u : (uint32_t)chunk);
-    LxFinishArgs a{jobs, &P, C, bounce0, spill, accum, c, uscene, order, cam_sched, tile, band};
+    LxFinishArgs a{jobs, &P, C, bounce0, spill, accum, c, uscene, order, cam_sched, tile, band, dl_query};
     dim3 g(threads / 128), b(128);
     if (count) {
         /* the counting replay: same paths and rays, per-lane scene and job */
-        if (width == 8)
-            hipLaunchKernelGGL((k_finish<true, 128, 0, 8, 0>), g, b, 0, s, jobs, P, C, bounce0, spill, accum, c, uscene,
-                               order, cam_sched, tile, band);
+        if (width == 8 && dl_query)
+            hipLaunchKernelGGL((k_finish<true, 128, 0, 8, 0, true>), g, b, 0, s, jobs, P, C, bounce0, spill, accum, c,
+                               uscene, order, cam_sched, tile, band);
+        else if (width == 8)
+            hipLaunchKernelGGL((k_finish<true, 128, 0, 8, 0, false>), g, b, 0, s, jobs, P, C, bounce0, spill, accum, c,
+                               uscene, order, cam_sched, tile, band);
+        else if (dl_query)
+            hipLaunchKernelGGL((k_finish<true, 128, 0, 4, 0, true>), g, b, 0, s, jobs, P, C, bounce0, spill, accum, c,
+                               uscene, order, cam_sched, tile, band);
         else
-            hipLaunchKernelGGL((k_finish<true, 128, 0, 4, 0>), g, b, 0, s, jobs, P, C, bounce0, spill, accum, c, uscene,
-                               order, cam_sched, tile, band);
+            hipLaunchKernelGGL((k_finish<true, 128, 0, 4, 0, false>), g, b, 0, s, jobs, P, C, bounce0, spill, accum, c,
+                               uscene, order, cam_sched, tile, band);
     } else if (uscene && single_job) {
         if (width == 8) lx_finish_w<8, 2>(a, threads, block, occ, s);
         else lx_finish_w<4, 2>(a, threads, block, occ, s);

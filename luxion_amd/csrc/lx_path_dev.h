@@ -119,8 +119,11 @@ LXD void lx_trace_path(const LxJob& J, const LxDevScene& S, f3 o, f3 d, uint32_t
  * the same traversal loop as this bounce's radiance ray, and this bounce's
  * environment shadow ray is not traced here but left in the record (so, ed)
  * for the next call.  has_rad false traces only the pending ray (a path that
- * ended with one outstanding).  The direct-light ray follows as before. */
-template <bool CNT, int W>
+ * ended with one outstanding).  The direct-light ray follows; with DLQ it is
+ * answered by lx_trace_dl, whose hit is kept only when it emits (a
+ * non-emitting closest hit adds nothing to the colour), so the per-bounce
+ * form's records, not this form's, carry the occluders. */
+template <bool CNT, int W, bool DLQ = false>
 LXD void lx_trace_path_d(const LxJob& J, const LxDevScene& S, f3 o, f3 d, uint32_t pix, int bounce, LxStack& st,
                          uint32_t* overflow, bool pend, f3 pend_o, f3 pend_d, bool has_rad, bool& pend_occluded,
                          LxRecSink& out, LxRayCount& rc, LxWork* wc, LxWork* wa) {
@@ -160,8 +163,11 @@ LXD void lx_trace_path_d(const LxJob& J, const LxDevScene& S, f3 o, f3 d, uint32
             f3 ld = lp - so;
             float dist = length(ld);
             ld = ld * (1.0f / dist);
-            LxHitD hd = lx_trace<false, CNT, W>(S, so, ld, dist * 1.0001f + 1e-4f, pix, (uint32_t)J.iter,
-                                                LX_INST_ALPHA, st, overflow, wc);
+            /* DLQ: only an emitting closest hit reaches the colour */
+            LxHitD hd = DLQ ? lx_trace_dl<CNT, W>(S, so, ld, dist * 1.0001f + 1e-4f, pix, (uint32_t)J.iter, st,
+                                                  overflow, wc)
+                            : lx_trace<false, CNT, W>(S, so, ld, dist * 1.0001f + 1e-4f, pix, (uint32_t)J.iter,
+                                                      LX_INST_ALPHA, st, overflow, wc);
             rc.dl++;
             if (hd.inst != 0xffffffffu) {
                 f3 dN;

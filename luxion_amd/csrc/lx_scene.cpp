@@ -39,6 +39,17 @@ static inline H3 hxf(const float* m, H3 p) {
               (m[8] * p.x + m[9] * p.y) + (m[10] * p.z + m[11]));
 }
 
+/* Whether a material can emit: lx_get_emission is emission colour x
+ * strength (times a texture), so a material whose product is zero in every
+ * channel adds exactly zero wherever it is seen. */
+static bool lx_material_emits(const lx_material_desc& m) {
+    for (int c = 0; c < 3; c++) {
+        float e = m.emission_color[c] * m.emission_strength;
+        if (e != 0.0f || e != e) return true;
+    }
+    return false;
+}
+
 static double now_ms() {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -589,10 +600,14 @@ void lx_scene_accel(lx_scene* s, const lx_build_opts* o) {
         std::memcpy(D.inv, &s->inv[12 * (size_t)G[0]], 48);
         D.blas_root = root;
         D.flags = 0u;
+        bool all_emit = true;
         for (uint32_t ii : G) {
             if (s->mats[s->insts[ii].material].alpha_mode != LX_ALPHA_OPAQUE) D.flags |= LX_INST_ALPHA;
+            if (lx_material_emits(s->mats[s->insts[ii].material])) D.flags |= LX_GROUP_EMIS_ANY;
+            else all_emit = false;
             inst_root[ii] = root;
         }
+        if (all_emit) D.flags |= LX_GROUP_EMIS_ALL;
         D.inst = G.size() == 1 ? G[0] : LX_GROUP_TAGGED;
         D.pad = 0u;
     }
@@ -654,7 +669,8 @@ void lx_scene_accel(lx_scene* s, const lx_build_opts* o) {
         I.mesh = g.mesh;
         I.material = g.material;
         I.blas_root = inst_root[i];
-        I.flags = s->mats[g.material].alpha_mode != LX_ALPHA_OPAQUE ? LX_INST_ALPHA : 0u;
+        I.flags = (s->mats[g.material].alpha_mode != LX_ALPHA_OPAQUE ? LX_INST_ALPHA : 0u) |
+                  (lx_material_emits(s->mats[g.material]) ? LX_INST_EMISSIVE : 0u);
         I.idx_off = idx_off[g.mesh];
         I.vtx_off = vtx_off[g.mesh];
         I.num_tris = (uint32_t)(s->meshes[g.mesh].idx.size() / 3);
@@ -670,6 +686,36 @@ void lx_scene_accel(lx_scene* s, const lx_build_opts* o) {
     /* TLAS over the groups */
     BvhBuildResult tl;
     lx_build_tlas_host(wboxes, opt.bvh_width, tl);
+    /* and a second TLAS over the groups with an emitting member, for
+     * direct-light shadow rays (whose closest hit only matters when it
+     * emits), appended to the first one's nodes with its leaves naming the
+     * groups' indices */
+    uint32_t emis_root_local = LX_CHILD_EMPTY;
+    {
+        std::vector<float> eboxes;
+        std::vector<uint32_t> egroup;
+        for (size_t gi = 0; gi < groups.size(); gi++)
+            if (dgroups[gi].flags & LX_GROUP_EMIS_ANY) {
+                eboxes.insert(eboxes.end(), wboxes.begin() + 6 * gi, wboxes.begin() + 6 * gi + 6);
+                egroup.push_back((uint32_t)gi);
+            }
+        if (!egroup.empty()) {
+            BvhBuildResult et;
+            lx_build_tlas_host(eboxes, opt.bvh_width, et);
+            const uint32_t off = (uint32_t)tl.nodes.size();
+            auto remap = [&](uint32_t v) {
+                if (v == LX_CHILD_EMPTY) return v;
+                if (v & LX_LEAF_BIT)
+                    return (v & ~LX_LEAF_FIRST_MASK) | egroup[v & LX_LEAF_FIRST_MASK];
+                return v + off;
+            };
+            for (LxWideNode n : et.nodes) {
+                for (int k = 0; k < 8; k++) n.child[k] = remap(n.child[k]);
+                tl.nodes.push_back(n);
+            }
+            emis_root_local = remap(et.root);
+        }
+    }
     const uint32_t tlas_first = node_total;
     LX_REQUIRE((size_t)tlas_first + tl.nodes.size() < LX_LEAF_BIT, LX_ERR_INVALID,
                "scene too large for the node encoding");
@@ -683,6 +729,9 @@ void lx_scene_accel(lx_scene* s, const lx_build_opts* o) {
             n.child[k] = v + tlas_first;
         }
     uint32_t tlas_root = (tl.root & LX_LEAF_BIT) ? tl.root : tl.root + tlas_first;
+    uint32_t emis_tlas_root = (emis_root_local == LX_CHILD_EMPTY || (emis_root_local & LX_LEAF_BIT))
+                                  ? emis_root_local
+                                  : emis_root_local + tlas_first;
 
     /* uploads: nodes packed at the traversal's width; BLASes the GPU builder
      * left on the device are copied there */
@@ -725,6 +774,7 @@ void lx_scene_accel(lx_scene* s, const lx_build_opts* o) {
     D.tri_cdf = s->d_tri_cdf.as<float>();
     D.tri_cdf_off = s->d_tri_cdf_off.as<uint32_t>();
     D.tlas_root = tlas_root;
+    D.emis_tlas_root = emis_tlas_root;
     D.num_inst = (uint32_t)s->insts.size();
     D.num_groups = (uint32_t)groups.size();
     D.num_emissive = (uint32_t)s->emissive.size();

@@ -456,6 +456,119 @@ LXD LxHitD lx_trace(const LxDevScene& S, f3 wo, f3 wd, float tmax, uint32_t key,
     return best;
 }
 
+/* Direct-light shadow rays.  sampleDirectLight's ray is a closest-hit ray,
+ * but shadePath uses its hit only through the hit material's emission
+ * (shade_path.cu:246-289), which is exactly zero on a material whose
+ * emission colour x strength is zero in every channel: such a closest hit
+ * adds +0 to a colour that is never -0, i.e. nothing.  So the ray is answered
+ * by two cheaper queries with the same result for the colour:
+ *   DARK  (lx_trace_q<1>): any accepted candidate on a non-emitting instance
+ *         that comes before `bound` in the lexicographic (t, instance,
+ *         primitive) order -- an occlusion query that stops at the first one;
+ *   LIGHT (lx_trace_q<2>): the closest accepted candidate on an emitting
+ *         instance before `bound`, over the emitter TLAS.
+ * Candidates are the same Moller-Trumbore and alpha any-hit acceptances as
+ * lx_trace's.  Returns whether a candidate was found (in best). */
+template <int MODE, bool CNT, int W>
+LXD bool lx_trace_q(const LxDevScene& S, uint32_t root, f3 wo, f3 wd, float tmax, const LxHitD& bound, uint32_t key,
+                    uint32_t iteration, uint32_t alpha_mask, LxStack& st, uint32_t* overflow, LxWork* work,
+                    LxHitD& best) {
+    best = bound;
+    f3 iwd = lx_rcp3(wd);
+    f3 o = wo, d = wd;
+    LxSlab sl = lx_slab(wo, iwd);
+    bool in_blas = false;
+    uint32_t cur_inst = 0, inst_flags = 0, emis = 0;
+    st.sp = 0;
+    uint32_t node = root;
+    bool found = false;
+    while (true) {
+        bool pop = false;
+        if ((node & LX_LEAF_BIT) && !in_blas) {
+            const uint32_t gf = lx_gptr(S.groups)[node & LX_LEAF_FIRST_MASK].flags;
+            /* an occlusion query skips groups that only emit */
+            if (MODE == 1 && (gf & LX_GROUP_EMIS_ALL)) pop = true;
+            else {
+                emis = gf & (LX_GROUP_EMIS_ANY | LX_GROUP_EMIS_ALL);
+                pop = !lx_enter_instance<CNT>(S, node, cur_inst, inst_flags, in_blas, o, d, sl, wo, wd, alpha_mask,
+                                              st, overflow, work);
+            }
+        }
+        if (!pop) {
+            if (node & LX_LEAF_BIT) {
+                if (in_blas) {
+                    const uint32_t first = node & LX_LEAF_FIRST_MASK;
+                    const uint32_t cnt = ((node >> 28) & 7u) + 1u;
+                    const LX_GLOBAL lx_v4f* tb = (const LX_GLOBAL lx_v4f*)lx_gptr(S.tris) + 3u * (size_t)first;
+                    for (uint32_t k = 0; k < cnt; k++) {
+                        const LX_GLOBAL lx_v4f* p0 = tb + 3u * k;
+                        float4 A = lx_ld4(p0), B = lx_ld4(p0 + 1), Cc = lx_ld4(p0 + 2);
+                        if (CNT) work->tris++;
+                        float t, u, v;
+                        if (!lx_mt(o, d, A, B, Cc, tmax, t, u, v)) continue;
+                        const uint32_t prim = lxm_f2u(A.w);
+                        const uint32_t inst = cur_inst == LX_GROUP_TAGGED ? lxm_f2u(B.w) : cur_inst;
+                        if (!lx_better(t, inst, prim, best)) continue;
+                        bool em;
+                        if (emis == (LX_GROUP_EMIS_ANY | LX_GROUP_EMIS_ALL)) em = true;
+                        else if (emis == 0u) em = false;
+                        else em = (lx_gptr(S.inst)[inst].flags & LX_INST_EMISSIVE) != 0u;
+                        if (MODE == 1 ? em : !em) continue;
+                        if (inst_flags & LX_INST_ALPHA) {
+                            const LX_GLOBAL LxInstance& I = lx_gptr(S.inst)[inst];
+                            if ((I.flags & LX_INST_ALPHA) && !lx_any_hit_accept(S, I, inst, prim, u, v, wd, key, iteration))
+                                continue;
+                        }
+                        best.t = t;
+                        best.inst = inst;
+                        best.prim = prim;
+                        best.u = u;
+                        best.v = v;
+                        found = true;
+                        if (MODE == 1) return true;
+                    }
+                    pop = true;
+                }
+            } else {
+                float tcut = (best.inst == 0xffffffffu ? tmax : best.t) * LX_CULL_MARGIN;
+                float tn[W];
+                uint32_t cs[W];
+                uint32_t m = lx_boxw<W>(S.nodes, node, sl, tcut, tn, cs);
+                if (CNT) work->nodes++;
+                if (m == 0u) pop = true;
+                else node = lx_order_children<W>(tn, cs, m, st, overflow);
+            }
+        }
+        if (pop && !lx_pop_next(st, node, in_blas, o, d, sl, wo, wd, iwd)) break;
+    }
+    return found;
+}
+
+/* The direct-light shadow ray (so, ld, tmax) as far as shading can tell:
+ * its closest hit if that emits, otherwise a miss. */
+template <bool CNT, int W>
+LXD LxHitD lx_trace_dl(const LxDevScene& S, f3 so, f3 ld, float tmax, uint32_t key, uint32_t iteration,
+                       LxStack& st, uint32_t* overflow, LxWork* work) {
+    LxHitD none;
+    none.t = __builtin_inff();
+    none.inst = 0xffffffffu;
+    none.prim = 0xffffffffu;
+    none.u = none.v = 0.0f;
+    LxHitD dark, light;
+    const bool occ = lx_trace_q<1, CNT, W>(S, S.tlas_root, so, ld, tmax, none, key, iteration, LX_INST_ALPHA, st,
+                                           overflow, work, dark);
+    if (S.emis_tlas_root == LX_CHILD_EMPTY ||
+        !lx_trace_q<2, CNT, W>(S, S.emis_tlas_root, so, ld, tmax, occ ? dark : none, key, iteration, LX_INST_ALPHA,
+                               st, overflow, work, light))
+        return none; /* no emitting candidate before the first occluder found */
+    /* an emitting candidate before it: the closest hit is that one unless a
+     * non-emitting candidate comes first */
+    if (occ && lx_trace_q<1, CNT, W>(S, S.tlas_root, so, ld, tmax, light, key, iteration, LX_INST_ALPHA, st, overflow,
+                                     work, dark))
+        return none;
+    return light;
+}
+
 /* Two rays of one lane in one traversal loop: first (when has0) an any-hit
  * ray (o0, d0, tmax0), then (when has1) a closest-hit ray (o1, d1, tmax1);
  * at least one of them must be present.  A lane that

@@ -62,7 +62,8 @@ struct alignas(16) LxInstance {
     uint32_t num_tris;
     uint32_t has_n_uv; /* bit 0: normals, bit 1: uvs */
 };
-#define LX_INST_ALPHA 1u /* material needs the any-hit alpha test */
+#define LX_INST_ALPHA 1u    /* material needs the any-hit alpha test */
+#define LX_INST_EMISSIVE 2u /* material emits: emission colour x strength not all zero */
 
 /* A TLAS leaf: instances that the traversal enters together.  Instances with
  * one bitwise-identical transform whose meshes have no other user share one
@@ -72,11 +73,14 @@ struct alignas(16) LxInstance {
 struct alignas(16) LxGroup {
     float inv[12];      /* world -> object of every member */
     uint32_t blas_root;
-    uint32_t flags;     /* LX_INST_ALPHA if any member needs the any-hit alpha test */
+    uint32_t flags;     /* LX_INST_ALPHA if any member needs the any-hit alpha test;
+                           LX_GROUP_EMIS_ANY / _ALL if some / every member emits */
     uint32_t inst;      /* the member instance, or LX_GROUP_TAGGED */
     uint32_t pad;
 };
 #define LX_GROUP_TAGGED 0xffffffffu
+#define LX_GROUP_EMIS_ANY 2u
+#define LX_GROUP_EMIS_ALL 4u
 
 struct LxTexDesc {
     const void* data;
@@ -102,6 +106,7 @@ struct LxDevScene {
     const float* tri_cdf;
     const uint32_t* tri_cdf_off;
     uint32_t tlas_root;
+    uint32_t emis_tlas_root; /* TLAS over the groups with an emitting member (LX_CHILD_EMPTY: none) */
     uint32_t num_inst;
     uint32_t num_emissive;
     float total_area;
