@@ -96,7 +96,8 @@ int lx_simulate_rounds(const std::vector<uint32_t>& npix, const std::vector<int3
     return rounds;
 }
 
-uint32_t lx_batch_path_cap(lx_context* ctx, const std::vector<uint32_t>& npix, bool dmis, uint64_t extra_bytes) {
+uint32_t lx_batch_path_cap(lx_context* ctx, const std::vector<uint32_t>& npix, bool dmis, uint64_t extra_bytes,
+                           bool paths_in_memory) {
     uint64_t total = 0, mx = 0;
     for (uint32_t p : npix) {
         total += p;
@@ -105,7 +106,7 @@ uint32_t lx_batch_path_cap(lx_context* ctx, const std::vector<uint32_t>& npix, b
     /* bytes per path slot: two path buffers + records (+ emitter records) + spill share */
     uint64_t per = 2 * 15 * 4 + (11 + 3 + (dmis ? 10 : 0)) * 4;
     uint64_t free_b = ctx->budget > ctx->in_use.load() + extra_bytes ? ctx->budget - ctx->in_use.load() - extra_bytes : 0;
-    uint64_t cap = std::min<uint64_t>(total, free_b / per / 2);
+    uint64_t cap = paths_in_memory ? std::min<uint64_t>(total, free_b / per / 2) : total;
     if (ctx->tuning.max_paths_per_round) cap = std::min<uint64_t>(cap, ctx->tuning.max_paths_per_round);
     cap = std::min<uint64_t>(cap, (1ull << 31) - 1);
     LX_REQUIRE(cap >= mx, LX_ERR_BUDGET, "HBM budget too small for the largest job's paths");
@@ -144,7 +145,11 @@ extern "C" lx_status lx_batch_run(lx_context* ctx, const lx_job* jobs, uint32_t 
         LX_REQUIRE(accum_pix < (1ull << 32), LX_ERR_INVALID, "batch too large");
         DevBuf d_jobs, d_spp, d_cnt, d_sched, d_spill, d_accum;
         d_accum.alloc(ctx, accum_pix * 12);
-        uint32_t cap = lx_batch_path_cap(ctx, npix, any_dmis, (uint64_t)lx_spill_words(ctx->num_cus * 16u) * 4 + (1u << 20));
+        /* a round whose tail kernel makes the camera paths keeps no path in
+         * memory: no path buffers, and the budget goes to the colour pool */
+        const bool paths_mem = !lx_tail_makes_camera(ctx, max_depth);
+        uint32_t cap = lx_batch_path_cap(ctx, npix, any_dmis, (uint64_t)lx_spill_words(ctx->num_cus * 16u) * 4 + (1u << 20),
+                                         paths_mem);
         /* wide rounds (tuning.round_frames = R > 1): each round takes up to R
          * samples of every job as R copies of it (copy k: samples k, k + R,
          * ... into region k of a colour pool added in sample order), when the
@@ -163,7 +168,8 @@ extern "C" lx_status lx_batch_run(lx_context* ctx, const lx_job* jobs, uint32_t 
             try {
                 uint32_t capw = lx_batch_path_cap(ctx, npw, any_dmis,
                                                   (uint64_t)lx_spill_words(ctx->num_cus * 16u) * 4 + (1u << 20) +
-                                                      accum_pix * R * 12);
+                                                      accum_pix * R * 12,
+                                                  paths_mem);
                 wide = (uint64_t)capw >= accum_pix * R;
                 if (wide) {
                     d_colw.alloc(ctx, accum_pix * R * 12);
@@ -175,7 +181,7 @@ extern "C" lx_status lx_batch_run(lx_context* ctx, const lx_job* jobs, uint32_t 
             }
         }
         WaveBufs wb;
-        wb.alloc(ctx, cap, any_dmis, true, any_tan);
+        if (paths_mem) wb.alloc(ctx, cap, any_dmis, true, any_tan);
         uint32_t waves = (uint32_t)std::max(1, ctx->tuning.waves_per_cu);
         uint32_t tg = std::min((uint32_t)ctx->num_cus * std::max(1u, waves * 64u / LX_TRACE_BLOCK),
                                std::max(1u, (cap + LX_TRACE_BLOCK - 1) / LX_TRACE_BLOCK));
@@ -226,7 +232,7 @@ extern "C" lx_status lx_batch_run(lx_context* ctx, const lx_job* jobs, uint32_t 
         std::vector<int32_t> hsp[2];
         if (overlap) {
             try {
-                wb1.alloc(ctx, cap, any_dmis, true, any_tan);
+                if (paths_mem) wb1.alloc(ctx, cap, any_dmis, true, any_tan);
                 d_spill1.alloc(ctx, d_spill.bytes);
                 d_cnt1.alloc(ctx, sizeof(LxCounters));
                 d_sched1.alloc(ctx, sizeof(LxSched));

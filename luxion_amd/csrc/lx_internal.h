@@ -323,6 +323,10 @@ void lx_counters_add(LxCounters& a, const LxCounters& b);
  * (tuning.wavefront_bounces; negative or above max_depth = all of them,
  * 0 = the tail kernel runs every bounce) */
 int lx_wavefront_bounces(const lx_context* ctx, int max_depth);
+/* Whether a round under the context's tuning starts with the tail kernel at
+ * bounce 0 in slot order, so that it makes the camera paths itself: such a
+ * round reads and writes no path buffers (WaveBufs stay unallocated). */
+bool lx_tail_makes_camera(const lx_context* ctx, int max_depth);
 
 
 
@@ -338,5 +342,8 @@ uint32_t lx_job_flags(int32_t debug_mode, int32_t shading_model);
 LxJob lx_make_dev_job(const lx_job& j, uint32_t accum_off);
 int lx_simulate_rounds(const std::vector<uint32_t>& npix, const std::vector<int32_t>& spp, uint32_t cap,
                        std::vector<int>* finish_round);
-uint32_t lx_batch_path_cap(lx_context* ctx, const std::vector<uint32_t>& npix, bool dmis, uint64_t extra_bytes);
+/* Paths one round of a batch may hold: the budget's share for path buffers
+ * when the round keeps paths in memory, else every pixel of the batch. */
+uint32_t lx_batch_path_cap(lx_context* ctx, const std::vector<uint32_t>& npix, bool dmis, uint64_t extra_bytes,
+                           bool paths_in_memory = true);
 #endif

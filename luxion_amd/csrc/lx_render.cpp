@@ -102,7 +102,7 @@ void lx_enqueue_round(lx_context* ctx, LxJob* d_jobs, const int32_t* d_spp, uint
     /* a tail kernel that starts at bounce 0 with the queue in slot order makes
      * the camera paths itself (the same rays, with no path-state round trip
      * through memory); otherwise the camera kernel writes them first */
-    const bool tail_camera = wf == 0 && wf < max_depth && !(sm & LX_SORT_RAYS);
+    const bool tail_camera = lx_tail_makes_camera(ctx, max_depth);
     LX_HIP(lx_launch_begin_round(d_jobs, d_spp, njobs, d_cnt, d_sched, path_cap, s));
     if (!tail_camera) LX_HIP(lx_launch_camera(d_jobs, d_sched, d_cnt, wb.A, cam_grid, s, tile, band));
     LxPathBuf* cur = &wb.A;
@@ -161,4 +161,9 @@ int lx_wavefront_bounces(const lx_context* ctx, int max_depth) {
     int wf = ctx->tuning.wavefront_bounces;
     if (wf < 0 || wf > max_depth) wf = max_depth;
     return wf;
+}
+
+bool lx_tail_makes_camera(const lx_context* ctx, int max_depth) {
+    const int wf = lx_wavefront_bounces(ctx, max_depth);
+    return wf == 0 && wf < max_depth && !(ctx->tuning.sort_mode & LX_SORT_RAYS);
 }
