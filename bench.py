@@ -493,7 +493,8 @@ def main():
         batched = dict(bres)
         batched.update({
             "form": "one launch per bounce for all jobs; rounds captured in a hipGraph and replayed; up to %d "
-                    "samples of every job per round; autotuned launch knobs" % WIDE_FRAMES,
+                    "samples of every job per round (at most 2048 job copies per round); autotuned launch "
+                    "knobs" % WIDE_FRAMES,
             "autotuned": {k: best[k] for k in ("block_size", "bvh_width", "queue_chunk", "waves_per_cu",
                                                "sort_mode", "pixel_tile")},
             "per_job_one_sample_per_launch_s": per1["per_job_s"],
