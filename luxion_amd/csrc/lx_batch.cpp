@@ -269,7 +269,7 @@ extern "C" lx_status lx_batch_run(lx_context* ctx, const lx_job* jobs, uint32_t 
                 LX_HIP(lx_launch_fill_u32(d_colw.as<uint32_t>(), accum_pix * R * 3, 0x80000000u, s));
                 lx_enqueue_round(ctx, d_jobsw.as<LxJob>(), d_sppw.as<int32_t>(), n * R, d_cnt.as<LxCounters>(),
                                  d_sched.as<LxSched>(), cap, wb, d_colw.as<float>(), d_spill.as<uint32_t>(), max_depth,
-                                 tg, sg, cg, width, uscene, nullptr, false, nullptr, any_dmis);
+                                 tg, sg, cg, width, uscene, nullptr, false, nullptr, any_dmis, true);
                 LX_HIP(lx_launch_add_regions(d_accum.as<float>(), d_colw.as<float>(), accum_pix * 3, R, s));
             } else {
                 lx_enqueue_round(ctx, d_jobs.as<LxJob>(), d_spp.as<int32_t>(), n, d_cnt.as<LxCounters>(),
@@ -338,7 +338,7 @@ extern "C" lx_status lx_batch_run(lx_context* ctx, const lx_job* jobs, uint32_t 
                 if (k >= 2) LX_HIP(hipStreamWaitEvent(ps[p], e_free[p], 0));
                 LX_HIP(lx_launch_fill_u32(d_col[p].as<uint32_t>(), accum_pix * 3, 0x80000000u, ps[p]));
                 lx_enqueue_round(ctx, pj[p], pspp[p], n, pcnt[p], psch[p], cap, *pwb[p], d_col[p].as<float>(),
-                                 pspill[p], max_depth, tg, sg, cg, width, uscene, nullptr, false, ps[p], any_dmis);
+                                 pspill[p], max_depth, tg, sg, cg, width, uscene, nullptr, false, ps[p], any_dmis, true);
                 LX_HIP(hipEventRecord(e_done[p], ps[p]));
                 LX_HIP(hipStreamWaitEvent(s, e_done[p], 0));
                 LX_HIP(lx_launch_add_f32(d_accum.as<float>(), d_col[p].as<float>(), accum_pix * 3, s));

@@ -508,7 +508,7 @@ static void iterate_overlapped(lx_render* r, uint32_t first_iter, uint32_t n, in
                          (O ? O->counters : r->counters).as<LxCounters>(), (O ? O->sched : r->sched).as<LxSched>(),
                          r->npix, O ? O->wb : r->wb, r->color[p].as<float>(), (O ? O->spill : r->spill).as<uint32_t>(),
                          r->depth, r->trace_grid, r->shade_grid, r->cam_grid, r->scene->dev.bvh_width,
-                         r->scene->d_scene.as<LxDevScene>(), nullptr, false, s, r->opts.dmis != 0);
+                         r->scene->d_scene.as<LxDevScene>(), nullptr, false, s, r->opts.dmis != 0, true);
         LX_HIP(hipEventRecord(r->ev_done[p], s));
         LX_HIP(hipStreamWaitEvent(ms, r->ev_done[p], 0));
         LX_HIP(lx_launch_add_f32(r->accum.as<float>(), r->color[p].as<float>(), nf, ms));
@@ -629,7 +629,7 @@ static void iterate_wide(lx_render* r, uint32_t first_iter, uint32_t n, int roun
         lx_enqueue_round(ctx, W.jobs.as<LxJob>(), W.job_spp.as<int32_t>(), R, cnt + p, W.sched.as<LxSched>(),
                          R * r->npix, W.wb, W.color.as<float>(), W.spill.as<uint32_t>(), r->depth, r->wide_grid[0],
                          r->wide_grid[1], r->wide_grid[2], r->scene->dev.bvh_width,
-                         r->scene->d_scene.as<LxDevScene>(), e ? e + 1 : nullptr, false, s, r->opts.dmis != 0);
+                         r->scene->d_scene.as<LxDevScene>(), e ? e + 1 : nullptr, false, s, r->opts.dmis != 0, true);
         if (e) LX_HIP(hipEventRecord(e[per + 1], s));
         LX_HIP(hipEventRecord(r->ev_done[p], s));
         LX_HIP(hipStreamWaitEvent(ms, r->ev_done[p], 0));

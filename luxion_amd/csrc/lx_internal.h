@@ -315,7 +315,9 @@ void lx_enqueue_round(lx_context* ctx, LxJob* d_jobs, const int32_t* d_spp, uint
                                         [2*max_depth] the round's end; other slots are not recorded */,
                       bool count = false, hipStream_t stream = nullptr /* NULL: the context's stream */,
                       bool dl_query = true /* some job may have -dmis: the tail kernel whose direct-light
-                                              rays are occlusion queries */);
+                                              rays are occlusion queries */,
+                      bool colour_fresh = false /* d_accum was filled with -0.0 for this round (one path
+                                                   per pixel lands in it): the tail kernel stores colours */);
 uint32_t lx_spill_words(uint32_t trace_grid);
 /* a += b for the cumulative fields (ray, bounce and work counts, overflow flag) */
 void lx_counters_add(LxCounters& a, const LxCounters& b);

@@ -69,7 +69,10 @@ hipError_t lx_launch_finish(const LxJob* jobs, const LxPathBuf& P, LxCounters* C
                             const LxSched* cam_sched = nullptr, uint32_t tile = 1, uint32_t band = 0,
                             /* some job has -dmis: the kernel whose direct-light rays
                              * are occlusion queries (same colours) */
-                            bool dl_query = false);
+                            bool dl_query = false,
+                            /* accum was filled with -0.0 for this round and gets one
+                             * path per pixel: colours are stored, not added */
+                            bool colour_fresh = false);
 hipError_t lx_launch_trace_rays(const LxDevScene* S, const lx_ray* rays, uint32_t n, int any, int alpha,
                                 uint32_t iteration, lx_hit* hits, uint8_t* occ, uint32_t* spill, uint32_t* overflow,
                                 uint32_t grid, int width, hipStream_t s);

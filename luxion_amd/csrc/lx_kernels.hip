@@ -319,12 +319,21 @@ LXD void lx_load_rec(const LxJob& J, const LxDevScene& S, const LxRecBuf& R, uin
     }
 }
 
-LXD void lx_accumulate(float* __restrict__ accum, const LxJob& J, uint32_t pix, f3 col) {
+/* store: the buffer was filled with -0.0 for this round and receives one
+ * path per pixel, so the add -0 + c would give c's bits: the colour is
+ * written without reading the buffer */
+LXD void lx_accumulate(float* __restrict__ accum, const LxJob& J, uint32_t pix, f3 col, bool store = false) {
     /* finalGather: the path's colour is final */
     LX_GLOBAL float* a = (LX_GLOBAL float*)(accum + 3u * (size_t)(J.accum_off + pix));
-    a[0] += col.x;
-    a[1] += col.y;
-    a[2] += col.z;
+    if (store) {
+        a[0] = col.x;
+        a[1] = col.y;
+        a[2] = col.z;
+    } else {
+        a[0] += col.x;
+        a[1] += col.y;
+        a[2] += col.z;
+    }
 }
 
 __global__ __launch_bounds__(LX_SHADE_BLOCK) void k_shade(const LxJob* __restrict__ jobs, LxPathBuf P, LxPathBuf Q,
@@ -435,7 +444,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC > 0 ?
 void k_finish(const LxJob* __restrict__ jobs, LxPathBuf P, LxCounters* __restrict__ C, int bounce0,
               uint32_t* __restrict__ spill, float* __restrict__ accum, uint32_t chunk,
               const LxDevScene* __restrict__ uscene, const uint32_t* __restrict__ order,
-              const LxSched* __restrict__ cam_sched, uint32_t cam_tile, uint32_t cam_band) {
+              const LxSched* __restrict__ cam_sched, uint32_t cam_tile, uint32_t cam_band, uint32_t acc_store) {
     __shared__ uint32_t s_stack[LX_STACK * BLOCK];
     /* per-lane path state that only shading reads: it waits in an LDS
      * column while the lane traverses, instead of holding registers */
@@ -518,7 +527,7 @@ void k_finish(const LxJob* __restrict__ jobs, LxPathBuf P, LxCounters* __restric
                 pend = false;
             }
             if (flush) {
-                lx_accumulate(accum, J, pix, col);
+                lx_accumulate(accum, J, pix, col, acc_store != 0u);
                 flush = false;
             } else {
                 int depth = b + 1;
@@ -545,7 +554,7 @@ void k_finish(const LxJob* __restrict__ jobs, LxPathBuf P, LxCounters* __restric
                         ps.put_col(col);
                         ps.put_pend_col(pend_col);
                     } else {
-                        lx_accumulate(accum, J, pix, col);
+                        lx_accumulate(accum, J, pix, col, acc_store != 0u);
                     }
                 }
             }
@@ -888,15 +897,18 @@ struct LxFinishArgs {
     const LxSched* cam;
     uint32_t tile, band;
     bool dlq;
+    bool store; /* colour buffer filled with -0.0 for this round */
 };
 template <int BLOCK, int OCC, int W, int UNI>
 static void lx_finish_at(const LxFinishArgs& a, uint32_t grid, hipStream_t s) {
     if (OCC > 0 && a.dlq)
         hipLaunchKernelGGL((k_finish<false, BLOCK, OCC, W, UNI, true>), dim3(grid), dim3(BLOCK), 0, s, a.jobs, *a.P,
-                           a.C, a.bounce0, a.spill, a.accum, a.chunk, a.us, a.order, a.cam, a.tile, a.band);
+                           a.C, a.bounce0, a.spill, a.accum, a.chunk, a.us, a.order, a.cam, a.tile, a.band,
+                           (uint32_t)a.store);
     else
         hipLaunchKernelGGL((k_finish<false, BLOCK, OCC, W, UNI, false>), dim3(grid), dim3(BLOCK), 0, s, a.jobs, *a.P,
-                           a.C, a.bounce0, a.spill, a.accum, a.chunk, a.us, a.order, a.cam, a.tile, a.band);
+                           a.C, a.bounce0, a.spill, a.accum, a.chunk, a.us, a.order, a.cam, a.tile, a.band,
+                           (uint32_t)a.store);
 }
 template <int W, int UNI>
 static void lx_finish_w(const LxFinishArgs& a, uint32_t threads, int block, int occ, hipStream_t s) {
@@ -911,26 +923,27 @@ hipError_t lx_launch_finish(const LxJob* jobs, const LxPathBuf& P, LxCounters* C
                             float* accum, uint32_t threads, int block, int occ, int width, int chunk,
                             const LxDevScene* uscene, bool single_job, hipStream_t s, bool count,
                             const uint32_t* order, const LxSched* cam_sched, uint32_t tile, uint32_t band,
-                            bool dl_query) {
+                            bool dl_query, bool colour_fresh) {
     /* threads = grid * block lanes, the same for every shape, so one spill
      * allocation serves them all */
     uint32_t c = chunk < 1 ? 1u : (chunk > 64 ? 64u : (uint32_t)chunk);
-    LxFinishArgs a{jobs, &P, C, bounce0, spill, accum, c, uscene, order, cam_sched, tile, band, dl_query};
+    LxFinishArgs a{jobs, &P, C, bounce0, spill, accum, c, uscene, order, cam_sched, tile, band, dl_query, colour_fresh};
+    const uint32_t st = colour_fresh ? 1u : 0u;
     dim3 g(threads / 128), b(128);
     if (count) {
         /* the counting replay: same paths and rays, per-lane scene and job */
         if (width == 8 && dl_query)
             hipLaunchKernelGGL((k_finish<true, 128, 0, 8, 0, true>), g, b, 0, s, jobs, P, C, bounce0, spill, accum, c,
-                               uscene, order, cam_sched, tile, band);
+                               uscene, order, cam_sched, tile, band, st);
         else if (width == 8)
             hipLaunchKernelGGL((k_finish<true, 128, 0, 8, 0, false>), g, b, 0, s, jobs, P, C, bounce0, spill, accum, c,
-                               uscene, order, cam_sched, tile, band);
+                               uscene, order, cam_sched, tile, band, st);
         else if (dl_query)
             hipLaunchKernelGGL((k_finish<true, 128, 0, 4, 0, true>), g, b, 0, s, jobs, P, C, bounce0, spill, accum, c,
-                               uscene, order, cam_sched, tile, band);
+                               uscene, order, cam_sched, tile, band, st);
         else
             hipLaunchKernelGGL((k_finish<true, 128, 0, 4, 0, false>), g, b, 0, s, jobs, P, C, bounce0, spill, accum, c,
-                               uscene, order, cam_sched, tile, band);
+                               uscene, order, cam_sched, tile, band, st);
     } else if (uscene && single_job) {
         if (width == 8) lx_finish_w<8, 2>(a, threads, block, occ, s);
         else lx_finish_w<4, 2>(a, threads, block, occ, s);

@@ -94,7 +94,8 @@ void WaveBufs::alloc(lx_context* ctx, uint32_t c, bool dmis, bool emis, bool tan
 void lx_enqueue_round(lx_context* ctx, LxJob* d_jobs, const int32_t* d_spp, uint32_t njobs, LxCounters* d_cnt,
                       LxSched* d_sched, uint32_t path_cap, WaveBufs& wb, float* d_accum, uint32_t* d_spill,
                       int max_depth, uint32_t trace_grid, uint32_t shade_grid, uint32_t cam_grid, int width,
-                      const LxDevScene* uscene, hipEvent_t* ev, bool count, hipStream_t stream, bool dl_query) {
+                      const LxDevScene* uscene, hipEvent_t* ev, bool count, hipStream_t stream, bool dl_query,
+                      bool colour_fresh) {
     hipStream_t s = stream ? stream : ctx->stream;
     int wf = lx_wavefront_bounces(ctx, max_depth);
     const int sm = ctx->tuning.sort_mode;
@@ -131,7 +132,7 @@ void lx_enqueue_round(lx_context* ctx, LxJob* d_jobs, const int32_t* d_spp, uint
         LX_HIP(lx_launch_finish(d_jobs, *cur, d_cnt, wf, d_spill, d_accum, trace_grid * LX_TRACE_BLOCK,
                                 ctx->tuning.block_size, ctx->tuning.tail_min_waves, width, ctx->tuning.queue_chunk,
                                 uscene, njobs == 1, s, count, of, tail_camera ? d_sched : nullptr, tile, band,
-                                dl_query));
+                                dl_query, colour_fresh));
         if (ev && 2 * wf + 1 < 2 * max_depth) LX_HIP(hipEventRecord(ev[2 * wf + 1], s));
     }
     if (ev) LX_HIP(hipEventRecord(ev[2 * max_depth], s));

@@ -415,7 +415,7 @@ void lx_server::loop_overlapped() {
                 LX_HIP(lx_launch_fill_ranges(P.col.as<uint32_t>(), P.rng.as<uint32_t>(), P.nr, 0x80000000u, P.s));
                 lx_enqueue_round(ctx, P.jobs.as<LxJob>(), P.spp.as<int32_t>(), slots, P.cnt.as<LxCounters>(),
                                  P.sched.as<LxSched>(), cap, P.wb, P.col.as<float>(), P.spill.as<uint32_t>(),
-                                 max_depth, tg, sg, cg, width, nullptr, nullptr, false, P.s);
+                                 max_depth, tg, sg, cg, width, nullptr, nullptr, false, P.s, true, true);
                 LX_HIP(hipEventRecord(P.done, P.s));
                 LX_HIP(hipStreamWaitEvent(ms, P.done, 0));
                 LX_HIP(lx_launch_add_ranges(d_accum.as<float>(), P.col.as<float>(), P.rng.as<uint32_t>(), P.nr, ms));

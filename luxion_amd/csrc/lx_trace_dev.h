@@ -460,8 +460,10 @@ LXD LxHitD lx_trace(const LxDevScene& S, f3 wo, f3 wd, float tmax, uint32_t key,
  * but shadePath uses its hit only through the hit material's emission
  * (shade_path.cu:246-289), which is exactly zero on a material whose
  * emission colour x strength is zero in every channel: such a closest hit
- * adds +0 to a colour that is never -0, i.e. nothing.  So the ray is answered
- * by two cheaper queries with the same result for the colour:
+ * adds +0 to a colour that is never -0, i.e. nothing, as long as the term's
+ * other factors are finite (where one overflows, the closest-hit form's
+ * 0 x inf would make the pixel NaN; this form skips the term).  So the ray is
+ * answered by two cheaper queries with the same result for the colour:
  *   DARK  (lx_trace_q<1>): any accepted candidate on a non-emitting instance
  *         that comes before `bound` in the lexicographic (t, instance,
  *         primitive) order -- an occlusion query that stops at the first one;
