@@ -199,25 +199,44 @@ LXD bool lx_shade_path(const LxJob& J, const LxDevScene& S, const LxRec& R, int 
     f3 N = mk3(0, 0, 0);
     f3 sample_dir = mk3(0, 0, 0);
     bool has_exr = S.has_env != 0;
+    /* The environment terms (the MIS weight of an emitter hit, the
+     * environment light sample at a hit, a miss's radiance) are requested
+     * per lane in the branches and evaluated after them, once for every lane
+     * of the wave: a wave whose lanes both hit and miss runs envmap_pdf and
+     * the lookup once instead of once per branch.  Same inputs, same
+     * expressions, same bits. */
+    int env_case = 0; /* 1: emitter hit under -emis, 2: environment light sample, 3: miss */
+    bool want_pe = false, want_L = false;
+    f3 pe_dir = mk3(0, 0, 0), L_dir = mk3(0, 0, 0);
+    f3 ev = mk3(0, 0, 0); /* case 1: the emission; case 2: the BRDF */
+    float e_pb = 0.0f, e_cs = 0.0f;
+    /* the hit's shading terms, used again by the throughput update below */
+    bool shaded = false, legacy = false, is_specular = false;
+    int mt = LX_MATERIAL_MICROFACET;
+    f3 color = mk3(0, 0, 0), normal = mk3(0, 0, 0);
+    float rough = 0.0f, metal = 0.0f, m_ior = 1.0f, m_tr = 0.0f;
+    LxLobes lobes{};
+    LxBrdfView bview{};
     if (t > 0.0f) {
         N = R.N;
         f2 uv = R.uv;
         const lx_material_desc& mat = S.mats[R.mat];
         /* legacy shading (UBER_SHADER 0) dispatches on the material class; the
          * uber shader treats every material as microfacet */
-        const bool legacy = (J.flags & LX_JF_LEGACY) != 0u;
-        const int mt = legacy ? mat.material_type : LX_MATERIAL_MICROFACET;
-        f3 color = lx_get_albedo(S, mat, uv);
+        legacy = (J.flags & LX_JF_LEGACY) != 0u;
+        mt = legacy ? mat.material_type : LX_MATERIAL_MICROFACET;
+        color = lx_get_albedo(S, mat, uv);
         /* the legacy loader gives an emissive material its emission colour as
          * base colour (src/scene/scene_load_gltf.cpp:211-213) */
         if (mt == LX_MATERIAL_EMISSIVE && mat.albedo_tex < 0)
             color = mk3(mat.emission_color[0], mat.emission_color[1], mat.emission_color[2]);
-        f3 normal = N;
+        normal = N;
         if (mat.normal_tex >= 0) normal = lx_get_normal(S, mat, N, R.Tg, uv);
-        float rough, metal;
         lx_get_mr(S, mat, uv, rough, metal);
+        m_ior = mat.ior;
+        m_tr = mat.transmission;
         f3 emission = lx_get_emission(S, mat, uv);
-        bool is_specular = legacy ? (mt == LX_MATERIAL_SPECULAR || mt == LX_MATERIAL_GLASS) : rough < 0.05f;
+        is_specular = legacy ? (mt == LX_MATERIAL_SPECULAR || mt == LX_MATERIAL_GLASS) : rough < 0.05f;
         f3 nf = dot(-d, normal) < 0.0f ? -normal : normal;
         const uint32_t dbg = J.flags & LX_JF_DEBUG_MASK;
         if (dbg != 0u) {
@@ -225,49 +244,38 @@ LXD bool lx_shade_path(const LxJob& J, const LxDevScene& S, const LxRec& R, int 
             col = lx_debug_color(S, mat, dbg, color, normal, uv, rough, metal);
             kill = true;
         } else {
+            shaded = true;
             /* the lobe probabilities and the BRDF's view-side terms of this
              * shading point, shared by the sampler and every PDF / BRDF below */
-            LxLobes lobes{};
-            LxBrdfView bview{};
             if (mt == LX_MATERIAL_MICROFACET) {
-                lobes = lx_lobes_of(-d, normal, color, metal, mat.ior, mat.transmission);
-                bview = lx_brdf_view(-d, normal, rough, mat.ior);
-                sample_dir = lx_sample_ct_q(-d, normal, rough, rng, mat.ior, lobes);
+                lobes = lx_lobes_of(-d, normal, color, metal, m_ior, m_tr);
+                bview = lx_brdf_view(-d, normal, rough, m_ior);
+                sample_dir = lx_sample_ct_q(-d, normal, rough, rng, m_ior, lobes);
             }
             else
-                sample_dir = lx_sample_legacy(mt, d, normal, mat.ior, rng);
+                sample_dir = lx_sample_legacy(mt, d, normal, m_ior, rng);
             bool emissive = legacy ? mt == LX_MATERIAL_EMISSIVE : length(emission) > 0.00001f;
             if (J.emis) {
                 if (emissive) {
-                    float w = 1.0f;
+                    env_case = 1;
+                    ev = emission;
                     if (depth > 1 && !spec) {
-                        float pb = last_pdf;
-                        float pe = lx_envmap_pdf(S, d);
-                        w = (pb * pb) / (pb * pb + pe * pe);
+                        want_pe = true;
+                        pe_dir = d;
+                        e_pb = last_pdf;
                     }
-                    col = col + thr * (w * emission);
                     kill = true;
                 } else if ((DEFER || R.env_t < 0.0f) && !is_specular && has_exr) {
                     f3 ed = R.ed;
-                    float pe = lx_envmap_pdf(S, ed);
-                    float pb = mt == LX_MATERIAL_MICROFACET
-                                   ? lx_pdf_q(-d, ed, normal, rough, mat.ior, lobes)
-                                   : lx_legacy_pdf(mt, ed, normal);
-                    float w = (pe * pe) / (pe * pe + pb * pb);
-                    f3 brdf = mt == LX_MATERIAL_MICROFACET
-                                  ? lx_brdf_v(bview, ed, color, metal, mat.transmission)
-                                  : lx_legacy_brdf(mt, color);
-                    f3 L = lx_env_lookup(S, normalize(ed), J.intensity);
-                    float cs = dot(nf, ed);
-                    if (cs > 0.0f && pe > 1e-8f) {
-                        f3 contrib = ((L * brdf) * cs) / pe;
-                        if (DEFER) {
-                            *pend_add = true;
-                            *pend_col = (thr * contrib) * w;
-                        } else {
-                            col = col + (thr * contrib) * w;
-                        }
-                    }
+                    env_case = 2;
+                    want_pe = want_L = true;
+                    pe_dir = ed;
+                    L_dir = normalize(ed);
+                    e_pb = mt == LX_MATERIAL_MICROFACET ? lx_pdf_q(-d, ed, normal, rough, m_ior, lobes)
+                                                        : lx_legacy_pdf(mt, ed, normal);
+                    ev = mt == LX_MATERIAL_MICROFACET ? lx_brdf_v(bview, ed, color, metal, m_tr)
+                                                      : lx_legacy_brdf(mt, color);
+                    e_cs = dot(nf, ed);
                 }
             } else if (J.dmis) {
                 if (emissive) {
@@ -298,11 +306,11 @@ LXD bool lx_shade_path(const LxJob& J, const LxDevScene& S, const LxRec& R, int 
                         float pa = 1.0f / S.total_area;
                         float psa = pa * (dsq / cl);
                         float pb = mt == LX_MATERIAL_MICROFACET
-                                       ? lx_pdf_q(-d, ld, normal, rough, mat.ior, lobes)
+                                       ? lx_pdf_q(-d, ld, normal, rough, m_ior, lobes)
                                        : lx_legacy_pdf(mt, ld, normal);
                         float w = (psa * psa) / (psa * psa + pb * pb);
                         f3 brdf = mt == LX_MATERIAL_MICROFACET
-                                      ? lx_brdf_v(bview, ld, color, metal, mat.transmission)
+                                      ? lx_brdf_v(bview, ld, color, metal, m_tr)
                                       : lx_legacy_brdf(mt, color);
                         f3 Le = lx_get_emission(S, S.mats[R.dlmat], R.dluv);
                         f3 contrib = ((Le * brdf) * cs) / psa;
@@ -313,40 +321,64 @@ LXD bool lx_shade_path(const LxJob& J, const LxDevScene& S, const LxRec& R, int 
                 col = col + thr * emission;
                 kill = true;
             }
-            if (!kill) {
-                /* update_throughput_materials, src/update_throughput_materials.cu:8-67 */
-                if (mt == LX_MATERIAL_MICROFACET) {
-                    /* only the uber build counts a transmission as specular */
-                    bool transmitted = !legacy && dot(-d, normal) * dot(sample_dir, normal) < 0.0f;
-                    bool nspec = is_specular || transmitted;
-                    f3 ct = lx_ct_eval_cos_v(bview, -d, normal, sample_dir, color, rough, metal, mat.ior,
-                                             mat.transmission);
-                    float pdf = lx_pdf_q(-d, sample_dir, normal, rough, mat.ior, lobes);
-                    thr = thr * (ct / pdf);
-                    last_pdf = pdf;
-                    spec = nspec;
-                } else {
-                    last_pdf = lx_legacy_update(mt, d, normal, sample_dir, color, mat.ior, mat.transmission, thr, last_pdf);
-                    spec = is_specular;
-                }
-                if (depth >= 3) {
-                    float mt = lxm_max(thr.x, lxm_max(thr.y, thr.z));
-                    float sv = lxm_clamp(mt, 0.0f, 1.0f);
-                    if (rng.u01() >= sv) kill = true;
-                    else thr = thr / sv;
-                }
-            }
         }
     } else if (has_exr) {
+        env_case = 3;
         f3 dn = normalize(d);
-        f3 L = lx_env_lookup(S, dn, J.intensity);
-        float w = 1.0f;
+        want_L = true;
+        L_dir = dn;
         if (J.emis && depth > 1 && !spec) {
-            float pb = last_pdf;
-            float pe = lx_envmap_pdf(S, dn);
-            w = (pb * pb) / (pb * pb + pe * pe);
+            want_pe = true;
+            pe_dir = dn;
+            e_pb = last_pdf;
         }
+    }
+    /* the environment terms, with the throughput before this bounce's update */
+    float pe = 0.0f;
+    f3 L = mk3(0, 0, 0);
+    if (want_pe) pe = lx_envmap_pdf(S, pe_dir);
+    if (want_L) L = lx_env_lookup(S, L_dir, J.intensity);
+    if (env_case == 1) {
+        float w = 1.0f;
+        if (want_pe) w = (e_pb * e_pb) / (e_pb * e_pb + pe * pe);
+        col = col + thr * (w * ev);
+    } else if (env_case == 2) {
+        float w = (pe * pe) / (pe * pe + e_pb * e_pb);
+        if (e_cs > 0.0f && pe > 1e-8f) {
+            f3 contrib = ((L * ev) * e_cs) / pe;
+            if (DEFER) {
+                *pend_add = true;
+                *pend_col = (thr * contrib) * w;
+            } else {
+                col = col + (thr * contrib) * w;
+            }
+        }
+    } else if (env_case == 3) {
+        float w = 1.0f;
+        if (want_pe) w = (e_pb * e_pb) / (e_pb * e_pb + pe * pe);
         col = col + (thr * L) * w;
+    }
+    if (shaded && !kill) {
+        /* update_throughput_materials, src/update_throughput_materials.cu:8-67 */
+        if (mt == LX_MATERIAL_MICROFACET) {
+            /* only the uber build counts a transmission as specular */
+            bool transmitted = !legacy && dot(-d, normal) * dot(sample_dir, normal) < 0.0f;
+            bool nspec = is_specular || transmitted;
+            f3 ct = lx_ct_eval_cos_v(bview, -d, normal, sample_dir, color, rough, metal, m_ior, m_tr);
+            float pdf = lx_pdf_q(-d, sample_dir, normal, rough, m_ior, lobes);
+            thr = thr * (ct / pdf);
+            last_pdf = pdf;
+            spec = nspec;
+        } else {
+            last_pdf = lx_legacy_update(mt, d, normal, sample_dir, color, m_ior, m_tr, thr, last_pdf);
+            spec = is_specular;
+        }
+        if (depth >= 3) {
+            float mx = lxm_max(thr.x, lxm_max(thr.y, thr.z));
+            float sv = lxm_clamp(mx, 0.0f, 1.0f);
+            if (rng.u01() >= sv) kill = true;
+            else thr = thr / sv;
+        }
     }
     if (t == -1.0f) {
         kill = true;
