@@ -131,18 +131,7 @@ LXD void lx_trace_path_d(const LxJob& J, const LxDevScene& S, f3 o, f3 d, uint32
                                       LX_INST_ALPHA, st, overflow, pend_occluded, wc, wa);
     if (pend) rc.env++;
     if (!has_rad) return;
-    /* sampleDirectLight: RNG keyed on the pre-increment depth.  The samples
-     * do not depend on the trace, so they are drawn after it (fewer registers
-     * live across the traversal loop; the same values). */
-    f3 lp = mk3(0, 0, 0), ed = mk3(0, 0, 0);
-    if (J.dmis || J.emis) {
-        LxRng rng = lx_make_rng(J.iter, (int)pix, bounce, J.seed);
-        if (J.dmis) lp = lx_sample_light_point(S, rng);
-        if (J.emis) ed = lx_sample_env_dir(S, rng);
-    }
     rc.rad++;
-    if (J.dmis) out.light(lp);
-    if (J.emis) out.env_dir(ed);
     /* the miss values first, overwritten below on hits: stores of the same
      * constant to different fields in different branches would otherwise
      * be merged into one store through a selected address, which keeps the
@@ -151,6 +140,19 @@ LXD void lx_trace_path_d(const LxJob& J, const LxDevScene& S, f3 o, f3 d, uint32
     if (J.dmis) out.dl_miss();
     out.env(-0.5f);
     if (h.inst != 0xffffffffu) {
+        /* sampleDirectLight: RNG keyed on the pre-increment depth, a stream
+         * of its own.  Its samples do not depend on the trace and only a hit
+         * uses them, so they are drawn after it and only on hits (fewer
+         * registers live across the traversal loop, no sampling in waves
+         * whose rays all miss; the same values) */
+        f3 lp = mk3(0, 0, 0), ed = mk3(0, 0, 0);
+        if (J.dmis || J.emis) {
+            LxRng rng = lx_make_rng(J.iter, (int)pix, bounce, J.seed);
+            if (J.dmis) lp = lx_sample_light_point(S, rng);
+            if (J.emis) ed = lx_sample_env_dir(S, rng);
+        }
+        if (J.dmis) out.light(lp);
+        if (J.emis) out.env_dir(ed);
         bool want_t = lx_gptr(S.mats)[lx_gptr(S.inst)[h.inst].material].normal_tex >= 0;
         f3 N, Tg;
         f2 uv;

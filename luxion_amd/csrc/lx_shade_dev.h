@@ -378,7 +378,23 @@ LXD f3 lx_cos_hemisphere(f3 n, LxRng& rng) {
     f3 p2 = normalize(cross(n, p1));
     return ((up * n) + ((lxm_cos(around) * over) * p1)) + ((lxm_sin(around) * over) * p2);
 }
-/* sampleCookTorrance, cook_torrance.cu:156-224, with the lobe probabilities given */
+/* lx_cos_hemisphere with its draws' sqrt(u1) and the sine and cosine of its
+ * azimuth u2 * 2pi given */
+LXD f3 lx_cos_hemisphere_at(f3 n, float up, float c_around, float s_around) {
+    float over = lxm_sqrt(1.0f - up * up);
+    f3 dnn;
+    if (lxm_abs(n.x) < LX_SQRT_THIRD) dnn = mk3(1, 0, 0);
+    else if (lxm_abs(n.y) < LX_SQRT_THIRD) dnn = mk3(0, 1, 0);
+    else dnn = mk3(0, 0, 1);
+    f3 p1 = normalize(cross(n, dnn));
+    f3 p2 = normalize(cross(n, p1));
+    return ((up * n) + ((c_around * over) * p1)) + ((s_around * over) * p2);
+}
+/* sampleCookTorrance, cook_torrance.cu:156-224, with the lobe probabilities
+ * given.  Both lobes draw two more numbers in the same order and take
+ * sqrt(u1) and the sine and cosine of the same azimuth 2pi * u2 (2.0f * pi_f
+ * is the float 2pi bit for bit), so these are computed once, before the
+ * lobe branch: a wave whose lanes pick both lobes evaluates them once. */
 LXD f3 lx_sample_ct_q(f3 wo, f3 n, float rough, LxRng& rng, float ior, const LxLobes& q) {
     float r = rng.u01();
     ior = lxm_max(ior, 1.0001f);
@@ -388,13 +404,15 @@ LXD f3 lx_sample_ct_q(f3 wo, f3 n, float rough, LxRng& rng, float ior, const LxL
     f3 nf = enter ? n : -n;
     float ei = enter ? 1.0f : ior, et = enter ? ior : 1.0f;
     const float pD = q.pD, pS = q.pS;
-    if (r > pS && r <= pS + pD) return lx_cos_hemisphere(nf, rng);
     float x1 = rng.u01();
     float x2 = rng.u01();
-    float th = lxm_atan((alpha * lxm_sqrt(x1)) / lxm_sqrt(1.0f - x1));
+    float sx1 = lxm_sqrt(x1);
     float ph = 2.0f * LX_PI_F * x2;
+    float cph = lxm_cos(ph), sph = lxm_sin(ph);
+    if (r > pS && r <= pS + pD) return lx_cos_hemisphere_at(nf, sx1, cph, sph);
+    float th = lxm_atan((alpha * sx1) / lxm_sqrt(1.0f - x1));
     float st = lxm_sin(th);
-    f3 h = normalize(mk3(1.0f * st * lxm_cos(ph), 1.0f * st * lxm_sin(ph), 1.0f * lxm_cos(th)));
+    f3 h = normalize(mk3(1.0f * st * cph, 1.0f * st * sph, 1.0f * lxm_cos(th)));
     f3 t = normalize(lxm_abs(nf.z) < 0.999f ? cross(mk3(0, 0, 1), nf) : cross(mk3(1, 0, 0), nf));
     f3 b = cross(nf, t);
     f3 hw = ((h.x * t) + (h.y * b)) + (h.z * nf);
