@@ -195,7 +195,9 @@ LXD bool lx_shade_path(const LxJob& J, const LxDevScene& S, const LxRec& R, int 
                        f3& thr, f3& col, float& last_pdf, bool& spec, uint32_t pix, bool* pend_add = nullptr,
                        f3* pend_col = nullptr) {
     if (DEFER) *pend_add = false;
-    LxRng rng = lx_make_rng(J.iter, (int)pix, depth, J.seed);
+    /* only a hit draws from the bounce's stream (sampling, roulette) */
+    LxRng rng;
+    rng.x = 1u;
     bool kill = false;
     float t = R.t;
     f3 N = mk3(0, 0, 0);
@@ -220,6 +222,7 @@ LXD bool lx_shade_path(const LxJob& J, const LxDevScene& S, const LxRec& R, int 
     LxLobes lobes{};
     LxBrdfView bview{};
     if (t > 0.0f) {
+        rng = lx_make_rng(J.iter, (int)pix, depth, J.seed);
         N = R.N;
         f2 uv = R.uv;
         const lx_material_desc& mat = S.mats[R.mat];
