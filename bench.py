@@ -42,12 +42,12 @@ SHADE_BYTES = 2 * 48 + 3 * 16 + 12 + 36 + 24 + 64 + 32 + 2 * 32 + 4
 # profiles of the same command time the same kernels): tail kernel for every
 # bounce (making its camera paths itself), 128-lane (HDRI) / 256-lane
 # (interior) blocks, 16 waves per CU, BVH4, no queue sort, refills once 48
-# (HDRI) / 16 (interior) lanes of a wave are idle; the headline renders one
+# (HDRI) / 8 (interior) lanes of a wave are idle; the headline renders one
 # frame per round with one more round in flight (HDRI) or two (interior)
 WORKLOAD_TUNING = {
     "hdri": dict(wavefront_bounces=0, block_size=128, waves_per_cu=16, queue_chunk=48, tail_min_waves=4, bvh_width=4,
                  sort_mode=0, pixel_tile=16, tile_band=8, overlap_rounds=1, round_frames=1),
-    "interior": dict(wavefront_bounces=0, block_size=256, waves_per_cu=16, queue_chunk=16, tail_min_waves=4,
+    "interior": dict(wavefront_bounces=0, block_size=256, waves_per_cu=16, queue_chunk=8, tail_min_waves=4,
                      bvh_width=4, sort_mode=0, pixel_tile=8, tile_band=0, overlap_rounds=2, round_frames=1),
 }
 WIDE_FRAMES = 20
