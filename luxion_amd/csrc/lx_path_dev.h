@@ -140,6 +140,12 @@ LXD void lx_trace_path_d(const LxJob& J, const LxDevScene& S, f3 o, f3 d, uint32
     if (J.dmis) out.dl_miss();
     out.env(-0.5f);
     if (h.inst != 0xffffffffu) {
+        bool want_t = lx_gptr(S.mats)[lx_gptr(S.inst)[h.inst].material].normal_tex >= 0;
+        f3 N, Tg;
+        f2 uv;
+        int mat;
+        lx_hit_attrs(S, h, want_t, N, Tg, uv, mat);
+        out.hit(h.t, N, uv, mat, want_t, Tg);
         /* sampleDirectLight: RNG keyed on the pre-increment depth, a stream
          * of its own.  Its samples do not depend on the trace and only a hit
          * uses them, so they are drawn after it and only on hits (fewer
@@ -153,12 +159,6 @@ LXD void lx_trace_path_d(const LxJob& J, const LxDevScene& S, f3 o, f3 d, uint32
         }
         if (J.dmis) out.light(lp);
         if (J.emis) out.env_dir(ed);
-        bool want_t = lx_gptr(S.mats)[lx_gptr(S.inst)[h.inst].material].normal_tex >= 0;
-        f3 N, Tg;
-        f2 uv;
-        int mat;
-        lx_hit_attrs(S, h, want_t, N, Tg, uv, mat);
-        out.hit(h.t, N, uv, mat, want_t, Tg);
         f3 hp = o + (h.t * d);
         f3 so = hp + (N * 1e-4f);
         if (J.dmis) {
