@@ -343,6 +343,14 @@ lx_status lx_trace_closest(lx_scene* scene, const lx_ray* rays, uint32_t n, int3
 /* Any hit: occluded[i] = 1 if some accepted triangle has t < tmax. */
 lx_status lx_trace_occluded(lx_scene* scene, const lx_ray* rays, uint32_t n, int32_t any_hit_alpha,
                             uint32_t iteration, uint8_t* occluded);
+/* No counterpart in the reference, whose direct-light ray is the closest-hit
+ * ray of lx_trace_closest (src/kernels/sample_direct_light.cu:9-66,
+ * src/optixshaders/optix_triangle.cu:259-305): the query the tail kernel
+ * answers that ray with for -dmis jobs.  hits[i] is lx_trace_closest's hit
+ * (alpha any-hit rules on, iteration as given) when the hit instance's
+ * material emits (emission colour x strength non-zero in a channel), else a
+ * miss; a test hook for the query's parity with the closest-hit ray. */
+lx_status lx_trace_emitting(lx_scene* scene, const lx_ray* rays, uint32_t n, uint32_t iteration, lx_hit* hits);
 
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that

@@ -183,6 +183,16 @@ def trace_occluded(scene, rays, alpha=True, iteration=1):
     return out
 
 
+def trace_emitting(scene, rays, iteration=1):
+    """The direct-light query of the tail kernel: trace_closest's hit (alpha
+    rules on) where its instance's material emits, a miss elsewhere."""
+    rays = np.ascontiguousarray(rays, np.float32)
+    out = np.zeros(len(rays), dtype=[("t", "<f4"), ("instance", "<u4"), ("prim", "<u4"), ("u", "<f4"),
+                                     ("v", "<f4")])
+    check(scene.lib.lx_trace_emitting(scene.h, rays.ctypes.data, len(rays), iteration, out.ctypes.data))
+    return out
+
+
 class Render:
     """pathtraceInit equivalent: per-render device buffers (path state, hit
     records, accumulation)."""

@@ -177,6 +177,7 @@ SIGNATURES = {
     "lx_render_destroy": (None, [_P]),
     "lx_trace_closest": (C.c_int, [_P, _P, C.c_uint32, C.c_int32, C.c_uint32, _P]),
     "lx_trace_occluded": (C.c_int, [_P, _P, C.c_uint32, C.c_int32, C.c_uint32, _P]),
+    "lx_trace_emitting": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, _P]),
     "lx_trace_closest_noalloc": (C.c_int, [_P, _P, C.c_uint32, C.c_int32, C.c_uint32, _P]),
     "lx_debug_bounce": (C.c_int, [_P, C.c_uint32, _P, _P, _P, _P]),
     "lx_debug_math_eval": (C.c_int, [_P, C.c_int32, _P, _P, _P, C.c_uint32]),

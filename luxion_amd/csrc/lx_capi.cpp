@@ -210,7 +210,7 @@ static lx_status trace_hook(lx_scene* s, const lx_ray* rays, uint32_t n, int any
                             lx_hit* hits, uint8_t* occ) {
     LX_TRY
     LX_REQUIRE(s && s->accel_built, LX_ERR_STATE, "call lx_scene_build_accel first");
-    LX_REQUIRE(rays && (any ? occ != nullptr : hits != nullptr), LX_ERR_INVALID, "null argument");
+    LX_REQUIRE(rays && (any == 1 ? occ != nullptr : hits != nullptr), LX_ERR_INVALID, "null argument");
     if (n == 0) return LX_OK;
     lx_context* ctx = s->ctx;
     std::lock_guard<std::mutex> lk(ctx->mu);
@@ -218,15 +218,15 @@ static lx_status trace_hook(lx_scene* s, const lx_ray* rays, uint32_t n, int any
     uint32_t grid = std::max(1u, std::min((n + LX_TRACE_BLOCK - 1) / LX_TRACE_BLOCK, (uint32_t)ctx->num_cus * 8u));
     DevBuf dr, dh, dsp, dov;
     dr.upload(ctx, rays, (size_t)n * sizeof(lx_ray));
-    dh.alloc(ctx, any ? (size_t)n : (size_t)n * sizeof(lx_hit));
+    dh.alloc(ctx, any == 1 ? (size_t)n : (size_t)n * sizeof(lx_hit));
     dsp.alloc(ctx, (size_t)lx_spill_words(grid) * 4);
     dov.alloc(ctx, 256);
     LX_HIP(lx_launch_fill_u32(dov.as<uint32_t>(), 64, 0u, ctx->stream));
     LX_HIP(lx_launch_trace_rays(s->d_scene.as<LxDevScene>(), dr.as<lx_ray>(), n, any, alpha, iteration,
-                                any ? nullptr : dh.as<lx_hit>(), any ? dh.as<uint8_t>() : nullptr,
+                                any == 1 ? nullptr : dh.as<lx_hit>(), any == 1 ? dh.as<uint8_t>() : nullptr,
                                 dsp.as<uint32_t>(), dov.as<uint32_t>(), grid, s->dev.bvh_width, ctx->stream));
     uint32_t ov = 0;
-    if (any) LX_HIP(hipMemcpyAsync(occ, dh.ptr, n, hipMemcpyDeviceToHost, ctx->stream));
+    if (any == 1) LX_HIP(hipMemcpyAsync(occ, dh.ptr, n, hipMemcpyDeviceToHost, ctx->stream));
     else LX_HIP(hipMemcpyAsync(hits, dh.ptr, (size_t)n * sizeof(lx_hit), hipMemcpyDeviceToHost, ctx->stream));
     LX_HIP(hipMemcpyAsync(&ov, dov.ptr, 4, hipMemcpyDeviceToHost, ctx->stream));
     LX_HIP(hipStreamSynchronize(ctx->stream));
@@ -242,6 +242,10 @@ lx_status lx_trace_closest(lx_scene* s, const lx_ray* rays, uint32_t n, int32_t 
 lx_status lx_trace_occluded(lx_scene* s, const lx_ray* rays, uint32_t n, int32_t alpha, uint32_t iteration,
                             uint8_t* occ) {
     return trace_hook(s, rays, n, 1, alpha, iteration, nullptr, occ);
+}
+
+lx_status lx_trace_emitting(lx_scene* s, const lx_ray* rays, uint32_t n, uint32_t iteration, lx_hit* hits) {
+    return trace_hook(s, rays, n, 2, 1, iteration, hits, nullptr);
 }
 
 /* The allocation-free variant of lx_trace_closest: device-resident rays and

@@ -721,11 +721,13 @@ __global__ __launch_bounds__(LX_TRACE_BLOCK) void k_trace_rays(const LxDevScene*
         lx_ray r = rays[i];
         f3 o = mk3(r.origin[0], r.origin[1], r.origin[2]);
         f3 d = mk3(r.dir[0], r.dir[1], r.dir[2]);
-        if (any) {
+        if (any == 1) {
             LxHitD h = lx_trace<true, false, W>(S, o, d, r.tmax, i, iteration, amask, st, overflow);
             occ[i] = h.inst != 0xffffffffu ? 1 : 0;
         } else {
-            LxHitD h = lx_trace<false, false, W>(S, o, d, r.tmax, i, iteration, amask, st, overflow);
+            /* any == 2: the tail kernel's direct-light query (alpha rules on) */
+            LxHitD h = any == 2 ? lx_trace_dl<false, W>(S, o, d, r.tmax, i, iteration, st, overflow, nullptr)
+                                : lx_trace<false, false, W>(S, o, d, r.tmax, i, iteration, amask, st, overflow);
             lx_hit o2;
             o2.t = h.inst == 0xffffffffu ? -1.0f : h.t;
             o2.instance = h.inst;
