@@ -324,15 +324,19 @@ struct LxWork {
  * restores the world-space ray and is consumed in the same step, so an
  * iteration of the traversal loops never spends itself on a sentinel.
  * Returns false when the stack is empty (the ray is done). */
+template <bool EARLY = false>
 LXD bool lx_pop_next(LxStack& st, uint32_t& node, bool& in_blas, f3& o, f3& d, LxSlab& sl, f3 wo, f3 wd, f3 iwd) {
     if (st.sp == 0) return false;
     node = st.pop();
     if (node == LX_SENTINEL) {
+        in_blas = false;
+        /* EARLY: with nothing left above the instance the ray is done, and
+         * the world ray need not be restored */
+        if (EARLY && st.sp == 0) return false;
         o = wo;
         d = wd;
         sl = lx_slab(wo, iwd);
-        in_blas = false;
-        if (st.sp == 0) return false;
+        if (!EARLY && st.sp == 0) return false;
         node = st.pop();
     }
     return true;
@@ -631,7 +635,7 @@ LXD LxHitD lx_trace_pair(const LxDevScene& S, bool has0, f3 o0, f3 d0, float tma
             }
         }
         bool more = true;
-        if (pop) more = lx_pop_next(st, node, in_blas, o, d, sl, wo, wd, iwd);
+        if (pop) more = lx_pop_next<true>(st, node, in_blas, o, d, sl, wo, wd, iwd);
         if (found_any || !more) {
             if (!any) break;
             /* first ray finished: record it and start the second */
